@@ -1,0 +1,708 @@
+// fr_api.hip -- implementation of the C ABI in include/fr.h: context,
+// device buffers, kernel dispatch and the solver verbs.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <type_traits>
+#include <utility>
+#include <vector>
+
+#include "../../include/fr.h"
+#include "fr_host.hpp"
+#include "fr_kernels.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                              \
+  do {                                                                             \
+    hipError_t e_ = (expr);                                                        \
+    if (e_ != hipSuccess) return fail(FR_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <typename F>
+bool dispatch(int D, int P, int NF, F &&f) {
+#define FR_CASE(d, p, nf)                                                                          \
+  if (D == d && P == p && NF == nf) {                                                              \
+    f(std::integral_constant<int, d>{}, std::integral_constant<int, p>{}, std::integral_constant<int, nf>{}); \
+    return true;                                                                                   \
+  }
+  FR_CASE(3, 1, 5) FR_CASE(3, 2, 5) FR_CASE(3, 3, 5) FR_CASE(3, 4, 5) FR_CASE(3, 5, 5)
+  FR_CASE(3, 1, 1) FR_CASE(3, 2, 1) FR_CASE(3, 3, 1) FR_CASE(3, 4, 1) FR_CASE(3, 5, 1)
+  FR_CASE(2, 1, 4) FR_CASE(2, 2, 4) FR_CASE(2, 3, 4) FR_CASE(2, 4, 4) FR_CASE(2, 5, 4)
+  FR_CASE(2, 1, 1) FR_CASE(2, 2, 1) FR_CASE(2, 3, 1) FR_CASE(2, 4, 1) FR_CASE(2, 5, 1)
+#undef FR_CASE
+  return false;
+}
+
+inline unsigned nblk(long n, int b = 256) { return (unsigned)((n + b - 1) / b); }
+
+}  // namespace
+
+struct fr_ctx {
+  fr_config cfg{};
+  int prec = FR_PREC_FP64;
+  frhost::Operators ops;
+  frhost::Connectivity conn;
+  int D = 3, P = 3, NS = 0, NFP = 0, NFPF = 0, NFACE = 0, NF = 0, nE = 0, nRK = 0;
+  double RKa[5] = {0}, RKb[4] = {0};
+  frdev::Phys ph{};
+  hipStream_t own = nullptr, stream = nullptr;
+  // device buffers
+  double *U = nullptr, *U0 = nullptr, *detJ = nullptr, *JG64 = nullptr, *fgeo64 = nullptr, *ownN = nullptr;
+  double *wts = nullptr, *oned64 = nullptr, *part = nullptr;
+  float *U32 = nullptr, *JG32 = nullptr, *fgeo32 = nullptr, *oned32 = nullptr;
+  void *Uf = nullptr;
+  void *divF[4] = {nullptr, nullptr, nullptr, nullptr};
+  int32_t *nbr = nullptr;
+  int8_t *fkind = nullptr;
+  size_t partCap = 0;
+  std::vector<double> X;  // host copy of element node coordinates [ele][node][3]
+  // timing
+  bool timing = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[3];
+  double ms[3] = {0, 0, 0};
+  long cnt[3] = {0, 0, 0};
+
+  size_t nState() const { return (size_t)nE * NF * NS; }
+  size_t tsize() const { return prec == FR_PREC_FP32 ? 4 : 8; }
+};
+
+namespace {
+
+void freeAll(fr_ctx *c) {
+  void *bufs[] = {c->U, c->U0, c->detJ, c->JG64, c->fgeo64, c->ownN, c->wts, c->oned64, c->part, c->U32,
+                  c->JG32, c->fgeo32, c->oned32, c->Uf, c->divF[0], c->divF[1], c->divF[2], c->divF[3],
+                  c->nbr, c->fkind};
+  for (void *b : bufs)
+    if (b) (void)hipFree(b);
+  for (auto &v : c->ev)
+    for (auto &pr : v) {
+      (void)hipEventDestroy(pr.first);
+      (void)hipEventDestroy(pr.second);
+    }
+  if (c->own) (void)hipStreamDestroy(c->own);
+}
+
+template <typename T>
+int dmalloc(T **p, size_t n) {
+  hipError_t e = hipMalloc((void **)p, (n ? n : 1) * sizeof(T));
+  if (e != hipSuccess) return fail(FR_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  return 0;
+}
+
+// Kernel timing bracket on the launch stream.
+struct TimeScope {
+  fr_ctx *c;
+  int id;
+  hipEvent_t a = nullptr, b = nullptr;
+  TimeScope(fr_ctx *c_, int id_) : c(c_), id(id_) {
+    if (c->timing) {
+      (void)hipEventCreate(&a);
+      (void)hipEventCreate(&b);
+      (void)hipEventRecord(a, c->stream);
+    }
+  }
+  ~TimeScope() {
+    if (c->timing) {
+      (void)hipEventRecord(b, c->stream);
+      c->ev[id].push_back({a, b});
+    }
+  }
+};
+
+template <typename T>
+int launchResidual(fr_ctx *c, int step) {
+  const T *U = std::is_same<T, float>::value ? (const T *)c->U32 : (const T *)c->U;
+  const T *JG = std::is_same<T, float>::value ? (const T *)c->JG32 : (const T *)c->JG64;
+  const T *fg = std::is_same<T, float>::value ? (const T *)c->fgeo32 : (const T *)c->fgeo64;
+  const T *o1 = std::is_same<T, float>::value ? (const T *)c->oned32 : (const T *)c->oned64;
+  T *Uf = (T *)c->Uf;
+  T *out = (T *)c->divF[step];
+  hipError_t err = hipSuccess;
+  bool ok = dispatch(c->D, c->P, c->NF, [&](auto Dc, auto Pc, auto NFc) {
+    constexpr int D = decltype(Dc)::value, P = decltype(Pc)::value, NF = decltype(NFc)::value;
+    using G = frdev::ElemGeo<D, P>;
+    {
+      TimeScope ts(c, 1);
+      hipLaunchKernelGGL((frdev::k_trace<D, P, NF, T>), dim3(nblk((long)c->nE * G::NFP)), dim3(256), 0, c->stream,
+                         U, Uf, o1, c->nE);
+    }
+    frdev::ResArgs<T> a;
+    a.U = U;
+    a.Uf = Uf;
+    a.JG = JG;
+    a.fgeo = fg;
+    a.nbr = c->nbr;
+    a.fkind = c->fkind;
+    a.oned = o1;
+    a.divF = out;
+    a.nEles = c->nE;
+    a.ph = c->ph;
+    {
+      TimeScope ts(c, 0);
+      hipLaunchKernelGGL((frdev::k_residual<D, P, NF, T>), dim3((c->nE + G::EPB - 1) / G::EPB), dim3(G::NTH), 0,
+                         c->stream, a);
+    }
+    err = hipGetLastError();
+  });
+  if (!ok) return fail(FR_EINVAL, "no kernel instance for this (nDims, order, nFields)");
+  if (err != hipSuccess) return fail(FR_EHIP, std::string("residual launch: ") + hipGetErrorString(err));
+  return 0;
+}
+
+int residual(fr_ctx *c, int step) {
+  if (step < 0 || step >= c->nRK) return fail(FR_EINVAL, "step out of range");
+  return c->prec == FR_PREC_FP32 ? launchResidual<float>(c, step) : launchResidual<double>(c, step);
+}
+
+template <typename TD>
+int rkA(fr_ctx *c, int step, double rk, int copy) {
+  long n = (long)c->nState();
+  TimeScope ts(c, 2);
+  hipLaunchKernelGGL((frdev::k_rk_a<TD>), dim3(nblk(n)), dim3(256), 0, c->stream, c->U, c->U0,
+                     (const TD *)c->divF[step], c->detJ, c->U32, rk, c->cfg.dt, copy, c->NF, c->NS, n);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+template <typename TD>
+int rkB(fr_ctx *c) {
+  long n = (long)c->nState();
+  TimeScope ts(c, 2);
+  const TD *d[4];
+  for (int s = 0; s < 4; s++) d[s] = (const TD *)c->divF[s < c->nRK ? s : 0];
+  hipLaunchKernelGGL((frdev::k_rk_b<TD>), dim3(nblk(n)), dim3(256), 0, c->stream, c->U, c->U0, d[0], d[1], d[2],
+                     d[3], c->detJ, c->U32, c->RKb[0], c->RKb[1], c->RKb[2], c->RKb[3], c->cfg.dt, c->nRK, c->NF,
+                     c->NS, n);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+// solver::update (solver.cpp:212-250) with the stage passes fused:
+// copy+timeStepA for stage 0, timeStepA for later stages, and one pass for
+// copyU0_Uspts + all timeStepB subtractions.
+int updateOnce(fr_ctx *c) {
+  int rc;
+  for (int step = 0; step < c->nRK - 1; step++) {
+    if ((rc = residual(c, step))) return rc;
+    rc = (c->prec == FR_PREC_FP32) ? rkA<float>(c, step, c->RKa[step + 1], step == 0)
+                                   : rkA<double>(c, step, c->RKa[step + 1], step == 0);
+    if (rc) return rc;
+  }
+  if ((rc = residual(c, c->nRK - 1))) return rc;
+  if (c->cfg.timeType < 5) {
+    rc = (c->prec == FR_PREC_FP32) ? rkB<float>(c) : rkB<double>(c);
+  } else {
+    rc = (c->prec == FR_PREC_FP32) ? rkA<float>(c, c->nRK - 1, c->RKa[c->nRK], 0)
+                                   : rkA<double>(c, c->nRK - 1, c->RKa[c->nRK], 0);
+  }
+  return rc;
+}
+
+// reference layout [pt][ele][field] <-> device [ele][field][pt]
+void refToDev(const double *src, double *dst, int nE, int NF, int NP) {
+  for (int p = 0; p < NP; p++)
+    for (int e = 0; e < nE; e++)
+      for (int k = 0; k < NF; k++) dst[((size_t)e * NF + k) * NP + p] = src[((size_t)p * nE + e) * NF + k];
+}
+void devToRef(const double *src, double *dst, int nE, int NF, int NP) {
+  for (int p = 0; p < NP; p++)
+    for (int e = 0; e < nE; e++)
+      for (int k = 0; k < NF; k++) dst[((size_t)p * nE + e) * NF + k] = src[((size_t)e * NF + k) * NP + p];
+}
+
+int syncU32(fr_ctx *c) {
+  if (c->prec != FR_PREC_FP32) return 0;
+  long n = (long)c->nState();
+  hipLaunchKernelGGL(frdev::k_to_float, dim3(nblk(n)), dim3(256), 0, c->stream, c->U, c->U32, n);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+int createImpl(const fr_config *cfg, const fr_mesh *m, int precision, fr_ctx *c) {
+  c->cfg = *cfg;
+  c->prec = precision;
+  if (precision != FR_PREC_FP64 && precision != FR_PREC_FP32) return fail(FR_EINVAL, "precision must be 0 or 1");
+  if (cfg->equation != 0 && cfg->equation != 1) return fail(FR_EINVAL, "equation must be 0 or 1");
+  if (cfg->nDims == 3 && cfg->equation == 1 && cfg->riemannType == 1)
+    return fail(FR_EINVAL, "Roe not implemented in 3D");
+  if (cfg->riemannType != 0 && cfg->riemannType != 1) return fail(FR_EINVAL, "riemannType must be 0 or 1");
+  if (cfg->viscous) return fail(FR_EINVAL, "viscous residual is not available in this build");
+  if (!m || m->nEles <= 0) return fail(FR_EMESH, "Cannot setup elements array - nEles = 0");
+  if (m->nNodes != (cfg->nDims == 3 ? 8 : 4)) return fail(FR_EINVAL, "only linear hexes (8 nodes) or quads (4)");
+  for (int i = 0; i < m->nEles * m->nNodes; i++)
+    if (m->c2v[i] < 0 || m->c2v[i] >= m->nVerts) return fail(FR_EMESH, "c2v vertex id out of range");
+  // RK tables (input.cpp:424-461)
+  switch (cfg->timeType) {
+    case 0: c->nRK = 1; c->RKb[0] = 1; break;
+    case 2: c->nRK = 2; c->RKa[1] = 1.; c->RKb[0] = 1. / 2.; c->RKb[1] = 1. / 2.; break;
+    case 4:
+      c->nRK = 4;
+      c->RKa[1] = .5; c->RKa[2] = .5; c->RKa[3] = 1.;
+      c->RKb[0] = 1. / 6.; c->RKb[1] = 1. / 3.; c->RKb[2] = 1. / 3.; c->RKb[3] = 1. / 6.;
+      break;
+    case 5: c->nRK = 4; c->RKa[1] = 1. / 4.; c->RKa[2] = 1. / 3.; c->RKa[3] = 1. / 2.; c->RKa[4] = 1.0; break;
+    case 6:
+      c->nRK = 4;
+      if (cfg->order <= 2) { c->RKa[1] = .153; c->RKa[2] = .442; c->RKa[3] = .931; c->RKa[4] = 1.0; }
+      else { c->RKa[1] = .158; c->RKa[2] = .477; c->RKa[3] = 1.; c->RKa[4] = 1.; }
+      break;
+    default: return fail(FR_EINVAL, "Time-Stepping type not supported.");
+  }
+  std::string err = frhost::buildOperators(*cfg, c->ops);
+  if (!err.empty()) return fail(FR_EINVAL, err);
+  err = frhost::buildConnectivity(*cfg, *m, c->ops, c->conn);
+  if (!err.empty()) return fail(FR_EMESH, err);
+  c->D = cfg->nDims;
+  c->P = cfg->order;
+  c->NS = c->ops.nSpts;
+  c->NFP = c->ops.nFpts;
+  c->NFPF = c->ops.nFptsFace;
+  c->NFACE = c->ops.nFacesEle;
+  c->NF = (cfg->equation == 0) ? 1 : c->D + 2;
+  c->nE = m->nEles;
+  if ((double)c->nE * c->NF * c->NFP >= 2147483647.0) return fail(FR_EINVAL, "mesh too large for 32-bit face indices");
+  frdev::Phys &ph = c->ph;
+  ph.gamma = cfg->gamma; ph.lambda = cfg->lambda; ph.diffD = cfg->diffD;
+  ph.advect[0] = cfg->advectVx; ph.advect[1] = cfg->advectVy; ph.advect[2] = cfg->advectVz;
+  ph.rhoBound = cfg->rhoBound; ph.uBound = cfg->uBound; ph.vBound = cfg->vBound; ph.wBound = cfg->wBound;
+  ph.pBound = cfg->pBound; ph.TWall = cfg->TWall; ph.RGas = cfg->RGas;
+  ph.uWall = cfg->uWall; ph.vWall = cfg->vWall; ph.wWall = cfg->wWall;
+  ph.penFact = cfg->penFact; ph.tau = cfg->tau; ph.mu_inf = cfg->mu_inf; ph.prandtl = cfg->prandtl;
+  ph.rt_inf = cfg->rt_inf; ph.c_sth = cfg->c_sth;
+  ph.equation = cfg->equation; ph.riemannType = cfg->riemannType; ph.fixVis = cfg->fixVis; ph.viscFix = cfg->viscFix;
+
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(FR_EHIP, "no HIP device available");
+  HIP_TRY(hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking));
+  c->stream = c->own;
+
+  const int D = c->D, nE = c->nE, NS = c->NS, NFP = c->NFP, nN = m->nNodes;
+  const size_t nSt = c->nState();
+  int rc;
+  if ((rc = dmalloc(&c->U, nSt)) || (rc = dmalloc(&c->U0, nSt)) || (rc = dmalloc(&c->detJ, (size_t)nE * NS)) ||
+      (rc = dmalloc(&c->JG64, (size_t)nE * D * D * NS)) || (rc = dmalloc(&c->fgeo64, (size_t)nE * (D + 1) * NFP)) ||
+      (rc = dmalloc(&c->ownN, (size_t)nE * D * NFP)) || (rc = dmalloc(&c->wts, NS)) ||
+      (rc = dmalloc(&c->oned64, c->ops.oned.size())) || (rc = dmalloc(&c->nbr, (size_t)nE * NFP)) ||
+      (rc = dmalloc(&c->fkind, (size_t)nE * c->NFACE)))
+    return rc;
+  HIP_TRY(hipMemset(c->U, 0, nSt * 8));
+  HIP_TRY(hipMemset(c->U0, 0, nSt * 8));
+  const size_t ts = c->tsize();
+  HIP_TRY(hipMalloc(&c->Uf, (size_t)nE * c->NF * NFP * ts));
+  for (int s = 0; s < c->nRK; s++) {
+    HIP_TRY(hipMalloc(&c->divF[s], nSt * ts));
+    HIP_TRY(hipMemset(c->divF[s], 0, nSt * ts));
+  }
+  if (c->prec == FR_PREC_FP32) {
+    if ((rc = dmalloc(&c->U32, nSt)) || (rc = dmalloc(&c->JG32, (size_t)nE * D * D * NS)) ||
+        (rc = dmalloc(&c->fgeo32, (size_t)nE * (D + 1) * NFP)) || (rc = dmalloc(&c->oned32, c->ops.oned.size())))
+      return rc;
+    HIP_TRY(hipMemset(c->U32, 0, nSt * 4));
+  }
+  // geometry on device
+  {
+    std::vector<double> &X = c->X;
+    X.assign((size_t)nE * nN * 3, 0.0);
+    for (int e = 0; e < nE; e++)
+      for (int n = 0; n < nN; n++)
+        for (int d = 0; d < 3; d++) X[((size_t)e * nN + n) * 3 + d] = m->xv[3 * (size_t)m->c2v[(size_t)e * nN + n] + d];
+    double *dX, *dS, *dF, *dT;
+    int *dNeg;
+    if ((rc = dmalloc(&dX, X.size())) || (rc = dmalloc(&dS, c->ops.dshapeSpts.size())) ||
+        (rc = dmalloc(&dF, c->ops.dshapeFpts.size())) || (rc = dmalloc(&dT, c->ops.tNorm.size())) ||
+        (rc = dmalloc(&dNeg, 1)))
+      return rc;
+    int32_t *pairEF;
+    if ((rc = dmalloc(&pairEF, (size_t)nE * NFP))) return rc;
+    HIP_TRY(hipMemcpy(dX, X.data(), X.size() * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dS, c->ops.dshapeSpts.data(), c->ops.dshapeSpts.size() * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dF, c->ops.dshapeFpts.data(), c->ops.dshapeFpts.size() * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dT, c->ops.tNorm.data(), c->ops.tNorm.size() * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemset(dNeg, 0, sizeof(int)));
+    HIP_TRY(hipMemcpy(pairEF, c->conn.nbr.data(), (size_t)nE * NFP * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->fkind, c->conn.fkind.data(), c->conn.fkind.size(), hipMemcpyHostToDevice));
+    long np = (long)nE * (NS + NFP);
+    if (D == 3)
+      hipLaunchKernelGGL(frdev::k_metrics<3>, dim3(nblk(np)), dim3(256), 0, c->stream, dX, dS, dF, dT, NS, NFP, nN,
+                         nE, c->JG64, c->detJ, c->ownN, c->fgeo64, dNeg);
+    else
+      hipLaunchKernelGGL(frdev::k_metrics<2>, dim3(nblk(np)), dim3(256), 0, c->stream, dX, dS, dF, dT, NS, NFP, nN,
+                         nE, c->JG64, c->detJ, c->ownN, c->fgeo64, dNeg);
+    HIP_TRY(hipGetLastError());
+    long nq = (long)nE * NFP;
+    if (D == 3)
+      hipLaunchKernelGGL(frdev::k_fgeo<3>, dim3(nblk(nq)), dim3(256), 0, c->stream, c->ownN, pairEF, c->fkind, NFP,
+                         c->NFPF, nE, c->fgeo64);
+    else
+      hipLaunchKernelGGL(frdev::k_fgeo<2>, dim3(nblk(nq)), dim3(256), 0, c->stream, c->ownN, pairEF, c->fkind, NFP,
+                         c->NFPF, nE, c->fgeo64);
+    HIP_TRY(hipGetLastError());
+    int neg = 0;
+    HIP_TRY(hipMemcpyAsync(&neg, dNeg, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    (void)hipFree(dX); (void)hipFree(dS); (void)hipFree(dF); (void)hipFree(dT); (void)hipFree(dNeg);
+    (void)hipFree(pairEF);
+    if (neg) return fail(FR_EJACOBIAN, "Negative Jacobian at solution points.");
+  }
+  // neighbour trace bases: (ele'*NFP + fpt') -> ele'*NF*NFP + fpt'
+  {
+    std::vector<int32_t> nb(c->conn.nbr.size());
+    for (size_t i = 0; i < nb.size(); i++) {
+      int32_t g = c->conn.nbr[i];
+      if (g < 0) { nb[i] = -1; continue; }
+      int32_t e2 = g / NFP, q2 = g - e2 * NFP;
+      nb[i] = e2 * c->NF * NFP + q2;
+    }
+    HIP_TRY(hipMemcpy(c->nbr, nb.data(), nb.size() * 4, hipMemcpyHostToDevice));
+  }
+  HIP_TRY(hipMemcpy(c->wts, c->ops.wts.data(), NS * 8, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(c->oned64, c->ops.oned.data(), c->ops.oned.size() * 8, hipMemcpyHostToDevice));
+  if (c->prec == FR_PREC_FP32) {
+    std::vector<float> o32(c->ops.oned.begin(), c->ops.oned.end());
+    HIP_TRY(hipMemcpy(c->oned32, o32.data(), o32.size() * 4, hipMemcpyHostToDevice));
+    long n1 = (long)nE * D * D * NS, n2 = (long)nE * (D + 1) * NFP;
+    hipLaunchKernelGGL(frdev::k_to_float, dim3(nblk(n1)), dim3(256), 0, c->stream, c->JG64, c->JG32, n1);
+    hipLaunchKernelGGL(frdev::k_to_float, dim3(nblk(n2)), dim3(256), 0, c->stream, c->fgeo64, c->fgeo32, n2);
+    HIP_TRY(hipGetLastError());
+  }
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int needPart(fr_ctx *c, size_t n) {
+  if (c->partCap >= n) return 0;
+  if (c->part) (void)hipFree(c->part);
+  c->part = nullptr;
+  c->partCap = 0;
+  int rc = dmalloc(&c->part, n);
+  if (!rc) c->partCap = n;
+  return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *fr_last_error(void) { return g_err.c_str(); }
+
+int fr_create(const fr_config *cfg, const fr_mesh *mesh, int precision, fr_ctx **out) {
+  if (!cfg || !mesh || !out) return fail(FR_EINVAL, "null argument");
+  *out = nullptr;
+  fr_ctx *c = new fr_ctx();
+  int rc = createImpl(cfg, mesh, precision, c);
+  if (rc) {
+    std::string msg = g_err;
+    freeAll(c);
+    delete c;
+    g_err = msg;
+    return rc;
+  }
+  *out = c;
+  return 0;
+}
+
+int fr_destroy(fr_ctx *c) {
+  if (!c) return 0;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  freeAll(c);
+  delete c;
+  return 0;
+}
+
+int fr_get_sizes(fr_ctx *c, int *o) {
+  if (!c || !o) return fail(FR_EINVAL, "null argument");
+  o[0] = c->NS; o[1] = c->NFP; o[2] = c->nE; o[3] = c->NF;
+  o[4] = c->D; o[5] = (int)c->conn.faces.size(); o[6] = c->conn.nIntFaces; o[7] = c->nRK;
+  return 0;
+}
+
+int fr_get_operator(fr_ctx *c, int which, double *out) {
+  if (!c || !out) return fail(FR_EINVAL, "null argument");
+  const std::vector<double> *v = nullptr;
+  switch (which) {
+    case 0: v = &c->ops.spts_to_fpts; break;
+    case 1: v = &c->ops.grad_spts; break;
+    case 2: v = &c->ops.extrapolateFn; break;
+    case 3: v = &c->ops.correction; break;
+    case 4: v = &c->ops.correctU; break;
+    default: return fail(FR_EINVAL, "unknown operator id");
+  }
+  std::memcpy(out, v->data(), v->size() * 8);
+  return 0;
+}
+
+int fr_get_geometry(fr_ctx *c, int which, double *out) {
+  if (!c || !out) return fail(FR_EINVAL, "null argument");
+  const int D = c->D, nE = c->nE, NS = c->NS, NFP = c->NFP;
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (which == 0) {
+    std::vector<double> h((size_t)nE * D * D * NS);
+    HIP_TRY(hipMemcpy(h.data(), c->JG64, h.size() * 8, hipMemcpyDeviceToHost));
+    for (int d1 = 0; d1 < D; d1++)
+      for (int s = 0; s < NS; s++)
+        for (int e = 0; e < nE; e++)
+          for (int d2 = 0; d2 < D; d2++)
+            out[(((size_t)d1 * NS + s) * nE + e) * D + d2] = h[((size_t)e * D * D + d1 * D + d2) * NS + s];
+  } else if (which == 1) {
+    std::vector<double> h((size_t)nE * NS);
+    HIP_TRY(hipMemcpy(h.data(), c->detJ, h.size() * 8, hipMemcpyDeviceToHost));
+    for (int s = 0; s < NS; s++)
+      for (int e = 0; e < nE; e++) out[(size_t)s * nE + e] = h[(size_t)e * NS + s];
+  } else if (which == 2) {
+    std::vector<double> h((size_t)nE * D * NFP);
+    HIP_TRY(hipMemcpy(h.data(), c->ownN, h.size() * 8, hipMemcpyDeviceToHost));
+    devToRef(h.data(), out, nE, D, NFP);
+  } else if (which == 3) {
+    std::vector<double> h((size_t)nE * (D + 1) * NFP);
+    HIP_TRY(hipMemcpy(h.data(), c->fgeo64, h.size() * 8, hipMemcpyDeviceToHost));
+    for (int q = 0; q < NFP; q++)
+      for (int e = 0; e < nE; e++) out[(size_t)q * nE + e] = h[((size_t)e * (D + 1) + D) * NFP + q];
+  } else if (which == 4) {
+    // pos_spts(spt, ele, dim) = sum_n shape(spt, n) * node(n, ele, dim)
+    // (solver::setPosSptsFpts, solver.cpp:904-951)
+    const int nN = c->ops.nNodes;
+    for (int s = 0; s < NS; s++)
+      for (int e = 0; e < nE; e++)
+        for (int d = 0; d < D; d++) {
+          double acc = 0.0;
+          for (int n = 0; n < nN; n++) acc += c->ops.shapeSpts[(size_t)s * nN + n] * c->X[((size_t)e * nN + n) * 3 + d];
+          out[((size_t)s * nE + e) * D + d] = acc;
+        }
+  } else {
+    return fail(FR_EINVAL, "unknown geometry id");
+  }
+  return 0;
+}
+
+int fr_get_faces(fr_ctx *c, int *o6) {
+  if (!c || !o6) return fail(FR_EINVAL, "null argument");
+  for (size_t f = 0; f < c->conn.faces.size(); f++) {
+    const frhost::FaceRec &r = c->conn.faces[f];
+    o6[6 * f + 0] = r.eL; o6[6 * f + 1] = r.fL; o6[6 * f + 2] = r.eR;
+    o6[6 * f + 3] = r.fR; o6[6 * f + 4] = r.rot; o6[6 * f + 5] = r.bc;
+  }
+  return 0;
+}
+
+int fr_set_state(fr_ctx *c, const double *U, int layout) {
+  if (!c || !U) return fail(FR_EINVAL, "null argument");
+  const size_t n = c->nState();
+  if (layout == FR_LAYOUT_REF) {
+    std::vector<double> h(n);
+    refToDev(U, h.data(), c->nE, c->NF, c->NS);
+    HIP_TRY(hipMemcpyAsync(c->U, h.data(), n * 8, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+  } else if (layout == FR_LAYOUT_DEVICE) {
+    HIP_TRY(hipMemcpyAsync(c->U, U, n * 8, hipMemcpyHostToDevice, c->stream));
+  } else {
+    return fail(FR_EINVAL, "unknown layout");
+  }
+  int rc = syncU32(c);
+  if (rc) return rc;
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int fr_get_state(fr_ctx *c, double *U, int layout) {
+  if (!c || !U) return fail(FR_EINVAL, "null argument");
+  const size_t n = c->nState();
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (layout == FR_LAYOUT_REF) {
+    std::vector<double> h(n);
+    HIP_TRY(hipMemcpy(h.data(), c->U, n * 8, hipMemcpyDeviceToHost));
+    devToRef(h.data(), U, c->nE, c->NF, c->NS);
+  } else {
+    HIP_TRY(hipMemcpy(U, c->U, n * 8, hipMemcpyDeviceToHost));
+  }
+  return 0;
+}
+
+int fr_calc_residual(fr_ctx *c, int step) {
+  if (!c) return fail(FR_EINVAL, "null argument");
+  return residual(c, step);
+}
+
+int fr_get_residual(fr_ctx *c, int step, double *out, int layout) {
+  if (!c || !out) return fail(FR_EINVAL, "null argument");
+  if (step < 0 || step >= c->nRK) return fail(FR_EINVAL, "step out of range");
+  const size_t n = c->nState();
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  std::vector<double> h(n);
+  if (c->prec == FR_PREC_FP32) {
+    std::vector<float> f(n);
+    HIP_TRY(hipMemcpy(f.data(), c->divF[step], n * 4, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; i++) h[i] = f[i];
+  } else {
+    HIP_TRY(hipMemcpy(h.data(), c->divF[step], n * 8, hipMemcpyDeviceToHost));
+  }
+  if (layout == FR_LAYOUT_REF) devToRef(h.data(), out, c->nE, c->NF, c->NS);
+  else std::memcpy(out, h.data(), n * 8);
+  return 0;
+}
+
+int fr_get_traces(fr_ctx *c, double *out) {
+  if (!c || !out) return fail(FR_EINVAL, "null argument");
+  const size_t n = (size_t)c->nE * c->NF * c->NFP;
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  std::vector<double> h(n);
+  if (c->prec == FR_PREC_FP32) {
+    std::vector<float> f(n);
+    HIP_TRY(hipMemcpy(f.data(), c->Uf, n * 4, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; i++) h[i] = f[i];
+  } else {
+    HIP_TRY(hipMemcpy(h.data(), c->Uf, n * 8, hipMemcpyDeviceToHost));
+  }
+  devToRef(h.data(), out, c->nE, c->NF, c->NFP);
+  return 0;
+}
+
+int fr_update(fr_ctx *c, int nSteps) {
+  if (!c) return fail(FR_EINVAL, "null argument");
+  for (int i = 0; i < nSteps; i++) {
+    int rc = updateOnce(c);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+int fr_time_step_a(fr_ctx *c, int step, double rkval) {
+  if (!c) return fail(FR_EINVAL, "null argument");
+  if (step < 0 || step >= c->nRK) return fail(FR_EINVAL, "step out of range");
+  return c->prec == FR_PREC_FP32 ? rkA<float>(c, step, rkval, 0) : rkA<double>(c, step, rkval, 0);
+}
+
+int fr_time_step_b(fr_ctx *c, int step) {
+  if (!c) return fail(FR_EINVAL, "null argument");
+  if (step < 0 || step >= c->nRK) return fail(FR_EINVAL, "step out of range");
+  long n = (long)c->nState();
+  if (c->prec == FR_PREC_FP32)
+    hipLaunchKernelGGL(frdev::k_rk_b1<float>, dim3(nblk(n)), dim3(256), 0, c->stream, c->U,
+                       (const float *)c->divF[step], c->detJ, c->U32, c->RKb[step], c->cfg.dt, c->NF, c->NS, n);
+  else
+    hipLaunchKernelGGL(frdev::k_rk_b1<double>, dim3(nblk(n)), dim3(256), 0, c->stream, c->U,
+                       (const double *)c->divF[step], c->detJ, c->U32, c->RKb[step], c->cfg.dt, c->NF, c->NS, n);
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+int fr_copy_u_u0(fr_ctx *c) {
+  if (!c) return fail(FR_EINVAL, "null argument");
+  HIP_TRY(hipMemcpyAsync(c->U0, c->U, c->nState() * 8, hipMemcpyDeviceToDevice, c->stream));
+  return 0;
+}
+
+int fr_copy_u0_u(fr_ctx *c) {
+  if (!c) return fail(FR_EINVAL, "null argument");
+  HIP_TRY(hipMemcpyAsync(c->U, c->U0, c->nState() * 8, hipMemcpyDeviceToDevice, c->stream));
+  return syncU32(c);
+}
+
+int fr_reduce(fr_ctx *c, int kind, double *out) {
+  if (!c || !out) return fail(FR_EINVAL, "null argument");
+  const int nb = (int)nblk(c->nE);
+  int rc;
+  if (kind >= 1 && kind <= 3) {
+    if ((rc = needPart(c, (size_t)nb * 5))) return rc;
+    if (c->prec == FR_PREC_FP32)
+      hipLaunchKernelGGL(frdev::k_norm<float>, dim3(nb), dim3(256), 0, c->stream, (const float *)c->divF[0],
+                         c->detJ, c->wts, kind, c->NF, c->NS, c->nE, c->part);
+    else
+      hipLaunchKernelGGL(frdev::k_norm<double>, dim3(nb), dim3(256), 0, c->stream, (const double *)c->divF[0],
+                         c->detJ, c->wts, kind, c->NF, c->NS, c->nE, c->part);
+    HIP_TRY(hipGetLastError());
+    std::vector<double> h((size_t)nb * 5);
+    HIP_TRY(hipMemcpyAsync(h.data(), c->part, h.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (int k = 0; k < c->NF; k++) {
+      double r = 0;
+      for (int b = 0; b < nb; b++) r = (kind == 3) ? std::fmax(r, h[(size_t)b * 5 + k]) : r + h[(size_t)b * 5 + k];
+      if (kind == 2) r = std::sqrt(r);
+      if (std::isnan(r)) return fail(FR_ENAN, "NaN Encountered in Solution Residual!");
+      out[k] = r;
+    }
+    return 0;
+  }
+  if (kind == 10) {
+    if (c->cfg.equation != 1) return fail(FR_EINVAL, "kinetic energy needs the Euler / Navier-Stokes equations");
+    if ((rc = needPart(c, (size_t)nb * 2))) return rc;
+    if (c->D == 3)
+      hipLaunchKernelGGL(frdev::k_ke<3>, dim3(nb), dim3(256), 0, c->stream, c->U, c->detJ, c->wts, c->NS, c->nE,
+                         c->part);
+    else
+      hipLaunchKernelGGL(frdev::k_ke<2>, dim3(nb), dim3(256), 0, c->stream, c->U, c->detJ, c->wts, c->NS, c->nE,
+                         c->part);
+    HIP_TRY(hipGetLastError());
+    std::vector<double> h((size_t)nb * 2);
+    HIP_TRY(hipMemcpyAsync(h.data(), c->part, h.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    double ke = 0, vol = 0;
+    for (int b = 0; b < nb; b++) {
+      ke += h[(size_t)b * 2];
+      vol += h[(size_t)b * 2 + 1];
+    }
+    out[0] = ke;
+    out[1] = vol;
+    return 0;
+  }
+  return fail(FR_EINVAL, "unknown reduction kind");
+}
+
+int fr_set_stream(fr_ctx *c, void *stream) {
+  if (!c) return fail(FR_EINVAL, "null argument");
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  c->stream = stream ? (hipStream_t)stream : c->own;
+  return 0;
+}
+
+int fr_synchronize(fr_ctx *c) {
+  if (!c) return fail(FR_EINVAL, "null argument");
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int fr_set_timing(fr_ctx *c, int on) {
+  if (!c) return fail(FR_EINVAL, "null argument");
+  c->timing = on != 0;
+  return 0;
+}
+
+int fr_kernel_times(fr_ctx *c, int kernel, double *ms, long *count) {
+  if (!c || kernel < 0 || kernel > 2) return fail(FR_EINVAL, "bad argument");
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  for (int k = 0; k < 3; k++) {
+    for (auto &pr : c->ev[k]) {
+      float t = 0;
+      HIP_TRY(hipEventElapsedTime(&t, pr.first, pr.second));
+      c->ms[k] += t;
+      c->cnt[k] += 1;
+      (void)hipEventDestroy(pr.first);
+      (void)hipEventDestroy(pr.second);
+    }
+    c->ev[k].clear();
+  }
+  if (ms) *ms = c->ms[kernel];
+  if (count) *count = c->cnt[kernel];
+  c->ms[kernel] = 0;
+  c->cnt[kernel] = 0;
+  return 0;
+}
+
+int fr_residual_bytes(fr_ctx *c, double *bytes) {
+  if (!c || !bytes) return fail(FR_EINVAL, "null argument");
+  const double s = (double)c->tsize(), F = (double)c->conn.faces.size();
+  *bytes = (double)c->nE * c->NS * (2.0 * c->NF * s + (double)c->D * c->D * s) +
+           F * c->NFPF * (c->D + 1) * s + 20.0 * F;
+  return 0;
+}
+
+}  // extern "C"

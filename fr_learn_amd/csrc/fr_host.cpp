@@ -1,0 +1,510 @@
+// fr_host.cpp -- FR operator construction and face connectivity (host side).
+//
+// Operators: the dense matrices are produced from 1D Lagrange / correction
+// factors exactly as the reference's oper::setupOperators does
+// (src/operators.cpp:13-111, 660-1092, 1544-1619; polynomials from
+// src/polynomials.cpp:9-118, 392-475), so every entry has the same value.  The
+// kernels never apply them densely: each row holds at most P+1 (extrapolation,
+// gradient) or 2*nDims (correction) nonzeros.  The kernels enumerate them with
+// the index maps of fr_layout.hpp, in ascending column order so that a sparse
+// row sum adds the same terms in the same order as the dense product, and
+// verifyIndexMaps() checks those maps against the dense matrices here.
+//
+// Connectivity: faces are matched on sorted canonical vertex lists (the
+// reference does the same with quadratic scans, src/geo.cpp:75-267); the left
+// element of an interior face is its lowest-index owner (geo.cpp:222-233), the
+// relative rotation follows geo::compareOrientation (geo.cpp:789-917) and the
+// right flux-point map follows intFace::setupRightState (intFace.cpp:29-65).
+#include "fr_host.hpp"
+
+#include "fr_layout.hpp"
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstring>
+#include <unordered_set>
+
+namespace frhost {
+
+namespace {
+
+// 1D abscissae / weights.  The values are the reference's 15-digit tables
+// (src/points.cpp:169-253 Gauss-Legendre, 254-330 Gauss-Lobatto, 376-469
+// weights), kept digit for digit because operator parity depends on them.
+const double kGL[7][7] = {
+    {0.0},
+    {-0.577350269189626, 0.577350269189626},
+    {-0.774596669241483, 0.000000000000000, 0.774596669241483},
+    {-0.861136311594053, -0.339981043584856, 0.339981043584856, 0.861136311594053},
+    {-0.906179845938664, -0.538469310105683, 0.000000000000000, 0.538469310105683, 0.906179845938664},
+    {-0.932469514203152, -0.661209386466264, -0.238619186083197, 0.238619186083197, 0.661209386466265,
+     0.932469514203152},
+    {-0.949107912342758, -0.741531185599395, -0.405845151377397, 0.000000000000000, 0.405845151377397,
+     0.741531185599394, 0.949107912342758}};
+const double kGLL[7][7] = {
+    {0.0},
+    {-1.000000000000000, 1.000000000000000},
+    {-1.000000000000000, -0.000000000000000, 1.000000000000000},
+    {-1.000000000000000, -0.447213595499958, 0.447213595499958, 1.000000000000000},
+    {-1.000000000000000, -0.654653670707977, -0.000000000000000, 0.654653670707977, 1.000000000000000},
+    {-1.000000000000000, -0.765055323929465, -0.285231516480645, 0.285231516480645, 0.765055323929465,
+     1.000000000000000},
+    {-1.000000000000000, -0.830223896278567, -0.468848793470714, 0.000000000000000, 0.468848793470714,
+     0.830223896278567, 1.000000000000000}};
+const double kGW[7][7] = {
+    {2.0},
+    {1.0, 1.0},
+    {0.5555555555555556, 0.8888888888888888, 0.5555555555555556},
+    {0.3478548451374538, 0.6521451548625461, 0.6521451548625461, 0.3478548451374538},
+    {0.2369268850561891, 0.4786286704993665, 0.5688888888888889, 0.4786286704993665, 0.2369268850561891},
+    {0.1713244923791704, 0.3607615730481386, 0.4679139345726910, 0.4679139345726910, 0.3607615730481386,
+     0.1713244923791704},
+    {0.1294849661688697, 0.2797053914892766, 0.3818300505051189, 0.4179591836734694, 0.3818300505051189,
+     0.2797053914892766, 0.1294849661688697}};
+
+// Lagrange basis l_mode(y) on nodes x (product form of polynomials.cpp:9-19).
+double lag(const std::vector<double> &x, double y, int mode) {
+  double v = 1.0;
+  for (size_t i = 0; i < x.size(); i++)
+    if ((int)i != mode) v = v * ((y - x[i]) / (x[mode] - x[i]));
+  return v;
+}
+// d/dy l_mode(y): sum over dropped factor (polynomials.cpp:21-47).
+double dlag(const std::vector<double> &x, double y, int mode) {
+  double s = 0.0;
+  int n = (int)x.size();
+  for (int i = 0; i < n; i++) {
+    if (i == mode) continue;
+    double num = 1.0, den = 1.0;
+    for (int j = 0; j < n; j++) {
+      if (j != mode && j != i) num = num * (y - x[j]);
+      if (j != mode) den = den * (x[mode] - x[j]);
+    }
+    s = s + num / den;
+  }
+  return s;
+}
+// Legendre P_n by the three-term recursion (polynomials.cpp:84-95).
+double leg(double r, int n) {
+  if (n < 0) return 0.;
+  if (n == 0) return 1.0;
+  if (n == 1) return r;
+  return ((2 * n - 1) * r * leg(r, n - 1) - (n - 1) * leg(r, n - 2)) / n;
+}
+// P_n'(r) from the identity (r^2-1) P_n' = n (r P_n - P_{n-1}); endpoint
+// closed forms at r = +-1 (polynomials.cpp:97-118).
+double dleg(double r, int n) {
+  if (n == 0) return 0;
+  if (r > -1.0 && r < 1.0) return n * ((r * leg(r, n)) - leg(r, n - 1)) / (r * r - 1.0);
+  double d = 0.;
+  if (r == -1.0) d = std::pow(-1.0, n - 1.0) * 0.5 * n * (n + 1.0);
+  if (r == 1.0) d = 0.5 * n * (n + 1.0);
+  return d;
+}
+int fact(int n) { return (n <= 1) ? 1 : fact(n - 1) * n; }
+// VCJH parameter eta for DG / SD / HU / C+ (polynomials.cpp:392-428).
+double vcjhEta(int scheme, int P) {
+  if (scheme == 0) return 0.0;
+  if (scheme == 1) return (1.0 * P) / (1.0 * (P + 1));
+  if (scheme == 2) return (1.0 * (P + 1)) / (1.0 * P);
+  double c = 0;
+  if (P == 2) c = 0.206;
+  else if (P == 3) c = 3.80e-3;
+  else if (P == 4) c = 4.67e-5;
+  else if (P == 5) c = 4.28e-7;
+  double ap = 1. / std::pow(2.0, P) * fact(2 * P) / (fact(P) * fact(P));
+  return c * (2 * P + 1) / 2 * (fact(P) * ap) * (fact(P) * ap);
+}
+// Derivative of the left (side 0) / right (side 1) VCJH correction function
+// (polynomials.cpp:447-475).
+double dvcjh(double r, int side, int P, double eta) {
+  double sgn = std::pow(-1.0, P);
+  if (side == 0) {
+    if (P == 0) return 0.5 * sgn * (dleg(r, P) - (dleg(r, P + 1) / (1.0 + eta)));
+    return 0.5 * sgn * (dleg(r, P) - ((eta * dleg(r, P - 1) + dleg(r, P + 1)) / (1.0 + eta)));
+  }
+  if (P == 0) return 0.5 * (dleg(r, P) + (dleg(r, P + 1) / (1.0 + eta)));
+  return 0.5 * (dleg(r, P) + ((eta * dleg(r, P - 1) + dleg(r, P + 1)) / (1.0 + eta)));
+}
+
+void faceNormal(int nDims, int face, double *t) {
+  static const double H[6][3] = {{0, 0, -1}, {0, 0, 1}, {-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}};
+  static const double Q[4][3] = {{0, -1, 0}, {1, 0, 0}, {0, 1, 0}, {-1, 0, 0}};
+  for (int d = 0; d < 3; d++) t[d] = (nDims == 3) ? H[face][d] : Q[face][d];
+}
+
+
+// Compare the kernels' sparse index maps against the dense operators.
+template <int D, int P>
+std::string verifyMapsT(const Operators &o) {
+  using frdev::OneD;
+  const int P1 = P + 1, nS = o.nSpts, nF = o.nFpts;
+  // extrapolation and reference-normal flux rows
+  for (int q = 0; q < nF; q++) {
+    int base, stride, side, dn;
+    frdev::fptLine<D, P>(q, base, stride, side, dn);
+    std::vector<double> row(nS, 0.0);
+    if (base < 0 || base + P * stride >= nS || dn < 0 || dn >= D) return "extrapolation index map out of range";
+    for (int m = 0; m < P1; m++) row[base + m * stride] = o.oned[(side ? OneD<P>::LR : OneD<P>::LL) + m];
+    for (int s = 0; s < nS; s++) {
+      if (row[s] != o.spts_to_fpts[(size_t)q * nS + s]) return "extrapolation index map mismatch";
+      for (int d = 0; d < D; d++) {
+        const double want = (d == dn) ? row[s] * (side ? 1.0 : -1.0) : 0.0;
+        if (want != o.extrapolateFn[((size_t)d * nF + q) * nS + s]) return "normal-flux index map mismatch";
+      }
+    }
+  }
+  // gradient rows
+  for (int s = 0; s < nS; s++) {
+    int ijk[3] = {s % P1, (s / P1) % P1, (D == 3) ? s / (P1 * P1) : 0};
+    for (int d = 0; d < D; d++) {
+      std::vector<double> row(nS, 0.0);
+      for (int m = 0; m < P1; m++) {
+        int c[3] = {ijk[0], ijk[1], ijk[2]};
+        c[d] = m;
+        const int col = c[0] + P1 * (c[1] + P1 * c[2]);
+        row[col] = o.oned[OneD<P>::DM + ijk[d] * P1 + m];
+      }
+      for (int b = 0; b < nS; b++)
+        if (row[b] != o.grad_spts[((size_t)d * nS + s) * nS + b]) return "gradient index map mismatch";
+    }
+  }
+  // correction rows
+  for (int s = 0; s < nS; s++) {
+    int q[6], side[6], ax[6];
+    frdev::corrEntries<D, P>(s, q, side, ax);
+    std::vector<double> row(nF, 0.0);
+    for (int f = 0; f < 2 * D; f++) {
+      if (q[f] < 0 || q[f] >= nF || ax[f] < 0 || ax[f] > P) return "correction index map out of range";
+      row[q[f]] = o.oned[(side[f] ? OneD<P>::GR : OneD<P>::GL) + ax[f]];
+    }
+    for (int c = 0; c < nF; c++)
+      if (row[c] != o.correction[(size_t)s * nF + c]) return "correction index map mismatch";
+  }
+  return "";
+}
+
+std::string verifyIndexMaps(const Operators &o) {
+  const int key = o.nDims * 10 + o.order;
+  switch (key) {
+    case 31: return verifyMapsT<3, 1>(o);
+    case 32: return verifyMapsT<3, 2>(o);
+    case 33: return verifyMapsT<3, 3>(o);
+    case 34: return verifyMapsT<3, 4>(o);
+    case 35: return verifyMapsT<3, 5>(o);
+    case 21: return verifyMapsT<2, 1>(o);
+    case 22: return verifyMapsT<2, 2>(o);
+    case 23: return verifyMapsT<2, 3>(o);
+    case 24: return verifyMapsT<2, 4>(o);
+    case 25: return verifyMapsT<2, 5>(o);
+  }
+  return "unsupported element order";
+}
+
+}  // namespace
+
+std::string buildOperators(const fr_config &cfg, Operators &o) {
+  const int D = cfg.nDims, P = cfg.order;
+  if (D != 2 && D != 3) return "nDims must be 2 or 3";
+  if (P < 1 || P > 5) return "order must be in 1..5";
+  if (cfg.sptsType != 0 && cfg.sptsType != 1) return "sptsType must be 0 (Legendre) or 1 (Lobatto)";
+  if (cfg.vcjhScheme < 0 || cfg.vcjhScheme > 3) return "Invalid VCJH scheme.";
+  if (cfg.vcjhScheme == 3 && P < 2) return "C_plus scheme not implemented for this order";
+  o.nDims = D;
+  o.order = P;
+  const int P1 = o.P1 = P + 1;
+  o.nSpts = (D == 3) ? P1 * P1 * P1 : P1 * P1;
+  o.nFptsFace = (D == 3) ? P1 * P1 : P1;
+  o.nFacesEle = 2 * D;
+  o.nFpts = o.nFacesEle * o.nFptsFace;
+  o.nNodes = (D == 3) ? 8 : 4;
+  const int nS = o.nSpts, nF = o.nFpts;
+  std::vector<double> x(P1);
+  for (int i = 0; i < P1; i++) x[i] = (cfg.sptsType == 0) ? kGL[P][i] : kGLL[P][i];
+  o.spts1D = x;
+
+  // point sets (points.cpp:37-128) and spt quadrature weights (points.cpp:347-374)
+  o.locSpts.assign(3 * nS, 0.0);
+  o.locFpts.assign(3 * nF, 0.0);
+  o.wts.assign(nS, 0.0);
+  for (int s = 0; s < nS; s++) {
+    int i = s % P1, j = (s / P1) % P1, k = s / (P1 * P1);
+    o.locSpts[3 * s + 0] = x[i];
+    o.locSpts[3 * s + 1] = x[j];
+    o.locSpts[3 * s + 2] = (D == 3) ? x[k] : 0.0;
+    o.wts[s] = (D == 3) ? kGW[P][k] * kGW[P][j] * kGW[P][i] : kGW[P][j] * kGW[P][i];
+  }
+  if (D == 3) {
+    const int F2 = P1 * P1;
+    for (int i = 0; i < P1; i++)
+      for (int j = 0; j < P1; j++) {
+        const int q = i + j * P1;
+        const double pts[6][3] = {{x[i], x[j], -1.},     {x[P - i], x[j], 1.},     {-1., x[i], x[j]},
+                                  {1., x[P - i], x[j]}, {x[P - i], -1., x[j]}, {x[i], 1., x[j]}};
+        for (int f = 0; f < 6; f++)
+          for (int d = 0; d < 3; d++) o.locFpts[3 * (q + f * F2) + d] = pts[f][d];
+      }
+  } else {
+    for (int i = 0; i < P1; i++) {
+      const double pts[4][2] = {{x[i], -1.}, {1., x[i]}, {x[P - i], 1.}, {-1., x[P - i]}};
+      for (int f = 0; f < 4; f++)
+        for (int d = 0; d < 2; d++) o.locFpts[3 * (i + f * P1) + d] = pts[f][d];
+    }
+  }
+  o.tNorm.assign(3 * nF, 0.0);
+  for (int q = 0; q < nF; q++) faceNormal(D, q / o.nFptsFace, &o.tNorm[3 * q]);
+
+  auto tripleIdx = [&](int s, int &i, int &j, int &k) {
+    if (D == 3) {
+      k = s / (P1 * P1);
+      j = (s - P1 * P1 * k) / P1;
+      i = s - P1 * j - P1 * P1 * k;
+    } else {
+      i = s % P1;
+      j = s / P1;
+      k = 0;
+    }
+  };
+  // extrapolation to flux points (operators.cpp:78-111)
+  o.spts_to_fpts.assign((size_t)nF * nS, 0.0);
+  for (int q = 0; q < nF; q++) {
+    const double *f = &o.locFpts[3 * q];
+    for (int s = 0; s < nS; s++) {
+      int i, j, k;
+      tripleIdx(s, i, j, k);
+      double v = lag(x, f[0], i) * lag(x, f[1], j);
+      if (D == 3) v = v * lag(x, f[2], k);
+      o.spts_to_fpts[(size_t)q * nS + s] = v;
+    }
+  }
+  // gradient at spts (operators.cpp:660-740)
+  o.grad_spts.assign((size_t)D * nS * nS, 0.0);
+  for (int d = 0; d < D; d++)
+    for (int g = 0; g < nS; g++) {
+      int ig, jg, kg;
+      tripleIdx(g, ig, jg, kg);
+      for (int b = 0; b < nS; b++) {
+        int ib, jb, kb;
+        tripleIdx(b, ib, jb, kb);
+        double v;
+        if (D == 3) {
+          if (d == 0) v = dlag(x, x[ig], ib) * lag(x, x[jg], jb) * lag(x, x[kg], kb);
+          else if (d == 1) v = dlag(x, x[jg], jb) * lag(x, x[ig], ib) * lag(x, x[kg], kb);
+          else v = dlag(x, x[kg], kb) * lag(x, x[ig], ib) * lag(x, x[jg], jb);
+        } else {
+          if (d == 0) v = dlag(x, x[ig], ib) * lag(x, x[jg], jb);
+          else v = dlag(x, x[jg], jb) * lag(x, x[ig], ib);
+        }
+        o.grad_spts[((size_t)d * nS + g) * nS + b] = v;
+      }
+    }
+  // reference-normal flux extrapolation (operators.cpp:807-889)
+  o.extrapolateFn.assign((size_t)D * nF * nS, 0.0);
+  for (int d = 0; d < D; d++)
+    for (int q = 0; q < nF; q++)
+      for (int s = 0; s < nS; s++)
+        o.extrapolateFn[((size_t)d * nF + q) * nS + s] = o.spts_to_fpts[(size_t)q * nS + s] * o.tNorm[3 * q + d];
+  // correction = divergence of the VCJH correction field (operators.cpp:891-914,
+  // divVCJH_hex 1575-1619, divVCJH_quad 1544-1573)
+  const double eta = (cfg.vcjhScheme == 0) ? 0. : vcjhEta(cfg.vcjhScheme, P);
+  o.correction.assign((size_t)nS * nF, 0.0);
+  for (int s = 0; s < nS; s++) {
+    const double *l = &o.locSpts[3 * s];
+    for (int q = 0; q < nF; q++) {
+      double v;
+      if (D == 3) {
+        const int F2 = P1 * P1, f = q / F2, j = (q - F2 * f) / P1, i = q - F2 * f - P1 * j;
+        switch (f) {
+          case 0: v = lag(x, l[0], i) * lag(x, l[1], j) * -dvcjh(l[2], 0, P, eta); break;
+          case 1: v = lag(x, l[0], P - i) * lag(x, l[1], j) * dvcjh(l[2], 1, P, eta); break;
+          case 2: v = lag(x, l[1], i) * lag(x, l[2], j) * -dvcjh(l[0], 0, P, eta); break;
+          case 3: v = lag(x, l[1], P - i) * lag(x, l[2], j) * dvcjh(l[0], 1, P, eta); break;
+          case 4: v = lag(x, l[0], P - i) * lag(x, l[2], j) * -dvcjh(l[1], 0, P, eta); break;
+          default: v = lag(x, l[0], i) * lag(x, l[2], j) * dvcjh(l[1], 1, P, eta); break;
+        }
+      } else {
+        const int f = q / P1, j = q - P1 * f;
+        switch (f) {
+          case 0: v = -lag(x, l[0], j) * dvcjh(l[1], 0, P, eta); break;
+          case 1: v = lag(x, l[1], j) * dvcjh(l[0], 1, P, eta); break;
+          case 2: v = lag(x, l[0], P - j) * dvcjh(l[1], 1, P, eta); break;
+          default: v = -lag(x, l[1], P - j) * dvcjh(l[0], 0, P, eta); break;
+        }
+      }
+      o.correction[(size_t)s * nF + q] = v;
+    }
+  }
+  // gradient correction (operators.cpp:1008-1092)
+  o.correctU.assign((size_t)D * nS * nF, 0.0);
+  for (int d = 0; d < D; d++)
+    for (int q = 0; q < nF; q++)
+      for (int s = 0; s < nS; s++)
+        o.correctU[((size_t)d * nS + s) * nF + q] = o.correction[(size_t)s * nF + q] * o.tNorm[3 * q + d];
+
+  // 1D factors for the kernels and a check that the kernels' index maps
+  // (fr_layout.hpp) enumerate exactly the nonzeros of the dense operators
+  {
+    const int L = 4 * P1 + P1 * P1;
+    o.oned.assign(L, 0.0);
+    for (int m = 0; m < P1; m++) {
+      o.oned[m] = lag(x, -1.0, m);
+      o.oned[P1 + m] = lag(x, 1.0, m);
+      o.oned[2 * P1 + m] = -dvcjh(x[m], 0, P, eta);
+      o.oned[3 * P1 + m] = dvcjh(x[m], 1, P, eta);
+      for (int i = 0; i < P1; i++) o.oned[4 * P1 + i * P1 + m] = dlag(x, x[i], m);
+    }
+    std::string err = verifyIndexMaps(o);
+    if (!err.empty()) return err;
+  }
+
+  // linear-element shape-function derivatives (Lagrange on {-1, 1}, Gmsh
+  // corner order: funcs.cpp:178-260, 663-800)
+  static const int CI[8] = {0, 1, 1, 0, 0, 1, 1, 0}, CJ[8] = {0, 0, 1, 1, 0, 0, 1, 1},
+                   CK[8] = {0, 0, 0, 0, 1, 1, 1, 1};
+  const std::vector<double> xl = {-1., 1.};
+  const int nN = o.nNodes;
+  auto fillDshape = [&](const std::vector<double> &loc, int nPts, std::vector<double> &out) {
+    out.assign((size_t)D * nPts * nN, 0.0);
+    for (int p = 0; p < nPts; p++) {
+      const double *r = &loc[3 * p];
+      for (int n = 0; n < nN; n++) {
+        double la = lag(xl, r[0], CI[n]), lb = lag(xl, r[1], CJ[n]);
+        double da = dlag(xl, r[0], CI[n]), db = dlag(xl, r[1], CJ[n]);
+        if (D == 3) {
+          double lc = lag(xl, r[2], CK[n]), dc = dlag(xl, r[2], CK[n]);
+          out[((size_t)0 * nPts + p) * nN + n] = da * lb * lc;
+          out[((size_t)1 * nPts + p) * nN + n] = la * db * lc;
+          out[((size_t)2 * nPts + p) * nN + n] = la * lb * dc;
+        } else {
+          out[((size_t)0 * nPts + p) * nN + n] = da * lb;
+          out[((size_t)1 * nPts + p) * nN + n] = la * db;
+        }
+      }
+    }
+  };
+  fillDshape(o.locSpts, nS, o.dshapeSpts);
+  o.shapeSpts.assign((size_t)nS * nN, 0.0);
+  for (int p = 0; p < nS; p++) {
+    const double *r = &o.locSpts[3 * p];
+    for (int n = 0; n < nN; n++) {
+      double v = lag(xl, r[0], CI[n]) * lag(xl, r[1], CJ[n]);
+      if (D == 3) v = v * lag(xl, r[2], CK[n]);
+      o.shapeSpts[(size_t)p * nN + n] = v;
+    }
+  }
+  fillDshape(o.locFpts, nF, o.dshapeFpts);
+  return "";
+}
+
+std::string buildConnectivity(const fr_config &cfg, const fr_mesh &m, const Operators &o, Connectivity &c) {
+  const int D = cfg.nDims, nE = m.nEles, nN = m.nNodes, nfe = o.nFacesEle, nfv = (D == 3) ? 4 : 2;
+  const int nfp = o.nFptsFace, P = o.order, nF = o.nFpts;
+  // face vertex lists: matching order (geo.cpp:86-91) and orientation order
+  // (geo.cpp:804-900); quad edges follow the quad flux-point faces
+  static const int HFV[6][4] = {{0, 1, 2, 3}, {4, 5, 6, 7}, {3, 0, 4, 7}, {2, 1, 5, 6}, {1, 0, 4, 5}, {3, 2, 6, 7}};
+  static const int HOR[6][4] = {{0, 1, 2, 3}, {5, 4, 7, 6}, {0, 3, 7, 4}, {2, 1, 5, 6}, {1, 0, 4, 5}, {3, 2, 6, 7}};
+  static const int QFV[4][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}};
+  auto cv = [&](int e, int n) -> int {
+    int v = m.c2v[(size_t)e * nN + n];
+    return m.vmap ? m.vmap[v] : v;
+  };
+  struct Key {
+    std::array<int, 4> v;
+    int e, f;
+  };
+  std::vector<Key> keys((size_t)nE * nfe);
+  for (int e = 0; e < nE; e++)
+    for (int f = 0; f < nfe; f++) {
+      Key &k = keys[(size_t)e * nfe + f];
+      k.v = {INT32_MAX, INT32_MAX, INT32_MAX, INT32_MAX};
+      for (int i = 0; i < nfv; i++) k.v[i] = (D == 3) ? cv(e, HFV[f][i]) : cv(e, QFV[f][i]);
+      std::sort(k.v.begin(), k.v.begin() + nfv);
+      k.e = e;
+      k.f = f;
+    }
+  std::sort(keys.begin(), keys.end(), [](const Key &a, const Key &b) {
+    if (a.v != b.v) return a.v < b.v;
+    if (a.e != b.e) return a.e < b.e;
+    return a.f < b.f;
+  });
+  std::vector<std::unordered_set<int>> bnd(m.nBounds);
+  for (int b = 0; b < m.nBounds; b++)
+    for (int q = m.bndOff[b]; q < m.bndOff[b + 1]; q++) {
+      int v = m.bndPts[q];
+      if (v < 0 || v >= m.nVerts) return "boundary vertex id out of range";
+      bnd[b].insert(m.vmap ? m.vmap[v] : v);
+    }
+  c.faces.clear();
+  std::vector<FaceRec> bfaces;
+  for (size_t a = 0; a < keys.size();) {
+    size_t b = a + 1;
+    while (b < keys.size() && keys[b].v == keys[a].v) b++;
+    size_t cnt = b - a;
+    if (cnt > 2) return "More than 2 cells for face";
+    if (cnt == 2) {
+      c.faces.push_back({keys[a].e, keys[a].f, keys[a + 1].e, keys[a + 1].f, 0, -1});
+    } else {
+      int bc = -1;
+      for (int bb = 0; bb < m.nBounds && bc < 0; bb++) {
+        bool ok = true;
+        for (int i = 0; i < nfv && ok; i++) ok = bnd[bb].count(keys[a].v[i]) > 0;
+        if (ok) bc = m.bcType[bb];
+      }
+      if (bc < 0) return "boundary face matches no boundary";
+      static const int known[] = {1, 2, 3, 4, 5, 8, 9, 10, 12, 13};
+      if (std::find(std::begin(known), std::end(known), bc) == std::end(known))
+        return "Boundary condition not recognized.";
+      bfaces.push_back({keys[a].e, keys[a].f, -1, -1, 0, bc});
+    }
+    a = b;
+  }
+  c.nIntFaces = (int)c.faces.size();
+  c.faces.insert(c.faces.end(), bfaces.begin(), bfaces.end());
+  const int nFaces = (int)c.faces.size();
+  c.fptR.assign((size_t)nFaces * nfp, -1);
+  c.nbr.assign((size_t)nE * nF, -1);
+  c.fkind.assign((size_t)nE * nfe, 0);
+  for (int fi = 0; fi < nFaces; fi++) {
+    FaceRec &fr = c.faces[fi];
+    if (fr.bc >= 0) {
+      c.fkind[(size_t)fr.eL * nfe + fr.fL] = (int8_t)(2 + fr.bc);
+      continue;
+    }
+    int *map = &c.fptR[(size_t)fi * nfp];
+    if (D == 3) {
+      const int v0 = cv(fr.eR, HOR[fr.fR][0]);
+      int rot = -1;
+      for (int i = 0; i < 4; i++)
+        if (cv(fr.eL, HOR[fr.fL][i]) == v0) {
+          rot = i;
+          break;
+        }
+      if (rot < 0) return "Internal faces improperly matched.";
+      fr.rot = rot;
+      const int start = nfp * fr.fR, end = nfp * (fr.fR + 1);
+      for (int i = 0; i < nfp; i++) {
+        const int a = i % (P + 1), b = i / (P + 1);
+        switch (rot) {
+          case 0: map[i] = start + a * (P + 1) + b; break;
+          case 1: map[i] = start + P - a + b * (P + 1); break;
+          case 2: map[i] = end - 1 - (a * (P + 1) + b); break;
+          default: map[i] = end - (P + 1) * (b + 1) + a; break;
+        }
+      }
+    } else {
+      fr.rot = 1;
+      for (int i = 0; i < nfp; i++) map[i] = fr.fR * nfp + nfp - 1 - i;
+    }
+    c.fkind[(size_t)fr.eL * nfe + fr.fL] = 0;
+    c.fkind[(size_t)fr.eR * nfe + fr.fR] = 1;
+    for (int i = 0; i < nfp; i++) {
+      const int qL = fr.fL * nfp + i, qR = map[i];
+      c.nbr[(size_t)fr.eL * nF + qL] = fr.eR * nF + qR;
+      c.nbr[(size_t)fr.eR * nF + qR] = fr.eL * nF + qL;
+    }
+  }
+  return "";
+}
+
+}  // namespace frhost

@@ -1,0 +1,56 @@
+// fr_host.hpp -- host-side setup of the FR engine: FR operators (dense, for
+// export, plus the tensor-sparse tables the kernels use) and face
+// connectivity.  Runs once per fr_create; the hot path is on the device.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/fr.h"
+
+namespace frhost {
+
+// Dense FR operators of one (element type, order) plus their sparse form.
+// Dense shapes follow oper::opp_* (reference include/operators.hpp:133-146).
+struct Operators {
+  int nDims = 3, order = 3, P1 = 4, nSpts = 0, nFpts = 0, nFptsFace = 0, nFacesEle = 0;
+  std::vector<double> spts1D, wts;           // 1D abscissae, per-spt quadrature weights
+  std::vector<double> locSpts, locFpts;      // [pt][3]
+  std::vector<double> tNorm;                 // [fpt][3]
+  std::vector<double> spts_to_fpts;          // nFpts x nSpts
+  std::vector<double> grad_spts;             // nDims x nSpts x nSpts
+  std::vector<double> extrapolateFn;         // nDims x nFpts x nSpts
+  std::vector<double> correction;            // nSpts x nFpts
+  std::vector<double> correctU;              // nDims x nSpts x nFpts
+  // Shape-function derivatives of the linear element at spts / fpts,
+  // [dim][pt][node] (reference solver.cpp:153-168, funcs.cpp dshape_hex).
+  std::vector<double> dshapeSpts, dshapeFpts;
+  std::vector<double> shapeSpts;  // [pt][node]
+  int nNodes = 8;
+  // 1D factors consumed by the kernels (layout of frdev::OneD): l_L, l_R,
+  // correction coefficients of left faces (-g_L') and right faces (g_R'), and
+  // the nodal derivative matrix Dm[i][m].
+  std::vector<double> oned;
+};
+
+// Builds everything above; returns "" or an error message.
+std::string buildOperators(const fr_config &cfg, Operators &ops);
+
+struct FaceRec {
+  int eL, fL, eR, fR, rot, bc;  // bc = -1 for interior faces
+};
+
+struct Connectivity {
+  std::vector<FaceRec> faces;   // interior faces first, then boundary faces
+  int nIntFaces = 0;
+  std::vector<int> fptR;        // [nFaces][nFptsFace] right-element local fpt per left fpt
+  // Element-centric tables consumed by the kernels:
+  std::vector<int32_t> nbr;     // [nEles][nFpts]: neighbor (ele*nFpts + fpt) or -1
+  std::vector<int8_t> fkind;    // [nEles][nFacesEle]: 0 left, 1 right, 2+bc boundary
+};
+
+std::string buildConnectivity(const fr_config &cfg, const fr_mesh &mesh, const Operators &ops,
+                              Connectivity &conn);
+
+}  // namespace frhost

@@ -1,0 +1,96 @@
+// fr_layout.hpp -- compile-time sizes of a tensor-product element and the
+// index maps that the kernels use instead of dense operator rows.
+//
+// Point numbering is the reference's (src/points.cpp:37-128): hex solution
+// point (i,j,k) is i + P1*(j + P1*k); hex faces are 0 z=-1, 1 z=+1, 2 x=-1,
+// 3 x=+1, 4 y=-1, 5 y=+1 with flux point face*P1^2 + a + b*P1 and the face-wise
+// index reversals of getLocFpts; quad faces are 0 y=-1, 1 x=+1, 2 y=+1,
+// 3 x=-1 (counter-clockwise).  Because every flux point shares its tangential
+// coordinates with a line of solution points, each row of the extrapolation,
+// gradient and correction operators is nonzero only on that line; the maps
+// below enumerate those entries in ascending column order.  fr_host.cpp
+// checks them against the dense operators at context creation.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace frdev {
+
+constexpr int nextPow2(int n) {
+  int p = 1;
+  while (p < n) p *= 2;
+  return p;
+}
+
+template <int D, int P>
+struct ElemGeo {
+  static constexpr int P1 = P + 1;
+  static constexpr int NS = (D == 3) ? P1 * P1 * P1 : P1 * P1;  // solution points
+  static constexpr int NFPF = (D == 3) ? P1 * P1 : P1;          // flux points per face
+  static constexpr int NFACE = 2 * D;
+  static constexpr int NFP = NFACE * NFPF;                      // flux points
+  static constexpr int TPE = nextPow2(NS) < 8 ? 8 : nextPow2(NS);  // threads per element
+  static constexpr int EPB = TPE >= 256 ? 1 : 256 / TPE;           // elements per block
+  static constexpr int NTH = TPE * EPB;
+};
+
+// Solution-point line behind flux point q: spts base + m*stride (m = 0..P)
+// with 1D weights l_R (side 1) or l_L (side 0); dn = reference normal axis.
+template <int D, int P>
+__host__ __device__ __forceinline__ void fptLine(int q, int &base, int &stride, int &side, int &dn) {
+  constexpr int P1 = P + 1;
+  if (D == 3) {
+    const int f = q / (P1 * P1), r = q - f * P1 * P1, a = r % P1, b = r / P1;
+    switch (f) {
+      case 0: base = a + P1 * b; stride = P1 * P1; side = 0; dn = 2; break;
+      case 1: base = (P - a) + P1 * b; stride = P1 * P1; side = 1; dn = 2; break;
+      case 2: base = P1 * (a + P1 * b); stride = 1; side = 0; dn = 0; break;
+      case 3: base = P1 * ((P - a) + P1 * b); stride = 1; side = 1; dn = 0; break;
+      case 4: base = (P - a) + P1 * P1 * b; stride = P1; side = 0; dn = 1; break;
+      default: base = a + P1 * P1 * b; stride = P1; side = 1; dn = 1; break;
+    }
+  } else {
+    const int f = q / P1, a = q - f * P1;
+    switch (f) {
+      case 0: base = a; stride = P1; side = 0; dn = 1; break;
+      case 1: base = P1 * a; stride = 1; side = 1; dn = 0; break;
+      case 2: base = P - a; stride = P1; side = 1; dn = 1; break;
+      default: base = P1 * (P - a); stride = 1; side = 0; dn = 0; break;
+    }
+  }
+}
+
+// Correction entries of solution point s, one per face in face order:
+// flux point q[f] and coefficient = sgn[f] * g_side[f](x_{axis[f]}), where
+// g_L / g_R are the derivatives of the left / right correction functions.
+template <int D, int P>
+__host__ __device__ __forceinline__ void corrEntries(int s, int *q, int *side, int *axisIdx) {
+  constexpr int P1 = P + 1;
+  if (D == 3) {
+    constexpr int F2 = P1 * P1;
+    const int k = s / F2, j = (s - F2 * k) / P1, i = s - F2 * k - P1 * j;
+    q[0] = i + j * P1;                side[0] = 0; axisIdx[0] = k;
+    q[1] = F2 + (P - i) + j * P1;     side[1] = 1; axisIdx[1] = k;
+    q[2] = 2 * F2 + j + k * P1;       side[2] = 0; axisIdx[2] = i;
+    q[3] = 3 * F2 + (P - j) + k * P1; side[3] = 1; axisIdx[3] = i;
+    q[4] = 4 * F2 + (P - i) + k * P1; side[4] = 0; axisIdx[4] = j;
+    q[5] = 5 * F2 + i + k * P1;       side[5] = 1; axisIdx[5] = j;
+  } else {
+    const int i = s % P1, j = s / P1;
+    q[0] = i;                side[0] = 0; axisIdx[0] = j;
+    q[1] = P1 + j;           side[1] = 1; axisIdx[1] = i;
+    q[2] = 2 * P1 + (P - i); side[2] = 1; axisIdx[2] = j;
+    q[3] = 3 * P1 + (P - j); side[3] = 0; axisIdx[3] = i;
+  }
+}
+
+// Layout of the per-order 1D factor table: l_L[P1], l_R[P1], g_L[P1], g_R[P1],
+// Dm[P1][P1] (Dm[i][m] = derivative of basis m at node i).
+template <int P>
+struct OneD {
+  static constexpr int P1 = P + 1;
+  static constexpr int LL = 0, LR = P1, GL = 2 * P1, GR = 3 * P1, DM = 4 * P1;
+  static constexpr int SIZE = 4 * P1 + P1 * P1;
+};
+
+}  // namespace frdev

@@ -1,0 +1,183 @@
+"""Linear hex / quad meshes for the FR engine.
+
+A mesh is what fr_create consumes (include/fr.h `fr_mesh`): vertex
+coordinates, element -> vertex lists in Gmsh corner order, an optional
+canonical-vertex map used for face matching (periodicity by vertex
+identification), and boundary vertex sets with a boundary-condition id each
+(the reference's mesh_bound / $PhysicalNames mapping, src/geo.cpp:448-518).
+"""
+import itertools
+
+import numpy as np
+
+# boundary-condition ids (reference include/global.hpp:63-79)
+BC = {
+    "char": 1, "sup_in": 2, "sup_out": 3, "sub_in": 4, "sub_out": 5, "slip_wall": 8,
+    "isothermal_noslip": 9, "adiabatic_noslip": 10, "symmetry": 12, "isothermal_noslip_moving": 13,
+}
+
+# Gmsh corner coordinates of the linear hex (funcs.cpp shape_hex node order)
+HEX_CORNERS = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0],
+                        [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]])
+QUAD_CORNERS = np.array([[0, 0], [1, 0], [1, 1], [0, 1]])
+
+
+class Mesh:
+    def __init__(self, nDims, xv, c2v, vmap=None, bounds=()):
+        self.nDims = nDims
+        self.xv = np.ascontiguousarray(xv, dtype=np.float64)
+        self.c2v = np.ascontiguousarray(c2v, dtype=np.int32)
+        self.vmap = None if vmap is None else np.ascontiguousarray(vmap, dtype=np.int32)
+        self.bounds = [(int(bc), np.ascontiguousarray(np.unique(v), dtype=np.int32)) for bc, v in bounds]
+
+    @property
+    def nEles(self):
+        return self.c2v.shape[0]
+
+    @property
+    def nNodes(self):
+        return self.c2v.shape[1]
+
+    def bnd_arrays(self):
+        bcs = np.array([b for b, _ in self.bounds], dtype=np.int32)
+        off = np.zeros(len(self.bounds) + 1, dtype=np.int32)
+        for i, (_, v) in enumerate(self.bounds):
+            off[i + 1] = off[i] + len(v)
+        pts = (np.concatenate([v for _, v in self.bounds]).astype(np.int32)
+               if self.bounds else np.zeros(1, dtype=np.int32))
+        return bcs, off, np.ascontiguousarray(pts)
+
+    def element_nodes(self):
+        """[nEles][nNodes][3] node coordinates."""
+        return self.xv[self.c2v]
+
+
+def _rotations3():
+    """The 24 proper rotations of the cube as signed permutation matrices."""
+    out = []
+    for perm in itertools.permutations(range(3)):
+        for signs in itertools.product((1, -1), repeat=3):
+            R = np.zeros((3, 3), dtype=int)
+            for r in range(3):
+                R[r, perm[r]] = signs[r]
+            if round(np.linalg.det(R)) == 1:
+                out.append(R)
+    return out
+
+
+def _node_perm(R, corners):
+    """Permutation p with new node n = old node p[n] for a cube rotation R."""
+    c = 2 * corners - 1
+    rc = (c @ R.T + 1) // 2
+    lookup = {tuple(v): i for i, v in enumerate(corners)}
+    return np.array([lookup[tuple(v)] for v in rc])
+
+
+def box(nDims, n, lo, hi, periodic=None, bc=None, perturb=0.0, seed=0, rotate=False, shuffle=False):
+    """Structured box of linear hexes (nDims=3) or quads (nDims=2).
+
+    n, lo, hi: cells and bounds per direction.  periodic: per-direction flags
+    (default all periodic).  bc: boundary condition id (or name) for every
+    non-periodic side, either one value or a dict keyed 'xmin', 'xmax', ...
+    perturb: random displacement of vertices as a fraction of the cell size
+    (the same displacement on periodic images, none normal to a wall).
+    rotate / shuffle: randomly re-orient each element's corner ordering / the
+    element order (exercises face rotations and left/right assignment).
+    """
+    D = nDims
+    n = [int(v) for v in n][:D]
+    lo = np.asarray(lo, dtype=np.float64)[:D]
+    hi = np.asarray(hi, dtype=np.float64)[:D]
+    periodic = [True] * D if periodic is None else list(periodic)[:D]
+    rng = np.random.default_rng(seed)
+    dims = [v + 1 for v in n]
+    grids = np.meshgrid(*[np.arange(d) for d in dims], indexing="ij")
+    ijk = np.stack([g.transpose(*reversed(range(D))).ravel() for g in grids], axis=1)  # x fastest
+    nv = ijk.shape[0]
+    h = (hi - lo) / np.asarray(n)
+    xv = np.zeros((nv, 3))
+    xv[:, :D] = lo + ijk * h
+    # canonical ids
+    can = ijk.copy()
+    for d in range(D):
+        if periodic[d]:
+            can[can[:, d] == n[d], d] = 0
+    strides = np.cumprod([1] + dims[:-1])
+    canon = (can * strides).sum(axis=1).astype(np.int32)
+    if perturb > 0:
+        disp = rng.uniform(-1, 1, size=(nv, D)) * perturb * h
+        disp = disp[canon]  # periodic images move together
+        for d in range(D):
+            if not periodic[d]:
+                on = (ijk[:, d] == 0) | (ijk[:, d] == n[d])
+                disp[on, d] = 0.0
+        xv[:, :D] += disp
+    vmap = canon if any(periodic) else None
+    # elements
+    corners = HEX_CORNERS if D == 3 else QUAD_CORNERS
+    egrid = np.meshgrid(*[np.arange(v) for v in n], indexing="ij")
+    eijk = np.stack([g.transpose(*reversed(range(D))).ravel() for g in egrid], axis=1)
+    vid = lambda c: (c * strides).sum(axis=-1)
+    c2v = np.stack([vid(eijk + corners[m]) for m in range(len(corners))], axis=1).astype(np.int32)
+    if rotate:
+        if D == 3:
+            perms = [_node_perm(R, HEX_CORNERS) for R in _rotations3()]
+        else:
+            perms = [np.roll(np.arange(4), -r) for r in range(4)]
+        choice = rng.integers(0, len(perms), size=c2v.shape[0])
+        c2v = np.stack([c2v[e, perms[c]] for e, c in enumerate(choice)]).astype(np.int32)
+    if shuffle:
+        c2v = c2v[rng.permutation(c2v.shape[0])]
+    # boundaries
+    bounds = []
+    names = ["x", "y", "z"]
+    for d in range(D):
+        if periodic[d]:
+            continue
+        for side, val in (("min", 0), ("max", n[d])):
+            key = names[d] + side
+            b = bc[key] if isinstance(bc, dict) else bc
+            if b is None:
+                raise ValueError(f"no boundary condition for {key}")
+            b = BC[b] if isinstance(b, str) else int(b)
+            bounds.append((b, np.nonzero(ijk[:, d] == val)[0]))
+    return Mesh(D, xv, c2v, vmap, bounds)
+
+
+def write_gmsh(mesh, path, names=None):
+    """Write a Gmsh 2.2 ASCII mesh with one physical volume "fluid" and one
+    physical surface per boundary (the format read by geo::readGmsh,
+    src/geo.cpp:448-770).  Boundary faces are emitted as linear quads/lines."""
+    D = mesh.nDims
+    names = names or [f"bnd{i}" for i in range(len(mesh.bounds))]
+    lines = ["$MeshFormat", "2.2 0 8", "$EndMeshFormat", "$PhysicalNames", str(len(mesh.bounds) + 1)]
+    for i, nm in enumerate(names):
+        lines.append(f"{D - 1} {i + 1} \"{nm}\"")
+    lines.append(f"{D} {len(mesh.bounds) + 1} \"fluid\"")
+    lines.append("$EndPhysicalNames")
+    lines.append("$Nodes")
+    lines.append(str(mesh.xv.shape[0]))
+    for i, x in enumerate(mesh.xv):
+        lines.append(f"{i + 1} {x[0]!r} {x[1]!r} {x[2]!r}")
+    lines.append("$EndNodes")
+    # boundary faces: element faces whose vertices all lie in one boundary set
+    fv = ([[0, 1, 2, 3], [4, 5, 6, 7], [3, 0, 4, 7], [2, 1, 5, 6], [1, 0, 4, 5], [3, 2, 6, 7]] if D == 3
+          else [[0, 1], [1, 2], [2, 3], [3, 0]])
+    elems = []
+    sets = [set(v.tolist()) for _, v in mesh.bounds]
+    for e in range(mesh.nEles):
+        for f in fv:
+            vs = mesh.c2v[e, f]
+            for b, s in enumerate(sets):
+                if all(int(v) in s for v in vs):
+                    elems.append((3 if D == 3 else 1, b + 1, vs))
+                    break
+    for e in range(mesh.nEles):
+        elems.append((5 if D == 3 else 3, len(mesh.bounds) + 1, mesh.c2v[e]))
+    lines.append("$Elements")
+    lines.append(str(len(elems)))
+    for i, (t, tag, vs) in enumerate(elems):
+        lines.append(f"{i + 1} {t} 2 {tag} {tag} " + " ".join(str(int(v) + 1) for v in vs))
+    lines.append("$EndElements")
+    with open(path, "w") as fh:
+        fh.write("\n".join(lines) + "\n")

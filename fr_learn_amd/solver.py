@@ -1,0 +1,165 @@
+"""Python mirror of the reference `solver` verbs over the libfrhip C ABI.
+
+Method names follow reference include/solver.hpp (setup, initializeSolution,
+calcResidual, update, timeStepA/B, ...); arrays use the reference layout
+U(spt, ele, field) (src/matrix.cpp:260-272).  Every call goes to the HIP
+library; there is no CPU path.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from .cases import n_fields
+
+
+class Solver:
+    def __init__(self, cfg, mesh, precision="fp64"):
+        lib = _lib.load()
+        self.cfg = dict(cfg)
+        self.mesh = mesh
+        c = _lib.FrConfig()
+        for name, _ in _lib.FrConfig._fields_:
+            setattr(c, name, self.cfg[name])
+        self._c = c
+        bcs, off, pts = mesh.bnd_arrays()
+        self._keep = (mesh.xv, mesh.c2v, mesh.vmap, bcs, off, pts)
+        m = _lib.FrMesh()
+        m.nVerts = mesh.xv.shape[0]
+        m.xv = _lib.dptr(mesh.xv)
+        m.nEles = mesh.nEles
+        m.nNodes = mesh.nNodes
+        m.c2v = _lib.iptr(mesh.c2v)
+        m.vmap = _lib.iptr(mesh.vmap) if mesh.vmap is not None else ctypes.POINTER(ctypes.c_int)()
+        m.nBounds = len(mesh.bounds)
+        m.bcType = _lib.iptr(bcs)
+        m.bndOff = _lib.iptr(off)
+        m.bndPts = _lib.iptr(pts)
+        self._m = m
+        prec = {"fp64": _lib.FR_PREC_FP64, "fp32": _lib.FR_PREC_FP32}[precision]
+        self.precision = precision
+        h = ctypes.c_void_p()
+        _lib.check(lib.fr_create(ctypes.byref(c), ctypes.byref(m), prec, ctypes.byref(h)))
+        self._h = h
+        self._lib = lib
+        sz = np.zeros(8, dtype=np.int32)
+        _lib.check(lib.fr_get_sizes(h, _lib.iptr(sz)))
+        (self.nSpts, self.nFpts, self.nEles, self.nFields, self.nDims, self.nFaces, self.nIntFaces,
+         self.nRKSteps) = [int(v) for v in sz]
+        assert self.nFields == n_fields(self.cfg)
+
+    # -- lifetime -------------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.fr_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- state ----------------------------------------------------------------
+    def state_shape(self):
+        return (self.nSpts, self.nEles, self.nFields)
+
+    def set_state(self, U):
+        U = np.ascontiguousarray(U, dtype=np.float64).reshape(self.state_shape())
+        _lib.check(self._lib.fr_set_state(self._h, _lib.dptr(U), _lib.FR_LAYOUT_REF))
+
+    def get_state(self):
+        out = np.zeros(self.state_shape())
+        _lib.check(self._lib.fr_get_state(self._h, _lib.dptr(out), _lib.FR_LAYOUT_REF))
+        return out
+
+    def positions(self):
+        out = np.zeros((self.nSpts, self.nEles, self.nDims))
+        _lib.check(self._lib.fr_get_geometry(self._h, 4, _lib.dptr(out)))
+        return out
+
+    def initialize(self, ic, **kw):
+        """solver::initializeSolution with an initial-condition function of the
+        solution-point positions (cases.ic_*)."""
+        x = self.positions()
+        if self.nDims == 2:
+            x = np.concatenate([x, np.zeros(x.shape[:-1] + (1,))], axis=-1)
+        U = ic(x, self.cfg, **kw)
+        self.set_state(U)
+        return U
+
+    # -- verbs ----------------------------------------------------------------
+    def calc_residual(self, step=0):
+        _lib.check(self._lib.fr_calc_residual(self._h, step))
+
+    def residual(self, step=0):
+        out = np.zeros(self.state_shape())
+        _lib.check(self._lib.fr_get_residual(self._h, step, _lib.dptr(out), _lib.FR_LAYOUT_REF))
+        return out
+
+    def traces(self):
+        out = np.zeros((self.nFpts, self.nEles, self.nFields))
+        _lib.check(self._lib.fr_get_traces(self._h, _lib.dptr(out)))
+        return out
+
+    def update(self, n=1):
+        _lib.check(self._lib.fr_update(self._h, n))
+
+    def time_step_a(self, step, rkval):
+        _lib.check(self._lib.fr_time_step_a(self._h, step, rkval))
+
+    def time_step_b(self, step):
+        _lib.check(self._lib.fr_time_step_b(self._h, step))
+
+    def copy_u_u0(self):
+        _lib.check(self._lib.fr_copy_u_u0(self._h))
+
+    def copy_u0_u(self):
+        _lib.check(self._lib.fr_copy_u0_u(self._h))
+
+    def synchronize(self):
+        _lib.check(self._lib.fr_synchronize(self._h))
+
+    # -- diagnostics ----------------------------------------------------------
+    def norm_residual(self, kind=2):
+        out = np.zeros(self.nFields)
+        _lib.check(self._lib.fr_reduce(self._h, kind, _lib.dptr(out)))
+        return out
+
+    def kinetic_energy(self):
+        out = np.zeros(2)
+        _lib.check(self._lib.fr_reduce(self._h, 10, _lib.dptr(out)))
+        return out[0], out[1]
+
+    def operator(self, which):
+        D, nS, nF = self.nDims, self.nSpts, self.nFpts
+        shape = {0: (nF, nS), 1: (D, nS, nS), 2: (D, nF, nS), 3: (nS, nF), 4: (D, nS, nF)}[which]
+        out = np.zeros(shape)
+        _lib.check(self._lib.fr_get_operator(self._h, which, _lib.dptr(out)))
+        return out
+
+    def geometry(self, which):
+        D, nS, nF, nE = self.nDims, self.nSpts, self.nFpts, self.nEles
+        shape = {0: (D, nS, nE, D), 1: (nS, nE), 2: (nF, nE, D), 3: (nF, nE), 4: (nS, nE, D)}[which]
+        out = np.zeros(shape)
+        _lib.check(self._lib.fr_get_geometry(self._h, which, _lib.dptr(out)))
+        return out
+
+    def faces(self):
+        out = np.zeros((self.nFaces, 6), dtype=np.int32)
+        _lib.check(self._lib.fr_get_faces(self._h, _lib.iptr(out)))
+        return out
+
+    def residual_bytes(self):
+        b = ctypes.c_double()
+        _lib.check(self._lib.fr_residual_bytes(self._h, ctypes.byref(b)))
+        return b.value
+
+    def set_timing(self, on):
+        _lib.check(self._lib.fr_set_timing(self._h, int(on)))
+
+    def kernel_times(self, kernel=0):
+        ms = ctypes.c_double()
+        cnt = ctypes.c_long()
+        _lib.check(self._lib.fr_kernel_times(self._h, kernel, ctypes.byref(ms), ctypes.byref(cnt)))
+        return ms.value, cnt.value

@@ -1,0 +1,161 @@
+/*
+ * fr.h -- C ABI of libfrhip, the MI355X (gfx950) flux-reconstruction residual
+ * engine.  Plain C types only: pointers, sizes, ints and doubles.
+ *
+ * The ABI replaces the hot path of FlurryPP's `solver` class (reference
+ * include/solver.hpp:25-289, src/solver.cpp).  Each entry point names the
+ * reference interface it stands in for.
+ *
+ * Conventions
+ *  - every call returns 0 on success or a negative FR_E* code; the message of
+ *    the last failure on the calling thread is in fr_last_error().  The
+ *    reference calls fatalError() -> exit(1) (include/error.hpp:10-16) for the
+ *    same conditions; this library never exits;
+ *  - an fr_ctx owns all device memory; host buffers belong to the caller and
+ *    are copied during the call;
+ *  - host arrays passed with layout FR_LAYOUT_REF use the reference layout
+ *    (src/matrix.cpp:260-272, last index fastest): U(spt, ele, field) at
+ *    U[field + nFields*(ele + nEles*spt)];
+ *  - device work is ordered on the context's HIP stream; getters synchronize;
+ *  - one context per host thread; a context is not thread-safe.
+ */
+#ifndef FR_H
+#define FR_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fr_ctx fr_ctx;
+
+enum {
+  FR_OK = 0,
+  FR_EINVAL = -1,   /* bad argument or unsupported configuration */
+  FR_EMESH = -2,    /* mesh / connectivity error (geo.cpp fatalError cases) */
+  FR_EJACOBIAN = -3,/* negative Jacobian (solver.cpp:1283-1284) */
+  FR_EHIP = -4,     /* HIP runtime error or missing device */
+  FR_ENAN = -5,     /* NaN in residual (output.cpp:833-839) */
+  FR_ENOMEM = -6
+};
+
+enum { FR_LAYOUT_REF = 0, FR_LAYOUT_DEVICE = 1 };
+
+/* boundary-condition ids: include/global.hpp:63-79 */
+enum {
+  FR_BC_CHAR = 1, FR_BC_SUP_IN = 2, FR_BC_SUP_OUT = 3, FR_BC_SUB_IN = 4,
+  FR_BC_SUB_OUT = 5, FR_BC_SLIP_WALL = 8, FR_BC_ISOTHERMAL_NOSLIP = 9,
+  FR_BC_ADIABATIC_NOSLIP = 10, FR_BC_SYMMETRY = 12,
+  FR_BC_ISOTHERMAL_NOSLIP_MOVING = 13
+};
+
+enum { FR_PREC_FP64 = 0, FR_PREC_FP32 = 1 };
+
+/* Parameters of the path; mirrors the `input` fields it reads
+ * (include/input.hpp, defaults src/input.cpp:214-481). */
+typedef struct fr_config {
+  int equation;    /* 0 advection-diffusion, 1 Euler / Navier-Stokes */
+  int nDims;       /* 2 (quads) or 3 (hexes) */
+  int order;       /* P, 1..5 */
+  int riemannType; /* 0 Rusanov, 1 Roe (2D) */
+  int viscous;
+  int timeType;    /* 0 FE, 2 RK2, 4 RK44 ("RK45"), 5, 6 */
+  int sptsType;    /* 0 Gauss-Legendre, 1 Gauss-Lobatto */
+  int vcjhScheme;  /* 0 DG, 1 SD, 2 HU, 3 C+ */
+  int fixVis;
+  int viscFix;     /* 0 reproduce reference viscous index defects, 1 corrected */
+  double dt;
+  double gamma, advectVx, advectVy, advectVz, lambda, diffD;
+  double rhoBound, uBound, vBound, wBound, pBound;
+  double penFact, tau, mu_inf, prandtl, rt_inf, c_sth;
+  double TWall, RGas, uWall, vWall, wWall;
+} fr_config;
+
+/* Unstructured linear mesh (hex: 8 Gmsh-ordered nodes, quad: 4).
+ * vmap (optional) gives a canonical vertex id used only for face matching:
+ * identifying the vertices of opposite box faces makes them periodic. */
+typedef struct fr_mesh {
+  int nVerts;
+  const double *xv;   /* [nVerts][3] */
+  int nEles;
+  int nNodes;         /* nodes per element: 8 (hex) or 4 (quad) */
+  const int *c2v;     /* [nEles][nNodes] */
+  const int *vmap;    /* [nVerts] or NULL */
+  int nBounds;
+  const int *bcType;  /* [nBounds] */
+  const int *bndOff;  /* [nBounds+1] CSR offsets into bndPts */
+  const int *bndPts;  /* vertex ids on each boundary */
+} fr_mesh;
+
+/* solver::setup(input*, order, geo*) -- solver.cpp:21-63: builds the FR
+ * operators (operators.cpp:13-76), face connectivity (geo.cpp:75-446,
+ * intFace.cpp:6-71) and metrics (solver.cpp:109-210, 1227-1356) on device. */
+int fr_create(const fr_config *cfg, const fr_mesh *mesh, int precision, fr_ctx **out);
+int fr_destroy(fr_ctx *ctx);
+const char *fr_last_error(void);
+
+/* sizes: out[0..7] = nSpts, nFpts, nEles, nFields, nDims, nFaces, nIntFaces, nRKSteps */
+int fr_get_sizes(fr_ctx *ctx, int *out8);
+
+/* oper::opp_* public matrices (operators.hpp:133-146) rebuilt on the host:
+ * which = 0 spts_to_fpts, 1 grad_spts[D], 2 extrapolateFn[D], 3 correction,
+ * 4 correctU[D] (dense, row-major, reference shapes) */
+int fr_get_operator(fr_ctx *ctx, int which, double *out);
+
+/* geometry in reference layout: 0 JGinv_spts(dim,spt,ele,dim),
+ * 1 detJac_spts(spt,ele), 2 norm_fpts(fpt,ele,dim), 3 dA_fpts(fpt,ele),
+ * 4 pos_spts(spt,ele,dim) (solver.hpp geometry arrays, solver.cpp:109-210) */
+int fr_get_geometry(fr_ctx *ctx, int which, double *out);
+
+/* face table as built by geo::setupElesFaces (geo.cpp:322-446): per face
+ * (L ele, L local face, R ele, R local face, relRot, bcType or -1) */
+int fr_get_faces(fr_ctx *ctx, int *out6);
+
+/* U_spts setter / getter (solver.hpp:55; solver::initializeSolution writes it) */
+int fr_set_state(fr_ctx *ctx, const double *U, int layout);
+int fr_get_state(fr_ctx *ctx, double *U, int layout);
+
+/* solver::calcResidual(step) -- solver.cpp:252-291: divF_spts[step] from the
+ * current U_spts */
+int fr_calc_residual(fr_ctx *ctx, int step);
+/* divF_spts[step] (solver.hpp:63) */
+int fr_get_residual(fr_ctx *ctx, int step, double *out, int layout);
+/* Fn_fpts / U_fpts / disFn-free intermediate views are not materialized by the
+ * fused kernels; only the face traces U_fpts can be read back: */
+int fr_get_traces(fr_ctx *ctx, double *out /* U_fpts(fpt,ele,field) */);
+
+/* solver::update() -- solver.cpp:212-250, repeated nSteps times */
+int fr_update(fr_ctx *ctx, int nSteps);
+/* solver::timeStepA / timeStepB / copyUspts_U0 / copyU0_Uspts
+ * (solver.cpp:307-399) */
+int fr_time_step_a(fr_ctx *ctx, int step, double rkval);
+int fr_time_step_b(fr_ctx *ctx, int step);
+int fr_copy_u_u0(fr_ctx *ctx);
+int fr_copy_u0_u(fr_ctx *ctx);
+
+/* reductions: kind 1,2,3 = residual norm of divF_spts[0] as ele::getNormResidual
+ * + writeResidual (ele.cpp:1882-1903, output.cpp:846-892), out[nFields];
+ * kind 10 = kinetic energy integral of 0.5*rho*|u|^2 (out[0]) and volume
+ * (out[1]) by the spt quadrature of solver.cpp:1510-1526 */
+int fr_reduce(fr_ctx *ctx, int kind, double *out);
+
+/* Stream control.  fr_set_stream(ctx, (void*)hipStream_t) makes the context
+ * launch on the caller's stream (NULL = the context's own stream). */
+int fr_set_stream(fr_ctx *ctx, void *stream);
+int fr_synchronize(fr_ctx *ctx);
+
+/* Kernel timing: with timing on, every launch of the residual kernel is
+ * bracketed by HIP events on the launch stream; fr_kernel_times returns the
+ * accumulated milliseconds and launch count since the last reset for
+ * kernel id 0 = residual, 1 = trace, 2 = RK update. */
+int fr_set_timing(fr_ctx *ctx, int on);
+int fr_kernel_times(fr_ctx *ctx, int kernel, double *ms, long *count);
+
+/* Algorithmic HBM bytes of one residual evaluation (the B_res model: state in
+ * and residual out, per-spt adjugate metrics, one-sided normal + dA per face
+ * flux point, 20-byte face record), in the context's precision. */
+int fr_residual_bytes(fr_ctx *ctx, double *bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,195 @@
+"""CPU FP64 oracle of the FR residual + RK path -- TEST INFRASTRUCTURE ONLY.
+
+ctypes binding of liboracle.so (oracle/fr_oracle.c, a plain-C restatement of
+the reference algorithm).  Only tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg may use it, and only as the checker or the timed CPU
+baseline; the product (fr_learn_amd) never imports it.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+
+class OrParams(ctypes.Structure):
+    _fields_ = [
+        ("equation", ctypes.c_int), ("nDims", ctypes.c_int), ("order", ctypes.c_int),
+        ("riemannType", ctypes.c_int), ("viscous", ctypes.c_int), ("timeType", ctypes.c_int),
+        ("sptsType", ctypes.c_int), ("vcjhScheme", ctypes.c_int), ("fixVis", ctypes.c_int),
+        ("viscFix", ctypes.c_int),
+        ("dt", ctypes.c_double),
+        ("gamma", ctypes.c_double), ("advectVx", ctypes.c_double), ("advectVy", ctypes.c_double),
+        ("advectVz", ctypes.c_double), ("lambda_", ctypes.c_double), ("diffD", ctypes.c_double),
+        ("rhoBound", ctypes.c_double), ("uBound", ctypes.c_double), ("vBound", ctypes.c_double),
+        ("wBound", ctypes.c_double), ("pBound", ctypes.c_double),
+        ("penFact", ctypes.c_double), ("tau", ctypes.c_double), ("mu_inf", ctypes.c_double),
+        ("prandtl", ctypes.c_double), ("rt_inf", ctypes.c_double), ("c_sth", ctypes.c_double),
+        ("TWall", ctypes.c_double), ("RGas", ctypes.c_double), ("uWall", ctypes.c_double),
+        ("vWall", ctypes.c_double), ("wWall", ctypes.c_double),
+    ]
+
+
+_DP = ctypes.POINTER(ctypes.c_double)
+_IP = ctypes.POINTER(ctypes.c_int)
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FileNotFoundError(f"{LIB_PATH} missing: run python oracle/build.py")
+        lib = ctypes.CDLL(LIB_PATH)
+        lib.or_create.restype = ctypes.c_void_p
+        lib.or_create.argtypes = [ctypes.POINTER(OrParams), ctypes.c_int, _DP, ctypes.c_int, ctypes.c_int, _IP, _IP,
+                                  ctypes.c_int, _IP, _IP, _IP]
+        lib.or_last_error.restype = ctypes.c_char_p
+        for name in ("or_destroy", "or_copy_u_u0", "or_copy_u0_u", "or_update"):
+            getattr(lib, name).argtypes = [ctypes.c_void_p]
+            getattr(lib, name).restype = None
+        lib.or_sizes.argtypes = [ctypes.c_void_p, _IP]
+        lib.or_get_operator.argtypes = [ctypes.c_void_p, ctypes.c_int, _DP]
+        lib.or_get_geometry.argtypes = [ctypes.c_void_p, ctypes.c_int, _DP]
+        lib.or_get_faces.argtypes = [ctypes.c_void_p, _IP, _IP]
+        lib.or_set_state.argtypes = [ctypes.c_void_p, _DP]
+        lib.or_get_state.argtypes = [ctypes.c_void_p, _DP]
+        lib.or_get_array.argtypes = [ctypes.c_void_p, ctypes.c_int, _DP]
+        lib.or_calc_residual.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        lib.or_time_step_a.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]
+        lib.or_time_step_b.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        lib.or_norm_residual.argtypes = [ctypes.c_void_p, ctypes.c_int, _DP]
+        lib.or_set_threads.argtypes = [ctypes.c_int]
+        _lib = lib
+    return _lib
+
+
+def _d(a):
+    return a.ctypes.data_as(_DP)
+
+
+def _i(a):
+    return a.ctypes.data_as(_IP)
+
+
+class Oracle:
+    """Reference-layout FP64 CPU solver (same verbs as fr_learn_amd.Solver)."""
+
+    def __init__(self, cfg, mesh, threads=None):
+        lib = load()
+        if threads:
+            lib.or_set_threads(int(threads))
+        p = OrParams()
+        for name, _ in OrParams._fields_:
+            setattr(p, name, cfg[name])
+        self.cfg = dict(cfg)
+        bcs, off, pts = mesh.bnd_arrays()
+        vmap = mesh.vmap
+        self._keep = (mesh.xv, mesh.c2v, vmap, bcs, off, pts)
+        h = lib.or_create(ctypes.byref(p), mesh.xv.shape[0], _d(mesh.xv), mesh.nEles, mesh.nNodes, _i(mesh.c2v),
+                          _i(vmap) if vmap is not None else _IP(), len(mesh.bounds), _i(bcs), _i(off), _i(pts))
+        if not h:
+            raise RuntimeError("oracle: " + lib.or_last_error().decode())
+        self._h = ctypes.c_void_p(h)
+        self._lib = lib
+        sz = np.zeros(8, dtype=np.int32)
+        lib.or_sizes(self._h, _i(sz))
+        (self.nSpts, self.nFpts, self.nEles, self.nFields, self.nDims, self.nFaces, self.nIntFaces,
+         self.nRKSteps) = [int(v) for v in sz]
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.or_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def state_shape(self):
+        return (self.nSpts, self.nEles, self.nFields)
+
+    def set_state(self, U):
+        U = np.ascontiguousarray(U, dtype=np.float64).reshape(self.state_shape())
+        self._lib.or_set_state(self._h, _d(U))
+
+    def get_state(self):
+        out = np.zeros(self.state_shape())
+        self._lib.or_get_state(self._h, _d(out))
+        return out
+
+    def positions(self):
+        out = np.zeros((self.nSpts, self.nEles, self.nDims))
+        self._lib.or_get_geometry(self._h, 4, _d(out))
+        return out
+
+    def calc_residual(self, step=0):
+        self._lib.or_calc_residual(self._h, step)
+
+    def residual(self, step=0):
+        return self.array(10 + step)
+
+    def array(self, which):
+        D, nS, nF, nE, K = self.nDims, self.nSpts, self.nFpts, self.nEles, self.nFields
+        shapes = {0: (nS, nE, K), 1: (nF, nE, K), 2: (D, nS, nE, K), 3: (nF, nE, K), 4: (nF, nE, K),
+                  5: (D, nS, nE, K), 6: (D, nF, nE, K), 7: (nF, nE, K), 8: (nS, nE, K)}
+        shape = shapes.get(which, (nS, nE, K))
+        out = np.zeros(shape)
+        if self._lib.or_get_array(self._h, which, _d(out)) != 0:
+            raise ValueError(f"oracle array {which} not available")
+        return out
+
+    def update(self, n=1):
+        for _ in range(n):
+            self._lib.or_update(self._h)
+
+    def time_step_a(self, step, rkval):
+        self._lib.or_time_step_a(self._h, step, rkval)
+
+    def time_step_b(self, step):
+        self._lib.or_time_step_b(self._h, step)
+
+    def copy_u_u0(self):
+        self._lib.or_copy_u_u0(self._h)
+
+    def copy_u0_u(self):
+        self._lib.or_copy_u0_u(self._h)
+
+    def norm_residual(self, kind=2):
+        out = np.zeros(self.nFields)
+        self._lib.or_norm_residual(self._h, kind, _d(out))
+        return out
+
+    def operator(self, which):
+        D, nS, nF, P1 = self.nDims, self.nSpts, self.nFpts, self.cfg["order"] + 1
+        shape = {0: (nF, nS), 1: (D, nS, nS), 2: (D, nF, nS), 3: (nS, nF), 4: (D, nS, nF), 5: (nS, 3),
+                 6: (nF, 3), 7: (P1,), 8: (nS,)}[which]
+        out = np.zeros(shape)
+        self._lib.or_get_operator(self._h, which, _d(out))
+        return out
+
+    def geometry(self, which):
+        D, nS, nF, nE = self.nDims, self.nSpts, self.nFpts, self.nEles
+        shape = {0: (D, nS, nE, D), 1: (nS, nE), 2: (nF, nE, D), 3: (nF, nE), 4: (nS, nE, D), 5: (D, nS, nE, D)}[which]
+        out = np.zeros(shape)
+        self._lib.or_get_geometry(self._h, which, _d(out))
+        return out
+
+    def faces(self):
+        nfp = self.nFpts // (2 * self.nDims)
+        out = np.zeros((self.nFaces, 6), dtype=np.int32)
+        fptR = np.zeros((self.nFaces, nfp), dtype=np.int32)
+        self._lib.or_get_faces(self._h, _i(out), _i(fptR))
+        return out, fptR
+
+    def initialize(self, ic, **kw):
+        x = self.positions()
+        if self.nDims == 2:
+            x = np.concatenate([x, np.zeros(x.shape[:-1] + (1,))], axis=-1)
+        U = ic(x, self.cfg, **kw)
+        self.set_state(U)
+        return U
