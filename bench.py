@@ -1,0 +1,165 @@
+"""Benchmark: residual GDOF/s of the 3D Euler Taylor-Green workload on one MI355X.
+
+Workload: periodic [0, 2pi]^3, 48^3 linear hexes, P = 3 (64 solution points /
+96 flux points per element), Ma = 0.1, rho0 = 1, Rusanov flux, classic RK4
+(the reference's timeType 4, "RK45"), dt = 2e-4, synthetic TGV initial state.
+A step is one RK4 step = 4 residual evaluations + stage updates;
+value = nEles * nSpts * 4 * steps / seconds / 1e9 (DOF counts solution points).
+
+Prints ONE JSON line.  Besides the contract keys it carries:
+  roofline      the fused residual kernel's algorithmic bytes per launch over
+                its average launch time (HIP events on the launch stream),
+                against MI355X's 8 TB/s;
+  cpu_baseline  the CPU oracle (same algorithm, dense operators like the
+                reference's cblas_dgemm calls) on one RK step of the same
+                workload, on the host's cores;
+  fp32          the FP32 residual path (FP64 state / RK / reductions) and its
+                relative L2 distance from the FP64 path after the same steps.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "residual GDOF/s, 3D Euler hex P3, FP64 vs FP32-MFMA; % HBM roofline"
+HBM_PEAK = 8.0e12
+
+
+def tgv_setup(n, order=3, dt=2e-4):
+    from fr_learn_amd import cases, mesh
+    cfg = cases.config(nDims=3, order=order, equation=1, riemannType=0, timeType=4, dt=dt, gamma=1.4)
+    m = mesh.box(3, (n, n, n), (0, 0, 0), (2 * np.pi,) * 3)
+    return cfg, m
+
+
+def run_gpu(cfg, m, steps, warmup, precision):
+    from fr_learn_amd import Solver, cases
+    s = Solver(cfg, m, precision=precision)
+    s.initialize(cases.ic_tgv)
+    s.update(warmup)
+    s.synchronize()
+    s.set_timing(True)
+    s.kernel_times(0)
+    s.kernel_times(1)
+    s.kernel_times(2)
+    t0 = time.perf_counter()
+    s.update(steps)
+    s.synchronize()
+    wall = time.perf_counter() - t0
+    res_ms, res_n = s.kernel_times(0)
+    tr_ms, tr_n = s.kernel_times(1)
+    rk_ms, rk_n = s.kernel_times(2)
+    s.set_timing(False)
+    out = dict(wall=wall, res_ms=res_ms / max(res_n, 1), res_n=res_n, trace_ms=tr_ms / max(tr_n, 1),
+               rk_ms=rk_ms / max(rk_n, 1), bytes=s.residual_bytes(), nEles=s.nEles, nSpts=s.nSpts,
+               state=s.get_state())
+    s.close()
+    return out
+
+
+def run_cpu_baseline(cfg, m):
+    """One RK4 step of the same workload through the CPU oracle."""
+    from fr_learn_amd import cases
+    from oracle import Oracle
+    threads = min(16, os.cpu_count() or 1)
+    o = Oracle(cfg, m, threads=threads)
+    o.initialize(cases.ic_tgv)
+    nsteps = 2
+    t0 = time.perf_counter()
+    o.update(nsteps)
+    dt = time.perf_counter() - t0
+    gdof = o.nEles * o.nSpts * 4 * nsteps / dt / 1e9
+    o.close()
+    return dict(value=gdof, unit="GDOF/s", cores=threads, kind="port",
+                sample=f"{nsteps} RK4 steps ({4 * nsteps} residuals) of the same 48^3 P3 TGV workload, "
+                       f"{dt:.1f} s, dense-operator oracle, {threads} OpenMP threads")
+
+
+def traffic_from_profile(precision):
+    """HBM bytes per launch of the residual kernel from the committed rocprofv3
+    PMC summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor), if any."""
+    path = os.path.join(ROOT, "profiles", f"k_residual_tgv48_p3_{precision}_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        return json.load(fh).get("bytes_per_launch")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=48, help="cells per direction")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 path")
+    ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU path is "
+              "not part of this build", file=sys.stderr)
+        sys.exit(2)
+
+    cfg, m = tgv_setup(args.n)
+    main_prec = args.precision
+    r = run_gpu(cfg, m, args.steps, args.warmup, main_prec)
+    dof = r["nEles"] * r["nSpts"]
+    value = dof * 4 * args.steps / r["wall"] / 1e9
+    achieved = r["bytes"] / (r["res_ms"] * 1e-3)
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "GDOF/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r["wall"] / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if main_prec == "fp64" else "f32",
+        "data": "synthetic (Taylor-Green initial state, generated mesh)",
+        "config": {
+            "workload": "3D Euler Taylor-Green vortex, periodic [0,2pi]^3, 48^3 linear hexes, P=3, Ma=0.1, "
+                        "Rusanov, RK4 (timeType 4), dt=2e-4",
+            "cells": [args.n] * 3, "order": 3, "nEles": r["nEles"], "dof_per_field": dof, "fields": 5,
+            "precision": main_prec,
+        },
+        "roofline": {
+            "bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK, "traffic": traffic_from_profile(main_prec),
+            "kernel": "k_residual<3,3,5>", "algorithmic_bytes_per_launch": r["bytes"],
+            "avg_launch_ms": r["res_ms"], "launches": r["res_n"],
+        },
+        "kernels_ms": {"residual": r["res_ms"], "trace": r["trace_ms"], "rk_update": r["rk_ms"]},
+    }
+    if not args.no_fp32 and main_prec == "fp64":
+        r32 = run_gpu(cfg, m, args.steps, args.warmup, "fp32")
+        v32 = dof * 4 * args.steps / r32["wall"] / 1e9
+        a, b = r32["state"], r["state"]
+        l2 = float(np.linalg.norm(a - b) / np.linalg.norm(b))
+        l2f = [float(np.linalg.norm(a[..., k] - b[..., k]) / np.linalg.norm(b[..., k])) for k in range(5)]
+        a32 = r32["bytes"] / (r32["res_ms"] * 1e-3)
+        line["fp32"] = {
+            "value": v32, "unit": "GDOF/s", "ms_per_step": r32["wall"] / args.steps * 1e3,
+            "speedup_vs_fp64": v32 / value, "rel_l2_vs_fp64": l2, "rel_l2_per_field_vs_fp64": l2f,
+            "roofline": {"bound": "hbm", "achieved": a32 / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                         "frac": a32 / HBM_PEAK, "avg_launch_ms": r32["res_ms"],
+                         "algorithmic_bytes_per_launch": r32["bytes"]},
+            "kernels_ms": {"residual": r32["res_ms"], "trace": r32["trace_ms"], "rk_update": r32["rk_ms"]},
+        }
+    if not args.no_cpu:
+        line["cpu_baseline"] = run_cpu_baseline(cfg, m)
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()
