@@ -89,6 +89,11 @@ def box(nDims, n, lo, hi, periodic=None, bc=None, perturb=0.0, seed=0, rotate=Fa
     lo = np.asarray(lo, dtype=np.float64)[:D]
     hi = np.asarray(hi, dtype=np.float64)[:D]
     periodic = [True] * D if periodic is None else list(periodic)[:D]
+    for d in range(D):
+        if periodic[d] and n[d] < 3:
+            # with fewer than 3 cells a periodic face is not identified uniquely
+            # by its vertex set (two faces share the same canonical vertices)
+            raise ValueError("a periodic direction needs at least 3 cells")
     rng = np.random.default_rng(seed)
     dims = [v + 1 for v in n]
     grids = np.meshgrid(*[np.arange(d) for d in dims], indexing="ij")

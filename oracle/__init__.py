@@ -62,8 +62,67 @@ def load():
         lib.or_time_step_b.argtypes = [ctypes.c_void_p, ctypes.c_int]
         lib.or_norm_residual.argtypes = [ctypes.c_void_p, ctypes.c_int, _DP]
         lib.or_set_threads.argtypes = [ctypes.c_int]
+        P = ctypes.POINTER(OrParams)
+        lib.or_point_inviscid_flux.argtypes = [P, _DP, _DP]
+        lib.or_point_common_flux.argtypes = [P, ctypes.c_int, _DP, _DP, _DP, _DP]
+        lib.or_point_boundary_state.argtypes = [P, ctypes.c_int, _DP, _DP, _DP]
+        lib.or_point_viscous_flux.argtypes = [P, _DP, _DP, _DP]
+        lib.or_point_dvcjh.argtypes = [ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_double]
+        lib.or_point_dvcjh.restype = ctypes.c_double
+        lib.or_point_eta.argtypes = [ctypes.c_int, ctypes.c_int]
+        lib.or_point_eta.restype = ctypes.c_double
         _lib = lib
     return _lib
+
+
+def params(cfg):
+    p = OrParams()
+    for name, _ in OrParams._fields_:
+        setattr(p, name, cfg[name])
+    return p
+
+
+class Point:
+    """Pointwise oracle routines (test hooks)."""
+
+    def __init__(self, cfg):
+        self.lib = load()
+        self.p = params(cfg)
+        self.cfg = cfg
+
+    def inviscid_flux(self, U):
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        F = np.zeros((3, 5))
+        self.lib.or_point_inviscid_flux(ctypes.byref(self.p), _d(U), _d(F))
+        return F
+
+    def common_flux(self, kind, UL, UR, n):
+        UL, UR = np.ascontiguousarray(UL, dtype=np.float64), np.ascontiguousarray(UR, dtype=np.float64)
+        n = np.ascontiguousarray(n, dtype=np.float64)
+        Fn = np.zeros(5)
+        self.lib.or_point_common_flux(ctypes.byref(self.p), kind, _d(UL), _d(UR), _d(n), _d(Fn))
+        return Fn
+
+    def boundary_state(self, bc, UL, n):
+        UL = np.ascontiguousarray(UL, dtype=np.float64)
+        n = np.ascontiguousarray(n, dtype=np.float64)
+        UR = np.zeros(5)
+        self.lib.or_point_boundary_state(ctypes.byref(self.p), bc, _d(UL), _d(n), _d(UR))
+        return UR
+
+    def viscous_flux(self, U, grad):
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        g = np.zeros((3, 5))
+        g[:grad.shape[0], :grad.shape[1]] = grad
+        F = np.zeros((3, 5))
+        self.lib.or_point_viscous_flux(ctypes.byref(self.p), _d(U), _d(g), _d(F))
+        return F
+
+    def dvcjh(self, r, mode, order, eta):
+        return self.lib.or_point_dvcjh(r, mode, order, eta)
+
+    def eta(self, scheme, order):
+        return self.lib.or_point_eta(scheme, order)
 
 
 def _d(a):

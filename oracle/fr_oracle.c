@@ -1416,3 +1416,32 @@ void or_norm_residual(const or_solver *s, int normType, double *out) {
   if (normType == 2)
     for (int k = 0; k < K; k++) out[k] = sqrt(out[k]);
 }
+
+/* ------------------------------------------------------------------------ */
+/* test hooks: pointwise routines exposed so that tests can check them        */
+/* against independent textbook implementations                              */
+/* ------------------------------------------------------------------------ */
+void or_point_inviscid_flux(const or_params *p, const double *U, double *F /* [3][5] */) {
+  double t[3][5] = {{0}};
+  inviscid_flux(p, U, t);
+  memcpy(F, t, sizeof t);
+}
+/* kind: 0 Rusanov, 1 Roe, 2 Lax-Friedrichs (advection) */
+void or_point_common_flux(const or_params *p, int kind, const double *UL, const double *UR, const double *n,
+                          double *Fn) {
+  if (kind == 0) rusanov(p, UL, UR, n, Fn);
+  else if (kind == 1) roe(p, UL, UR, n, Fn);
+  else lax_friedrichs(p, UL, UR, n, Fn);
+}
+void or_point_boundary_state(const or_params *p, int bc, const double *UL, const double *n, double *UR) {
+  apply_bc(p, bc, UL, n, UR);
+}
+void or_point_viscous_flux(const or_params *p, const double *U, const double *grad /* [3][5] */,
+                           double *F /* [3][5] */) {
+  double g[3][5], t[3][5] = {{0}};
+  memcpy(g, grad, sizeof g);
+  viscous_flux(p, U, (const double(*)[5])g, t);
+  memcpy(F, t, sizeof t);
+}
+double or_point_dvcjh(double r, int mode, int order, double eta) { return dvcjh_1d(r, mode, order, eta); }
+double or_point_eta(int scheme, int order) { return compute_eta(scheme, order); }

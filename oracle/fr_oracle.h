@@ -101,6 +101,16 @@ void or_copy_u0_u(or_solver *s);
 void or_norm_residual(const or_solver *s, int normType, double *out);
 void or_set_threads(int n);
 
+/* pointwise test hooks (flux.cpp / face.cpp / boundFace.cpp / polynomials.cpp
+ * restatements), so tests can pin them against textbook formulas */
+void or_point_inviscid_flux(const or_params *p, const double *U, double *F /* [3][5] */);
+void or_point_common_flux(const or_params *p, int kind, const double *UL, const double *UR, const double *n,
+                          double *Fn); /* kind 0 Rusanov, 1 Roe, 2 Lax-Friedrichs */
+void or_point_boundary_state(const or_params *p, int bc, const double *UL, const double *n, double *UR);
+void or_point_viscous_flux(const or_params *p, const double *U, const double *grad, double *F);
+double or_point_dvcjh(double r, int mode, int order, double eta);
+double or_point_eta(int scheme, int order);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,479 @@
+"""Pins the CPU oracle (oracle/fr_oracle.c) against independent checks.
+
+The reference ships no tests, fixtures or golden vectors, and it cannot be
+built in this image (every translation unit includes cblas.h, absent here), so
+each oracle function is pinned by closed forms, textbook implementations
+(tests/independent.py) and invariants, chosen so that a plausible mistake --
+a dropped term, a wrong sign or index, a transposed operand -- fails one of
+them.  Tolerances are stated per test."""
+import numpy as np
+import pytest
+
+from fr_learn_amd import cases
+from oracle import Oracle, Point
+
+import independent as ind
+from helpers import (QUAD_BCS, advection_cfg, euler_cfg, hex_box, perturbed_state, quad_box, rel_err)
+
+L2PI = 2 * np.pi
+
+
+# ---------------------------------------------------------------------------
+# 1D points, weights, correction functions (points.cpp, polynomials.cpp)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("P", [1, 2, 3, 4, 5])
+def test_gauss_points_and_weights(P):
+    o = Oracle(euler_cfg(3, P), hex_box(3, perturb=0, rotate=False, shuffle=False))
+    x, w = ind.gauss_legendre(P)
+    # the 15-decimal tables agree with the Gauss-Legendre nodes to rounding of the 15th decimal
+    assert np.abs(o.operator(7) - x).max() < 1e-15
+    wq = o.operator(8)
+    idx, _ = ind.tensor_points(x, 3)
+    assert np.abs(wq - np.prod(w[idx], axis=1)).max() < 1e-15
+    assert abs(wq.sum() - 8.0) < 1e-13
+
+
+@pytest.mark.parametrize("P", [1, 2, 3, 4, 5])
+def test_lobatto_points(P):
+    cfg = euler_cfg(3, P, sptsType=1)
+    o = Oracle(cfg, hex_box(3, perturb=0, rotate=False, shuffle=False))
+    x = o.operator(7)
+    # interior Gauss-Lobatto nodes are the roots of P_P'
+    c = np.zeros(P + 1)
+    c[P] = 1
+    assert x[0] == -1 and x[-1] == 1
+    if P > 1:
+        assert np.abs(np.polynomial.legendre.legval(x[1:-1], np.polynomial.legendre.legder(c))).max() < 1e-13
+
+
+@pytest.mark.parametrize("scheme", [0, 1, 2, 3])
+@pytest.mark.parametrize("P", [2, 3, 4, 5])
+def test_vcjh_correction_derivative(scheme, P):
+    pt = Point(euler_cfg(3, P))
+    eta = pt.eta(scheme, P)
+    assert abs(eta - ind.vcjh_eta(scheme, P)) <= 1e-12 * max(1.0, abs(eta))
+    xs = np.linspace(-0.95, 0.95, 7)
+    right = np.array([pt.dvcjh(v, 1, P, eta) for v in xs])
+    left = np.array([pt.dvcjh(v, 0, P, eta) for v in xs])
+    ref = ind.vcjh_right_derivative(xs, P, eta)
+    assert np.abs(right - ref).max() < 1e-11
+    # g_L(x) = g_R(-x)  =>  g_L'(x) = -g_R'(-x)
+    assert np.abs(left + ind.vcjh_right_derivative(-xs, P, eta)).max() < 1e-11
+
+
+# ---------------------------------------------------------------------------
+# operators (operators.cpp)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("D", [2, 3])
+@pytest.mark.parametrize("P", [1, 3, 5])
+def test_operators_polynomial_exactness(D, P):
+    cfg = euler_cfg(D, P)
+    m = hex_box(3, perturb=0, rotate=False, shuffle=False) if D == 3 else quad_box(3, perturb=0, rotate=False,
+                                                                                    shuffle=False)
+    o = Oracle(cfg, m)
+    ext, grad = o.operator(0), o.operator(1)
+    locs, locf = o.operator(5)[:, :D], o.operator(6)[:, :D]
+    rng = np.random.default_rng(P)
+    for _ in range(3):
+        # random tensor polynomial of degree <= P in each variable
+        C = rng.standard_normal((P + 1,) * D)
+
+        def poly(x, dd=None):
+            val = np.zeros(len(x))
+            for idx in np.ndindex(C.shape):
+                term = C[idx] * np.ones(len(x))
+                for d in range(D):
+                    e = idx[d]
+                    if d == dd:
+                        term = term * (e * x[:, d] ** (e - 1) if e > 0 else 0 * x[:, d])
+                    else:
+                        term = term * x[:, d] ** e
+                val += term
+            return val
+
+        ps = poly(locs)
+        assert np.abs(ext @ ps - poly(locf)).max() < 1e-12 * np.abs(ps).max()
+        for d in range(D):
+            assert np.abs(grad[d] @ ps - poly(locs, d)).max() < 1e-10 * np.abs(ps).max()
+    # reference-normal extrapolation = extrapolation times the face's reference normal
+    tn = np.array([[0, 0, -1], [0, 0, 1], [-1, 0, 0], [1, 0, 0], [0, -1, 0], [0, 1, 0]] if D == 3 else
+                  [[0, -1], [1, 0], [0, 1], [-1, 0]])
+    nfpf = o.nFpts // (2 * D)
+    efn = o.operator(2)
+    for d in range(D):
+        assert np.array_equal(efn[d], ext * tn[np.arange(o.nFpts) // nfpf, d][:, None])
+
+
+@pytest.mark.parametrize("D", [2, 3])
+@pytest.mark.parametrize("P", [1, 2, 3, 4, 5])
+def test_dg_correction_closed_form(D, P):
+    """DG correction with Gauss points: each nonzero of opp_correction equals
+    l_m(+-1)/w_m of the 1D node m on the line normal to the face (Huynh 2007)."""
+    cfg = euler_cfg(D, P)
+    m = hex_box(3, perturb=0, rotate=False, shuffle=False) if D == 3 else quad_box(3, perturb=0, rotate=False,
+                                                                                    shuffle=False)
+    o = Oracle(cfg, m)
+    corr = o.operator(3)
+    x, w = ind.gauss_legendre(P)
+    lm = ind.lagrange_values(x, [-1.0, 1.0])  # [side][node]
+    locs, locf = o.operator(5), o.operator(6)
+    nfpf = o.nFpts // (2 * D)
+    # face normal axis and side from the flux-point location
+    for q in range(o.nFpts):
+        f = q // nfpf
+        ax = [2, 2, 0, 0, 1, 1][f] if D == 3 else [1, 0, 1, 0][f]
+        side = locf[q, ax] > 0
+        for s in range(o.nSpts):
+            same_line = all(abs(locs[s, d] - locf[q, d]) < 1e-15 for d in range(D) if d != ax)
+            if not same_line:
+                assert corr[s, q] == 0.0
+                continue
+            node = int(np.argmin(np.abs(x - locs[s, ax])))
+            want = lm[1 if side else 0, node] / w[node]
+            assert abs(corr[s, q] - want) < 2e-13 * abs(want)
+
+
+# ---------------------------------------------------------------------------
+# geometry (solver.cpp:904-951, 1227-1356)
+# ---------------------------------------------------------------------------
+def test_affine_box_metrics():
+    dx, dy, dz = 0.5, 0.25, 2.0
+    from fr_learn_amd import mesh
+    m = mesh.box(3, (3, 3, 3), (0, 0, 0), (1.5, 0.75, 6.0))
+    o = Oracle(euler_cfg(3, 3), m)
+    JG, det = o.geometry(0), o.geometry(1)
+    want = np.diag([dy * dz / 4, dx * dz / 4, dx * dy / 4])
+    # vertex coordinates lo + i*h carry rounding of a few ulps
+    assert np.abs(JG - want[:, None, None, :]).max() < 1e-14
+    assert np.abs(det - dx * dy * dz / 8).max() < 1e-14
+    nrm, dA = o.geometry(2), o.geometry(3)
+    ax = np.array([2, 2, 0, 0, 1, 1])[np.arange(o.nFpts) // 16]
+    sg = np.array([-1, 1, -1, 1, -1, 1])[np.arange(o.nFpts) // 16]
+    for q in range(o.nFpts):
+        e = np.zeros(3)
+        e[ax[q]] = sg[q]
+        assert np.abs(nrm[q] - e).max() < 1e-14
+        area = [dy * dz, dx * dz, dx * dy][ax[q]] / 4
+        assert np.abs(dA[q] - area).max() < 1e-14
+
+
+@pytest.mark.parametrize("D", [2, 3])
+def test_curved_metrics_vs_analytic(D):
+    m = hex_box(3, perturb=0.2, seed=5) if D == 3 else quad_box(4, perturb=0.2, seed=5)
+    o = Oracle(euler_cfg(D, 3), m)
+    x1, _ = ind.gauss_legendre(3)
+    _, xi = ind.tensor_points(x1, D)
+    X = m.element_nodes()
+    pos, J = ind.multilinear(X, xi)
+    det = np.linalg.det(J)
+    adj = det[..., None, None] * np.linalg.inv(J)  # [e][s][ref][phys]
+    assert np.abs(o.geometry(1).T - det).max() < 1e-13
+    assert np.abs(np.transpose(o.geometry(0), (2, 1, 0, 3)) - adj).max() < 1e-13
+    assert np.abs(np.transpose(o.positions(), (1, 0, 2)) - pos).max() < 1e-13
+    # normals: orthogonal to the face tangents, outward, |N| = surface element
+    locf = o.operator(6)[:, :D]
+    _, Jf = ind.multilinear(X, locf)
+    nrm, dA = o.geometry(2), o.geometry(3)
+    nfpf = o.nFpts // (2 * D)
+    for q in range(o.nFpts):
+        f = q // nfpf
+        ax = [2, 2, 0, 0, 1, 1][f] if D == 3 else [1, 0, 1, 0][f]
+        tang = [r for r in range(D) if r != ax]
+        for t in tang:
+            assert np.abs((Jf[:, q, :, t] * nrm[q]).sum(-1)).max() < 1e-13
+        if D == 3:
+            area = np.linalg.norm(np.cross(Jf[:, q, :, tang[0]], Jf[:, q, :, tang[1]]), axis=-1)
+        else:
+            area = np.linalg.norm(Jf[:, q, :, tang[0]], axis=-1)
+        assert np.abs(dA[q] - area).max() < 1e-13
+        outward = (nrm[q] * Jf[:, q, :, ax]).sum(-1) * (1 if locf[q, ax] > 0 else -1)
+        assert (outward > 0).all()
+
+
+# ---------------------------------------------------------------------------
+# face connectivity (geo.cpp, intFace.cpp)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("D", [2, 3])
+@pytest.mark.parametrize("periodic", [True, False])
+def test_face_pairing_is_geometric(D, periodic):
+    """Left flux point i and right flux point fptR[i] of every interior face
+    sit at the same physical point (modulo the period): pins relRot
+    (compareOrientation) and the right-point map on rotated, shuffled meshes."""
+    kw = dict(perturb=0.15, rotate=True, shuffle=True, seed=11, periodic=periodic,
+              bc=None if periodic else "slip_wall")
+    m = hex_box(3, **kw) if D == 3 else quad_box(4, **kw)
+    cfg = euler_cfg(D, 3)
+    o = Oracle(cfg, m)
+    faces, fptR = o.faces()
+    locf = o.operator(6)[:, :D]
+    xf, _ = ind.multilinear(m.element_nodes(), locf)  # [e][fpt][D]
+    Lbox = L2PI if D == 3 else 10.0
+    nfpf = o.nFpts // (2 * D)
+    nint = 0
+    for f, (eL, fL, eR, fR, rot, bc) in enumerate(faces):
+        if bc >= 0:
+            assert eR == -1
+            continue
+        nint += 1
+        assert eL < eR or (eL == eR and fL < fR)
+        a = xf[eL, fL * nfpf:(fL + 1) * nfpf]
+        b = xf[eR, fptR[f]]
+        diff = a - b
+        if periodic:
+            diff = diff - Lbox * np.round(diff / Lbox)
+        assert np.abs(diff).max() < 1e-12
+    assert nint == o.nIntFaces
+    if D == 3:
+        assert set(faces[:o.nIntFaces, 4]) == {0, 1, 2, 3}
+
+
+# ---------------------------------------------------------------------------
+# pointwise physics (flux.cpp, face.cpp, boundFace.cpp)
+# ---------------------------------------------------------------------------
+def _rand_states(D, n, seed):
+    rng = np.random.default_rng(seed)
+    rho = rng.uniform(0.5, 2.0, n)
+    vel = rng.uniform(-0.8, 0.8, (n, D))
+    p = rng.uniform(0.5, 2.0, n)
+    return cases.conservative(rho, [vel[:, d] for d in range(D)], p, 1.4)
+
+
+def _rand_normals(D, n, seed):
+    v = np.random.default_rng(seed).standard_normal((n, D))
+    return v / np.linalg.norm(v, axis=1, keepdims=True)
+
+
+@pytest.mark.parametrize("D", [2, 3])
+def test_euler_flux_and_rusanov(D):
+    cfg = euler_cfg(D, 3)
+    pt = Point(cfg)
+    UL, UR, n = _rand_states(D, 20, 1), _rand_states(D, 20, 2), _rand_normals(D, 20, 3)
+    F = ind.euler_flux(UL, D, 1.4)
+    for i in range(20):
+        assert np.allclose(pt.inviscid_flux(UL[i])[:D, :D + 2], F[i], rtol=1e-14, atol=1e-14)
+        got = pt.common_flux(0, UL[i], UR[i], n[i])[:D + 2]
+        want = ind.rusanov(UL[i], UR[i], n[i], D, 1.4)
+        assert np.allclose(got, want, rtol=1e-13, atol=1e-13)
+        # consistency: Fn(U, U, n) = F(U).n
+        assert np.allclose(pt.common_flux(0, UL[i], UL[i], n[i])[:D + 2], F[i].T @ n[i], rtol=1e-13, atol=1e-13)
+
+
+def test_roe_2d_vs_eigen_decomposition():
+    cfg = euler_cfg(2, 3)
+    pt = Point(cfg)
+    UL, UR, n = _rand_states(2, 40, 4), _rand_states(2, 40, 5), _rand_normals(2, 40, 6)
+    # flows tangent to the face (normal velocity ~ 0) so that the entropy fix
+    # is active on the contact/shear wave for these points
+    for i in range(10):
+        v = UL[i, 1:3] / UL[i, 0]
+        n[i] = np.array([-v[1], v[0]]) / np.linalg.norm(v)
+        UR[i] = UL[i] * (1 + 0.05 * (i + 1) / 10)
+    for i in range(40):
+        got = pt.common_flux(1, UL[i], UR[i], n[i])[:4]
+        want = ind.roe_2d(UL[i], UR[i], n[i], 1.4)
+        assert np.allclose(got, want, rtol=1e-12, atol=1e-12), i
+
+
+def test_lax_friedrichs_advection():
+    cfg = advection_cfg(3, 3, lambda_=0.7)
+    pt = Point(cfg)
+    a = np.array([1.0, 0.5, -0.25])
+    n = _rand_normals(3, 10, 7)
+    rng = np.random.default_rng(8)
+    for i in range(10):
+        uL, uR = rng.standard_normal(1), rng.standard_normal(1)
+        got = pt.common_flux(2, uL, uR, n[i])[0]
+        want = ind.lax_friedrichs_adv(uL[None], uR[None], n[i][None], a, 0.7)[0, 0]
+        assert abs(got - want) < 1e-14
+
+
+@pytest.mark.parametrize("D", [2, 3])
+@pytest.mark.parametrize("bc", [2, 3, 4, 5, 8, 9, 10, 12, 13])
+def test_boundary_states(D, bc):
+    cfg = euler_cfg(D, 3)
+    pt = Point(cfg)
+    UL, n = _rand_states(D, 10, 9), _rand_normals(D, 10, 10)
+    for i in range(10):
+        got = pt.boundary_state(bc, UL[i], n[i])[:D + 2]
+        want = ind.boundary_state(bc, UL[i], n[i], cfg)
+        assert np.allclose(got, want, rtol=1e-13, atol=1e-13)
+
+
+@pytest.mark.parametrize("D", [2, 3])
+def test_characteristic_boundary(D):
+    cfg = euler_cfg(D, 3)
+    pt = Point(cfg)
+    UL, n = _rand_states(D, 20, 12), _rand_normals(D, 20, 13)
+    g = 1.4
+    for i in range(20):
+        rho, vel, _, p = ind.primitives(UL[i], D, g)
+        vB = np.array([cfg["uBound"], cfg["vBound"], cfg["wBound"]])[:D]
+        if abs(vB @ n[i]) >= np.sqrt(g * cfg["pBound"] / cfg["rhoBound"]):
+            continue
+        got = pt.boundary_state(1, UL[i], n[i])[:D + 2]
+        want = ind.char_state_subsonic(UL[i], n[i], cfg)
+        assert np.allclose(got, want, rtol=1e-12, atol=1e-12)
+    # a free stream passes through unchanged (subsonic and supersonic)
+    for cfgk in (cfg, euler_cfg(D, 3, uBound=2.0, vBound=0.3, wBound=0.1)):
+        pk = Point(cfgk)
+        vB = [cfgk["uBound"], cfgk["vBound"], cfgk["wBound"]][:D]
+        Ub = cases.conservative(np.array(cfgk["rhoBound"]), [np.array(v) for v in vB],
+                                np.array(cfgk["pBound"]), g)
+        for i in range(10):
+            assert np.allclose(pk.boundary_state(1, Ub, n[i])[:D + 2], Ub, rtol=1e-12, atol=1e-12)
+
+
+# ---------------------------------------------------------------------------
+# assembled residual vs the weak form of nodal DG
+# ---------------------------------------------------------------------------
+WEAK_TOL = 1e-11
+
+
+@pytest.mark.parametrize("P", [1, 2, 3, 4])
+def test_residual_hex_euler_periodic_vs_weak_form(P):
+    cfg = euler_cfg(3, P)
+    m = hex_box(3, perturb=0.15, seed=2)
+    o = Oracle(cfg, m)
+    U = perturbed_state(o.initialize(cases.ic_tgv), seed=1)
+    o.set_state(U)
+    o.calc_residual(0)
+    w = ind.weak_form_residual(cfg, m, U, box_lo=[0] * 3, box_hi=[L2PI] * 3, periodic=[True] * 3)
+    assert rel_err(o.residual(0), w).max() < WEAK_TOL
+
+
+@pytest.mark.parametrize("P", [1, 3, 5])
+@pytest.mark.parametrize("riemann", [0, 1])
+def test_residual_quad_euler_periodic_vs_weak_form(P, riemann):
+    cfg = euler_cfg(2, P, riemannType=riemann)
+    m = quad_box(4, seed=3)
+    o = Oracle(cfg, m)
+    U = perturbed_state(o.initialize(lambda x, c: cases.ic_vortex(x, c)), seed=1)
+    o.set_state(U)
+    o.calc_residual(0)
+    w = ind.weak_form_residual(cfg, m, U, box_lo=[-5, -5], box_hi=[5, 5], periodic=[True, True])
+    assert rel_err(o.residual(0), w).max() < WEAK_TOL
+
+
+def _box_bc(lo, hi, spec):
+    names = ["x", "y", "z"]
+
+    def bc_of(p):
+        from fr_learn_amd.mesh import BC
+        for d in range(len(p)):
+            for side, val in (("min", lo[d]), ("max", hi[d])):
+                if abs(p[d] - val) < 1e-9:
+                    b = spec[names[d] + side] if isinstance(spec, dict) else spec
+                    return BC[b] if isinstance(b, str) else b
+        raise AssertionError("point not on the box boundary")
+
+    return bc_of
+
+
+@pytest.mark.parametrize("bcs", [
+    {"xmin": "sup_in", "xmax": "sup_out", "ymin": "slip_wall", "ymax": "symmetry", "zmin": "sub_in",
+     "zmax": "sub_out"},
+    {"xmin": "isothermal_noslip", "xmax": "adiabatic_noslip", "ymin": "isothermal_noslip_moving",
+     "ymax": "char", "zmin": "slip_wall", "zmax": "sup_out"}])
+def test_residual_hex_euler_walls_vs_weak_form(bcs):
+    cfg = euler_cfg(3, 3)
+    m = hex_box(3, periodic=False, bc=bcs, L=1.0, seed=4)
+    o = Oracle(cfg, m)
+    U = perturbed_state(o.initialize(lambda x, c: cases.ic_uniform(x, c, vel=[0.3, 0.1, -0.05], p=1 / 1.4)),
+                        seed=2, amp=2e-2)
+    o.set_state(U)
+    o.calc_residual(0)
+    w = ind.weak_form_residual(cfg, m, U, bc_of=_box_bc([0] * 3, [1.0] * 3, bcs))
+    assert rel_err(o.residual(0), w).max() < WEAK_TOL
+
+
+@pytest.mark.parametrize("D", [2, 3])
+def test_residual_advection_vs_weak_form(D):
+    cfg = advection_cfg(D, 3)
+    if D == 3:
+        m = hex_box(3, periodic=False, bc="sup_out", L=2.0, seed=5)
+        lo, hi = [0] * 3, [2.0] * 3
+    else:
+        m = quad_box(4, periodic=False, bc=QUAD_BCS, seed=5)
+        lo, hi = [-5, -5], [5, 5]
+    o = Oracle(cfg, m)
+    U = o.initialize(lambda x, c: cases.ic_gaussian(x - 1.0, c))
+    o.calc_residual(0)
+    w = ind.weak_form_residual(cfg, m, U, bc_of=lambda p: 0)
+    assert rel_err(o.residual(0), w).max() < WEAK_TOL
+
+
+# ---------------------------------------------------------------------------
+# invariants
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("D", [2, 3])
+def test_free_stream_preservation_and_conservation(D):
+    cfg = euler_cfg(D, 3, uBound=0.4, vBound=-0.2, wBound=0.1)
+    m = hex_box(3, perturb=0.2, seed=8) if D == 3 else quad_box(4, perturb=0.2, seed=8)
+    o = Oracle(cfg, m)
+    o.initialize(cases.ic_uniform)
+    o.calc_residual(0)
+    assert np.abs(o.residual(0)).max() < 1e-13
+    # periodic conservation: sum_e sum_spt w_spt divF = 0
+    ic = cases.ic_tgv if D == 3 else (lambda x, c: cases.ic_vortex(x, c))
+    U = perturbed_state(o.initialize(ic), seed=3)
+    o.set_state(U)
+    o.calc_residual(0)
+    r = o.residual(0)
+    w = o.operator(8)
+    total = (r * w[:, None, None]).sum(axis=(0, 1))
+    scale = (np.abs(r) * w[:, None, None]).sum(axis=(0, 1))
+    assert (np.abs(total) < 1e-12 * scale).all()
+
+
+# ---------------------------------------------------------------------------
+# time integration (solver.cpp:212-399; RK tables input.cpp:424-461)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("timeType,coef", [
+    (0, [1, 1]), (2, [1, 1, 1 / 2]), (4, [1, 1, 1 / 2, 1 / 6, 1 / 24]), (5, [1, 1, 1 / 2, 1 / 6, 1 / 24])])
+def test_rk_stability_polynomial_linear_advection(timeType, coef):
+    """For the linear advection residual dU/dt = -A U, one step of an s-stage
+    explicit RK scheme applies its stability polynomial: U1 = sum_k c_k (-dt A)^k U0
+    (Taylor coefficients 1/k! up to the scheme's order for FE, Heun, RK4)."""
+    dt = 0.05
+    cfg = advection_cfg(2, 2, timeType=timeType, dt=dt)
+    m = quad_box(3, periodic=True, seed=6)
+    o = Oracle(cfg, m)
+    U0 = o.initialize(cases.ic_gaussian)
+    J = o.geometry(1)[..., None]
+
+    def A(v):
+        o.set_state(v)
+        o.calc_residual(0)
+        return o.residual(0) / J
+
+    want = np.zeros_like(U0)
+    term = U0.copy()
+    for k, c in enumerate(coef):
+        if k > 0:
+            term = -dt * A(term)
+        want += c * term
+    o.set_state(U0)
+    o.update(1)
+    assert np.abs(o.get_state() - want).max() < 1e-14 * np.abs(U0).max() * 10
+
+
+def test_residual_norms_quadrature():
+    """Residual norms are Gauss quadratures of divF / detJ over the domain."""
+    cfg = euler_cfg(3, 3)
+    m = hex_box(3, seed=9)
+    o = Oracle(cfg, m)
+    U = perturbed_state(o.initialize(cases.ic_tgv), seed=4)
+    o.set_state(U)
+    o.calc_residual(0)
+    r = o.residual(0)
+    x, w = ind.gauss_legendre(3)
+    idx, xi = ind.tensor_points(x, 3)
+    ws = np.prod(w[idx], axis=1)
+    _, J = ind.multilinear(m.element_nodes(), xi)
+    det = np.linalg.det(J).T  # [spt][ele]
+    ref2 = np.sqrt((ws[:, None, None] * det[..., None] * (r / det[..., None]) ** 2).sum(axis=(0, 1)))
+    ref1 = (ws[:, None, None] * np.abs(r)).sum(axis=(0, 1))
+    refi = (np.abs(r) / det[..., None]).max(axis=(0, 1))
+    assert np.allclose(o.norm_residual(2), ref2, rtol=1e-13)
+    assert np.allclose(o.norm_residual(1), ref1, rtol=1e-13)
+    assert np.allclose(o.norm_residual(3), refi, rtol=1e-13)
