@@ -60,17 +60,21 @@ struct fr_ctx {
   double *U = nullptr, *U0 = nullptr, *detJ = nullptr, *JG64 = nullptr, *fgeo64 = nullptr, *ownN = nullptr;
   double *wts = nullptr, *oned64 = nullptr, *part = nullptr;
   float *U32 = nullptr, *JG32 = nullptr, *fgeo32 = nullptr, *oned32 = nullptr;
-  void *Uf = nullptr;
+  void *Us = nullptr;                 // RK stage state (residual precision)
+  void *Uf[2] = {nullptr, nullptr};   // face traces, double-buffered
+  int cur = 0;                        // Uf[cur] holds the traces of `traceOf`
+  int traceOf = -1;                   // 0: U_spts, 1: stage state Us, -1: stale
   void *divF[4] = {nullptr, nullptr, nullptr, nullptr};
-  int32_t *nbr = nullptr;
-  int8_t *fkind = nullptr;
+  void *dUf = nullptr;                // viscous: gradient traces [ele][dim][field][fpt]
+  int32_t *fnbr = nullptr;
+  int8_t *finfo = nullptr;
   size_t partCap = 0;
   std::vector<double> X;  // host copy of element node coordinates [ele][node][3]
   // timing
   bool timing = false;
-  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[3];
-  double ms[3] = {0, 0, 0};
-  long cnt[3] = {0, 0, 0};
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[4];
+  double ms[4] = {0, 0, 0, 0};
+  long cnt[4] = {0, 0, 0, 0};
 
   size_t nState() const { return (size_t)nE * NF * NS; }
   size_t tsize() const { return prec == FR_PREC_FP32 ? 4 : 8; }
@@ -79,9 +83,10 @@ struct fr_ctx {
 namespace {
 
 void freeAll(fr_ctx *c) {
-  void *bufs[] = {c->U, c->U0, c->detJ, c->JG64, c->fgeo64, c->ownN, c->wts, c->oned64, c->part, c->U32,
-                  c->JG32, c->fgeo32, c->oned32, c->Uf, c->divF[0], c->divF[1], c->divF[2], c->divF[3],
-                  c->nbr, c->fkind};
+  void *bufs[] = {c->U,       c->U0,      c->detJ,    c->JG64,    c->fgeo64, c->ownN,  c->wts,
+                  c->oned64,  c->part,    c->U32,     c->JG32,    c->fgeo32, c->oned32, c->Us,
+                  c->Uf[0],   c->Uf[1],   c->divF[0], c->divF[1], c->divF[2], c->divF[3], c->fnbr,
+                  c->finfo,   c->dUf};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   for (auto &v : c->ev)
@@ -120,93 +125,135 @@ struct TimeScope {
 };
 
 template <typename T>
-int launchResidual(fr_ctx *c, int step) {
-  const T *U = std::is_same<T, float>::value ? (const T *)c->U32 : (const T *)c->U;
-  const T *JG = std::is_same<T, float>::value ? (const T *)c->JG32 : (const T *)c->JG64;
-  const T *fg = std::is_same<T, float>::value ? (const T *)c->fgeo32 : (const T *)c->fgeo64;
+const T *stateIn(fr_ctx *c, int which) {
+  if (which == 1) return (const T *)c->Us;
+  return std::is_same<T, float>::value ? (const T *)c->U32 : (const T *)c->U;
+}
+
+// Make Uf[cur] hold the traces of state `which` (0 = U_spts, 1 = Us).
+template <typename T>
+int ensureTraces(fr_ctx *c, int which) {
+  if (c->traceOf == which) return 0;
+  const T *U = stateIn<T>(c, which);
   const T *o1 = std::is_same<T, float>::value ? (const T *)c->oned32 : (const T *)c->oned64;
-  T *Uf = (T *)c->Uf;
-  T *out = (T *)c->divF[step];
   hipError_t err = hipSuccess;
   bool ok = dispatch(c->D, c->P, c->NF, [&](auto Dc, auto Pc, auto NFc) {
     constexpr int D = decltype(Dc)::value, P = decltype(Pc)::value, NF = decltype(NFc)::value;
     using G = frdev::ElemGeo<D, P>;
-    {
-      TimeScope ts(c, 1);
-      hipLaunchKernelGGL((frdev::k_trace<D, P, NF, T>), dim3(nblk((long)c->nE * G::NFP)), dim3(256), 0, c->stream,
-                         U, Uf, o1, c->nE);
-    }
-    frdev::ResArgs<T> a;
-    a.U = U;
-    a.Uf = Uf;
-    a.JG = JG;
-    a.fgeo = fg;
-    a.nbr = c->nbr;
-    a.fkind = c->fkind;
-    a.oned = o1;
-    a.divF = out;
-    a.nEles = c->nE;
-    a.ph = c->ph;
-    {
+    TimeScope ts(c, 1);
+    hipLaunchKernelGGL((frdev::k_trace<D, P, NF, T>), dim3(nblk((long)c->nE * G::NFP)), dim3(256), 0, c->stream, U,
+                       (T *)c->Uf[c->cur], o1, c->nE);
+    err = hipGetLastError();
+  });
+  if (!ok) return fail(FR_EINVAL, "no kernel instance for this (nDims, order, nFields)");
+  if (err != hipSuccess) return fail(FR_EHIP, std::string("trace launch: ") + hipGetErrorString(err));
+  c->traceOf = which;
+  return 0;
+}
+
+// One launch of k_stage.  in: 0 = U_spts, 1 = Us.  mode as in k_stage.
+template <typename T>
+int launchStage(fr_ctx *c, int in, int mode, int step, double rk) {
+  int rc = ensureTraces<T>(c, in);
+  if (rc) return rc;
+  const bool f32 = std::is_same<T, float>::value;
+  frdev::StageArgs<T> a{};
+  a.Uin = stateIn<T>(c, in);
+  a.UfIn = (const T *)c->Uf[c->cur];
+  a.UfOut = (T *)c->Uf[c->cur ^ 1];
+  a.JG = f32 ? (const T *)c->JG32 : (const T *)c->JG64;
+  a.fgeo = f32 ? (const T *)c->fgeo32 : (const T *)c->fgeo64;
+  a.fnbr = c->fnbr;
+  a.finfo = c->finfo;
+  a.oned = f32 ? (const T *)c->oned32 : (const T *)c->oned64;
+  a.divF = (T *)c->divF[step];
+  a.Ubase = c->U;
+  a.detJ = c->detJ;
+  a.Unext = (T *)c->Us;
+  a.Uout = c->U;
+  a.Uout32 = f32 ? c->U32 : nullptr;
+  a.dprev0 = (const T *)c->divF[0];
+  a.dprev1 = (const T *)c->divF[c->nRK > 1 ? 1 : 0];
+  a.dprev2 = (const T *)c->divF[c->nRK > 2 ? 2 : 0];
+  a.c0 = c->RKb[0];
+  a.c1 = c->RKb[1];
+  a.c2 = c->RKb[2];
+  a.c3 = c->RKb[3];
+  a.rk = rk;
+  a.dt = c->cfg.dt;
+  a.dUf = (const T *)c->dUf;
+  a.mode = mode;
+  a.nprev = step;
+  a.nEles = c->nE;
+  a.ph = c->ph;
+  const bool bnd = c->conn.nIntFaces < (int)c->conn.faces.size();
+  const bool vis = c->cfg.viscous != 0;
+  hipError_t err = hipSuccess;
+  bool ok = dispatch(c->D, c->P, c->NF, [&](auto Dc, auto Pc, auto NFc) {
+    constexpr int D = decltype(Dc)::value, P = decltype(Pc)::value, NF = decltype(NFc)::value;
+    using G = frdev::ElemGeo<D, P>;
+    // grid padded to a multiple of the 8 XCDs (the kernels remap block ids)
+    const unsigned nb = (unsigned)(((c->nE + G::EPB - 1) / G::EPB + 7) / 8 * 8);
+    if (vis) {
+      frdev::GradArgs<T> g{a.Uin, a.UfIn, a.JG, a.detJ, a.fgeo, a.fnbr, a.finfo, a.oned, (T *)c->dUf, a.nEles, a.ph};
+      {
+        TimeScope ts(c, 3);
+        if (bnd) hipLaunchKernelGGL((frdev::k_grad<D, P, NF, T, true>), dim3(nb), dim3(G::NTH), 0, c->stream, g);
+        else hipLaunchKernelGGL((frdev::k_grad<D, P, NF, T, false>), dim3(nb), dim3(G::NTH), 0, c->stream, g);
+      }
+      if ((err = hipGetLastError()) != hipSuccess) return;
       TimeScope ts(c, 0);
-      hipLaunchKernelGGL((frdev::k_residual<D, P, NF, T>), dim3((c->nE + G::EPB - 1) / G::EPB), dim3(G::NTH), 0,
-                         c->stream, a);
+      if (bnd) hipLaunchKernelGGL((frdev::k_stage<D, P, NF, T, true, true>), dim3(nb), dim3(G::NTH), 0, c->stream, a);
+      else hipLaunchKernelGGL((frdev::k_stage<D, P, NF, T, false, true>), dim3(nb), dim3(G::NTH), 0, c->stream, a);
+    } else {
+      TimeScope ts(c, 0);
+      if (bnd) hipLaunchKernelGGL((frdev::k_stage<D, P, NF, T, true, false>), dim3(nb), dim3(G::NTH), 0, c->stream, a);
+      else hipLaunchKernelGGL((frdev::k_stage<D, P, NF, T, false, false>), dim3(nb), dim3(G::NTH), 0, c->stream, a);
     }
     err = hipGetLastError();
   });
   if (!ok) return fail(FR_EINVAL, "no kernel instance for this (nDims, order, nFields)");
-  if (err != hipSuccess) return fail(FR_EHIP, std::string("residual launch: ") + hipGetErrorString(err));
+  if (err != hipSuccess) return fail(FR_EHIP, std::string("stage launch: ") + hipGetErrorString(err));
+  if (mode != 0) {
+    c->cur ^= 1;
+    c->traceOf = (mode == 1) ? 1 : 0;
+  }
   return 0;
 }
 
 int residual(fr_ctx *c, int step) {
   if (step < 0 || step >= c->nRK) return fail(FR_EINVAL, "step out of range");
-  return c->prec == FR_PREC_FP32 ? launchResidual<float>(c, step) : launchResidual<double>(c, step);
+  return c->prec == FR_PREC_FP32 ? launchStage<float>(c, 0, 0, step, 0.0) : launchStage<double>(c, 0, 0, step, 0.0);
 }
 
+// solver::update (solver.cpp:212-250): every stage is one fused launch that
+// evaluates the residual of the stage state, applies the stage update and
+// extrapolates the new state to the flux points.  U_spts holds the step's
+// starting state (the reference's U0) until the final stage overwrites it.
+template <typename T>
+int updateT(fr_ctx *c) {
+  int rc;
+  for (int step = 0; step < c->nRK - 1; step++)
+    if ((rc = launchStage<T>(c, step == 0 ? 0 : 1, 1, step, c->RKa[step + 1]))) return rc;
+  const int last = c->nRK - 1;
+  const int in = (c->nRK == 1) ? 0 : 1;
+  if (c->cfg.timeType < 5) return launchStage<T>(c, in, 2, last, 0.0);
+  return launchStage<T>(c, in, 3, last, c->RKa[c->nRK]);
+}
+
+// timeStepA as a separate verb (the fused update does not use it)
 template <typename TD>
-int rkA(fr_ctx *c, int step, double rk, int copy) {
+int rkA(fr_ctx *c, int step, double rk) {
   long n = (long)c->nState();
   TimeScope ts(c, 2);
   hipLaunchKernelGGL((frdev::k_rk_a<TD>), dim3(nblk(n)), dim3(256), 0, c->stream, c->U, c->U0,
-                     (const TD *)c->divF[step], c->detJ, c->U32, rk, c->cfg.dt, copy, c->NF, c->NS, n);
+                     (const TD *)c->divF[step], c->detJ, c->U32, rk, c->cfg.dt, 0, c->NF, c->NS, n);
   HIP_TRY(hipGetLastError());
+  c->traceOf = -1;
   return 0;
 }
 
-template <typename TD>
-int rkB(fr_ctx *c) {
-  long n = (long)c->nState();
-  TimeScope ts(c, 2);
-  const TD *d[4];
-  for (int s = 0; s < 4; s++) d[s] = (const TD *)c->divF[s < c->nRK ? s : 0];
-  hipLaunchKernelGGL((frdev::k_rk_b<TD>), dim3(nblk(n)), dim3(256), 0, c->stream, c->U, c->U0, d[0], d[1], d[2],
-                     d[3], c->detJ, c->U32, c->RKb[0], c->RKb[1], c->RKb[2], c->RKb[3], c->cfg.dt, c->nRK, c->NF,
-                     c->NS, n);
-  HIP_TRY(hipGetLastError());
-  return 0;
-}
-
-// solver::update (solver.cpp:212-250) with the stage passes fused:
-// copy+timeStepA for stage 0, timeStepA for later stages, and one pass for
-// copyU0_Uspts + all timeStepB subtractions.
-int updateOnce(fr_ctx *c) {
-  int rc;
-  for (int step = 0; step < c->nRK - 1; step++) {
-    if ((rc = residual(c, step))) return rc;
-    rc = (c->prec == FR_PREC_FP32) ? rkA<float>(c, step, c->RKa[step + 1], step == 0)
-                                   : rkA<double>(c, step, c->RKa[step + 1], step == 0);
-    if (rc) return rc;
-  }
-  if ((rc = residual(c, c->nRK - 1))) return rc;
-  if (c->cfg.timeType < 5) {
-    rc = (c->prec == FR_PREC_FP32) ? rkB<float>(c) : rkB<double>(c);
-  } else {
-    rc = (c->prec == FR_PREC_FP32) ? rkA<float>(c, c->nRK - 1, c->RKa[c->nRK], 0)
-                                   : rkA<double>(c, c->nRK - 1, c->RKa[c->nRK], 0);
-  }
-  return rc;
-}
+int updateOnce(fr_ctx *c) { return c->prec == FR_PREC_FP32 ? updateT<float>(c) : updateT<double>(c); }
 
 // reference layout [pt][ele][field] <-> device [ele][field][pt]
 void refToDev(const double *src, double *dst, int nE, int NF, int NP) {
@@ -236,7 +283,7 @@ int createImpl(const fr_config *cfg, const fr_mesh *m, int precision, fr_ctx *c)
   if (cfg->nDims == 3 && cfg->equation == 1 && cfg->riemannType == 1)
     return fail(FR_EINVAL, "Roe not implemented in 3D");
   if (cfg->riemannType != 0 && cfg->riemannType != 1) return fail(FR_EINVAL, "riemannType must be 0 or 1");
-  if (cfg->viscous) return fail(FR_EINVAL, "viscous residual is not available in this build");
+  if (cfg->viscous != 0 && cfg->viscous != 1) return fail(FR_EINVAL, "viscous must be 0 or 1");
   if (!m || m->nEles <= 0) return fail(FR_EMESH, "Cannot setup elements array - nEles = 0");
   if (m->nNodes != (cfg->nDims == 3 ? 8 : 4)) return fail(FR_EINVAL, "only linear hexes (8 nodes) or quads (4)");
   for (int i = 0; i < m->nEles * m->nNodes; i++)
@@ -292,13 +339,16 @@ int createImpl(const fr_config *cfg, const fr_mesh *m, int precision, fr_ctx *c)
   if ((rc = dmalloc(&c->U, nSt)) || (rc = dmalloc(&c->U0, nSt)) || (rc = dmalloc(&c->detJ, (size_t)nE * NS)) ||
       (rc = dmalloc(&c->JG64, (size_t)nE * D * D * NS)) || (rc = dmalloc(&c->fgeo64, (size_t)nE * (D + 1) * NFP)) ||
       (rc = dmalloc(&c->ownN, (size_t)nE * D * NFP)) || (rc = dmalloc(&c->wts, NS)) ||
-      (rc = dmalloc(&c->oned64, c->ops.oned.size())) || (rc = dmalloc(&c->nbr, (size_t)nE * NFP)) ||
-      (rc = dmalloc(&c->fkind, (size_t)nE * c->NFACE)))
+      (rc = dmalloc(&c->oned64, c->ops.oned.size())) || (rc = dmalloc(&c->fnbr, (size_t)nE * c->NFACE)) ||
+      (rc = dmalloc(&c->finfo, (size_t)nE * c->NFACE)))
     return rc;
   HIP_TRY(hipMemset(c->U, 0, nSt * 8));
   HIP_TRY(hipMemset(c->U0, 0, nSt * 8));
   const size_t ts = c->tsize();
-  HIP_TRY(hipMalloc(&c->Uf, (size_t)nE * c->NF * NFP * ts));
+  for (int b = 0; b < 2; b++) HIP_TRY(hipMalloc(&c->Uf[b], (size_t)nE * c->NF * NFP * ts));
+  HIP_TRY(hipMalloc(&c->Us, nSt * ts));
+  if (cfg->viscous) HIP_TRY(hipMalloc(&c->dUf, (size_t)nE * D * c->NF * NFP * ts));
+  HIP_TRY(hipMemset(c->Us, 0, nSt * ts));
   for (int s = 0; s < c->nRK; s++) {
     HIP_TRY(hipMalloc(&c->divF[s], nSt * ts));
     HIP_TRY(hipMemset(c->divF[s], 0, nSt * ts));
@@ -330,7 +380,8 @@ int createImpl(const fr_config *cfg, const fr_mesh *m, int precision, fr_ctx *c)
     HIP_TRY(hipMemcpy(dT, c->ops.tNorm.data(), c->ops.tNorm.size() * 8, hipMemcpyHostToDevice));
     HIP_TRY(hipMemset(dNeg, 0, sizeof(int)));
     HIP_TRY(hipMemcpy(pairEF, c->conn.nbr.data(), (size_t)nE * NFP * 4, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(c->fkind, c->conn.fkind.data(), c->conn.fkind.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->finfo, c->conn.finfo.data(), c->conn.finfo.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->fnbr, c->conn.fnbr.data(), c->conn.fnbr.size() * 4, hipMemcpyHostToDevice));
     long np = (long)nE * (NS + NFP);
     if (D == 3)
       hipLaunchKernelGGL(frdev::k_metrics<3>, dim3(nblk(np)), dim3(256), 0, c->stream, dX, dS, dF, dT, NS, NFP, nN,
@@ -341,10 +392,10 @@ int createImpl(const fr_config *cfg, const fr_mesh *m, int precision, fr_ctx *c)
     HIP_TRY(hipGetLastError());
     long nq = (long)nE * NFP;
     if (D == 3)
-      hipLaunchKernelGGL(frdev::k_fgeo<3>, dim3(nblk(nq)), dim3(256), 0, c->stream, c->ownN, pairEF, c->fkind, NFP,
+      hipLaunchKernelGGL(frdev::k_fgeo<3>, dim3(nblk(nq)), dim3(256), 0, c->stream, c->ownN, pairEF, c->finfo, NFP,
                          c->NFPF, nE, c->fgeo64);
     else
-      hipLaunchKernelGGL(frdev::k_fgeo<2>, dim3(nblk(nq)), dim3(256), 0, c->stream, c->ownN, pairEF, c->fkind, NFP,
+      hipLaunchKernelGGL(frdev::k_fgeo<2>, dim3(nblk(nq)), dim3(256), 0, c->stream, c->ownN, pairEF, c->finfo, NFP,
                          c->NFPF, nE, c->fgeo64);
     HIP_TRY(hipGetLastError());
     int neg = 0;
@@ -353,17 +404,6 @@ int createImpl(const fr_config *cfg, const fr_mesh *m, int precision, fr_ctx *c)
     (void)hipFree(dX); (void)hipFree(dS); (void)hipFree(dF); (void)hipFree(dT); (void)hipFree(dNeg);
     (void)hipFree(pairEF);
     if (neg) return fail(FR_EJACOBIAN, "Negative Jacobian at solution points.");
-  }
-  // neighbour trace bases: (ele'*NFP + fpt') -> ele'*NF*NFP + fpt'
-  {
-    std::vector<int32_t> nb(c->conn.nbr.size());
-    for (size_t i = 0; i < nb.size(); i++) {
-      int32_t g = c->conn.nbr[i];
-      if (g < 0) { nb[i] = -1; continue; }
-      int32_t e2 = g / NFP, q2 = g - e2 * NFP;
-      nb[i] = e2 * c->NF * NFP + q2;
-    }
-    HIP_TRY(hipMemcpy(c->nbr, nb.data(), nb.size() * 4, hipMemcpyHostToDevice));
   }
   HIP_TRY(hipMemcpy(c->wts, c->ops.wts.data(), NS * 8, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(c->oned64, c->ops.oned.data(), c->ops.oned.size() * 8, hipMemcpyHostToDevice));
@@ -509,6 +549,7 @@ int fr_set_state(fr_ctx *c, const double *U, int layout) {
   }
   int rc = syncU32(c);
   if (rc) return rc;
+  c->traceOf = -1;
   HIP_TRY(hipStreamSynchronize(c->stream));
   return 0;
 }
@@ -553,14 +594,16 @@ int fr_get_residual(fr_ctx *c, int step, double *out, int layout) {
 int fr_get_traces(fr_ctx *c, double *out) {
   if (!c || !out) return fail(FR_EINVAL, "null argument");
   const size_t n = (size_t)c->nE * c->NF * c->NFP;
+  int rc = (c->prec == FR_PREC_FP32) ? ensureTraces<float>(c, 0) : ensureTraces<double>(c, 0);
+  if (rc) return rc;
   HIP_TRY(hipStreamSynchronize(c->stream));
   std::vector<double> h(n);
   if (c->prec == FR_PREC_FP32) {
     std::vector<float> f(n);
-    HIP_TRY(hipMemcpy(f.data(), c->Uf, n * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(f.data(), c->Uf[c->cur], n * 4, hipMemcpyDeviceToHost));
     for (size_t i = 0; i < n; i++) h[i] = f[i];
   } else {
-    HIP_TRY(hipMemcpy(h.data(), c->Uf, n * 8, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(h.data(), c->Uf[c->cur], n * 8, hipMemcpyDeviceToHost));
   }
   devToRef(h.data(), out, c->nE, c->NF, c->NFP);
   return 0;
@@ -578,7 +621,7 @@ int fr_update(fr_ctx *c, int nSteps) {
 int fr_time_step_a(fr_ctx *c, int step, double rkval) {
   if (!c) return fail(FR_EINVAL, "null argument");
   if (step < 0 || step >= c->nRK) return fail(FR_EINVAL, "step out of range");
-  return c->prec == FR_PREC_FP32 ? rkA<float>(c, step, rkval, 0) : rkA<double>(c, step, rkval, 0);
+  return c->prec == FR_PREC_FP32 ? rkA<float>(c, step, rkval) : rkA<double>(c, step, rkval);
 }
 
 int fr_time_step_b(fr_ctx *c, int step) {
@@ -592,6 +635,7 @@ int fr_time_step_b(fr_ctx *c, int step) {
     hipLaunchKernelGGL(frdev::k_rk_b1<double>, dim3(nblk(n)), dim3(256), 0, c->stream, c->U,
                        (const double *)c->divF[step], c->detJ, c->U32, c->RKb[step], c->cfg.dt, c->NF, c->NS, n);
   HIP_TRY(hipGetLastError());
+  c->traceOf = -1;
   return 0;
 }
 
@@ -604,6 +648,7 @@ int fr_copy_u_u0(fr_ctx *c) {
 int fr_copy_u0_u(fr_ctx *c) {
   if (!c) return fail(FR_EINVAL, "null argument");
   HIP_TRY(hipMemcpyAsync(c->U, c->U0, c->nState() * 8, hipMemcpyDeviceToDevice, c->stream));
+  c->traceOf = -1;
   return syncU32(c);
 }
 
@@ -677,9 +722,9 @@ int fr_set_timing(fr_ctx *c, int on) {
 }
 
 int fr_kernel_times(fr_ctx *c, int kernel, double *ms, long *count) {
-  if (!c || kernel < 0 || kernel > 2) return fail(FR_EINVAL, "bad argument");
+  if (!c || kernel < 0 || kernel > 3) return fail(FR_EINVAL, "bad argument");
   HIP_TRY(hipStreamSynchronize(c->stream));
-  for (int k = 0; k < 3; k++) {
+  for (int k = 0; k < 4; k++) {
     for (auto &pr : c->ev[k]) {
       float t = 0;
       HIP_TRY(hipEventElapsedTime(&t, pr.first, pr.second));

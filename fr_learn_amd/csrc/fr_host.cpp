@@ -185,6 +185,42 @@ std::string verifyMapsT(const Operators &o) {
   return "";
 }
 
+// The kernels derive each flux point's partner from the per-face record
+// (neighbourFpt); check that against the explicit pairing built from
+// intFace::setupRightState's maps.
+template <int D, int P>
+std::string verifyNeighbourT(const Operators &o, const Connectivity &c, int nE) {
+  const int nF = o.nFpts, nfpf = o.nFptsFace, nfe = o.nFacesEle;
+  for (int e = 0; e < nE; e++)
+    for (int q = 0; q < nF; q++) {
+      const int info = c.finfo[(size_t)e * nfe + q / nfpf];
+      const int32_t want = c.nbr[(size_t)e * nF + q];
+      if (info >= frdev::FINFO_BOUNDARY) {
+        if (want != -1) return "boundary face with a paired flux point";
+        continue;
+      }
+      const int32_t got = c.fnbr[(size_t)e * nfe + q / nfpf] * nF + frdev::neighbourFpt<D, P>(q, info);
+      if (got != want) return "neighbour flux-point map mismatch";
+    }
+  return "";
+}
+
+std::string verifyNeighbourMaps(const Operators &o, const Connectivity &c, int nE) {
+  switch (o.nDims * 10 + o.order) {
+    case 31: return verifyNeighbourT<3, 1>(o, c, nE);
+    case 32: return verifyNeighbourT<3, 2>(o, c, nE);
+    case 33: return verifyNeighbourT<3, 3>(o, c, nE);
+    case 34: return verifyNeighbourT<3, 4>(o, c, nE);
+    case 35: return verifyNeighbourT<3, 5>(o, c, nE);
+    case 21: return verifyNeighbourT<2, 1>(o, c, nE);
+    case 22: return verifyNeighbourT<2, 2>(o, c, nE);
+    case 23: return verifyNeighbourT<2, 3>(o, c, nE);
+    case 24: return verifyNeighbourT<2, 4>(o, c, nE);
+    case 25: return verifyNeighbourT<2, 5>(o, c, nE);
+  }
+  return "unsupported element order";
+}
+
 std::string verifyIndexMaps(const Operators &o) {
   const int key = o.nDims * 10 + o.order;
   switch (key) {
@@ -464,11 +500,12 @@ std::string buildConnectivity(const fr_config &cfg, const fr_mesh &m, const Oper
   const int nFaces = (int)c.faces.size();
   c.fptR.assign((size_t)nFaces * nfp, -1);
   c.nbr.assign((size_t)nE * nF, -1);
-  c.fkind.assign((size_t)nE * nfe, 0);
+  c.fnbr.assign((size_t)nE * nfe, -1);
+  c.finfo.assign((size_t)nE * nfe, -1);
   for (int fi = 0; fi < nFaces; fi++) {
     FaceRec &fr = c.faces[fi];
     if (fr.bc >= 0) {
-      c.fkind[(size_t)fr.eL * nfe + fr.fL] = (int8_t)(2 + fr.bc);
+      c.finfo[(size_t)fr.eL * nfe + fr.fL] = (int8_t)(frdev::FINFO_BOUNDARY + fr.bc);
       continue;
     }
     int *map = &c.fptR[(size_t)fi * nfp];
@@ -496,15 +533,19 @@ std::string buildConnectivity(const fr_config &cfg, const fr_mesh &m, const Oper
       fr.rot = 1;
       for (int i = 0; i < nfp; i++) map[i] = fr.fR * nfp + nfp - 1 - i;
     }
-    c.fkind[(size_t)fr.eL * nfe + fr.fL] = 0;
-    c.fkind[(size_t)fr.eR * nfe + fr.fR] = 1;
+    c.fnbr[(size_t)fr.eL * nfe + fr.fL] = fr.eR;
+    c.fnbr[(size_t)fr.eR * nfe + fr.fR] = fr.eL;
+    c.finfo[(size_t)fr.eL * nfe + fr.fL] = (int8_t)frdev::packFinfo(fr.fR, fr.rot, 0);
+    c.finfo[(size_t)fr.eR * nfe + fr.fR] = (int8_t)frdev::packFinfo(fr.fL, fr.rot, 1);
     for (int i = 0; i < nfp; i++) {
       const int qL = fr.fL * nfp + i, qR = map[i];
       c.nbr[(size_t)fr.eL * nF + qL] = fr.eR * nF + qR;
       c.nbr[(size_t)fr.eR * nF + qR] = fr.eL * nF + qL;
     }
   }
-  return "";
+  for (size_t i = 0; i < c.finfo.size(); i++)
+    if (c.finfo[i] < 0) return "element face without a face record";
+  return verifyNeighbourMaps(o, c, nE);
 }
 
 }  // namespace frhost

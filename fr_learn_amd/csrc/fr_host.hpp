@@ -45,9 +45,10 @@ struct Connectivity {
   std::vector<FaceRec> faces;   // interior faces first, then boundary faces
   int nIntFaces = 0;
   std::vector<int> fptR;        // [nFaces][nFptsFace] right-element local fpt per left fpt
-  // Element-centric tables consumed by the kernels:
-  std::vector<int32_t> nbr;     // [nEles][nFpts]: neighbor (ele*nFpts + fpt) or -1
-  std::vector<int8_t> fkind;    // [nEles][nFacesEle]: 0 left, 1 right, 2+bc boundary
+  // Element-centric tables:
+  std::vector<int32_t> nbr;     // [nEles][nFpts]: paired (ele*nFpts + fpt) or -1 (setup only)
+  std::vector<int32_t> fnbr;    // [nEles][nFacesEle]: neighbour element or -1   (kernels)
+  std::vector<int8_t> finfo;    // [nEles][nFacesEle]: packed face record, see fr_layout.hpp
 };
 
 std::string buildConnectivity(const fr_config &cfg, const fr_mesh &mesh, const Operators &ops,

@@ -8,8 +8,9 @@
 //   Uf            [ele][field][fpt]                extrapolated face traces
 //   fgeo          [ele][c][fpt]   c < D: unit normal of the face's LEFT element,
 //                                 c == D: this element's own dA
-//   nbr           [ele][fpt]      neighbour trace base (ele'*NF*NFP + fpt') or -1
-//   fkind         [ele][face]     0 left side, 1 right side, 2+bc boundary
+//   fnbr          [ele][face]     neighbour element (interior faces)
+//   finfo         [ele][face]     packed face record (fr_layout.hpp packFinfo)
+//   dUf           [ele][dim][field][fpt]  gradient traces (viscous only)
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -21,20 +22,6 @@
 
 namespace frdev {
 
-template <typename T>
-struct ResArgs {
-  const T *__restrict__ U;
-  const T *__restrict__ Uf;
-  const T *__restrict__ JG;
-  const T *__restrict__ fgeo;
-  const int32_t *__restrict__ nbr;
-  const int8_t *__restrict__ fkind;
-  const T *__restrict__ oned;
-  T *__restrict__ divF;
-  int nEles;
-  Phys ph;
-};
-
 // Common normal flux at one face point, (UL, UR) in the left element's frame.
 template <int D, int NF, typename T>
 __device__ __forceinline__ void commonFlux(const Phys &ph, bool boundary, const T *UL, const T *UR, const T *n,
@@ -43,9 +30,9 @@ __device__ __forceinline__ void commonFlux(const Phys &ph, bool boundary, const 
     laxFriedrichs<D, T>(ph, UL, UR, n, Fn);
   } else if constexpr (D == 2) {
     if (!boundary && ph.riemannType == 1) roe2d<T>(ph, UL, UR, n, Fn);
-    else rusanov<D, T>(ph, UL, UR, n, Fn);
+    else rusanovN<D, T>(ph, UL, UR, n, Fn);
   } else {
-    rusanov<D, T>(ph, UL, UR, n, Fn);
+    rusanovN<D, T>(ph, UL, UR, n, Fn);
   }
 }
 
@@ -76,136 +63,534 @@ __global__ __launch_bounds__(256) void k_trace(const T *__restrict__ U, T *__res
 }
 
 // ---------------------------------------------------------------------------
-// Fused inviscid residual of one element per TPE-thread group:
+// One RK stage of one element per TPE-thread group:
 //   calcInviscidFlux_spts (solver.cpp:514-539), calcInviscidFlux_faces
 //   (face.cpp:132-175 + intFace/boundFace hooks), extrapolateNormalFlux
-//   (solver.cpp:764-786), calcDivF_spts (solver.cpp:707-727) and
-//   correctDivFlux (solver.cpp:789-806).
-// Output divF = reference-space residual divF_spts[step].
+//   (solver.cpp:764-786), calcDivF_spts (solver.cpp:707-727),
+//   correctDivFlux (solver.cpp:789-806), then the stage update
+//   (timeStepA / timeStepB, solver.cpp:307-385) and the face traces of the
+//   updated state for the next stage (extrapolateU, solver.cpp:401-412).
+// mode 0: residual only (divF out).
+// mode 1: stage update Unext = Ubase - rk * divF / detJ * dt.
+// mode 2: final update U = Ubase - sum_r RKb[r] divF_r / detJ * dt, the
+//         subtractions in stage order as the timeStepB passes do.
+// mode 3: final Jameson-type update U = Ubase - rk * divF / detJ * dt.
+// The state an element reads (Uin / Ubase) is read only by that element, so
+// Unext / Uout may alias them; traces are double-buffered (UfIn != UfOut).
 // ---------------------------------------------------------------------------
+template <typename T>
+struct StageArgs {
+  const T *Uin;
+  const T *__restrict__ UfIn;
+  T *__restrict__ UfOut;
+  const T *__restrict__ JG;
+  const T *__restrict__ fgeo;
+  const int32_t *__restrict__ fnbr;  // [ele][face] neighbour element
+  const int8_t *__restrict__ finfo;  // [ele][face] packed face record
+  const T *__restrict__ oned;
+  T *divF;
+  const double *Ubase;
+  const double *__restrict__ detJ;
+  T *Unext;
+  double *Uout;
+  float *Uout32;
+  const T *dprev0, *dprev1, *dprev2;
+  const T *__restrict__ dUf;  // viscous: gradient traces [ele][dim][field][fpt]
+  double c0, c1, c2, c3, rk, dt;
+  int mode, nprev, nEles;
+  Phys ph;
+};
+
+// Face f of the two faces normal to reference axis d (hex: x 2,3; y 4,5;
+// z 0,1 -- quad: x 3,1; y 0,2).
+template <int D>
+__device__ __forceinline__ int dirFace(int d, int which) {
+  if (D == 3) return (d == 0) ? 2 + which : (d == 1) ? 4 + which : which;
+  return (d == 0) ? (which ? 1 : 3) : (which ? 2 : 0);
+}
+
+// Reference axis normal to face f.
+template <int D>
+__device__ __forceinline__ int faceAxis(int f) {
+  if (D == 3) return (f < 2) ? 2 : (f < 4) ? 0 : 1;
+  return (f & 1) ? 0 : 1;
+}
+
+// LDG common solution at one flux point (face::ldgSolution, face.cpp:582-607)
+// returned as the jump dUc = UC - own trace (face.cpp:159-168,
+// intFace.cpp:122-126).  uo / un are this element's and the neighbour's (or
+// ghost) traces; right: this element is the face's right element.
+template <int D, int NF, typename T>
+__device__ __forceinline__ void ldgJump(const Phys &ph, bool bnd, bool right, const T *uo, const T *un, const T *n,
+                                        T *dUc) {
+  const T pen = bnd ? (T)0 : ldgPenalty<D, T>(ph, n);
+#pragma unroll
+  for (int k = 0; k < NF; k++) {
+    const T UL = right ? un[k] : uo[k], UR = right ? uo[k] : un[k];
+    const T UC = bnd ? (T)0.5 * (UL + UR) : (T)0.5 * (UL + UR) - pen * (UL - UR);
+    dUc[k] = UC - uo[k];
+  }
+}
+
+// Corrected physical gradient at solution point t (calcGradU_spts,
+// solver.cpp:808-821; correctGradU 823-879): the reference-space gradient
+// sum_m Dm u along each axis, plus correctU = correction * tNorm applied to the
+// flux-point jumps dUc (only the two faces normal to the axis are nonzero, in
+// ascending flux-point order), then transformed with JGinv / detJ.
+// sUe: [NF][NS] state, sCe: [NF][NFP] jumps, J: JGinv(dimRef, dimPhys).
 template <int D, int P, int NF, typename T>
-__global__ __launch_bounds__((ElemGeo<D, P>::NTH)) void k_residual(ResArgs<T> a) {
+__device__ __forceinline__ void correctedGradient(const T *s1, const T *sUe, const T *sCe, int t, int si, int sj,
+                                                  int sk, const T *J, T detJ, T g[D][NF]) {
+  using G = ElemGeo<D, P>;
+  using O = OneD<P>;
+  constexpr int NS = G::NS, NFP = G::NFP, P1 = G::P1, NFACE = G::NFACE;
+  int cq[NFACE], cside[NFACE], cax[NFACE];
+  corrEntries<D, P>(t, cq, cside, cax);
+  T gr[D][NF];
+#pragma unroll
+  for (int d = 0; d < D; d++) {
+    const int id = (d == 0) ? si : (d == 1) ? sj : sk;
+#pragma unroll
+    for (int k = 0; k < NF; k++) {
+      T acc = (T)0;
+#pragma unroll
+      for (int m = 0; m < P1; m++) {
+        const int sp = (d == 0) ? m + P1 * (sj + P1 * sk) : (d == 1) ? si + P1 * (m + P1 * sk) : si + P1 * (sj + P1 * m);
+        acc += s1[O::DM + id * P1 + m] * sUe[k * NS + sp];
+      }
+      T corr = (T)0;
+#pragma unroll
+      for (int f = 0; f < NFACE; f++)
+        if (faceAxis<D>(f) == d)
+          corr += (s1[(cside[f] ? O::GR : O::GL) + cax[f]] * (cside[f] ? (T)1 : (T)-1)) * sCe[k * NFP + cq[f]];
+      gr[d][k] = corr + acc;
+    }
+  }
+  const T invDet = (T)1. / detJ;
+#pragma unroll
+  for (int d1 = 0; d1 < D; d1++)
+#pragma unroll
+    for (int k = 0; k < NF; k++) {
+      T acc = gr[0][k] * J[0 * D + d1] + gr[1][k] * J[1 * D + d1];
+      if constexpr (D == 3) acc = acc + gr[2][k] * J[2 * D + d1];
+      g[d1][k] = invDet * acc;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Gradient traces for the viscous path: LDG common solution at every flux
+// point, corrected physical gradient at the solution points, and its
+// extrapolation to the flux points (extrapolateGradU, solver.cpp:881-895),
+// written to dUf[ele][dim][field][fpt] for the neighbours' viscous face flux.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct GradArgs {
+  const T *__restrict__ Uin;
+  const T *__restrict__ UfIn;
+  const T *__restrict__ JG;
+  const double *__restrict__ detJ;
+  const T *__restrict__ fgeo;
+  const int32_t *__restrict__ fnbr;
+  const int8_t *__restrict__ finfo;
+  const T *__restrict__ oned;
+  T *__restrict__ dUf;
+  int nEles;
+  Phys ph;
+};
+
+template <int D, int P, int NF, typename T, bool BND>
+__global__ __launch_bounds__((ElemGeo<D, P>::NTH)) void k_grad(GradArgs<T> a) {
   using G = ElemGeo<D, P>;
   using O = OneD<P>;
   constexpr int NS = G::NS, NFP = G::NFP, NFPF = G::NFPF, P1 = G::P1, TPE = G::TPE, EPB = G::EPB;
-  constexpr int NFACE = G::NFACE;
+  constexpr int NFACE = G::NFACE, NQ = (NFP + TPE - 1) / TPE;
   __shared__ T sU[EPB][NF][NS];
-  __shared__ T sF[EPB][D][NF][NS];
-  __shared__ T sJ[EPB][NF][NFP];
+  __shared__ T sC[EPB][NF][NFP];
+  __shared__ T sG[EPB][D][NF][NS];
   __shared__ T s1[O::SIZE];
+  const int b = (blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);
   const int slot = threadIdx.x / TPE, t = threadIdx.x - slot * TPE;
-  const int e = blockIdx.x * EPB + slot;
+  const int e = b * EPB + slot;
   const bool act = e < a.nEles;
+  const bool hasS = act && t < NS;
   for (int i = threadIdx.x; i < O::SIZE; i += blockDim.x) s1[i] = a.oned[i];
   if (act) {
-    const T *Ue = a.U + (size_t)e * NF * NS;
+    const T *Ue = a.Uin + (size_t)e * NF * NS;
     for (int i = t; i < NF * NS; i += TPE) (&sU[slot][0][0])[i] = Ue[i];
   }
+  T pJ[D * D];
+  T pdet = (T)1;
+  if (hasS) {
+#pragma unroll
+    for (int c = 0; c < D * D; c++) pJ[c] = a.JG[((size_t)e * D * D + c) * NS + t];
+    pdet = (T)a.detJ[(size_t)e * NS + t];
+  }
   __syncthreads();
-
-  // (1) point fluxes, transformed to reference space F~_d = sum JGinv(d,d2) F_d2
-  if (act) {
-    for (int s = t; s < NS; s += TPE) {
-      T u[NF], F[D][NF], J[D * D];
+  // LDG jump at every flux point
 #pragma unroll
-      for (int k = 0; k < NF; k++) u[k] = sU[slot][k][s];
-      inviscidFlux<D, NF, T>(a.ph, u, F);
+  for (int r = 0; r < NQ; r++) {
+    const int q = t + r * TPE;
+    if (act && q < NFP) {
+      int base, stride, side, dn;
+      fptLine<D, P>(q, base, stride, side, dn);
+      const T *l1 = s1 + (side ? O::LR : O::LL);
+      T uo[NF], un[NF], n[D], dUc[NF];
 #pragma unroll
-      for (int c = 0; c < D * D; c++) J[c] = a.JG[((size_t)e * D * D + c) * NS + s];
+      for (int k = 0; k < NF; k++) {
+        T acc = (T)0;
 #pragma unroll
-      for (int d1 = 0; d1 < D; d1++)
+        for (int m = 0; m < P1; m++) acc += l1[m] * sU[slot][k][base + m * stride];
+        uo[k] = acc;
+      }
 #pragma unroll
-        for (int k = 0; k < NF; k++) {
-          T acc = (T)0;
+      for (int c = 0; c < D; c++) n[c] = a.fgeo[((size_t)e * (D + 1) + c) * NFP + q];
+      const int fi = a.finfo[(size_t)e * NFACE + q / NFPF];
+      const bool bnd = BND && fi >= FINFO_BOUNDARY;
+      if (bnd) {
+        boundaryState<D, NF, T>(a.ph, fi - FINFO_BOUNDARY, uo, n, un);
+      } else {
+        const T *Ur = a.UfIn + (size_t)a.fnbr[(size_t)e * NFACE + q / NFPF] * NF * NFP + neighbourFpt<D, P>(q, fi);
 #pragma unroll
-          for (int d2 = 0; d2 < D; d2++) acc += J[d1 * D + d2] * F[d2][k];
-          sF[slot][d1][k][s] = acc;
-        }
+        for (int k = 0; k < NF; k++) un[k] = Ur[k * NFP];
+      }
+      ldgJump<D, NF, T>(a.ph, bnd, !bnd && ((fi >> 5) & 1), uo, un, n, dUc);
+#pragma unroll
+      for (int k = 0; k < NF; k++) sC[slot][k][q] = dUc[k];
     }
   }
   __syncthreads();
-
-  // (2) face points: own trace, discontinuous normal flux, common flux, jump
+  if (hasS) {
+    int si, sj, sk;
+    if (D == 3) {
+      sk = t / (P1 * P1);
+      sj = (t - P1 * P1 * sk) / P1;
+      si = t - P1 * sj - P1 * P1 * sk;
+    } else {
+      si = t % P1;
+      sj = t / P1;
+      sk = 0;
+    }
+    T g[D][NF];
+    correctedGradient<D, P, NF, T>(s1, &sU[slot][0][0], &sC[slot][0][0], t, si, sj, sk, pJ, pdet, g);
+#pragma unroll
+    for (int d = 0; d < D; d++)
+#pragma unroll
+      for (int k = 0; k < NF; k++) sG[slot][d][k][t] = g[d][k];
+  }
+  __syncthreads();
   if (act) {
+    T *Ge = a.dUf + (size_t)e * D * NF * NFP;
+#pragma unroll 1
     for (int q = t; q < NFP; q += TPE) {
       int base, stride, side, dn;
       fptLine<D, P>(q, base, stride, side, dn);
       const T *l1 = s1 + (side ? O::LR : O::LL);
-      const T sg = side ? (T)1 : (T)-1;
-      T uo[NF], dis[NF];
+#pragma unroll
+      for (int d = 0; d < D; d++)
+#pragma unroll
+        for (int k = 0; k < NF; k++) {
+          T acc = (T)0;
+#pragma unroll
+          for (int m = 0; m < P1; m++) acc += l1[m] * sG[slot][d][k][base + m * stride];
+          Ge[(d * NF + k) * NFP + q] = acc;
+        }
+    }
+  }
+}
+
+// BND = false is the variant for meshes without boundary faces (no ghost
+// states, which keeps its register allocation small).  VIS adds the viscous
+// terms (LDG): the corrected gradient at the solution points is recomputed
+// here from the state and the neighbours' traces exactly as k_grad forms it,
+// and the viscous common flux reads the gradient traces k_grad wrote.
+template <int D, int P, int NF, typename T, bool BND, bool VIS>
+__global__ __launch_bounds__((ElemGeo<D, P>::NTH)) void k_stage(StageArgs<T> a) {
+  using G = ElemGeo<D, P>;
+  using O = OneD<P>;
+  constexpr int NS = G::NS, NFP = G::NFP, NFPF = G::NFPF, P1 = G::P1, TPE = G::TPE, EPB = G::EPB;
+  constexpr int NFACE = G::NFACE;
+  static_assert(TPE >= NS && TPE >= 2 * NFPF, "one solution point per thread");
+  __shared__ T sU[EPB][NF][NS];   // stage state, later the updated state
+  __shared__ T sFd[EPB][NF][NS];  // contravariant flux F~_d of one reference axis
+  __shared__ T sJ[EPB][NF][NFP];  // normal-flux jump Fn*dA - disFn at the flux points
+  __shared__ T s1[O::SIZE];
+  // Workgroups are dealt to the 8 XCDs in turn; remap so that each XCD walks a
+  // contiguous range of elements and face neighbours share its L2.  The host
+  // pads the grid to a multiple of 8.
+  const int b = (blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+  const int slot = threadIdx.x / TPE, t = threadIdx.x - slot * TPE;
+  const int e = b * EPB + slot;
+  const bool act = e < a.nEles;
+  const bool hasS = act && t < NS;
+  for (int i = threadIdx.x; i < O::SIZE; i += blockDim.x) s1[i] = a.oned[i];
+  if (act) {
+    const T *Ue = a.Uin + (size_t)e * NF * NS;
+    for (int i = t; i < NF * NS; i += TPE) (&sU[slot][0][0])[i] = Ue[i];
+  }
+  // Prefetch: every other global read of the element is issued here, before
+  // the first barrier, so the loads stay in flight through the LDS phases
+  // (barriers wait on LDS traffic only).  NQ flux points per thread.
+  constexpr int NQ = (NFP + TPE - 1) / TPE;
+  T pJ[D * D];  // adjugate metrics of this solution point
+  if (hasS) {
+#pragma unroll
+    for (int c = 0; c < D * D; c++) pJ[c] = a.JG[((size_t)e * D * D + c) * NS + t];
+  }
+  T pn[NQ][D + 1], pu[NQ][NF];  // face normal + dA, neighbour trace
+  int pfi[NQ];
+  if (act) {
+#pragma unroll
+    for (int r = 0; r < NQ; r++) {
+      const int q = t + r * TPE;
+      if (q < NFP) {
+        const int fi = a.finfo[(size_t)e * NFACE + q / NFPF];
+        pfi[r] = fi;
+#pragma unroll
+        for (int c = 0; c <= D; c++) pn[r][c] = a.fgeo[((size_t)e * (D + 1) + c) * NFP + q];
+        if (!(BND && fi >= FINFO_BOUNDARY)) {
+          const T *Ur =
+              a.UfIn + (size_t)a.fnbr[(size_t)e * NFACE + q / NFPF] * NF * NFP + neighbourFpt<D, P>(q, fi);
+#pragma unroll
+          for (int k = 0; k < NF; k++) pu[r][k] = Ur[k * NFP];
+        }
+      }
+    }
+  }
+  double pb[NF], pdet = 1.0;  // stage base state and detJ for the update
+  if (hasS && (a.mode != 0 || VIS)) pdet = a.detJ[(size_t)e * NS + t];
+  if (hasS && a.mode != 0) {
+#pragma unroll
+    for (int k = 0; k < NF; k++) pb[k] = a.Ubase[((size_t)e * NF + k) * NS + t];
+  }
+  __syncthreads();
+
+  int si, sj, sk;  // tensor index of this thread's solution point
+  if (D == 3) {
+    sk = t / (P1 * P1);
+    sj = (t - P1 * P1 * sk) / P1;
+    si = t - P1 * sj - P1 * P1 * sk;
+  } else {
+    si = t % P1;
+    sj = t / P1;
+    sk = 0;
+  }
+  // (0) viscous: LDG jumps at the flux points (held in sJ until the first
+  // barrier of (2), which every read of them precedes)
+  if constexpr (VIS) {
+#pragma unroll
+    for (int r = 0; r < NQ; r++) {
+      const int q = t + r * TPE;
+      if (act && q < NFP) {
+        int base, stride, side, dn;
+        fptLine<D, P>(q, base, stride, side, dn);
+        const T *l1 = s1 + (side ? O::LR : O::LL);
+        T uo[NF], un[NF], dUc[NF];
+#pragma unroll
+        for (int k = 0; k < NF; k++) {
+          T acc = (T)0;
+#pragma unroll
+          for (int m = 0; m < P1; m++) acc += l1[m] * sU[slot][k][base + m * stride];
+          uo[k] = acc;
+        }
+        const int fi = pfi[r];
+        const bool bnd = BND && fi >= FINFO_BOUNDARY;
+        if (bnd) {
+          boundaryState<D, NF, T>(a.ph, fi - FINFO_BOUNDARY, uo, pn[r], un);
+        } else {
+#pragma unroll
+          for (int k = 0; k < NF; k++) un[k] = pu[r][k];
+        }
+        ldgJump<D, NF, T>(a.ph, bnd, !bnd && ((fi >> 5) & 1), uo, un, pn[r], dUc);
+#pragma unroll
+        for (int k = 0; k < NF; k++) sJ[slot][k][q] = dUc[k];
+      }
+    }
+    __syncthreads();
+  }
+  // (1) physical point flux (calcInviscidFlux_spts, solver.cpp:514-539; with
+  // calcViscousFlux_spts, solver.cpp:562-596, added for VIS)
+  T F[D][NF], div[NF];
+  if (hasS) {
+    T u[NF];
+#pragma unroll
+    for (int k = 0; k < NF; k++) u[k] = sU[slot][k][t];
+    inviscidFlux<D, NF, T>(a.ph, u, F);
+    if constexpr (VIS) {
+      T g[D][NF], Fv[D][NF];
+      correctedGradient<D, P, NF, T>(s1, &sU[slot][0][0], &sJ[slot][0][0], t, si, sj, sk, pJ, (T)pdet, g);
+      viscousFlux<D, NF, T>(a.ph, u, g, Fv);
+#pragma unroll
+      for (int d = 0; d < D; d++)
+#pragma unroll
+        for (int k = 0; k < NF; k++) F[d][k] = Fv[d][k] + F[d][k];
+    }
+  }
+  // (2) per reference axis d: F~_d = sum_d2 JGinv(d,d2) F_d2, its derivative
+  // along d (calcDivF_spts) and its extrapolation to the two faces normal to d
+  // (extrapolateNormalFlux), stored negated as the start of the jump
+#pragma unroll
+  for (int d = 0; d < D; d++) {
+    if (hasS) {
 #pragma unroll
       for (int k = 0; k < NF; k++) {
-        T acc = (T)0, acc2 = (T)0;
+        T acc = (T)0;
+#pragma unroll
+        for (int d2 = 0; d2 < D; d2++) acc += pJ[d * D + d2] * F[d2][k];
+        sFd[slot][k][t] = acc;
+      }
+    }
+    __syncthreads();
+    if (hasS) {
+      const int id = (d == 0) ? si : (d == 1) ? sj : sk;
+#pragma unroll
+      for (int k = 0; k < NF; k++) {
+        T acc = (T)0;
 #pragma unroll
         for (int m = 0; m < P1; m++) {
-          const int s = base + m * stride;
-          acc += l1[m] * sU[slot][k][s];
-          acc2 += (l1[m] * sg) * sF[slot][dn][k][s];
+          const int sp = (d == 0) ? m + P1 * (sj + P1 * sk) : (d == 1) ? si + P1 * (m + P1 * sk) : si + P1 * (sj + P1 * m);
+          acc += s1[O::DM + id * P1 + m] * sFd[slot][k][sp];
         }
+        div[k] = (d == 0) ? acc : acc + div[k];
+      }
+    }
+    if (act && t < 2 * NFPF) {
+      const int q = dirFace<D>(d, t / NFPF) * NFPF + t % NFPF;
+      int base, stride, side, dn;
+      fptLine<D, P>(q, base, stride, side, dn);
+      const T *l1 = s1 + (side ? O::LR : O::LL);
+      const T sg = side ? (T)1 : (T)-1;
+#pragma unroll
+      for (int k = 0; k < NF; k++) {
+        T acc = (T)0;
+#pragma unroll
+        for (int m = 0; m < P1; m++) acc += (l1[m] * sg) * sFd[slot][k][base + m * stride];
+        sJ[slot][k][q] = -acc;
+      }
+    }
+    __syncthreads();
+  }
+
+  // (3) common flux at every flux point (face::calcInviscidFlux,
+  // face.cpp:132-175; intFace.cpp:85-120; boundFace.cpp:33-277)
+#pragma unroll
+  for (int r = 0; r < NQ; r++) {
+    const int q = t + r * TPE;
+    if (act && q < NFP) {
+      int base, stride, side, dn;
+      fptLine<D, P>(q, base, stride, side, dn);
+      const T *l1 = s1 + (side ? O::LR : O::LL);
+      T uo[NF];
+#pragma unroll
+      for (int k = 0; k < NF; k++) {
+        T acc = (T)0;
+#pragma unroll
+        for (int m = 0; m < P1; m++) acc += l1[m] * sU[slot][k][base + m * stride];
         uo[k] = acc;
-        dis[k] = acc2;
       }
-      const int fk = a.fkind[(size_t)e * NFACE + q / NFPF];
-      T n[D];
-#pragma unroll
-      for (int c = 0; c < D; c++) n[c] = a.fgeo[((size_t)e * (D + 1) + c) * NFP + q];
-      const T dA = a.fgeo[((size_t)e * (D + 1) + D) * NFP + q];
+      const int fi = pfi[r];
+      const bool bnd = BND && fi >= FINFO_BOUNDARY;
+      const bool right = !bnd && ((fi >> 5) & 1);
+      const T *n = pn[r];
+      const T dA = pn[r][D];
       T un[NF], Fn[NF];
-      if (fk >= 2) {
-        boundaryState<D, NF, T>(a.ph, fk - 2, uo, n, un);
+      if (bnd) {
+        boundaryState<D, NF, T>(a.ph, fi - FINFO_BOUNDARY, uo, n, un);
       } else {
-        const int32_t g = a.nbr[(size_t)e * NFP + q];
 #pragma unroll
-        for (int k = 0; k < NF; k++) un[k] = a.Uf[(size_t)g + k * NFP];
+        for (int k = 0; k < NF; k++) un[k] = pu[r][k];
       }
-      if (fk == 1) commonFlux<D, NF, T>(a.ph, false, un, uo, n, Fn);
-      else commonFlux<D, NF, T>(a.ph, fk >= 2, uo, un, n, Fn);
-      const T scale = (fk == 1) ? -dA : dA;
+      if (right) commonFlux<D, NF, T>(a.ph, false, un, uo, n, Fn);
+      else commonFlux<D, NF, T>(a.ph, bnd, uo, un, n, Fn);
+      if constexpr (VIS) {
+        // face::calcViscousFlux (face.cpp:177-287) with the gradient traces;
+        // boundaries take boundFace::applyViscousBCs (boundFace.cpp:279-320),
+        // advection-diffusion boundaries a zero outside gradient
+        T go[D][NF], gn[D][NF];
+        const T *Go = a.dUf + (size_t)e * D * NF * NFP + q;
 #pragma unroll
-      for (int k = 0; k < NF; k++) sJ[slot][k][q] = Fn[k] * scale - dis[k];
+        for (int d = 0; d < D; d++)
+#pragma unroll
+          for (int k = 0; k < NF; k++) go[d][k] = Go[(d * NF + k) * NFP];
+        int vb = 0;
+        if (bnd) {
+          const int bc = fi - FINFO_BOUNDARY;
+          vb = (bc == 10) ? 2 : 1;
+          if constexpr (NF > 1) {
+            boundaryGradient<D, NF, T>(a.ph, bc, uo, un, go, gn);
+          } else {
+#pragma unroll
+            for (int d = 0; d < D; d++) gn[d][0] = (T)0;
+          }
+        } else {
+          const T *Gn = a.dUf + (size_t)a.fnbr[(size_t)e * NFACE + q / NFPF] * D * NF * NFP + neighbourFpt<D, P>(q, fi);
+#pragma unroll
+          for (int d = 0; d < D; d++)
+#pragma unroll
+            for (int k = 0; k < NF; k++) gn[d][k] = Gn[(d * NF + k) * NFP];
+        }
+        if (right) ldgViscousFlux<D, NF, T>(a.ph, 0, un, uo, gn, go, n, Fn);
+        else ldgViscousFlux<D, NF, T>(a.ph, vb, uo, un, go, gn, n, Fn);
+      }
+      const T scale = right ? -dA : dA;
+#pragma unroll
+      for (int k = 0; k < NF; k++) sJ[slot][k][q] = Fn[k] * scale + sJ[slot][k][q];
     }
   }
   __syncthreads();
 
-  // (3) divergence of F~ plus correction of the normal-flux jump
-  if (act) {
-    T *out = a.divF + (size_t)e * NF * NS;
-    for (int s = t; s < NS; s += TPE) {
-      int i, j, kk;
-      if (D == 3) {
-        kk = s / (P1 * P1);
-        j = (s - P1 * P1 * kk) / P1;
-        i = s - P1 * j - P1 * P1 * kk;
-      } else {
-        i = s % P1;
-        j = s / P1;
-        kk = 0;
-      }
-      int cq[NFACE], cside[NFACE], cax[NFACE];
-      corrEntries<D, P>(s, cq, cside, cax);
-      T cval[NFACE];
+  // (4) correction of the jump (correctDivFlux, solver.cpp:789-806) and the
+  // stage update; the updated state goes back to sU for its traces
+  if (hasS) {
+    int cq[NFACE], cside[NFACE], cax[NFACE];
+    corrEntries<D, P>(t, cq, cside, cax);
+    const size_t eo = (size_t)e * NF * NS;
 #pragma unroll
-      for (int f = 0; f < NFACE; f++) cval[f] = s1[(cside[f] ? O::GR : O::GL) + cax[f]];
+    for (int k = 0; k < NF; k++) {
+      T corr = (T)0;
+#pragma unroll
+      for (int f = 0; f < NFACE; f++) corr += s1[(cside[f] ? O::GR : O::GL) + cax[f]] * sJ[slot][k][cq[f]];
+      const T dv = corr + div[k];
+      const size_t gi = eo + (size_t)k * NS + t;
+      if (a.divF) a.divF[gi] = dv;
+      if (a.mode == 0) continue;
+      // stage terms RK * divF / detJ * dt, with dt / detJ formed once per point
+      const double dtJ = a.dt / pdet;
+      double u = pb[k];
+      if (a.mode == 2) {
+        if (a.nprev > 0) u -= a.c0 * (double)a.dprev0[gi] * dtJ;
+        if (a.nprev > 1) u -= a.c1 * (double)a.dprev1[gi] * dtJ;
+        if (a.nprev > 2) u -= a.c2 * (double)a.dprev2[gi] * dtJ;
+        const double cl = (a.nprev == 0) ? a.c0 : (a.nprev == 1) ? a.c1 : (a.nprev == 2) ? a.c2 : a.c3;
+        u -= cl * (double)dv * dtJ;
+      } else {
+        u = u - a.rk * (double)dv * dtJ;
+      }
+      if (a.mode == 1) {
+        a.Unext[gi] = (T)u;
+      } else {
+        a.Uout[gi] = u;
+        if (a.Uout32) a.Uout32[gi] = (float)u;
+      }
+      sU[slot][k][t] = (T)u;
+    }
+  }
+  if (a.mode == 0) return;
+  __syncthreads();
+
+  // (5) face traces of the updated state for the next stage
+  if (act) {
+    T *Fe = a.UfOut + (size_t)e * NF * NFP;
+#pragma unroll 1
+    for (int q = t; q < NFP; q += TPE) {
+      int base, stride, side, dn;
+      fptLine<D, P>(q, base, stride, side, dn);
+      const T *l1 = s1 + (side ? O::LR : O::LL);
 #pragma unroll
       for (int k = 0; k < NF; k++) {
-        T div, acc = (T)0;
+        T acc = (T)0;
 #pragma unroll
-        for (int m = 0; m < P1; m++) acc += s1[O::DM + i * P1 + m] * sF[slot][0][k][m + P1 * (j + P1 * kk)];
-        div = acc;
-        acc = (T)0;
-#pragma unroll
-        for (int m = 0; m < P1; m++) acc += s1[O::DM + j * P1 + m] * sF[slot][1][k][i + P1 * (m + P1 * kk)];
-        div = acc + div;
-        if constexpr (D == 3) {
-          acc = (T)0;
-#pragma unroll
-          for (int m = 0; m < P1; m++) acc += s1[O::DM + kk * P1 + m] * sF[slot][2][k][i + P1 * (j + P1 * m)];
-          div = acc + div;
-        }
-        T corr = (T)0;
-#pragma unroll
-        for (int f = 0; f < NFACE; f++) corr += cval[f] * sJ[slot][k][cq[f]];
-        out[k * NS + s] = corr + div;
+        for (int m = 0; m < P1; m++) acc += l1[m] * sU[slot][k][base + m * stride];
+        Fe[k * NFP + q] = acc;
       }
     }
   }
@@ -281,13 +666,13 @@ __global__ void k_metrics(const double *__restrict__ X, const double *__restrict
 // face.cpp:289-356, and scales by -dAR, intFace.cpp:116-120).
 template <int D>
 __global__ void k_fgeo(const double *__restrict__ ownN, const int32_t *__restrict__ pairEF,
-                       const int8_t *__restrict__ fkind, int NFP, int NFPF, int nEles, double *__restrict__ fgeo) {
+                       const int8_t *__restrict__ finfo, int NFP, int NFPF, int nEles, double *__restrict__ fgeo) {
   const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (long)nEles * NFP) return;
   const int e = (int)(idx / NFP), q = (int)(idx - (long)e * NFP);
-  const int fk = fkind[(size_t)e * 2 * D + q / NFPF];
+  const int fi = finfo[(size_t)e * 2 * D + q / NFPF];
   int se = e, sq = q;
-  if (fk == 1) {
+  if (fi < FINFO_BOUNDARY && ((fi >> 5) & 1)) {
     const int32_t g = pairEF[(size_t)e * NFP + q];
     se = g / NFP;
     sq = g - se * NFP;
@@ -319,28 +704,6 @@ __global__ void k_rk_a(double *__restrict__ U, double *__restrict__ U0, const TD
   const double v = u0 - rk * (double)divF[i] / detJ[e * NS + s] * dt;
   U[i] = v;
   if (U32) U32[i] = (float)v;
-}
-
-// Final combination: U = U0 (copyU0_Uspts, solver.cpp:394-399, skipped for a
-// one-stage scheme), then U -= RKb[s] * divF[s] / detJ * dt for s = 0..nRK-1
-// in order -- the same operations as nRK separate timeStepB passes.
-template <typename TD>
-__global__ void k_rk_b(double *__restrict__ U, const double *__restrict__ U0, const TD *__restrict__ d0,
-                       const TD *__restrict__ d1, const TD *__restrict__ d2, const TD *__restrict__ d3,
-                       const double *__restrict__ detJ, float *__restrict__ U32, double b0, double b1, double b2,
-                       double b3, double dt, int nRK, int NF, int NS, long n) {
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const long e = i / ((long)NF * NS);
-  const int s = (int)(i % NS);
-  const double J = detJ[e * NS + s];
-  double u = (nRK > 1) ? U0[i] : U[i];
-  u -= b0 * (double)d0[i] / J * dt;
-  if (nRK > 1) u -= b1 * (double)d1[i] / J * dt;
-  if (nRK > 2) u -= b2 * (double)d2[i] / J * dt;
-  if (nRK > 3) u -= b3 * (double)d3[i] / J * dt;
-  U[i] = u;
-  if (U32) U32[i] = (float)u;
 }
 
 // Single timeStepB pass (solver.cpp:347-385) for the ABI entry point.

@@ -84,6 +84,41 @@ __host__ __device__ __forceinline__ void corrEntries(int s, int *q, int *side, i
   }
 }
 
+// Per element-face record of the neighbour: an int32 neighbour element and an
+// int8 `finfo`: interior faces pack the neighbour's local face (bits 0-2), the
+// relative rotation (bits 3-4) and the side (bit 5: 0 = this element is the
+// face's left element, 1 = right); boundary faces are 64 + bc id.
+constexpr int FINFO_BOUNDARY = 64;
+
+__host__ __device__ __forceinline__ int packFinfo(int nbrFace, int rot, int side) {
+  return nbrFace | (rot << 3) | (side << 5);
+}
+
+// Neighbour-local flux point that pairs with flux point q of this element.
+// The right-element maps of intFace::setupRightState (intFace.cpp:29-65) are
+// involutions -- (a,b) -> (b,a), (P-a,b), (P-b,P-a), (a,P-b) -- so the same
+// map takes a left point to its right partner and back.
+template <int D, int P>
+__host__ __device__ __forceinline__ int neighbourFpt(int q, int finfo) {
+  constexpr int P1 = P + 1;
+  constexpr int NFPF = (D == 3) ? P1 * P1 : P1;
+  const int f = q / NFPF, i = q - f * NFPF;
+  const int nf = finfo & 7, rot = (finfo >> 3) & 3;
+  int j;
+  if (D == 3) {
+    const int a = i % P1, b = i / P1;
+    switch (rot) {
+      case 0: j = a * P1 + b; break;
+      case 1: j = (P - a) + b * P1; break;
+      case 2: j = NFPF - 1 - (a * P1 + b); break;
+      default: j = a + P1 * (P - b); break;
+    }
+  } else {
+    j = P - i;
+  }
+  return nf * NFPF + j;
+}
+
 // Layout of the per-order 1D factor table: l_L[P1], l_R[P1], g_L[P1], g_R[P1],
 // Dm[P1][P1] (Dm[i][m] = derivative of basis m at node i).
 template <int P>

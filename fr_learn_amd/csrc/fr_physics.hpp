@@ -25,8 +25,11 @@ __device__ __forceinline__ void inviscidFlux(const Phys &ph, const T *U, T F[D][
 #pragma unroll
     for (int d = 0; d < D; d++) F[d][0] = (T)ph.advect[d] * U[0];
   } else if constexpr (D == 2) {
+    // velocities use one reciprocal of rho instead of a division each (the
+    // reference divides; the difference is at most an ulp per velocity)
     const T g = (T)ph.gamma;
-    T rho = U[0], u = U[1] / rho, v = U[2] / rho, E = U[3];
+    const T rho = U[0], ri = (T)1 / rho;
+    T u = U[1] * ri, v = U[2] * ri, E = U[3];
     T p = (g - (T)1.0) * (E - (T)0.5 * rho * (u * u + v * v));
     F[0][0] = rho * u;         F[1][0] = rho * v;
     F[0][1] = rho * u * u + p; F[1][1] = rho * u * v;
@@ -34,7 +37,8 @@ __device__ __forceinline__ void inviscidFlux(const Phys &ph, const T *U, T F[D][
     F[0][3] = (E + p) * u;     F[1][3] = (E + p) * v;
   } else {
     const T g = (T)ph.gamma;
-    T rho = U[0], u = U[1] / rho, v = U[2] / rho, w = U[3] / rho, E = U[4];
+    const T rho = U[0], ri = (T)1 / rho;
+    T u = U[1] * ri, v = U[2] * ri, w = U[3] * ri, E = U[4];
     T p = (g - (T)1.0) * (E - (T)0.5 * rho * (u * u + v * v + w * w));
     F[0][0] = rho * u;         F[1][0] = rho * v;         F[2][0] = rho * w;
     F[0][1] = rho * u * u + p; F[1][1] = rho * u * v;     F[2][1] = rho * u * w;
@@ -44,43 +48,45 @@ __device__ __forceinline__ void inviscidFlux(const Phys &ph, const T *U, T F[D][
   }
 }
 
+// Normal Euler flux F(U).n of one side plus its normal velocity and squared
+// sound speed: each inviscidFlux term F[d][i] (flux.cpp:37-88) multiplied by
+// n[d] and summed over d in order, as face::rusanovFlux does, without
+// holding the D x NF flux table.  One reciprocal of rho replaces the
+// reference's divisions (an ulp-level difference).
+template <int D, typename T>
+__device__ __forceinline__ void normalFlux(const T g, const T *U, const T *n, T *Fn, T &vn, T &csq) {
+  constexpr int NF = D + 2;
+  const T rho = U[0], E = U[D + 1], ri = (T)1 / rho;
+  T v[D];
+#pragma unroll
+  for (int d = 0; d < D; d++) v[d] = U[d + 1] * ri;
+  T q = v[0] * v[0] + v[1] * v[1];
+  if (D == 3) q = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+  const T p = (g - (T)1.0) * (E - (T)0.5 * rho * q);
+#pragma unroll
+  for (int i = 0; i < NF; i++) Fn[i] = (T)0;
+  vn = (T)0;
+#pragma unroll
+  for (int d = 0; d < D; d++) {
+    vn += n[d] * v[d];
+    Fn[0] += n[d] * (rho * v[d]);
+#pragma unroll
+    for (int c = 0; c < D; c++) Fn[1 + c] += n[d] * ((c == d) ? rho * v[c] * v[d] + p : rho * v[c] * v[d]);
+    Fn[D + 1] += n[d] * ((E + p) * v[d]);
+  }
+  csq = fmax(g * p * ri, (T)0.0);
+}
+
 // Rusanov / local Lax-Friedrichs flux for Euler (face::rusanovFlux,
 // src/face.cpp:289-356, static mesh).
 template <int D, typename T>
-__device__ __forceinline__ void rusanov(const Phys &ph, const T *UL, const T *UR, const T *n, T *Fn) {
+__device__ __forceinline__ void rusanovN(const Phys &ph, const T *UL, const T *UR, const T *n, T *Fn) {
   constexpr int NF = D + 2;
-  T FL[D][NF], FR[D][NF];
-  inviscidFlux<D, NF, T>(ph, UL, FL);
-  inviscidFlux<D, NF, T>(ph, UR, FR);
   const T g = (T)ph.gamma;
-  T FnL[NF], FnR[NF];
-#pragma unroll
-  for (int i = 0; i < NF; i++) FnL[i] = FnR[i] = (T)0;
-  T rhoL = UL[0], rhoR = UR[0];
-  T uL = UL[1] / rhoL, uR = UR[1] / rhoR, vL = UL[2] / rhoL, vR = UR[2] / rhoR;
-  T pL, pR;
-  if constexpr (D == 2) {
-    pL = (g - (T)1.0) * (UL[3] - (T)0.5 * rhoL * (uL * uL + vL * vL));
-    pR = (g - (T)1.0) * (UR[3] - (T)0.5 * rhoR * (uR * uR + vR * vR));
-  } else {
-    T wL = UL[3] / rhoL, wR = UR[3] / rhoR;
-    pL = (g - (T)1.0) * (UL[4] - (T)0.5 * rhoL * (uL * uL + vL * vL + wL * wL));
-    pR = (g - (T)1.0) * (UR[4] - (T)0.5 * rhoR * (uR * uR + vR * vR + wR * wR));
-  }
-  T vnL = (T)0, vnR = (T)0;
-#pragma unroll
-  for (int d = 0; d < D; d++) {
-    vnL += n[d] * UL[d + 1] / rhoL;
-    vnR += n[d] * UR[d + 1] / rhoR;
-#pragma unroll
-    for (int i = 0; i < NF; i++) {
-      FnL[i] += n[d] * FL[d][i];
-      FnR[i] += n[d] * FR[d][i];
-    }
-  }
-  T csqL = fmax(g * pL / rhoL, (T)0.0), csqR = fmax(g * pR / rhoR, (T)0.0);
-  T eigL = fabs(vnL) + sqrt(csqL), eigR = fabs(vnR) + sqrt(csqR);
-  T eig = fmax(eigL, eigR);
+  T FnL[NF], FnR[NF], vnL, vnR, csqL, csqR;
+  normalFlux<D, T>(g, UL, n, FnL, vnL, csqL);
+  normalFlux<D, T>(g, UR, n, FnR, vnR, csqR);
+  const T eig = fmax(fabs(vnL) + sqrt(csqL), fabs(vnR) + sqrt(csqR));
 #pragma unroll
   for (int i = 0; i < NF; i++) Fn[i] = (T)0.5 * (FnL[i] + FnR[i] - eig * (UR[i] - UL[i]));
 }
@@ -248,6 +254,200 @@ __device__ __forceinline__ void boundaryState(const Phys &ph, int bc, const T *U
     for (int i = 0; i < D; i++) UR[i + 1] = rhoR * vR[i];
     UR[D + 1] = ER;
   }
+}
+
+// ---------------------------------------------------------------------------
+// viscous terms
+// ---------------------------------------------------------------------------
+// Navier-Stokes viscous flux Fv[d][k] from U and the physical gradient
+// g[d][k] (viscousFlux, reference src/flux.cpp:90-226).  viscFix = 0 keeps the
+// reference's index defects: tauxz = mu(du/dz + dv/dx), tauyz = mu(du/dz +
+// dv/dy) (flux.cpp:197-198) and Fv[0][3] = -tauzz (flux.cpp:216); viscFix = 1
+// uses the physical tauxz = mu(du/dz + dw/dx), tauyz = mu(dv/dz + dw/dy),
+// Fv[0][3] = -tauxz.  Advection-diffusion: Fv = -diffD grad u
+// (viscousFluxAD, flux.cpp:361-366).
+template <int D, int NF, typename T>
+__device__ __forceinline__ void viscousFlux(const Phys &ph, const T *U, const T g[D][NF], T Fv[D][NF]) {
+  if constexpr (NF == 1) {
+#pragma unroll
+    for (int d = 0; d < D; d++) Fv[d][0] = -(T)ph.diffD * g[d][0];
+  } else {
+    const T rho = U[0], u = U[1] / rho, v = U[2] / rho;
+    T e = U[D + 1] / rho - (T)0.5 * (u * u + v * v);
+    T w = (T)0;
+    if constexpr (D == 3) {
+      w = U[3] / rho;
+      e -= (T)0.5 * (w * w);
+    }
+    const T dRho_dx = g[0][0], dRhoU_dx = g[0][1], dRhoV_dx = g[0][2], dE_dx = g[0][D + 1];
+    const T dRho_dy = g[1][0], dRhoU_dy = g[1][1], dRhoV_dy = g[1][2], dE_dy = g[1][D + 1];
+    T dRho_dz = 0, dRhoU_dz = 0, dRhoV_dz = 0, dRhoW_dx = 0, dRhoW_dy = 0, dRhoW_dz = 0, dE_dz = 0;
+    if constexpr (D == 3) {
+      dRho_dz = g[2][0];
+      dRhoU_dz = g[2][1];
+      dRhoV_dz = g[2][2];
+      dRhoW_dx = g[0][3];
+      dRhoW_dy = g[1][3];
+      dRhoW_dz = g[2][3];
+      dE_dz = g[2][4];
+    }
+    T mu = (T)ph.mu_inf;
+    if (!ph.fixVis) {  // Sutherland's law
+      const T rt = ((T)ph.gamma - (T)1.0) * e / (T)ph.rt_inf;
+      mu *= pow(rt, (T)1.5) * ((T)1. + (T)ph.c_sth) / (rt + (T)ph.c_sth);
+    }
+    const T du_dx = (dRhoU_dx - dRho_dx * u) / rho, du_dy = (dRhoU_dy - dRho_dy * u) / rho;
+    const T dv_dx = (dRhoV_dx - dRho_dx * v) / rho, dv_dy = (dRhoV_dy - dRho_dy * v) / rho;
+    T du_dz = 0, dv_dz = 0, dw_dx = 0, dw_dy = 0, dw_dz = 0;
+    if constexpr (D == 3) {
+      du_dz = (dRhoU_dz - dRho_dz * u) / rho;
+      dv_dz = (dRhoV_dz - dRho_dz * v) / rho;
+      dw_dx = (dRhoW_dx - dRho_dx * w) / rho;
+      dw_dy = (dRhoW_dy - dRho_dy * w) / rho;
+      dw_dz = (dRhoW_dz - dRho_dz * w) / rho;
+    }
+    T dK_dx, dK_dy, dK_dz = 0;
+    if constexpr (D == 2) {
+      dK_dx = (T)0.5 * (u * u + v * v) * dRho_dx + rho * (u * du_dx + v * dv_dx);
+      dK_dy = (T)0.5 * (u * u + v * v) * dRho_dy + rho * (u * du_dy + v * dv_dy);
+    } else {
+      dK_dx = (T)0.5 * (u * u + v * v + w * w) * dRho_dx + rho * (u * du_dx + v * dv_dx + w * dw_dx);
+      dK_dy = (T)0.5 * (u * u + v * v + w * w) * dRho_dy + rho * (u * du_dy + v * dv_dy + w * dw_dy);
+      dK_dz = (T)0.5 * (u * u + v * v + w * w) * dRho_dz + rho * (u * du_dz + v * dv_dz + w * dw_dz);
+    }
+    const T de_dx = (dE_dx - dK_dx - dRho_dx * e) / rho;
+    const T de_dy = (dE_dy - dK_dy - dRho_dy * e) / rho;
+    T de_dz = 0;
+    if constexpr (D == 3) de_dz = (dE_dz - dK_dz - dRho_dz * e) / rho;
+    const T diag = (du_dx + dv_dy + dw_dz) / (T)3.0;
+    const T tauxx = (T)2.0 * mu * (du_dx - diag), tauyy = (T)2.0 * mu * (dv_dy - diag);
+    const T tauxy = mu * (du_dy + dv_dx);
+    T tauxz = 0, tauyz = 0, tauzz = 0;
+    if constexpr (D == 3) {
+      if (ph.viscFix) {
+        tauxz = mu * (du_dz + dw_dx);
+        tauyz = mu * (dv_dz + dw_dy);
+      } else {
+        tauxz = mu * (du_dz + dv_dx);
+        tauyz = mu * (du_dz + dv_dy);
+      }
+      tauzz = (T)2.0 * mu * (dw_dz - diag);
+    }
+    const T kq = (mu / (T)ph.prandtl) * (T)ph.gamma;
+    Fv[0][0] = (T)0.0;
+    Fv[0][1] = -tauxx;
+    Fv[0][2] = -tauxy;
+    Fv[0][D + 1] = -(u * tauxx + v * tauxy + w * tauxz + kq * de_dx);
+    Fv[1][0] = (T)0.0;
+    Fv[1][1] = -tauxy;
+    Fv[1][2] = -tauyy;
+    Fv[1][D + 1] = -(u * tauxy + v * tauyy + w * tauyz + kq * de_dy);
+    if constexpr (D == 3) {
+      Fv[0][3] = ph.viscFix ? -tauxz : -tauzz;
+      Fv[1][3] = -tauyz;
+      Fv[2][0] = (T)0.0;
+      Fv[2][1] = -tauxz;
+      Fv[2][2] = -tauyz;
+      Fv[2][3] = -tauzz;
+      Fv[2][4] = -(u * tauxz + v * tauyz + w * tauzz + kq * de_dz);
+    }
+  }
+}
+
+// Adiabatic no-slip wall gradient state (boundFace::applyViscousBCs,
+// boundFace.cpp:279-320): gR = gL, then the energy gradient is rebuilt with
+// zero temperature gradient.  Other boundaries keep gR = gL.
+template <int D, int NF, typename T>
+__device__ __forceinline__ void boundaryGradient(const Phys &ph, int bc, const T *UL, const T *UR, const T gL[D][NF],
+                                                 T gR[D][NF]) {
+#pragma unroll
+  for (int d = 0; d < D; d++)
+#pragma unroll
+    for (int k = 0; k < NF; k++) gR[d][k] = gL[d][k];
+  if constexpr (NF > 1) {
+    if (bc != 10) return;
+    T rhovSq = 0;
+    for (int d = 0; d < D; d++) rhovSq += UL[d + 1] * UL[d + 1];
+    const T pL = ((T)ph.gamma - (T)1.) * (UL[D + 1] - (T)0.5 * rhovSq / UL[0]);
+    const T e = pL / (((T)ph.gamma - (T)1.) * UR[0]);
+    T gv[D][D];
+    for (int d1 = 0; d1 < D; d1++)
+      for (int d2 = 0; d2 < D; d2++) gv[d1][d2] = (gR[d1][d2 + 1] - gR[d1][0] * UR[d2 + 1] / UR[0]) / UR[0];
+    T vSq = 0;
+    for (int d = 0; d < D; d++) vSq += UR[d + 1] * UR[d + 1] / (UR[0] * UR[0]);
+    for (int d1 = 0; d1 < D; d1++) {
+      gR[d1][D + 1] = (e + (T)0.5 * vSq) * gR[d1][0];
+      for (int d2 = 0; d2 < D; d2++) gR[d1][D + 1] += UR[d2 + 1] * gv[d2][d1];
+    }
+  }
+}
+
+// LDG penalty direction switch (face.cpp:587-597 / 206-215): -penFact when
+// n.(1, 1, sqrt 2) < 0 (n.(1,1) in 2D).
+template <int D, typename T>
+__device__ __forceinline__ T ldgPenalty(const Phys &ph, const T *n) {
+  T pen = (T)ph.penFact;
+  if (D == 2) {
+    if (n[0] + n[1] < 0) pen = -(T)ph.penFact;
+  } else if (n[0] + n[1] + (T)1.4142135623730951 * n[2] < 0) {
+    pen = -(T)ph.penFact;
+  }
+  return pen;
+}
+
+// Adds the common LDG viscous normal flux sum_d Fc[d] n[d] to the inviscid
+// common flux Fn (face::calcViscousFlux, face.cpp:177-287), in the left
+// element's frame.  bnd: 0 interior, 1 Dirichlet-type boundary, 2 adiabatic
+// wall.
+template <int D, int NF, typename T>
+__device__ __forceinline__ void ldgViscousFlux(const Phys &ph, int bnd, const T *UL, const T *UR, const T gL[D][NF],
+                                               const T gR[D][NF], const T *n, T *Fn) {
+  T Fc[D][NF];
+  if constexpr (NF == 1) {
+    // advection-diffusion: central flux of -diffD grad u (face.cpp:270-279)
+    T FL[D][NF], FR[D][NF];
+    viscousFlux<D, NF, T>(ph, UL, gL, FL);
+    viscousFlux<D, NF, T>(ph, UR, gR, FR);
+    T t = 0;
+#pragma unroll
+    for (int d = 0; d < D; d++) t += (T)0.5 * (FL[d][0] + FR[d][0]) * n[d];
+    Fn[0] += t;
+    return;
+  }
+  if (bnd) {
+    T Fb[D][NF];
+    if (bnd > 1) viscousFlux<D, NF, T>(ph, UR, gR, Fb);
+    else viscousFlux<D, NF, T>(ph, UL, gL, Fb);
+#pragma unroll
+    for (int d = 0; d < D; d++)
+#pragma unroll
+      for (int k = 0; k < NF; k++) Fc[d][k] = Fb[d][k] + (T)ph.tau * n[d] * (UL[k] - UR[k]);
+  } else {
+    T FL[D][NF], FR[D][NF];
+    viscousFlux<D, NF, T>(ph, UL, gL, FL);
+    viscousFlux<D, NF, T>(ph, UR, gR, FR);
+    const T pen = ldgPenalty<D, T>(ph, n);
+#pragma unroll
+    for (int k = 0; k < NF; k++) {
+      const T dF0 = FL[0][k] - FR[0][k], dF1 = FL[1][k] - FR[1][k];
+      const T dU = UL[k] - UR[k];
+      T nd;
+      if constexpr (D == 2) {
+        nd = n[0] * dF0 + n[1] * dF1;
+      } else {
+        // face.cpp:245 takes tempFL[1] here; viscFix = 1 takes tempFL[2]
+        const T dF2 = (ph.viscFix ? FL[2][k] : FL[1][k]) - FR[2][k];
+        nd = n[0] * dF0 + n[1] * dF1 + n[2] * dF2;
+      }
+#pragma unroll
+      for (int d = 0; d < D; d++)
+        Fc[d][k] = (T)0.5 * (FL[d][k] + FR[d][k]) + pen * n[d] * nd + (T)ph.tau * n[d] * dU;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NF; k++)
+#pragma unroll
+    for (int d = 0; d < D; d++) Fn[k] += Fc[d][k] * n[d];
 }
 
 }  // namespace frdev

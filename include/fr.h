@@ -146,7 +146,7 @@ int fr_synchronize(fr_ctx *ctx);
 /* Kernel timing: with timing on, every launch of the residual kernel is
  * bracketed by HIP events on the launch stream; fr_kernel_times returns the
  * accumulated milliseconds and launch count since the last reset for
- * kernel id 0 = residual, 1 = trace, 2 = RK update. */
+ * kernel id 0 = residual, 1 = trace, 2 = RK update, 3 = gradient (viscous). */
 int fr_set_timing(fr_ctx *ctx, int on);
 int fr_kernel_times(fr_ctx *ctx, int kernel, double *ms, long *count);
 

@@ -67,6 +67,8 @@ def load():
         lib.or_point_common_flux.argtypes = [P, ctypes.c_int, _DP, _DP, _DP, _DP]
         lib.or_point_boundary_state.argtypes = [P, ctypes.c_int, _DP, _DP, _DP]
         lib.or_point_viscous_flux.argtypes = [P, _DP, _DP, _DP]
+        lib.or_point_viscous_common.argtypes = [P, ctypes.c_int, _DP, _DP, _DP, _DP, _DP, _DP]
+        lib.or_point_viscous_bc_gradient.argtypes = [P, ctypes.c_int, _DP, _DP, _DP, _DP]
         lib.or_point_dvcjh.argtypes = [ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_double]
         lib.or_point_dvcjh.restype = ctypes.c_double
         lib.or_point_eta.argtypes = [ctypes.c_int, ctypes.c_int]
@@ -117,6 +119,27 @@ class Point:
         F = np.zeros((3, 5))
         self.lib.or_point_viscous_flux(ctypes.byref(self.p), _d(U), _d(g), _d(F))
         return F
+
+    def viscous_common(self, isBnd, UL, UR, gL, gR, n, Fn_inviscid=None):
+        """Inviscid normal flux Fn_inviscid (default 0) plus the common LDG
+        viscous normal flux; isBnd 0 interior, 1 boundary, 2 adiabatic wall."""
+        UL, UR = np.ascontiguousarray(UL, dtype=np.float64), np.ascontiguousarray(UR, dtype=np.float64)
+        n = np.ascontiguousarray(n, dtype=np.float64)
+        a, b = np.zeros((3, 5)), np.zeros((3, 5))
+        a[:gL.shape[0], :gL.shape[1]] = gL
+        b[:gR.shape[0], :gR.shape[1]] = gR
+        Fn = np.zeros(5) if Fn_inviscid is None else np.array(Fn_inviscid, dtype=np.float64).copy()
+        self.lib.or_point_viscous_common(ctypes.byref(self.p), int(isBnd), _d(UL), _d(UR), _d(a), _d(b), _d(n),
+                                         _d(Fn))
+        return Fn
+
+    def viscous_bc_gradient(self, bc, UL, UR, gL):
+        UL, UR = np.ascontiguousarray(UL, dtype=np.float64), np.ascontiguousarray(UR, dtype=np.float64)
+        a = np.zeros((3, 5))
+        a[:gL.shape[0], :gL.shape[1]] = gL
+        gR = np.zeros((3, 5))
+        self.lib.or_point_viscous_bc_gradient(ctypes.byref(self.p), int(bc), _d(UL), _d(UR), _d(a), _d(gR))
+        return gR
 
     def dvcjh(self, r, mode, order, eta):
         return self.lib.or_point_dvcjh(r, mode, order, eta)

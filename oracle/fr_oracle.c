@@ -1181,6 +1181,60 @@ static void inviscid_faces(or_solver *s) {
   }
 }
 
+/* Common LDG viscous flux added to the inviscid normal flux Fn, in the left
+ * element's frame (face::calcViscousFlux, face.cpp:177-287).  isBnd: 0
+ * interior, 1 boundary, 2 adiabatic wall (flux from the ghost state). */
+static void ldg_viscous_normal(const or_params *p, int isBnd, const double *UL, const double *UR,
+                               const double gL[3][5], const double gR[3][5], const double *n, double *Fn) {
+  int D = p->nDims, K = (p->equation == OR_NAVIER_STOKES) ? D + 2 : 1;
+  double FL[3][5] = {{0}}, FR[3][5] = {{0}};
+  if (p->equation == OR_NAVIER_STOKES) {
+    double Fc[3][5];
+    if (isBnd) {
+      if (isBnd > 1) {
+        viscous_flux(p, UR, gR, FR);
+        for (int d = 0; d < D; d++)
+          for (int k = 0; k < K; k++) Fc[d][k] = FR[d][k] + p->tau * n[d] * (UL[k] - UR[k]);
+      } else {
+        viscous_flux(p, UL, gL, FL);
+        for (int d = 0; d < D; d++)
+          for (int k = 0; k < K; k++) Fc[d][k] = FL[d][k] + p->tau * n[d] * (UL[k] - UR[k]);
+      }
+    } else {
+      viscous_flux(p, UL, gL, FL);
+      viscous_flux(p, UR, gR, FR);
+      double pen = p->penFact;
+      if (D == 2) { if (n[0] + n[1] < 0) pen = -p->penFact; }
+      else if (n[0] + n[1] + sqrt(2.) * n[2] < 0) pen = -p->penFact;
+      double nx = n[0], ny = n[1], nz = (D == 3) ? n[2] : 0;
+      for (int k = 0; k < K; k++) {
+        double dF0 = FL[0][k] - FR[0][k], dF1 = FL[1][k] - FR[1][k];
+        double dU = UL[k] - UR[k];
+        if (D == 2) {
+          Fc[0][k] = 0.5 * (FL[0][k] + FR[0][k]) + pen * nx * (nx * (dF0) + ny * (dF1)) + p->tau * nx * (dU);
+          Fc[1][k] = 0.5 * (FL[1][k] + FR[1][k]) + pen * ny * (nx * (dF0) + ny * (dF1)) + p->tau * ny * (dU);
+        } else {
+          /* face.cpp:245 uses tempFL[1] here; viscFix=1 uses tempFL[2] */
+          double dF2 = (p->viscFix ? FL[2][k] : FL[1][k]) - FR[2][k];
+          double nd = nx * (dF0) + ny * (dF1) + nz * (dF2);
+          Fc[0][k] = 0.5 * (FL[0][k] + FR[0][k]) + pen * nx * nd + p->tau * nx * (dU);
+          Fc[1][k] = 0.5 * (FL[1][k] + FR[1][k]) + pen * ny * nd + p->tau * ny * (dU);
+          Fc[2][k] = 0.5 * (FL[2][k] + FR[2][k]) + pen * nz * nd + p->tau * nz * (dU);
+        }
+      }
+    }
+    for (int d = 0; d < D; d++)
+      for (int k = 0; k < K; k++) Fn[k] += Fc[d][k] * n[d];
+  } else {
+    /* advection-diffusion: centralFlux of viscousFluxAD (face.cpp:270-279, flux.cpp:376-386) */
+    viscous_flux_ad(p, gL, FL);
+    viscous_flux_ad(p, gR, FR);
+    double t = 0;
+    for (int d = 0; d < D; d++) t += 0.5 * (FL[d][0] + FR[d][0]) * n[d];
+    Fn[0] += t;
+  }
+}
+
 /* face::calcViscousFlux, face.cpp:177-287 (+ intFace/boundFace gradient hooks) */
 static void viscous_faces(or_solver *s) {
   const or_params *p = &s->p;
@@ -1192,7 +1246,7 @@ static void viscous_faces(or_solver *s) {
     for (int i = 0; i < nfp; i++) {
       int fptL = s->flocL[f] * nfp + i;
       int fptR = isBnd ? -1 : s->fptR[(size_t)f * nfp + i];
-      double UL[5], UR[5], n[3] = {0, 0, 0}, gL[3][5] = {{0}}, gR[3][5] = {{0}}, FL[3][5] = {{0}}, FR[3][5] = {{0}};
+      double UL[5], UR[5], n[3] = {0, 0, 0}, gL[3][5] = {{0}}, gR[3][5] = {{0}};
       for (int k = 0; k < K; k++) UL[k] = UF(s, fptL, eL, k);
       for (int d = 0; d < D; d++) n[d] = s->norm_fpts[((size_t)fptL * nE + eL) * D + d];
       double dAL = s->dA_fpts[(size_t)fptL * nE + eL];
@@ -1207,51 +1261,7 @@ static void viscous_faces(or_solver *s) {
           for (int k = 0; k < K; k++) gR[d][k] = s->dU_fpts[(((size_t)d * nF + fptR) * nE + eR) * K + k];
       }
       double *Fn = s->faceFn + ((size_t)f * nfp + i) * K;
-      if (p->equation == OR_NAVIER_STOKES) {
-        double Fc[3][5];
-        if (isBnd) {
-          if (isBnd > 1) {
-            viscous_flux(p, UR, (const double(*)[5])gR, FR);
-            for (int d = 0; d < D; d++)
-              for (int k = 0; k < K; k++) Fc[d][k] = FR[d][k] + p->tau * n[d] * (UL[k] - UR[k]);
-          } else {
-            viscous_flux(p, UL, (const double(*)[5])gL, FL);
-            for (int d = 0; d < D; d++)
-              for (int k = 0; k < K; k++) Fc[d][k] = FL[d][k] + p->tau * n[d] * (UL[k] - UR[k]);
-          }
-        } else {
-          viscous_flux(p, UL, (const double(*)[5])gL, FL);
-          viscous_flux(p, UR, (const double(*)[5])gR, FR);
-          double pen = p->penFact;
-          if (D == 2) { if (n[0] + n[1] < 0) pen = -p->penFact; }
-          else if (n[0] + n[1] + sqrt(2.) * n[2] < 0) pen = -p->penFact;
-          double nx = n[0], ny = n[1], nz = (D == 3) ? n[2] : 0;
-          for (int k = 0; k < K; k++) {
-            double dF0 = FL[0][k] - FR[0][k], dF1 = FL[1][k] - FR[1][k];
-            double dU = UL[k] - UR[k];
-            if (D == 2) {
-              Fc[0][k] = 0.5 * (FL[0][k] + FR[0][k]) + pen * nx * (nx * (dF0) + ny * (dF1)) + p->tau * nx * (dU);
-              Fc[1][k] = 0.5 * (FL[1][k] + FR[1][k]) + pen * ny * (nx * (dF0) + ny * (dF1)) + p->tau * ny * (dU);
-            } else {
-              /* face.cpp:245 uses tempFL[1] here; viscFix=1 uses tempFL[2] */
-              double dF2 = (p->viscFix ? FL[2][k] : FL[1][k]) - FR[2][k];
-              double nd = nx * (dF0) + ny * (dF1) + nz * (dF2);
-              Fc[0][k] = 0.5 * (FL[0][k] + FR[0][k]) + pen * nx * nd + p->tau * nx * (dU);
-              Fc[1][k] = 0.5 * (FL[1][k] + FR[1][k]) + pen * ny * nd + p->tau * ny * (dU);
-              Fc[2][k] = 0.5 * (FL[2][k] + FR[2][k]) + pen * nz * nd + p->tau * nz * (dU);
-            }
-          }
-        }
-        for (int d = 0; d < D; d++)
-          for (int k = 0; k < K; k++) Fn[k] += Fc[d][k] * n[d];
-      } else {
-        /* advection-diffusion: centralFlux of viscousFluxAD (face.cpp:270-279, flux.cpp:376-386) */
-        viscous_flux_ad(p, (const double(*)[5])gL, FL);
-        viscous_flux_ad(p, (const double(*)[5])gR, FR);
-        double t = 0;
-        for (int d = 0; d < D; d++) t += 0.5 * (FL[d][0] + FR[d][0]) * n[d];
-        Fn[0] += t;
-      }
+      ldg_viscous_normal(p, isBnd, UL, UR, (const double(*)[5])gL, (const double(*)[5])gR, n, Fn);
       for (int k = 0; k < K; k++) s->Fn_fpts[IDX3(fptL, eL, k, nE, K)] = Fn[k] * dAL;
       if (!isBnd) {
         double dAR = s->dA_fpts[(size_t)fptR * nE + eR];
@@ -1440,8 +1450,24 @@ void or_point_viscous_flux(const or_params *p, const double *U, const double *gr
                            double *F /* [3][5] */) {
   double g[3][5], t[3][5] = {{0}};
   memcpy(g, grad, sizeof g);
-  viscous_flux(p, U, (const double(*)[5])g, t);
+  if (p->equation == OR_NAVIER_STOKES) viscous_flux(p, U, (const double(*)[5])g, t);
+  else viscous_flux_ad(p, (const double(*)[5])g, t);
   memcpy(F, t, sizeof t);
+}
+void or_point_viscous_common(const or_params *p, int isBnd, const double *UL, const double *UR,
+                             const double *gL /* [3][5] */, const double *gR /* [3][5] */, const double *n,
+                             double *Fn) {
+  double a[3][5], b[3][5];
+  memcpy(a, gL, sizeof a);
+  memcpy(b, gR, sizeof b);
+  ldg_viscous_normal(p, isBnd, UL, UR, (const double(*)[5])a, (const double(*)[5])b, n, Fn);
+}
+void or_point_viscous_bc_gradient(const or_params *p, int bc, const double *UL, const double *UR,
+                                  const double *gL /* [3][5] */, double *gR /* [3][5] */) {
+  double a[3][5], b[3][5];
+  memcpy(a, gL, sizeof a);
+  apply_viscous_bc(p, bc, UL, UR, (const double(*)[5])a, b);
+  memcpy(gR, b, sizeof b);
 }
 double or_point_dvcjh(double r, int mode, int order, double eta) { return dvcjh_1d(r, mode, order, eta); }
 double or_point_eta(int scheme, int order) { return compute_eta(scheme, order); }

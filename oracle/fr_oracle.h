@@ -108,6 +108,10 @@ void or_point_common_flux(const or_params *p, int kind, const double *UL, const 
                           double *Fn); /* kind 0 Rusanov, 1 Roe, 2 Lax-Friedrichs */
 void or_point_boundary_state(const or_params *p, int bc, const double *UL, const double *n, double *UR);
 void or_point_viscous_flux(const or_params *p, const double *U, const double *grad, double *F);
+void or_point_viscous_common(const or_params *p, int isBnd, const double *UL, const double *UR, const double *gL,
+                             const double *gR, const double *n, double *Fn);
+void or_point_viscous_bc_gradient(const or_params *p, int bc, const double *UL, const double *UR, const double *gL,
+                                  double *gR);
 double or_point_dvcjh(double r, int mode, int order, double eta);
 double or_point_eta(int scheme, int order);
 

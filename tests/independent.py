@@ -277,21 +277,116 @@ def char_state_subsonic(UL, n, cfg):
 
 
 # ----------------------------------------------------------------------------
+# viscous terms (textbook Navier-Stokes / diffusion)
+# ----------------------------------------------------------------------------
+
+
+def _velocity_energy(U, D):
+    """v_j = m_j / rho and specific internal energy e = E / rho - |v|^2 / 2
+    (complex-safe, for complex-step differentiation)."""
+    rho = U[..., 0]
+    v = U[..., 1:D + 1] / rho[..., None]
+    e = U[..., D + 1] / rho - 0.5 * (v * v).sum(-1)
+    return v, e
+
+
+def viscous_flux(U, q, cfg, defects=False):
+    """Viscous flux with the sign convention F_total = F_inviscid + Fv, i.e.
+    Fv[d] = (0, -tau_d., -(tau_d. v + kappa de/dx_d)) for Navier-Stokes with
+    Stokes' hypothesis, kappa = mu gamma / Pr (heat flux in terms of e = cv T),
+    mu constant (fixVis) or Sutherland's law mu_inf (T/T_ref)^1.5 (1 + S) /
+    (T/T_ref + S) with T/T_ref = (gamma - 1) e / rt_inf; Fv = -diffD grad u
+    for advection-diffusion.  q[..., dim, field] is the physical gradient of
+    the conserved variables; the primitive gradients are taken by complex-step
+    differentiation (exact to rounding), not by the quotient rule.
+    defects=True reproduces three index substitutions of the reference's 3D
+    flux (tau_xz = mu(du/dz + dv/dx), tau_yz = mu(du/dz + dv/dy), and the
+    x-flux of z-momentum -tau_zz)."""
+    D = cfg["nDims"]
+    if cfg["equation"] == 0:
+        return -cfg["diffD"] * q
+    g = cfg["gamma"]
+    v, e = _velocity_energy(U, D)
+    h = 1e-30
+    dv = np.zeros(U.shape[:-1] + (D, D))  # [..., dim, comp]
+    de = np.zeros(U.shape[:-1] + (D,))
+    for d in range(D):
+        vc, ec = _velocity_energy(U + 1j * h * q[..., d, :], D)
+        dv[..., d, :] = vc.imag / h
+        de[..., d] = ec.imag / h
+    mu = cfg["mu_inf"] * np.ones(e.shape)
+    if not cfg["fixVis"]:
+        T = (g - 1.0) * e / cfg["rt_inf"]
+        mu = cfg["mu_inf"] * T ** 1.5 * (1.0 + cfg["c_sth"]) / (T + cfg["c_sth"])
+    div = np.trace(dv, axis1=-2, axis2=-1)
+    tau = mu[..., None, None] * (dv + np.swapaxes(dv, -1, -2))
+    for d in range(D):
+        tau[..., d, d] -= 2.0 / 3.0 * mu * div
+    if defects and D == 3:
+        txz = mu * (dv[..., 2, 0] + dv[..., 0, 1])
+        tyz = mu * (dv[..., 2, 0] + dv[..., 1, 1])
+        tau[..., 0, 2] = tau[..., 2, 0] = txz
+        tau[..., 1, 2] = tau[..., 2, 1] = tyz
+    kappa = mu * g / cfg["prandtl"]
+    F = np.zeros(U.shape[:-1] + (D, D + 2))
+    for d in range(D):
+        F[..., d, 1:D + 1] = -tau[..., d, :]
+        F[..., d, D + 1] = -((tau[..., d, :] * v).sum(-1) + kappa * de[..., d])
+    if defects and D == 3:
+        F[..., 0, 3] = -tau[..., 2, 2]
+    return F
+
+
+def ldg_switch(n):
+    """Direction switch of the LDG one-sided terms: +1 where n . (1, 1, sqrt 2)
+    >= 0 (n . (1, 1) in 2D), else -1."""
+    w = np.array([1.0, 1.0, np.sqrt(2.0)])[:n.shape[-1]]
+    return np.where((n * w).sum(-1) < 0, -1.0, 1.0)
+
+
+def adiabatic_gradient(q, UL, Ug, cfg):
+    """Ghost gradient at an adiabatic no-slip wall: the interior gradient with
+    the energy gradient replaced by that of E = rho e with zero temperature
+    gradient and the ghost's (zero) velocity: dE = e_g drho, e_g = p_L /
+    ((gamma - 1) rho_g)."""
+    D = cfg["nDims"]
+    g = cfg["gamma"]
+    _, _, _, pL = primitives(UL, D, g)
+    eg = pL / ((g - 1.0) * Ug[..., 0])
+    vg = Ug[..., 1:D + 1] / Ug[..., 0:1]
+    assert np.all(vg == 0.0)
+    qg = q.copy()
+    qg[..., :, D + 1] = eg[..., None] * q[..., :, 0]
+    return qg
+
+
+# ----------------------------------------------------------------------------
 # weak-form DG residual
 # ----------------------------------------------------------------------------
 
 
-def weak_form_residual(cfg, mesh, U, box_lo=None, box_hi=None, periodic=None, bc_of=None):
+def weak_form_residual(cfg, mesh, U, box_lo=None, box_hi=None, periodic=None, bc_of=None, defects=False):
     """Reference-space residual divF (layout [spt][ele][field]) of nodal DG in
     weak form with Gauss-Legendre collocation.  Interior face points are
     paired by physical position (modulo the box period in periodic
-    directions); boundary points get the ghost state of bc_of(position)."""
+    directions); boundary points get the ghost state of bc_of(position).
+
+    With cfg["viscous"], the local DG (LDG) method: the reference-space
+    gradient is lifted in strong form,
+
+        w_i q_r,i = w_i d_r u(xi_i) + oint l_i (u^ - u) nref_r dS_ref,
+
+    with u^ = {u} - s(n) beta [u] inside and (u + u_ghost) / 2 on boundaries,
+    mapped to physical space with J^-T at the solution point, and the common
+    viscous normal flux is {F}.n + s(n) beta n.[F] + tau [u] (Navier-Stokes)
+    or {F}.n (advection-diffusion), [a] = own minus neighbour, s(n) the
+    ldg_switch of the own outward normal."""
     D, P = cfg["nDims"], cfg["order"]
     g = cfg["gamma"]
     adv = cfg["equation"] == 0
+    viscous = bool(cfg.get("viscous", 0))
     NF = 1 if adv else D + 2
     x1, w1 = gauss_legendre(P)
-    P1 = P + 1
     sidx, xis = tensor_points(x1, D)
     NS = len(xis)
     ws = np.prod(w1[sidx], axis=1)
@@ -301,29 +396,24 @@ def weak_form_residual(cfg, mesh, U, box_lo=None, box_hi=None, periodic=None, bc
     _, J = multilinear(X, xis)
     det = np.linalg.det(J)
     adj = det[..., None, None] * np.linalg.inv(J)  # [e][s][ref][phys]
-    # physical flux at spts
-    if adv:
-        a = np.array([cfg["advectVx"], cfg["advectVy"], cfg["advectVz"]])[:D]
-        F = a[None, None, :, None] * Ue[:, :, None, :]
-    else:
-        F = euler_flux(Ue, D, g)
-    Ft = np.einsum("esrp,espk->esrk", adj, F)
-    # volume term: G_r[i, s] = w_s * d l_i / d xi_r (xi_s)
     Dm = derivative_matrix(x1)
-    vol = np.zeros((nE, NS, NF))
-    for r in range(D):
-        G = np.zeros((NS, NS))
+
+    def line_matrix(r):
+        """M[i, s] = d l_s / d xi_r (xi_i) on the tensor lines along r."""
+        M = np.zeros((NS, NS))
         for i in range(NS):
-            for s in range(NS):
-                if all(sidx[i, d] == sidx[s, d] for d in range(D) if d != r):
-                    G[i, s] = ws[s] * Dm[sidx[s, r], sidx[i, r]]
-        vol += np.einsum("is,esk->eik", G, Ft[:, :, r, :])
-    # faces
+            for t in range(NS):
+                if all(sidx[i, d] == sidx[t, d] for d in range(D) if d != r):
+                    M[i, t] = Dm[sidx[i, r], sidx[t, r]]
+        return M
+
+    Dr = [line_matrix(r) for r in range(D)]
+    # faces: extrapolation, quadrature, positions, normals, own traces
     faces = []
     for d in range(D):
         for side in (-1.0, 1.0):
             other = [c for c in range(D) if c != d]
-            fidx, fx = tensor_points(x1, D - 1) if D > 1 else (None, None)
+            fidx, fx = tensor_points(x1, D - 1)
             xi = np.zeros((fx.shape[0], D))
             xi[:, d] = side
             for c, oc in enumerate(other):
@@ -338,7 +428,7 @@ def weak_form_residual(cfg, mesh, U, box_lo=None, box_hi=None, periodic=None, bc
             dA = np.linalg.norm(N, axis=-1)
             nrm = N / dA[..., None]
             UL = np.einsum("fs,esk->efk", E, Ue)
-            faces.append(dict(E=E, wf=wf, x=xp, n=nrm, dA=dA, UL=UL))
+            faces.append(dict(axis=d, side=side, E=E, wf=wf, x=xp, n=nrm, dA=dA, UL=UL))
     # pair face points by position
     allx = np.concatenate([f["x"].reshape(-1, D) for f in faces])
     key = allx.copy()
@@ -365,28 +455,59 @@ def weak_form_residual(cfg, mesh, U, box_lo=None, box_hi=None, periodic=None, bc
             raise RuntimeError("face point matched more than twice")
         i = j + 1
     allU = np.concatenate([f["UL"].reshape(-1, NF) for f in faces])
-    face_term = np.zeros((nE, NS, NF))
+    # neighbour or ghost states
     off = 0
     for f in faces:
-        nfp = f["x"].shape[1]
-        cnt = nE * nfp
+        cnt = f["UL"].shape[0] * f["UL"].shape[1]
         UL = f["UL"].reshape(-1, NF)
         n = f["n"].reshape(-1, D)
         pr = partner[off:off + cnt]
         UR = np.empty_like(UL)
         has = pr >= 0
         UR[has] = allU[pr[has]]
+        bcs = np.full(cnt, -1)
         if (~has).any():
             if bc_of is None:
                 raise RuntimeError("unpaired face point without a boundary condition")
             xb = f["x"].reshape(-1, D)[~has]
-            bcs = np.array([bc_of(p) for p in xb])
-            for b in np.unique(bcs):
-                sel = np.nonzero(~has)[0][bcs == b]
+            bcs[~has] = np.array([bc_of(p) for p in xb])
+            for b in np.unique(bcs[~has]):
+                sel = np.nonzero(bcs == b)[0]
                 if b == 1:
                     UR[sel] = char_state_subsonic(UL[sel], n[sel], cfg)
                 else:
                     UR[sel] = boundary_state(int(b), UL[sel], n[sel], cfg)
+        f.update(pr=pr, has=has, bcs=bcs, UR=UR, off=off)
+        off += cnt
+    # physical flux at spts
+    if adv:
+        a = np.array([cfg["advectVx"], cfg["advectVy"], cfg["advectVz"]])[:D]
+        F = a[None, None, :, None] * Ue[:, :, None, :]
+    else:
+        F = euler_flux(Ue, D, g)
+    if viscous:
+        # LDG: common solution, lifted reference gradient, physical gradient
+        qref = np.stack([np.einsum("is,esk->eik", Dr[r], Ue) for r in range(D)], axis=2)  # [e][s][r][k]
+        for f in faces:
+            UL, UR, has = f["UL"].reshape(-1, NF), f["UR"], f["has"]
+            sw = ldg_switch(f["n"].reshape(-1, D))[:, None]
+            beta = cfg["penFact"]
+            UC = np.where(has[:, None], 0.5 * (UL + UR) - sw * beta * (UL - UR), 0.5 * (UL + UR))
+            dU = (UC - UL).reshape(nE, -1, NF)
+            qref[:, :, f["axis"], :] += f["side"] * np.einsum("f,fi,efk->eik", f["wf"], f["E"], dU) / ws[None, :, None]
+        q = np.einsum("esrp,esrk->espk", adj, qref) / det[:, :, None, None]  # [e][s][phys][k]
+        F = F + viscous_flux(Ue, q, cfg, defects)
+        allQ = np.concatenate([np.einsum("fs,esdk->efdk", f["E"], q).reshape(-1, D, NF) for f in faces])
+    Ft = np.einsum("esrp,espk->esrk", adj, F)
+    # volume term: sum_s w_s d l_i / d xi_r (xi_s) F~_r(xi_s)
+    vol = np.zeros((nE, NS, NF))
+    for r in range(D):
+        vol += np.einsum("si,s,esk->eik", Dr[r], ws, Ft[:, :, r, :])
+    face_term = np.zeros((nE, NS, NF))
+    for f in faces:
+        nfp = f["x"].shape[1]
+        UL, UR, has = f["UL"].reshape(-1, NF), f["UR"], f["has"]
+        n = f["n"].reshape(-1, D)
         if adv:
             a = np.array([cfg["advectVx"], cfg["advectVy"], cfg["advectVz"]])[:D]
             Fs = lax_friedrichs_adv(UL, UR, n, a, cfg["lambda_"])
@@ -394,8 +515,26 @@ def weak_form_residual(cfg, mesh, U, box_lo=None, box_hi=None, periodic=None, bc
             Fs = np.where(has[:, None], roe_2d(UL, UR, n, g), rusanov(UL, UR, n, D, g))
         else:
             Fs = rusanov(UL, UR, n, D, g)
+        if viscous:
+            qL = allQ[f["off"]:f["off"] + len(UL)]
+            qR = np.zeros_like(qL)
+            qR[has] = allQ[f["pr"][has]]
+            FvL = np.einsum("fdk,fd->fk", viscous_flux(UL, qL, cfg, defects), n)
+            if adv:
+                FvR = np.einsum("fdk,fd->fk", viscous_flux(UR, qR, cfg, defects), n)
+                Fs = Fs + 0.5 * (FvL + FvR)  # boundary: zero outside gradient
+            else:
+                FvR = np.einsum("fdk,fd->fk", viscous_flux(UR, qR, cfg, defects), n)
+                sw = ldg_switch(n)[:, None]
+                inner = 0.5 * (FvL + FvR) + sw * cfg["penFact"] * (FvL - FvR) + cfg["tau"] * (UL - UR)
+                bnd = FvL + cfg["tau"] * (UL - UR)
+                adia = f["bcs"] == 10
+                if adia.any():
+                    qg = adiabatic_gradient(qL[adia], UL[adia], UR[adia], cfg)
+                    Fg = np.einsum("fdk,fd->fk", viscous_flux(UR[adia], qg, cfg, defects), n[adia])
+                    bnd[adia] = Fg + cfg["tau"] * (UL[adia] - UR[adia])
+                Fs = Fs + np.where(has[:, None], inner, bnd)
         Fs = (Fs * f["dA"].reshape(-1)[:, None]).reshape(nE, nfp, NF)
         face_term += np.einsum("f,fi,efk->eik", f["wf"], f["E"], Fs)
-        off += cnt
     divF = (face_term - vol) / ws[None, :, None]
     return np.transpose(divF, (1, 0, 2))

@@ -324,6 +324,120 @@ def test_characteristic_boundary(D):
 
 
 # ---------------------------------------------------------------------------
+# viscous terms (flux.cpp:90-226, 361-366; face.cpp:177-287, 582-607;
+# boundFace.cpp:279-320)
+# ---------------------------------------------------------------------------
+def ns_cfg(D, P=3, **kw):
+    base = dict(viscous=1, mu_inf=0.05, prandtl=0.72, penFact=0.5, tau=1.0, fixVis=0, c_sth=0.4, rt_inf=0.3)
+    base.update(kw)
+    return euler_cfg(D, P, **base)
+
+
+def _rand_grads(D, n, seed):
+    return np.random.default_rng(seed).standard_normal((n, D, D + 2))
+
+
+@pytest.mark.parametrize("D", [2, 3])
+@pytest.mark.parametrize("fixVis", [0, 1])
+@pytest.mark.parametrize("viscFix", [0, 1])
+def test_viscous_flux_textbook(D, fixVis, viscFix):
+    """Stokes-hypothesis stress, Fourier heat flux in e, Sutherland viscosity,
+    primitive gradients by complex step; viscFix = 0 in 3D carries the three
+    index substitutions of flux.cpp:197-198, 216 (rel. 1e-12)."""
+    cfg = ns_cfg(D, fixVis=fixVis, viscFix=viscFix)
+    pt = Point(cfg)
+    U, q = _rand_states(D, 20, 21), _rand_grads(D, 20, 22)
+    want = ind.viscous_flux(U, q, cfg, defects=(viscFix == 0))
+    for i in range(20):
+        got = pt.viscous_flux(U[i], q[i])[:D, :D + 2]
+        assert np.allclose(got, want[i], rtol=1e-12, atol=1e-12 * np.abs(want[i]).max()), i
+    if D == 3 and viscFix == 0:
+        # the substitutions matter: the physical flux differs
+        assert np.abs(ind.viscous_flux(U, q, cfg) - want).max() > 1e-3
+
+
+def test_viscous_flux_advection_diffusion():
+    cfg = advection_cfg(3, 3, viscous=1, diffD=0.3)
+    pt = Point(cfg)
+    q = np.random.default_rng(23).standard_normal((3, 1))
+    assert np.allclose(pt.viscous_flux(np.ones(1), q)[:3, :1], -0.3 * q, rtol=1e-15, atol=0)
+
+
+@pytest.mark.parametrize("D", [2, 3])
+def test_ldg_common_viscous_flux(D):
+    """Interior: {F}.n + s(n) beta n.(FL - FR) + tau (UL - UR) (viscFix = 1);
+    antisymmetric under (L, R, n) -> (R, L, -n), so the face conserves.
+    viscFix = 0 in 3D differs by exactly beta s(n) n_z (FL_y - FL_z), the
+    tempFL[1] of face.cpp:245.  Boundaries: the interior (isBnd 1) or ghost
+    (adiabatic, isBnd 2) flux plus the tau jump (rel. 1e-12)."""
+    cfg = ns_cfg(D, viscFix=1, penFact=0.37, tau=0.8)
+    pt = Point(cfg)
+    K = D + 2
+    UL, UR, n = _rand_states(D, 20, 24), _rand_states(D, 20, 25), _rand_normals(D, 20, 26)
+    gL, gR = _rand_grads(D, 20, 27), _rand_grads(D, 20, 28)
+    FL = np.einsum("fdk,fd->fk", ind.viscous_flux(UL, gL, cfg), n)
+    FR = np.einsum("fdk,fd->fk", ind.viscous_flux(UR, gR, cfg), n)
+    sw = ind.ldg_switch(n)[:, None]
+    want = 0.5 * (FL + FR) + sw * 0.37 * (FL - FR) + 0.8 * (UL - UR)
+    for i in range(20):
+        got = pt.viscous_common(0, UL[i], UR[i], gL[i], gR[i], n[i])[:K]
+        assert np.allclose(got, want[i], rtol=1e-12, atol=1e-12 * np.abs(want[i]).max()), i
+        back = pt.viscous_common(0, UR[i], UL[i], gR[i], gL[i], -n[i])[:K]
+        assert np.allclose(back, -got, rtol=1e-12, atol=1e-12 * np.abs(got).max()), i
+        # the inviscid normal flux passed in is kept and the viscous part added
+        base = np.arange(1.0, K + 1)
+        assert np.allclose(pt.viscous_common(0, UL[i], UR[i], gL[i], gR[i], n[i], np.r_[base, [0] * (5 - K)])[:K],
+                           base + got, rtol=1e-13, atol=1e-12)
+        b1 = pt.viscous_common(1, UL[i], UR[i], gL[i], gR[i], n[i])[:K]
+        assert np.allclose(b1, FL[i] + 0.8 * (UL[i] - UR[i]), rtol=1e-12, atol=1e-12 * np.abs(b1).max())
+        b2 = pt.viscous_common(2, UL[i], UR[i], gL[i], gR[i], n[i])[:K]
+        assert np.allclose(b2, FR[i] + 0.8 * (UL[i] - UR[i]), rtol=1e-12, atol=1e-12 * np.abs(b2).max())
+    if D == 3:
+        c0 = ns_cfg(3, viscFix=0, penFact=0.37, tau=0.8)
+        p0 = Point(c0)
+        Fx = ind.viscous_flux(UL, gL, c0, defects=True)  # [f][dim][k]
+        for i in range(20):
+            got0 = p0.viscous_common(0, UL[i], UR[i], gL[i], gR[i], n[i])[:K]
+            FLd = np.einsum("dk,d->k", Fx[i], n[i])
+            FRd = np.einsum("dk,d->k", ind.viscous_flux(UR[i], gR[i], c0, defects=True), n[i])
+            plain = 0.5 * (FLd + FRd) + sw[i] * 0.37 * (FLd - FRd) + 0.8 * (UL[i] - UR[i])
+            extra = sw[i] * 0.37 * n[i][2] * (Fx[i][1] - Fx[i][2])
+            assert np.allclose(got0, plain + extra, rtol=1e-12, atol=1e-12 * np.abs(got0).max()), i
+
+
+@pytest.mark.parametrize("D", [2, 3])
+def test_ldg_advection_diffusion_common_flux(D):
+    cfg = advection_cfg(D, 3, viscous=1, diffD=0.2, penFact=0.5)
+    pt = Point(cfg)
+    rng = np.random.default_rng(29)
+    n = _rand_normals(D, 10, 30)
+    for i in range(10):
+        gL, gR = rng.standard_normal((D, 1)), rng.standard_normal((D, 1))
+        got = pt.viscous_common(0, np.ones(1), np.ones(1), gL, gR, n[i])[0]
+        assert abs(got - (-0.2 * 0.5 * (gL + gR)[:, 0] @ n[i])) < 1e-15
+
+
+@pytest.mark.parametrize("D", [2, 3])
+def test_adiabatic_wall_gradient(D):
+    """Zero temperature gradient with the wall's zero ghost velocity: dE = e_g
+    drho; every other boundary keeps the interior gradient (rel. 1e-13)."""
+    cfg = ns_cfg(D)
+    pt = Point(cfg)
+    UL, n, gL = _rand_states(D, 10, 31), _rand_normals(D, 10, 32), _rand_grads(D, 10, 33)
+    for i in range(10):
+        Ug = pt.boundary_state(10, UL[i], n[i])[:D + 2]
+        got = pt.viscous_bc_gradient(10, UL[i], Ug, gL[i])[:D, :D + 2]
+        want = ind.adiabatic_gradient(gL[i], UL[i], Ug, cfg)
+        assert np.allclose(got, want, rtol=1e-13, atol=1e-13 * np.abs(want).max())
+        # a temperature gradient of zero: de/dx = 0 from the ghost state
+        _, e = ind._velocity_energy(Ug + 1j * 1e-30 * got[0], D)
+        assert abs(e.imag / 1e-30) < 1e-12 * np.abs(got).max()
+        for bc in (2, 3, 8, 9):
+            Ur = pt.boundary_state(bc, UL[i], n[i])[:D + 2]
+            assert np.array_equal(pt.viscous_bc_gradient(bc, UL[i], Ur, gL[i])[:D, :D + 2], gL[i])
+
+
+# ---------------------------------------------------------------------------
 # assembled residual vs the weak form of nodal DG
 # ---------------------------------------------------------------------------
 WEAK_TOL = 1e-11
@@ -395,6 +509,80 @@ def test_residual_advection_vs_weak_form(D):
     else:
         m = quad_box(4, periodic=False, bc=QUAD_BCS, seed=5)
         lo, hi = [-5, -5], [5, 5]
+    o = Oracle(cfg, m)
+    U = o.initialize(lambda x, c: cases.ic_gaussian(x - 1.0, c))
+    o.calc_residual(0)
+    w = ind.weak_form_residual(cfg, m, U, bc_of=lambda p: 0)
+    assert rel_err(o.residual(0), w).max() < WEAK_TOL
+
+
+@pytest.mark.parametrize("P", [1, 2, 3, 4])
+def test_residual_hex_ns_periodic_vs_weak_form(P):
+    """LDG Navier-Stokes residual vs LDG in weak form (viscFix = 1, the
+    physical stress; the viscFix = 0 substitutions are pinned pointwise)."""
+    cfg = ns_cfg(3, P, viscFix=1)
+    m = hex_box(3, perturb=0.15, seed=2)
+    o = Oracle(cfg, m)
+    U = perturbed_state(o.initialize(cases.ic_tgv), seed=1)
+    o.set_state(U)
+    o.calc_residual(0)
+    w = ind.weak_form_residual(cfg, m, U, box_lo=[0] * 3, box_hi=[L2PI] * 3, periodic=[True] * 3)
+    assert rel_err(o.residual(0), w).max() < WEAK_TOL
+    # the viscous terms are O(1) of the residual here
+    e = ind.weak_form_residual(dict(cfg, viscous=0), m, U, box_lo=[0] * 3, box_hi=[L2PI] * 3, periodic=[True] * 3)
+    assert rel_err(e, w).max() > 1e-2
+
+
+@pytest.mark.parametrize("P", [1, 3, 5])
+@pytest.mark.parametrize("riemann", [0, 1])
+@pytest.mark.parametrize("fixVis", [0, 1])
+def test_residual_quad_ns_periodic_vs_weak_form(P, riemann, fixVis):
+    cfg = ns_cfg(2, P, riemannType=riemann, fixVis=fixVis, penFact=-0.2, tau=0.3)
+    m = quad_box(4, seed=3)
+    o = Oracle(cfg, m)
+    U = perturbed_state(o.initialize(lambda x, c: cases.ic_vortex(x, c)), seed=1)
+    o.set_state(U)
+    o.calc_residual(0)
+    w = ind.weak_form_residual(cfg, m, U, box_lo=[-5, -5], box_hi=[5, 5], periodic=[True, True])
+    assert rel_err(o.residual(0), w).max() < WEAK_TOL
+
+
+@pytest.mark.parametrize("bcs", [
+    {"xmin": "sup_in", "xmax": "sup_out", "ymin": "slip_wall", "ymax": "symmetry", "zmin": "sub_in",
+     "zmax": "sub_out"},
+    {"xmin": "isothermal_noslip", "xmax": "adiabatic_noslip", "ymin": "isothermal_noslip_moving",
+     "ymax": "char", "zmin": "adiabatic_noslip", "zmax": "sup_out"}])
+def test_residual_hex_ns_walls_vs_weak_form(bcs):
+    cfg = ns_cfg(3, 3, viscFix=1)
+    m = hex_box(3, periodic=False, bc=bcs, L=1.0, seed=4)
+    o = Oracle(cfg, m)
+    U = perturbed_state(o.initialize(lambda x, c: cases.ic_uniform(x, c, vel=[0.3, 0.1, -0.05], p=1 / 1.4)),
+                        seed=2, amp=2e-2)
+    o.set_state(U)
+    o.calc_residual(0)
+    w = ind.weak_form_residual(cfg, m, U, bc_of=_box_bc([0] * 3, [1.0] * 3, bcs))
+    assert rel_err(o.residual(0), w).max() < WEAK_TOL
+
+
+def test_residual_quad_ns_walls_vs_weak_form():
+    bcs = dict(QUAD_BCS, ymin="adiabatic_noslip", ymax="isothermal_noslip_moving")
+    cfg = ns_cfg(2, 4, riemannType=1)
+    m = quad_box(4, periodic=False, bc=bcs, seed=5)
+    o = Oracle(cfg, m)
+    U = perturbed_state(o.initialize(lambda x, c: cases.ic_vortex(x, c, beta=2.0, mach=0.4)), seed=2)
+    o.set_state(U)
+    o.calc_residual(0)
+    w = ind.weak_form_residual(cfg, m, U, bc_of=_box_bc([-5, -5], [5, 5], bcs))
+    assert rel_err(o.residual(0), w).max() < WEAK_TOL
+
+
+@pytest.mark.parametrize("D", [2, 3])
+def test_residual_advection_diffusion_vs_weak_form(D):
+    cfg = advection_cfg(D, 3, viscous=1, diffD=0.1, penFact=0.5)
+    if D == 3:
+        m = hex_box(3, periodic=False, bc="sup_out", L=2.0, seed=5)
+    else:
+        m = quad_box(4, periodic=False, bc=QUAD_BCS, seed=5)
     o = Oracle(cfg, m)
     U = o.initialize(lambda x, c: cases.ic_gaussian(x - 1.0, c))
     o.calc_residual(0)
