@@ -62,6 +62,7 @@ SIGNATURES = {
     "fr_get_operator": (ctypes.c_int, [_P, ctypes.c_int, _DP]),
     "fr_get_geometry": (ctypes.c_int, [_P, ctypes.c_int, _DP]),
     "fr_get_faces": (ctypes.c_int, [_P, _IP]),
+    "fr_get_element_order": (ctypes.c_int, [_P, _IP]),
     "fr_set_state": (ctypes.c_int, [_P, _DP, ctypes.c_int]),
     "fr_get_state": (ctypes.c_int, [_P, _DP, ctypes.c_int]),
     "fr_calc_residual": (ctypes.c_int, [_P, ctypes.c_int]),

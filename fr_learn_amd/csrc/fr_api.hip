@@ -69,7 +69,10 @@ struct fr_ctx {
   int32_t *fnbr = nullptr;
   int8_t *finfo = nullptr;
   size_t partCap = 0;
-  std::vector<double> X;  // host copy of element node coordinates [ele][node][3]
+  std::vector<double> X;  // host copy of element node coordinates [device ele][node][3]
+  std::vector<int> perm;  // device element -> mesh element (blocks of EPB face-connected elements)
+  std::vector<int> iperm; // mesh element -> device element
+  int epb = 1;
   // timing
   bool timing = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[4];
@@ -255,16 +258,22 @@ int rkA(fr_ctx *c, int step, double rk) {
 
 int updateOnce(fr_ctx *c) { return c->prec == FR_PREC_FP32 ? updateT<float>(c) : updateT<double>(c); }
 
-// reference layout [pt][ele][field] <-> device [ele][field][pt]
-void refToDev(const double *src, double *dst, int nE, int NF, int NP) {
+// reference layout [pt][ele][field] <-> device [device ele][field][pt]
+void refToDev(const fr_ctx *c, const double *src, double *dst, int NF, int NP) {
+  const int nE = c->nE;
   for (int p = 0; p < NP; p++)
-    for (int e = 0; e < nE; e++)
-      for (int k = 0; k < NF; k++) dst[((size_t)e * NF + k) * NP + p] = src[((size_t)p * nE + e) * NF + k];
+    for (int e = 0; e < nE; e++) {
+      const size_t de = (size_t)c->iperm[e];
+      for (int k = 0; k < NF; k++) dst[(de * NF + k) * NP + p] = src[((size_t)p * nE + e) * NF + k];
+    }
 }
-void devToRef(const double *src, double *dst, int nE, int NF, int NP) {
+void devToRef(const fr_ctx *c, const double *src, double *dst, int NF, int NP) {
+  const int nE = c->nE;
   for (int p = 0; p < NP; p++)
-    for (int e = 0; e < nE; e++)
-      for (int k = 0; k < NF; k++) dst[((size_t)p * nE + e) * NF + k] = src[((size_t)e * NF + k) * NP + p];
+    for (int e = 0; e < nE; e++) {
+      const size_t de = (size_t)c->iperm[e];
+      for (int k = 0; k < NF; k++) dst[((size_t)p * nE + e) * NF + k] = src[(de * NF + k) * NP + p];
+    }
 }
 
 int syncU32(fr_ctx *c) {
@@ -359,13 +368,30 @@ int createImpl(const fr_config *cfg, const fr_mesh *m, int precision, fr_ctx *c)
       return rc;
     HIP_TRY(hipMemset(c->U32, 0, nSt * 4));
   }
+  // device element order: blocks of EPB face-connected elements
+  dispatch(D, c->P, c->NF, [&](auto Dc, auto Pc, auto) {
+    c->epb = frdev::ElemGeo<decltype(Dc)::value, decltype(Pc)::value>::EPB;
+  });
+  {
+    std::vector<double> cen((size_t)nE * 3, 0.0);
+    for (int e = 0; e < nE; e++)
+      for (int n = 0; n < nN; n++)
+        for (int d = 0; d < 3; d++) cen[3 * (size_t)e + d] += m->xv[3 * (size_t)m->c2v[(size_t)e * nN + n] + d] / nN;
+    c->perm = frhost::blockOrder(c->conn, cen, nE, c->NFACE, c->epb);
+    c->iperm.assign(nE, -1);
+    for (int i = 0; i < nE; i++) c->iperm[c->perm[i]] = i;
+    for (int e = 0; e < nE; e++)
+      if (c->iperm[e] < 0) return fail(FR_EMESH, "element order is not a permutation");
+  }
   // geometry on device
   {
     std::vector<double> &X = c->X;
     X.assign((size_t)nE * nN * 3, 0.0);
-    for (int e = 0; e < nE; e++)
+    for (int de = 0; de < nE; de++) {
+      const size_t e = (size_t)c->perm[de];
       for (int n = 0; n < nN; n++)
-        for (int d = 0; d < 3; d++) X[((size_t)e * nN + n) * 3 + d] = m->xv[3 * (size_t)m->c2v[(size_t)e * nN + n] + d];
+        for (int d = 0; d < 3; d++) X[((size_t)de * nN + n) * 3 + d] = m->xv[3 * (size_t)m->c2v[e * nN + n] + d];
+    }
     double *dX, *dS, *dF, *dT;
     int *dNeg;
     if ((rc = dmalloc(&dX, X.size())) || (rc = dmalloc(&dS, c->ops.dshapeSpts.size())) ||
@@ -379,9 +405,27 @@ int createImpl(const fr_config *cfg, const fr_mesh *m, int precision, fr_ctx *c)
     HIP_TRY(hipMemcpy(dF, c->ops.dshapeFpts.data(), c->ops.dshapeFpts.size() * 8, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(dT, c->ops.tNorm.data(), c->ops.tNorm.size() * 8, hipMemcpyHostToDevice));
     HIP_TRY(hipMemset(dNeg, 0, sizeof(int)));
-    HIP_TRY(hipMemcpy(pairEF, c->conn.nbr.data(), (size_t)nE * NFP * 4, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(c->finfo, c->conn.finfo.data(), c->conn.finfo.size(), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(c->fnbr, c->conn.fnbr.data(), c->conn.fnbr.size() * 4, hipMemcpyHostToDevice));
+    {
+      // per-element tables in device order, neighbour ids renumbered
+      const int NFACE = c->NFACE;
+      std::vector<int32_t> pe((size_t)nE * NFP), fn((size_t)nE * NFACE);
+      std::vector<int8_t> fi((size_t)nE * NFACE);
+      for (int de = 0; de < nE; de++) {
+        const size_t e = (size_t)c->perm[de];
+        for (int q = 0; q < NFP; q++) {
+          const int32_t v = c->conn.nbr[e * NFP + q];
+          pe[(size_t)de * NFP + q] = v < 0 ? v : c->iperm[v / NFP] * NFP + v % NFP;
+        }
+        for (int f = 0; f < NFACE; f++) {
+          const int32_t v = c->conn.fnbr[e * NFACE + f];
+          fn[(size_t)de * NFACE + f] = v < 0 ? v : c->iperm[v];
+          fi[(size_t)de * NFACE + f] = c->conn.finfo[e * NFACE + f];
+        }
+      }
+      HIP_TRY(hipMemcpy(pairEF, pe.data(), pe.size() * 4, hipMemcpyHostToDevice));
+      HIP_TRY(hipMemcpy(c->finfo, fi.data(), fi.size(), hipMemcpyHostToDevice));
+      HIP_TRY(hipMemcpy(c->fnbr, fn.data(), fn.size() * 4, hipMemcpyHostToDevice));
+    }
     long np = (long)nE * (NS + NFP);
     if (D == 3)
       hipLaunchKernelGGL(frdev::k_metrics<3>, dim3(nblk(np)), dim3(256), 0, c->stream, dX, dS, dF, dT, NS, NFP, nN,
@@ -492,21 +536,21 @@ int fr_get_geometry(fr_ctx *c, int which, double *out) {
       for (int s = 0; s < NS; s++)
         for (int e = 0; e < nE; e++)
           for (int d2 = 0; d2 < D; d2++)
-            out[(((size_t)d1 * NS + s) * nE + e) * D + d2] = h[((size_t)e * D * D + d1 * D + d2) * NS + s];
+            out[(((size_t)d1 * NS + s) * nE + e) * D + d2] = h[((size_t)c->iperm[e] * D * D + d1 * D + d2) * NS + s];
   } else if (which == 1) {
     std::vector<double> h((size_t)nE * NS);
     HIP_TRY(hipMemcpy(h.data(), c->detJ, h.size() * 8, hipMemcpyDeviceToHost));
     for (int s = 0; s < NS; s++)
-      for (int e = 0; e < nE; e++) out[(size_t)s * nE + e] = h[(size_t)e * NS + s];
+      for (int e = 0; e < nE; e++) out[(size_t)s * nE + e] = h[(size_t)c->iperm[e] * NS + s];
   } else if (which == 2) {
     std::vector<double> h((size_t)nE * D * NFP);
     HIP_TRY(hipMemcpy(h.data(), c->ownN, h.size() * 8, hipMemcpyDeviceToHost));
-    devToRef(h.data(), out, nE, D, NFP);
+    devToRef(c, h.data(), out, D, NFP);
   } else if (which == 3) {
     std::vector<double> h((size_t)nE * (D + 1) * NFP);
     HIP_TRY(hipMemcpy(h.data(), c->fgeo64, h.size() * 8, hipMemcpyDeviceToHost));
     for (int q = 0; q < NFP; q++)
-      for (int e = 0; e < nE; e++) out[(size_t)q * nE + e] = h[((size_t)e * (D + 1) + D) * NFP + q];
+      for (int e = 0; e < nE; e++) out[(size_t)q * nE + e] = h[((size_t)c->iperm[e] * (D + 1) + D) * NFP + q];
   } else if (which == 4) {
     // pos_spts(spt, ele, dim) = sum_n shape(spt, n) * node(n, ele, dim)
     // (solver::setPosSptsFpts, solver.cpp:904-951)
@@ -515,12 +559,19 @@ int fr_get_geometry(fr_ctx *c, int which, double *out) {
       for (int e = 0; e < nE; e++)
         for (int d = 0; d < D; d++) {
           double acc = 0.0;
-          for (int n = 0; n < nN; n++) acc += c->ops.shapeSpts[(size_t)s * nN + n] * c->X[((size_t)e * nN + n) * 3 + d];
+          for (int n = 0; n < nN; n++)
+            acc += c->ops.shapeSpts[(size_t)s * nN + n] * c->X[((size_t)c->iperm[e] * nN + n) * 3 + d];
           out[((size_t)s * nE + e) * D + d] = acc;
         }
   } else {
     return fail(FR_EINVAL, "unknown geometry id");
   }
+  return 0;
+}
+
+int fr_get_element_order(fr_ctx *c, int *order) {
+  if (!c || !order) return fail(FR_EINVAL, "null argument");
+  std::memcpy(order, c->perm.data(), c->perm.size() * sizeof(int));
   return 0;
 }
 
@@ -539,7 +590,7 @@ int fr_set_state(fr_ctx *c, const double *U, int layout) {
   const size_t n = c->nState();
   if (layout == FR_LAYOUT_REF) {
     std::vector<double> h(n);
-    refToDev(U, h.data(), c->nE, c->NF, c->NS);
+    refToDev(c, U, h.data(), c->NF, c->NS);
     HIP_TRY(hipMemcpyAsync(c->U, h.data(), n * 8, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
   } else if (layout == FR_LAYOUT_DEVICE) {
@@ -561,7 +612,7 @@ int fr_get_state(fr_ctx *c, double *U, int layout) {
   if (layout == FR_LAYOUT_REF) {
     std::vector<double> h(n);
     HIP_TRY(hipMemcpy(h.data(), c->U, n * 8, hipMemcpyDeviceToHost));
-    devToRef(h.data(), U, c->nE, c->NF, c->NS);
+    devToRef(c, h.data(), U, c->NF, c->NS);
   } else {
     HIP_TRY(hipMemcpy(U, c->U, n * 8, hipMemcpyDeviceToHost));
   }
@@ -586,7 +637,7 @@ int fr_get_residual(fr_ctx *c, int step, double *out, int layout) {
   } else {
     HIP_TRY(hipMemcpy(h.data(), c->divF[step], n * 8, hipMemcpyDeviceToHost));
   }
-  if (layout == FR_LAYOUT_REF) devToRef(h.data(), out, c->nE, c->NF, c->NS);
+  if (layout == FR_LAYOUT_REF) devToRef(c, h.data(), out, c->NF, c->NS);
   else std::memcpy(out, h.data(), n * 8);
   return 0;
 }
@@ -594,6 +645,9 @@ int fr_get_residual(fr_ctx *c, int step, double *out, int layout) {
 int fr_get_traces(fr_ctx *c, double *out) {
   if (!c || !out) return fail(FR_EINVAL, "null argument");
   const size_t n = (size_t)c->nE * c->NF * c->NFP;
+  // the stage kernels exchange the traces of faces inside an element block
+  // through LDS and leave them out of the trace buffer: extrapolate afresh
+  c->traceOf = -1;
   int rc = (c->prec == FR_PREC_FP32) ? ensureTraces<float>(c, 0) : ensureTraces<double>(c, 0);
   if (rc) return rc;
   HIP_TRY(hipStreamSynchronize(c->stream));
@@ -605,7 +659,7 @@ int fr_get_traces(fr_ctx *c, double *out) {
   } else {
     HIP_TRY(hipMemcpy(h.data(), c->Uf[c->cur], n * 8, hipMemcpyDeviceToHost));
   }
-  devToRef(h.data(), out, c->nE, c->NF, c->NFP);
+  devToRef(c, h.data(), out, c->NF, c->NFP);
   return 0;
 }
 

@@ -548,4 +548,85 @@ std::string buildConnectivity(const fr_config &cfg, const fr_mesh &m, const Oper
   return verifyNeighbourMaps(o, c, nE);
 }
 
+// Morton (Z-order) key of a point quantised to 21 bits per axis.
+static uint64_t mortonKey(const double *x, const double *lo, const double *span) {
+  uint64_t key = 0;
+  uint32_t q[3];
+  for (int d = 0; d < 3; d++) {
+    double t = span[d] > 0 ? (x[d] - lo[d]) / span[d] : 0.0;
+    t = std::min(std::max(t, 0.0), 1.0);
+    q[d] = (uint32_t)(t * 2097151.0);
+  }
+  for (int bit = 20; bit >= 0; bit--)
+    for (int d = 2; d >= 0; d--) key = (key << 1) | ((q[d] >> bit) & 1u);
+  return key;
+}
+
+std::vector<int> blockOrder(const Connectivity &c, const std::vector<double> &centroid, int nE, int nFaceEle,
+                            int epb) {
+  std::vector<int> order;
+  order.reserve(nE);
+  if (epb <= 1) {
+    for (int e = 0; e < nE; e++) order.push_back(e);
+    return order;
+  }
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300}, span[3];
+  for (int e = 0; e < nE; e++)
+    for (int d = 0; d < 3; d++) {
+      lo[d] = std::min(lo[d], centroid[3 * (size_t)e + d]);
+      hi[d] = std::max(hi[d], centroid[3 * (size_t)e + d]);
+    }
+  for (int d = 0; d < 3; d++) span[d] = hi[d] - lo[d];
+  std::vector<std::pair<uint64_t, int>> keyed(nE);
+  for (int e = 0; e < nE; e++) keyed[e] = {mortonKey(&centroid[3 * (size_t)e], lo, span), e};
+  std::sort(keyed.begin(), keyed.end());
+  std::vector<int> rank(nE), byRank(nE), cluster(nE, -1);
+  for (int i = 0; i < nE; i++) {
+    rank[keyed[i].second] = i;
+    byRank[i] = keyed[i].second;
+  }
+  std::vector<int> members;
+  size_t next = 0;  // first rank that may still be unassigned
+  int id = 0;
+  for (int i = 0; i < nE; i++) {
+    const int seed = byRank[i];
+    if (cluster[seed] >= 0) continue;
+    members.assign(1, seed);
+    cluster[seed] = id;
+    // grow by the unassigned face neighbour with the most faces into the
+    // cluster (ties: lowest Z-order rank)
+    while ((int)members.size() < epb) {
+      int best = -1, bestCnt = 0;
+      for (int m : members)
+        for (int f = 0; f < nFaceEle; f++) {
+          const int n = c.fnbr[(size_t)m * nFaceEle + f];
+          if (n < 0 || cluster[n] >= 0) continue;
+          int cnt = 0;
+          for (int g = 0; g < nFaceEle; g++) {
+            const int o = c.fnbr[(size_t)n * nFaceEle + g];
+            if (o >= 0 && cluster[o] == id) cnt++;
+          }
+          if (cnt > bestCnt || (cnt == bestCnt && best >= 0 && rank[n] < rank[best])) {
+            best = n;
+            bestCnt = cnt;
+          }
+        }
+      if (best < 0) break;
+      cluster[best] = id;
+      members.push_back(best);
+    }
+    // a cluster that cannot grow is filled up in Z-order, so that every block
+    // of epb device elements is one cluster
+    while ((int)members.size() < epb) {
+      while (next < (size_t)nE && cluster[byRank[next]] >= 0) next++;
+      if (next == (size_t)nE) break;
+      cluster[byRank[next]] = id;
+      members.push_back(byRank[next]);
+    }
+    for (int m : members) order.push_back(m);
+    id++;
+  }
+  return order;
+}
+
 }  // namespace frhost

@@ -54,4 +54,11 @@ struct Connectivity {
 std::string buildConnectivity(const fr_config &cfg, const fr_mesh &mesh, const Operators &ops,
                               Connectivity &conn);
 
+// Device element order (device index -> mesh element): consecutive blocks of
+// `epb` elements are face-connected clusters, grown greedily from seeds taken
+// in Z-order of the element centroids ([nE][3]), so that the faces inside a
+// block are exchanged through LDS and the blocks follow a space-filling curve.
+std::vector<int> blockOrder(const Connectivity &conn, const std::vector<double> &centroid, int nE, int nFaceEle,
+                            int epb);
+
 }  // namespace frhost

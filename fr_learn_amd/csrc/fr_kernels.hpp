@@ -109,6 +109,21 @@ __device__ __forceinline__ int dirFace(int d, int which) {
   return (d == 0) ? (which ? 1 : 3) : (which ? 2 : 0);
 }
 
+// Synchronisation of the threads of one element between LDS phases: a
+// wavefront-level barrier with wavefront-scope fences when each element's
+// threads lie inside one wavefront (TPE <= 64), else the workgroup barrier.
+// Only the phases that read another element's LDS data need the workgroup.
+template <int TPE>
+__device__ __forceinline__ void elementSync() {
+  if constexpr (TPE <= 64) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
+  }
+}
+
 // Reference axis normal to face f.
 template <int D>
 __device__ __forceinline__ int faceAxis(int f) {
@@ -258,7 +273,7 @@ __global__ __launch_bounds__((ElemGeo<D, P>::NTH)) void k_grad(GradArgs<T> a) {
       for (int k = 0; k < NF; k++) sC[slot][k][q] = dUc[k];
     }
   }
-  __syncthreads();
+  elementSync<TPE>();
   if (hasS) {
     int si, sj, sk;
     if (D == 3) {
@@ -277,7 +292,7 @@ __global__ __launch_bounds__((ElemGeo<D, P>::NTH)) void k_grad(GradArgs<T> a) {
 #pragma unroll
       for (int k = 0; k < NF; k++) sG[slot][d][k][t] = g[d][k];
   }
-  __syncthreads();
+  elementSync<TPE>();
   if (act) {
     T *Ge = a.dUf + (size_t)e * D * NF * NFP;
 #pragma unroll 1
@@ -322,45 +337,71 @@ __global__ __launch_bounds__((ElemGeo<D, P>::NTH)) void k_stage(StageArgs<T> a) 
   const int e = b * EPB + slot;
   const bool act = e < a.nEles;
   const bool hasS = act && t < NS;
-  for (int i = threadIdx.x; i < O::SIZE; i += blockDim.x) s1[i] = a.oned[i];
-  if (act) {
-    const T *Ue = a.Uin + (size_t)e * NF * NS;
-    for (int i = t; i < NF * NS; i += TPE) (&sU[slot][0][0])[i] = Ue[i];
+  // Faces between two elements of this block take the neighbour's trace from
+  // its state in LDS, and their traces are not written to HBM (the element
+  // order groups face neighbours into blocks).  The viscous variant keeps the
+  // full trace exchange, which k_grad reads.
+  const int blockBase = b * EPB;
+  auto inBlock = [&](int en) { return !VIS && (unsigned)(en - blockBase) < (unsigned)EPB; };
+  // Loads.  Vector-memory counters retire in issue order, so the face records
+  // (which the neighbour-trace addresses depend on) go first, the state
+  // (needed in LDS before the barrier) next, then every other read of the
+  // element; only the records and the state are waited for before the
+  // barrier, the rest stays in flight through the LDS phases.  The loads are
+  // branch-free (idle lanes read clamped, in-bounds indices; lanes with no
+  // neighbour trace to fetch read the 1D table, which is cached), so that the
+  // compiler's wait counts are exact.
+  constexpr int NQ = (NFP + TPE - 1) / TPE;  // flux points per thread
+  constexpr int NU = (NF * NS + TPE - 1) / TPE;
+  static_assert(O::SIZE <= G::NTH, "one thread per 1D table entry");
+  const size_t ec = (size_t)(act ? e : a.nEles - 1);  // clamped element
+  const int tc = t < NS ? t : NS - 1;                 // clamped solution point
+  int pfi[NQ], pnb[NQ];  // face record, neighbour element (-1: boundary)
+#pragma unroll
+  for (int r = 0; r < NQ; r++) {
+    const int qc = min(t + r * TPE, NFP - 1);
+    pfi[r] = a.finfo[ec * NFACE + qc / NFPF];
+    pnb[r] = a.fnbr[ec * NFACE + qc / NFPF];
   }
-  // Prefetch: every other global read of the element is issued here, before
-  // the first barrier, so the loads stay in flight through the LDS phases
-  // (barriers wait on LDS traffic only).  NQ flux points per thread.
-  constexpr int NQ = (NFP + TPE - 1) / TPE;
+  const T o1 = a.oned[min((int)threadIdx.x, O::SIZE - 1)];
+  T ul[NU];
+  {
+    const T *Ue = a.Uin + ec * NF * NS;
+#pragma unroll
+    for (int j = 0; j < NU; j++) ul[j] = Ue[min(t + j * TPE, NF * NS - 1)];
+  }
   T pJ[D * D];  // adjugate metrics of this solution point
-  if (hasS) {
 #pragma unroll
-    for (int c = 0; c < D * D; c++) pJ[c] = a.JG[((size_t)e * D * D + c) * NS + t];
-  }
+  for (int c = 0; c < D * D; c++) pJ[c] = a.JG[(ec * D * D + c) * NS + tc];
   T pn[NQ][D + 1], pu[NQ][NF];  // face normal + dA, neighbour trace
-  int pfi[NQ];
+#pragma unroll
+  for (int r = 0; r < NQ; r++) {
+    const int qc = min(t + r * TPE, NFP - 1);
+#pragma unroll
+    for (int c = 0; c <= D; c++) pn[r][c] = a.fgeo[(ec * (D + 1) + c) * NFP + qc];
+  }
+  // stage base state and detJ for the update (read in every mode: in mode 0
+  // the base is the input state, whose lines are already in cache)
+  double pb[NF];
+  const double pdet = a.detJ[ec * NS + tc];
+#pragma unroll
+  for (int k = 0; k < NF; k++) pb[k] = a.Ubase[(ec * NF + k) * NS + tc];
+#pragma unroll
+  for (int r = 0; r < NQ; r++) {
+    const int qc = min(t + r * TPE, NFP - 1);
+    const bool bndFace = BND && pfi[r] >= FINFO_BOUNDARY;
+    const bool fetch = !bndFace && !inBlock(pnb[r]);
+    const T *Ur = fetch ? a.UfIn + (size_t)pnb[r] * NF * NFP + neighbourFpt<D, P>(qc, pfi[r]) : a.oned;
+    const int st = fetch ? NFP : 0;
+#pragma unroll
+    for (int k = 0; k < NF; k++) pu[r][k] = Ur[k * st];
+    if (bndFace) pnb[r] = -1;
+  }
+  if (threadIdx.x < O::SIZE) s1[threadIdx.x] = o1;
   if (act) {
 #pragma unroll
-    for (int r = 0; r < NQ; r++) {
-      const int q = t + r * TPE;
-      if (q < NFP) {
-        const int fi = a.finfo[(size_t)e * NFACE + q / NFPF];
-        pfi[r] = fi;
-#pragma unroll
-        for (int c = 0; c <= D; c++) pn[r][c] = a.fgeo[((size_t)e * (D + 1) + c) * NFP + q];
-        if (!(BND && fi >= FINFO_BOUNDARY)) {
-          const T *Ur =
-              a.UfIn + (size_t)a.fnbr[(size_t)e * NFACE + q / NFPF] * NF * NFP + neighbourFpt<D, P>(q, fi);
-#pragma unroll
-          for (int k = 0; k < NF; k++) pu[r][k] = Ur[k * NFP];
-        }
-      }
-    }
-  }
-  double pb[NF], pdet = 1.0;  // stage base state and detJ for the update
-  if (hasS && (a.mode != 0 || VIS)) pdet = a.detJ[(size_t)e * NS + t];
-  if (hasS && a.mode != 0) {
-#pragma unroll
-    for (int k = 0; k < NF; k++) pb[k] = a.Ubase[((size_t)e * NF + k) * NS + t];
+    for (int j = 0; j < NU; j++)
+      if (t + j * TPE < NF * NS) (&sU[slot][0][0])[t + j * TPE] = ul[j];
   }
   __syncthreads();
 
@@ -405,7 +446,7 @@ __global__ __launch_bounds__((ElemGeo<D, P>::NTH)) void k_stage(StageArgs<T> a) 
         for (int k = 0; k < NF; k++) sJ[slot][k][q] = dUc[k];
       }
     }
-    __syncthreads();
+    elementSync<TPE>();
   }
   // (1) physical point flux (calcInviscidFlux_spts, solver.cpp:514-539; with
   // calcViscousFlux_spts, solver.cpp:562-596, added for VIS)
@@ -439,7 +480,7 @@ __global__ __launch_bounds__((ElemGeo<D, P>::NTH)) void k_stage(StageArgs<T> a) 
         sFd[slot][k][t] = acc;
       }
     }
-    __syncthreads();
+    elementSync<TPE>();
     if (hasS) {
       const int id = (d == 0) ? si : (d == 1) ? sj : sk;
 #pragma unroll
@@ -467,7 +508,7 @@ __global__ __launch_bounds__((ElemGeo<D, P>::NTH)) void k_stage(StageArgs<T> a) 
         sJ[slot][k][q] = -acc;
       }
     }
-    __syncthreads();
+    elementSync<TPE>();
   }
 
   // (3) common flux at every flux point (face::calcInviscidFlux,
@@ -495,6 +536,19 @@ __global__ __launch_bounds__((ElemGeo<D, P>::NTH)) void k_stage(StageArgs<T> a) 
       T un[NF], Fn[NF];
       if (bnd) {
         boundaryState<D, NF, T>(a.ph, fi - FINFO_BOUNDARY, uo, n, un);
+      } else if (inBlock(pnb[r])) {
+        // the neighbour's trace from its state in LDS, as it would extrapolate it
+        int nbase, nstride, nside, ndn;
+        fptLine<D, P>(neighbourFpt<D, P>(q, fi), nbase, nstride, nside, ndn);
+        const T *ln = s1 + (nside ? O::LR : O::LL);
+        const T *Un = &sU[pnb[r] - blockBase][0][0];
+#pragma unroll
+        for (int k = 0; k < NF; k++) {
+          T acc = (T)0;
+#pragma unroll
+          for (int m = 0; m < P1; m++) acc += ln[m] * Un[k * NS + nbase + m * nstride];
+          un[k] = acc;
+        }
       } else {
 #pragma unroll
         for (int k = 0; k < NF; k++) un[k] = pu[r][k];
@@ -536,10 +590,11 @@ __global__ __launch_bounds__((ElemGeo<D, P>::NTH)) void k_stage(StageArgs<T> a) 
       for (int k = 0; k < NF; k++) sJ[slot][k][q] = Fn[k] * scale + sJ[slot][k][q];
     }
   }
-  __syncthreads();
+  elementSync<TPE>();
 
   // (4) correction of the jump (correctDivFlux, solver.cpp:789-806) and the
-  // stage update; the updated state goes back to sU for its traces
+  // stage update; the updated state goes to sFd (free after (2)) for its
+  // traces, so that sU keeps the stage state other elements of the block read
   if (hasS) {
     int cq[NFACE], cside[NFACE], cax[NFACE];
     corrEntries<D, P>(t, cq, cside, cax);
@@ -571,17 +626,21 @@ __global__ __launch_bounds__((ElemGeo<D, P>::NTH)) void k_stage(StageArgs<T> a) 
         a.Uout[gi] = u;
         if (a.Uout32) a.Uout32[gi] = (float)u;
       }
-      sU[slot][k][t] = (T)u;
+      sFd[slot][k][t] = (T)u;
     }
   }
   if (a.mode == 0) return;
-  __syncthreads();
+  elementSync<TPE>();
 
-  // (5) face traces of the updated state for the next stage
+  // (5) face traces of the updated state for the next stage, for the faces
+  // whose neighbour is in another block
   if (act) {
     T *Fe = a.UfOut + (size_t)e * NF * NFP;
-#pragma unroll 1
-    for (int q = t; q < NFP; q += TPE) {
+#pragma unroll
+    for (int r = 0; r < NQ; r++) {
+      const int q = t + r * TPE;
+      if (q >= NFP) continue;
+      if (!VIS && (pnb[r] < 0 || inBlock(pnb[r]))) continue;
       int base, stride, side, dn;
       fptLine<D, P>(q, base, stride, side, dn);
       const T *l1 = s1 + (side ? O::LR : O::LL);
@@ -589,7 +648,7 @@ __global__ __launch_bounds__((ElemGeo<D, P>::NTH)) void k_stage(StageArgs<T> a) 
       for (int k = 0; k < NF; k++) {
         T acc = (T)0;
 #pragma unroll
-        for (int m = 0; m < P1; m++) acc += l1[m] * sU[slot][k][base + m * stride];
+        for (int m = 0; m < P1; m++) acc += l1[m] * sFd[slot][k][base + m * stride];
         Fe[k * NFP + q] = acc;
       }
     }

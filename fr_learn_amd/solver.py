@@ -150,6 +150,12 @@ class Solver:
         _lib.check(self._lib.fr_get_faces(self._h, _lib.iptr(out)))
         return out
 
+    def element_order(self):
+        """order[i] = mesh element at device position i (FR_LAYOUT_DEVICE order)."""
+        out = np.zeros(self.nEles, dtype=np.int32)
+        _lib.check(self._lib.fr_get_element_order(self._h, _lib.iptr(out)))
+        return out
+
     def residual_bytes(self):
         b = ctypes.c_double()
         _lib.check(self._lib.fr_residual_bytes(self._h, ctypes.byref(b)))

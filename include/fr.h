@@ -38,6 +38,8 @@ enum {
   FR_ENOMEM = -6
 };
 
+/* FR_LAYOUT_DEVICE: the library's own [ele][field][pt] layout with the
+ * elements in device order (fr_get_element_order) */
 enum { FR_LAYOUT_REF = 0, FR_LAYOUT_DEVICE = 1 };
 
 /* boundary-condition ids: include/global.hpp:63-79 */
@@ -105,6 +107,11 @@ int fr_get_operator(fr_ctx *ctx, int which, double *out);
  * 1 detJac_spts(spt,ele), 2 norm_fpts(fpt,ele,dim), 3 dA_fpts(fpt,ele),
  * 4 pos_spts(spt,ele,dim) (solver.hpp geometry arrays, solver.cpp:109-210) */
 int fr_get_geometry(fr_ctx *ctx, int which, double *out);
+
+/* Device element order: order[i] = mesh element stored at device position i.
+ * The library groups face-connected elements into the blocks its kernels
+ * share (no reference counterpart; geo keeps the mesh order). */
+int fr_get_element_order(fr_ctx *ctx, int *order /* nEles */);
 
 /* face table as built by geo::setupElesFaces (geo.cpp:322-446): per face
  * (L ele, L local face, R ele, R local face, relRot, bcType or -1) */
