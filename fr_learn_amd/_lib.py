@@ -63,6 +63,9 @@ SIGNATURES = {
     "fr_get_geometry": (ctypes.c_int, [_P, ctypes.c_int, _DP]),
     "fr_get_faces": (ctypes.c_int, [_P, _IP]),
     "fr_get_element_order": (ctypes.c_int, [_P, _IP]),
+    "fr_set_dt": (ctypes.c_int, [_P, ctypes.c_double]),
+    "fr_calc_dt": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_double, _DP]),
+    "fr_get_dt": (ctypes.c_int, [_P, _DP]),
     "fr_set_state": (ctypes.c_int, [_P, _DP, ctypes.c_int]),
     "fr_get_state": (ctypes.c_int, [_P, _DP, ctypes.c_int]),
     "fr_calc_residual": (ctypes.c_int, [_P, ctypes.c_int]),

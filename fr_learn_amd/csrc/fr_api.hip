@@ -66,6 +66,9 @@ struct fr_ctx {
   int traceOf = -1;                   // 0: U_spts, 1: stage state Us, -1: stale
   void *divF[4] = {nullptr, nullptr, nullptr, nullptr};
   void *dUf = nullptr;                // viscous: gradient traces [ele][dim][field][fpt]
+  double *dtG = nullptr;              // the global time step (one value)
+  double *dtE = nullptr;              // per-element time steps (local time stepping)
+  int dtType = 0;                     // 0 fixed dt, 1 global CFL dt, 2 local CFL dt
   int32_t *fnbr = nullptr;
   int8_t *finfo = nullptr;
   size_t partCap = 0;
@@ -89,7 +92,7 @@ void freeAll(fr_ctx *c) {
   void *bufs[] = {c->U,       c->U0,      c->detJ,    c->JG64,    c->fgeo64, c->ownN,  c->wts,
                   c->oned64,  c->part,    c->U32,     c->JG32,    c->fgeo32, c->oned32, c->Us,
                   c->Uf[0],   c->Uf[1],   c->divF[0], c->divF[1], c->divF[2], c->divF[3], c->fnbr,
-                  c->finfo,   c->dUf};
+                  c->finfo,   c->dUf,     c->dtG,     c->dtE};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   for (auto &v : c->ev)
@@ -126,6 +129,12 @@ struct TimeScope {
     }
   }
 };
+
+// Time step source of the update kernels: dtE[ele * stride].
+std::pair<const double *, int> dtSource(const fr_ctx *c) {
+  if (c->dtType == 2) return {c->dtE, 1};
+  return {c->dtG, 0};
+}
 
 template <typename T>
 const T *stateIn(fr_ctx *c, int which) {
@@ -183,7 +192,8 @@ int launchStage(fr_ctx *c, int in, int mode, int step, double rk) {
   a.c2 = c->RKb[2];
   a.c3 = c->RKb[3];
   a.rk = rk;
-  a.dt = c->cfg.dt;
+  a.dtE = dtSource(c).first;
+  a.dtStride = dtSource(c).second;
   a.dUf = (const T *)c->dUf;
   a.mode = mode;
   a.nprev = step;
@@ -250,7 +260,8 @@ int rkA(fr_ctx *c, int step, double rk) {
   long n = (long)c->nState();
   TimeScope ts(c, 2);
   hipLaunchKernelGGL((frdev::k_rk_a<TD>), dim3(nblk(n)), dim3(256), 0, c->stream, c->U, c->U0,
-                     (const TD *)c->divF[step], c->detJ, c->U32, rk, c->cfg.dt, 0, c->NF, c->NS, n);
+                     (const TD *)c->divF[step], c->detJ, c->U32, rk, dtSource(c).first, dtSource(c).second, 0,
+                     c->NF, c->NS, n);
   HIP_TRY(hipGetLastError());
   c->traceOf = -1;
   return 0;
@@ -356,6 +367,8 @@ int createImpl(const fr_config *cfg, const fr_mesh *m, int precision, fr_ctx *c)
   const size_t ts = c->tsize();
   for (int b = 0; b < 2; b++) HIP_TRY(hipMalloc(&c->Uf[b], (size_t)nE * c->NF * NFP * ts));
   HIP_TRY(hipMalloc(&c->Us, nSt * ts));
+  if ((rc = dmalloc(&c->dtG, 1))) return rc;
+  HIP_TRY(hipMemcpy(c->dtG, &cfg->dt, sizeof(double), hipMemcpyHostToDevice));
   if (cfg->viscous) HIP_TRY(hipMalloc(&c->dUf, (size_t)nE * D * c->NF * NFP * ts));
   HIP_TRY(hipMemset(c->Us, 0, nSt * ts));
   for (int s = 0; s < c->nRK; s++) {
@@ -684,10 +697,12 @@ int fr_time_step_b(fr_ctx *c, int step) {
   long n = (long)c->nState();
   if (c->prec == FR_PREC_FP32)
     hipLaunchKernelGGL(frdev::k_rk_b1<float>, dim3(nblk(n)), dim3(256), 0, c->stream, c->U,
-                       (const float *)c->divF[step], c->detJ, c->U32, c->RKb[step], c->cfg.dt, c->NF, c->NS, n);
+                       (const float *)c->divF[step], c->detJ, c->U32, c->RKb[step], dtSource(c).first,
+                       dtSource(c).second, c->NF, c->NS, n);
   else
     hipLaunchKernelGGL(frdev::k_rk_b1<double>, dim3(nblk(n)), dim3(256), 0, c->stream, c->U,
-                       (const double *)c->divF[step], c->detJ, c->U32, c->RKb[step], c->cfg.dt, c->NF, c->NS, n);
+                       (const double *)c->divF[step], c->detJ, c->U32, c->RKb[step], dtSource(c).first,
+                       dtSource(c).second, c->NF, c->NS, n);
   HIP_TRY(hipGetLastError());
   c->traceOf = -1;
   return 0;
@@ -754,6 +769,65 @@ int fr_reduce(fr_ctx *c, int kind, double *out) {
     return 0;
   }
   return fail(FR_EINVAL, "unknown reduction kind");
+}
+
+int fr_set_dt(fr_ctx *c, double dt) {
+  if (!c) return fail(FR_EINVAL, "null argument");
+  if (!(dt > 0)) return fail(FR_EINVAL, "dt must be positive");
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  c->cfg.dt = dt;
+  c->dtType = 0;
+  HIP_TRY(hipMemcpy(c->dtG, &dt, sizeof(double), hipMemcpyHostToDevice));
+  return 0;
+}
+
+int fr_calc_dt(fr_ctx *c, int dtType, double CFL, double *dtMin) {
+  if (!c) return fail(FR_EINVAL, "null argument");
+  if (dtType != 1 && dtType != 2) return fail(FR_EINVAL, "dtType must be 1 (global) or 2 (local)");
+  // getCFLLimit (global.cpp:45-68)
+  static const double lim[6] = {1.393, 0.464, 0.235, 0.1453, 0.100, 0.0736};
+  if (c->P < 0 || c->P > 5) return fail(FR_EINVAL, "CFL limit not available for this order!");
+  int rc;
+  if (!c->dtE && (rc = dmalloc(&c->dtE, (size_t)c->nE))) return rc;
+  double *wsurf;
+  if ((rc = dmalloc(&wsurf, c->ops.wtsFace.size()))) return rc;
+  HIP_TRY(hipMemcpyAsync(wsurf, c->ops.wtsFace.data(), c->ops.wtsFace.size() * 8, hipMemcpyHostToDevice, c->stream));
+  hipError_t err = hipSuccess;
+  dispatch(c->D, c->P, c->NF, [&](auto Dc, auto Pc, auto NFc) {
+    constexpr int D = decltype(Dc)::value, P = decltype(Pc)::value, NF = decltype(NFc)::value;
+    hipLaunchKernelGGL((frdev::k_dt<D, P, NF>), dim3(nblk(c->nE)), dim3(256), 0, c->stream, c->U, c->ownN, c->fgeo64,
+                       c->detJ, c->wts, wsurf, c->oned64, CFL * lim[c->P] * 2, c->ph, c->nE, c->dtE);
+    err = hipGetLastError();
+  });
+  if (err != hipSuccess) {
+    (void)hipFree(wsurf);
+    return fail(FR_EHIP, std::string("k_dt launch: ") + hipGetErrorString(err));
+  }
+  std::vector<double> h(c->nE);
+  HIP_TRY(hipMemcpyAsync(h.data(), c->dtE, h.size() * 8, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  (void)hipFree(wsurf);
+  double m = INFINITY;
+  for (double v : h) m = std::fmin(m, v);
+  if (!(m > 0) || std::isinf(m)) return fail(FR_ENAN, "CFL time step is not positive and finite");
+  c->cfg.dt = m;
+  c->dtType = dtType;
+  HIP_TRY(hipMemcpy(c->dtG, &m, sizeof(double), hipMemcpyHostToDevice));
+  if (dtMin) *dtMin = m;
+  return 0;
+}
+
+int fr_get_dt(fr_ctx *c, double *dtEle) {
+  if (!c || !dtEle) return fail(FR_EINVAL, "null argument");
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (c->dtType != 2) {
+    for (int e = 0; e < c->nE; e++) dtEle[e] = c->cfg.dt;
+    return 0;
+  }
+  std::vector<double> h(c->nE);
+  HIP_TRY(hipMemcpy(h.data(), c->dtE, h.size() * 8, hipMemcpyDeviceToHost));
+  for (int e = 0; e < c->nE; e++) dtEle[e] = h[c->iperm[e]];
+  return 0;
 }
 
 int fr_set_stream(fr_ctx *c, void *stream) {

@@ -271,6 +271,11 @@ std::string buildOperators(const fr_config &cfg, Operators &o) {
     o.locSpts[3 * s + 2] = (D == 3) ? x[k] : 0.0;
     o.wts[s] = (D == 3) ? kGW[P][k] * kGW[P][j] * kGW[P][i] : kGW[P][j] * kGW[P][i];
   }
+  // face weights indexed by face-local flux point, getQptWeights(order,
+  // nDims - 1) as ele::calcDt uses them (ele.cpp:1087, 1100)
+  const int nFF = (D == 3) ? P1 * P1 : P1;
+  o.wtsFace.assign(nFF, 0.0);
+  for (int m = 0; m < nFF; m++) o.wtsFace[m] = (D == 3) ? kGW[P][m / P1] * kGW[P][m % P1] : kGW[P][m];
   if (D == 3) {
     const int F2 = P1 * P1;
     for (int i = 0; i < P1; i++)

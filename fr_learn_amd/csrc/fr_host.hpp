@@ -16,6 +16,7 @@ namespace frhost {
 struct Operators {
   int nDims = 3, order = 3, P1 = 4, nSpts = 0, nFpts = 0, nFptsFace = 0, nFacesEle = 0;
   std::vector<double> spts1D, wts;           // 1D abscissae, per-spt quadrature weights
+  std::vector<double> wtsFace;               // quadrature weights of one face's flux points
   std::vector<double> locSpts, locFpts;      // [pt][3]
   std::vector<double> tNorm;                 // [fpt][3]
   std::vector<double> spts_to_fpts;          // nFpts x nSpts

@@ -96,7 +96,9 @@ struct StageArgs {
   float *Uout32;
   const T *dprev0, *dprev1, *dprev2;
   const T *__restrict__ dUf;  // viscous: gradient traces [ele][dim][field][fpt]
-  double c0, c1, c2, c3, rk, dt;
+  const double *dtE;          // time step: dtE[ele * dtStride] (stride 0: one global dt)
+  int dtStride;
+  double c0, c1, c2, c3, rk;
   int mode, nprev, nEles;
   Phys ph;
 };
@@ -318,8 +320,15 @@ __global__ __launch_bounds__((ElemGeo<D, P>::NTH)) void k_grad(GradArgs<T> a) {
 // terms (LDG): the corrected gradient at the solution points is recomputed
 // here from the state and the neighbours' traces exactly as k_grad forms it,
 // and the viscous common flux reads the gradient traces k_grad wrote.
+// Waves per SIMD the register allocation of k_stage is held to: 4 for the
+// inviscid FP64 kernel (at most 128 VGPRs), the compiler's choice otherwise.
+template <typename T, bool VIS>
+constexpr int stageWaves() {
+  return (!VIS && sizeof(T) == 8) ? 4 : 1;
+}
+
 template <int D, int P, int NF, typename T, bool BND, bool VIS>
-__global__ __launch_bounds__((ElemGeo<D, P>::NTH)) void k_stage(StageArgs<T> a) {
+__global__ __launch_bounds__((ElemGeo<D, P>::NTH), (stageWaves<T, VIS>())) void k_stage(StageArgs<T> a) {
   using G = ElemGeo<D, P>;
   using O = OneD<P>;
   constexpr int NS = G::NS, NFP = G::NFP, NFPF = G::NFPF, P1 = G::P1, TPE = G::TPE, EPB = G::EPB;
@@ -380,10 +389,11 @@ __global__ __launch_bounds__((ElemGeo<D, P>::NTH)) void k_stage(StageArgs<T> a) 
 #pragma unroll
     for (int c = 0; c <= D; c++) pn[r][c] = a.fgeo[(ec * (D + 1) + c) * NFP + qc];
   }
-  // stage base state and detJ for the update (read in every mode: in mode 0
-  // the base is the input state, whose lines are already in cache)
+  // stage base state, detJ and time step for the update (read in every mode:
+  // in mode 0 the base is the input state, whose lines are already in cache)
   double pb[NF];
   const double pdet = a.detJ[ec * NS + tc];
+  const double pdt = a.dtE[ec * a.dtStride];
 #pragma unroll
   for (int k = 0; k < NF; k++) pb[k] = a.Ubase[(ec * NF + k) * NS + tc];
 #pragma unroll
@@ -609,7 +619,7 @@ __global__ __launch_bounds__((ElemGeo<D, P>::NTH)) void k_stage(StageArgs<T> a) 
       if (a.divF) a.divF[gi] = dv;
       if (a.mode == 0) continue;
       // stage terms RK * divF / detJ * dt, with dt / detJ formed once per point
-      const double dtJ = a.dt / pdet;
+      const double dtJ = pdt / pdet;
       double u = pb[k];
       if (a.mode == 2) {
         if (a.nprev > 0) u -= a.c0 * (double)a.dprev0[gi] * dtJ;
@@ -747,8 +757,8 @@ __global__ void k_fgeo(const double *__restrict__ ownN, const int32_t *__restric
 // (copyUspts_U0, solver.cpp:387-392).  U32 (optional) receives a float copy.
 template <typename TD>
 __global__ void k_rk_a(double *__restrict__ U, double *__restrict__ U0, const TD *__restrict__ divF,
-                       const double *__restrict__ detJ, float *__restrict__ U32, double rk, double dt, int copy,
-                       int NF, int NS, long n) {
+                       const double *__restrict__ detJ, float *__restrict__ U32, double rk,
+                       const double *__restrict__ dtE, int dtStride, int copy, int NF, int NS, long n) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const long e = i / ((long)NF * NS);
@@ -760,7 +770,7 @@ __global__ void k_rk_a(double *__restrict__ U, double *__restrict__ U0, const TD
   } else {
     u0 = U0[i];
   }
-  const double v = u0 - rk * (double)divF[i] / detJ[e * NS + s] * dt;
+  const double v = u0 - rk * (double)divF[i] / detJ[e * NS + s] * dtE[e * dtStride];
   U[i] = v;
   if (U32) U32[i] = (float)v;
 }
@@ -768,14 +778,66 @@ __global__ void k_rk_a(double *__restrict__ U, double *__restrict__ U0, const TD
 // Single timeStepB pass (solver.cpp:347-385) for the ABI entry point.
 template <typename TD>
 __global__ void k_rk_b1(double *__restrict__ U, const TD *__restrict__ dF, const double *__restrict__ detJ,
-                        float *__restrict__ U32, double b, double dt, int NF, int NS, long n) {
+                        float *__restrict__ U32, double b, const double *__restrict__ dtE, int dtStride, int NF,
+                        int NS, long n) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const long e = i / ((long)NF * NS);
   const int s = (int)(i % NS);
-  const double u = U[i] - b * (double)dF[i] / detJ[e * NS + s] * dt;
+  const double u = U[i] - b * (double)dF[i] / detJ[e * NS + s] * dtE[e * dtStride];
   U[i] = u;
   if (U32) U32[i] = (float)u;
+}
+
+// CFL time step of each element from the current state, FP64: the traces
+// (extrapolateU), the wave speed |v.n| + c (Euler) or |a.n| (advection) at
+// every flux point (ele::calcWaveSpFpts, ele.cpp:1031-1084, static grids) and
+// dt = CFL * limit(P) * 2 * vol / sum(waveSp * wSurf * dA) (ele::calcDt,
+// ele.cpp:1086-1105).  One thread per element; runs only when asked for.
+template <int D, int P, int NF>
+__global__ __launch_bounds__(256) void k_dt(const double *__restrict__ U, const double *__restrict__ ownN,
+                                            const double *__restrict__ fgeo, const double *__restrict__ detJ,
+                                            const double *__restrict__ wts, const double *__restrict__ wsurf,
+                                            const double *__restrict__ oned, double factor, Phys ph, int nEles,
+                                            double *__restrict__ dtE) {
+  using G = ElemGeo<D, P>;
+  using O = OneD<P>;
+  constexpr int NS = G::NS, NFP = G::NFP, NFPF = G::NFPF, P1 = G::P1;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nEles) return;
+  double vol = 0;
+  for (int s = 0; s < NS; s++) vol += detJ[(size_t)e * NS + s] * wts[s];
+  const double *Ue = U + (size_t)e * NF * NS;
+  double intWave = 0;
+  for (int q = 0; q < NFP; q++) {
+    double n[D];
+    for (int c = 0; c < D; c++) n[c] = ownN[((size_t)e * D + c) * NFP + q];
+    double ws;
+    if constexpr (NF == 1) {
+      double vN = ph.advect[0] * n[0] + ph.advect[1] * n[1];
+      if (D == 3) vN += ph.advect[2] * n[2];
+      ws = fabs(vN);
+    } else {
+      int base, stride, side, dn;
+      fptLine<D, P>(q, base, stride, side, dn);
+      const double *l1 = oned + (side ? O::LR : O::LL);
+      double u[NF];
+      for (int k = 0; k < NF; k++) {
+        double acc = 0;
+        for (int m = 0; m < P1; m++) acc += l1[m] * Ue[k * NS + base + m * stride];
+        u[k] = acc;
+      }
+      const double rho = u[0], vx = u[1] / rho, vy = u[2] / rho, vz = (D == 3) ? u[3] / rho : 0.;
+      const double rhoVSq = rho * (vx * vx + vy * vy + vz * vz);
+      const double pr = (ph.gamma - 1) * (u[D + 1] - 0.5 * rhoVSq);
+      double vN = vx * n[0] + vy * n[1];
+      if (D == 3) vN += vz * n[2];
+      const double csq = fmax(ph.gamma * pr / rho, 0.0);
+      ws = fabs(vN) + sqrt(csq);
+    }
+    intWave += ws * wsurf[q % NFPF] * fgeo[((size_t)e * (D + 1) + D) * NFP + q];
+  }
+  dtE[e] = factor * vol / intWave;
 }
 
 __global__ void k_to_float(const double *__restrict__ a, float *__restrict__ b, long n) {

@@ -150,6 +150,25 @@ class Solver:
         _lib.check(self._lib.fr_get_faces(self._h, _lib.iptr(out)))
         return out
 
+    def set_dt(self, dt):
+        """Fixed time step params->dt (dtType 0)."""
+        _lib.check(self._lib.fr_set_dt(self._h, float(dt)))
+        self.cfg["dt"] = float(dt)
+
+    def calc_dt(self, dtType, CFL):
+        """CFL time step from the current state (solver::calcDt); dtType 1
+        global, 2 local per element.  Returns the minimum over elements."""
+        out = ctypes.c_double()
+        _lib.check(self._lib.fr_calc_dt(self._h, int(dtType), float(CFL), ctypes.byref(out)))
+        self.cfg["dt"] = out.value
+        return out.value
+
+    def dt(self):
+        """The time step each element advances with (mesh element order)."""
+        out = np.zeros(self.nEles)
+        _lib.check(self._lib.fr_get_dt(self._h, _lib.dptr(out)))
+        return out
+
     def element_order(self):
         """order[i] = mesh element at device position i (FR_LAYOUT_DEVICE order)."""
         out = np.zeros(self.nEles, dtype=np.int32)

@@ -145,6 +145,19 @@ int fr_copy_u0_u(fr_ctx *ctx);
  * (out[1]) by the spt quadrature of solver.cpp:1510-1526 */
 int fr_reduce(fr_ctx *ctx, int kind, double *out);
 
+/* Time step.  fr_set_dt: fixed params->dt (dtType 0).  fr_calc_dt: CFL time
+ * step from the current U_spts -- ele::calcWaveSpFpts (ele.cpp:1031-1084,
+ * static grids), ele::calcDt (1086-1105), solver::calcDt (solver.cpp:293-305);
+ * dtType 1: params->dt = min over elements; dtType 2: every element advances
+ * with its own dt (timeStepA/B with eles[e]->dt, solver.cpp:307-385) and
+ * params->dt = the minimum.  The reference evaluates the wave speeds only
+ * from the initial state (solver.cpp:294-297, 1497-1503), so one call after
+ * fr_set_state reproduces its dt for the whole run.  fr_get_dt: the dt each
+ * element advances with, in mesh element order. */
+int fr_set_dt(fr_ctx *ctx, double dt);
+int fr_calc_dt(fr_ctx *ctx, int dtType, double CFL, double *dtMin);
+int fr_get_dt(fr_ctx *ctx, double *dtEle /* nEles */);
+
 /* Stream control.  fr_set_stream(ctx, (void*)hipStream_t) makes the context
  * launch on the caller's stream (NULL = the context's own stream). */
 int fr_set_stream(fr_ctx *ctx, void *stream);

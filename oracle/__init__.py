@@ -59,6 +59,10 @@ def load():
         lib.or_get_array.argtypes = [ctypes.c_void_p, ctypes.c_int, _DP]
         lib.or_calc_residual.argtypes = [ctypes.c_void_p, ctypes.c_int]
         lib.or_time_step_a.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]
+        lib.or_calc_dt.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]
+        lib.or_calc_dt.restype = ctypes.c_double
+        lib.or_get_dt.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        lib.or_get_dt.restype = ctypes.c_double
         lib.or_time_step_b.argtypes = [ctypes.c_void_p, ctypes.c_int]
         lib.or_norm_residual.argtypes = [ctypes.c_void_p, ctypes.c_int, _DP]
         lib.or_set_threads.argtypes = [ctypes.c_int]
@@ -224,6 +228,15 @@ class Oracle:
         if self._lib.or_get_array(self._h, which, _d(out)) != 0:
             raise ValueError(f"oracle array {which} not available")
         return out
+
+    def calc_dt(self, dtType, CFL):
+        """CFL time step from the current state (ele::calcWaveSpFpts, ele::calcDt,
+        solver::calcDt); returns the minimum over elements."""
+        return self._lib.or_calc_dt(self._h, int(dtType), float(CFL))
+
+    def dt(self, e=-1):
+        """params->dt (e < 0) or eles[e]->dt under local time stepping."""
+        return self._lib.or_get_dt(self._h, int(e))
 
     def update(self, n=1):
         for _ in range(n):

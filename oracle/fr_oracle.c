@@ -199,6 +199,8 @@ struct or_solver {
   double *faceFn; /* [nFaces][nFptsFace][nFields]: each face's common flux kept between
                      the inviscid and viscous passes (face.hpp Fn member) */
   double *faceUC; /* [nFaces][nFptsFace][nFields] LDG common solution */
+  int dtType;      /* 0 fixed, 1 global CFL dt, 2 local (per-element) CFL dt */
+  double *dtEle;   /* [nEles] ele::dt for dtType 2 */
 };
 
 #define IDX3(a, b, c, B, C) ((c) + (C) * ((b) + (B) * (a)))
@@ -724,6 +726,7 @@ void or_destroy(or_solver *s) {
                     s->dU_fpts, s->dUc_fpts, s->faceFn, s->faceUC};
   for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; i++) free(ptrs[i]);
   for (int r = 0; r < 4; r++) free(s->divF[r]);
+  free(s->dtEle);
   int *iptrs[] = {s->fL, s->flocL, s->fR, s->flocR, s->frot, s->fbc, s->fptR};
   for (size_t i = 0; i < sizeof iptrs / sizeof iptrs[0]; i++) free(iptrs[i]);
   free(s);
@@ -1365,27 +1368,89 @@ void or_calc_residual(or_solver *s, int step) {
 /* ------------------------------------------------------------------------ */
 /* time integration, src/solver.cpp:212-250, 307-399 (dtType 0, no PMG)       */
 /* ------------------------------------------------------------------------ */
+/* timeStepA / timeStepB: params->dt, or eles[e]->dt for dtType 2
+ * (solver.cpp:307-385) */
 void or_time_step_a(or_solver *s, int step, double RKval) {
   int nS = s->nSpts, nE = s->nEles, K = s->nFields;
-  double dt = s->p.dt;
 #pragma omp parallel for schedule(static)
   for (int spt = 0; spt < nS; spt++)
-    for (int e = 0; e < nE; e++)
+    for (int e = 0; e < nE; e++) {
+      double dt = (s->dtType == 2) ? s->dtEle[e] : s->p.dt;
       for (int k = 0; k < K; k++) {
         size_t o = IDX3(spt, e, k, nE, K);
         s->U_spts[o] = s->U0[o] - RKval * s->divF[step][o] / s->detJac_spts[(size_t)spt * nE + e] * dt;
       }
+    }
 }
 void or_time_step_b(or_solver *s, int step) {
   int nS = s->nSpts, nE = s->nEles, K = s->nFields;
-  double dt = s->p.dt, b = s->RKb[step];
+  double b = s->RKb[step];
 #pragma omp parallel for schedule(static)
   for (int spt = 0; spt < nS; spt++)
-    for (int e = 0; e < nE; e++)
+    for (int e = 0; e < nE; e++) {
+      double dt = (s->dtType == 2) ? s->dtEle[e] : s->p.dt;
       for (int k = 0; k < K; k++) {
         size_t o = IDX3(spt, e, k, nE, K);
         s->U_spts[o] -= b * s->divF[step][o] / s->detJac_spts[(size_t)spt * nE + e] * dt;
       }
+    }
+}
+
+double or_get_dt(const or_solver *s, int e) {
+  return (e >= 0 && s->dtEle && s->dtType == 2) ? s->dtEle[e] : s->p.dt;
+}
+
+/* getCFLLimit, global.cpp:45-68 */
+static double cfl_limit(int order) {
+  static const double lim[6] = {1.393, 0.464, 0.235, 0.1453, 0.100, 0.0736};
+  return (order >= 0 && order <= 5) ? lim[order] : -1.0;
+}
+
+/* CFL time step from the current state: extrapolateU, ele::calcWaveSpFpts
+ * (ele.cpp:1031-1084, static grids), ele::calcDt (1086-1105) and
+ * solver::calcDt (solver.cpp:293-305).  dtType 1 sets params->dt to the
+ * minimum over elements; dtType 2 also keeps each element's dt for the
+ * stage updates; 0 returns to the fixed dt given.  Returns the minimum. */
+double or_calc_dt(or_solver *s, int dtType, double CFL) {
+  const or_params *p = &s->p;
+  int D = s->nDims, K = s->nFields, nS = s->nSpts, nF = s->nFpts, nE = s->nEles, nfp = s->nFptsFace, P1 = s->P1;
+  oracle_dgemm(nF, nE * K, nS, 1.0, s->opp_spts_to_fpts, s->U_spts, 0.0, s->U_fpts);
+  double wS[64];
+  for (int m = 0; m < nfp; m++) {
+    const double *w1 = GL_WTS[s->order];
+    wS[m] = (D == 3) ? w1[m / P1] * w1[m % P1] : w1[m];
+  }
+  if (!s->dtEle) s->dtEle = xcalloc(nE, 8);
+  double dtMin = INFINITY;
+  for (int e = 0; e < nE; e++) {
+    double vol = 0;
+    for (int spt = 0; spt < nS; spt++) vol += s->detJac_spts[(size_t)spt * nE + e] * s->wts[spt];
+    double intWave = 0;
+    for (int fpt = 0; fpt < nF; fpt++) {
+      const double *n = s->norm_fpts + ((size_t)fpt * nE + e) * D;
+      double ws;
+      if (p->equation == OR_ADVECTION_DIFFUSION) {
+        double vN = p->advectVx * n[0] + p->advectVy * n[1];
+        if (D == 3) vN += p->advectVz * n[2];
+        ws = fabs(vN);
+      } else {
+        const double *U = s->U_fpts + IDX3(fpt, e, 0, nE, K);
+        double rho = U[0], u = U[1] / rho, v = U[2] / rho, w = (D == 3) ? U[3] / rho : 0.;
+        double rhoVSq = rho * (u * u + v * v + w * w);
+        double pr = (p->gamma - 1) * (U[D + 1] - 0.5 * rhoVSq);
+        double vN = u * n[0] + v * n[1];
+        if (D == 3) vN += w * n[2];
+        double csq = fmax(p->gamma * pr / rho, 0.0);
+        ws = fabs(vN) + sqrt(csq);
+      }
+      intWave += ws * wS[fpt % nfp] * s->dA_fpts[(size_t)fpt * nE + e];
+    }
+    s->dtEle[e] = CFL * cfl_limit(s->order) * 2 * vol / intWave;
+    dtMin = fmin(dtMin, s->dtEle[e]);
+  }
+  s->dtType = dtType;
+  if (dtType != 0) s->p.dt = dtMin;
+  return dtMin;
 }
 void or_copy_u_u0(or_solver *s) { memcpy(s->U0, s->U_spts, (size_t)s->nSpts * s->nEles * s->nFields * 8); }
 void or_copy_u0_u(or_solver *s) { memcpy(s->U_spts, s->U0, (size_t)s->nSpts * s->nEles * s->nFields * 8); }

@@ -103,6 +103,8 @@ void or_set_threads(int n);
 
 /* pointwise test hooks (flux.cpp / face.cpp / boundFace.cpp / polynomials.cpp
  * restatements), so tests can pin them against textbook formulas */
+double or_calc_dt(or_solver *s, int dtType, double CFL);
+double or_get_dt(const or_solver *s, int e);
 void or_point_inviscid_flux(const or_params *p, const double *U, double *F /* [3][5] */);
 void or_point_common_flux(const or_params *p, int kind, const double *UL, const double *UR, const double *n,
                           double *Fn); /* kind 0 Rusanov, 1 Roe, 2 Lax-Friedrichs */

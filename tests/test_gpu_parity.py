@@ -156,3 +156,22 @@ def test_fp32_residual_and_steps():
     l2 = np.linalg.norm(Ug - Uo) / np.linalg.norm(Uo)
     print("fp32 state rel L2", l2)
     assert l2 < 1e-6
+
+
+@pytest.mark.parametrize("dtType", [1, 2])
+def test_cfl_time_step_and_local_update(dtType):
+    """CFL dt (global min or per element) and RK4 steps with it."""
+    cfg = euler_cfg(3, 3, timeType=4)
+    o, g, _ = _pair(cfg, hex_box(3, perturb=0.2, seed=3), cases.ic_tgv, amp=2e-2)
+    dto = o.calc_dt(dtType, 0.7)
+    dtg = g.calc_dt(dtType, 0.7)
+    assert abs(dtg - dto) <= 1e-12 * dto
+    dto_e = np.array([o.dt(e if dtType == 2 else -1) for e in range(o.nEles)])
+    assert np.abs(g.dt() - dto_e).max() <= 1e-12 * dto
+    o.update(2)
+    g.update(2)
+    err = rel_err(g.get_state(), o.get_state())
+    print("state rel err", err)
+    assert err.max() <= TOL_FP64
+    g.set_dt(1e-3)
+    assert np.all(g.dt() == 1e-3)

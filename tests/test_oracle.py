@@ -645,6 +645,53 @@ def test_rk_stability_polynomial_linear_advection(timeType, coef):
     assert np.abs(o.get_state() - want).max() < 1e-14 * np.abs(U0).max() * 10
 
 
+@pytest.mark.parametrize("D", [2, 3])
+@pytest.mark.parametrize("equation", [0, 1])
+def test_cfl_time_step_closed_form(D, equation):
+    """Uniform state on an affine box: every element has vol = h^D and
+    sum over faces of (|v.n| + c) * area (Euler) or |a.n| * area
+    (advection), so dt = CFL * limit(P) * 2 * vol / that (ele.cpp:1086-1105)."""
+    P = 3
+    cfg = euler_cfg(D, P) if equation == 1 else advection_cfg(D, P)
+    m = hex_box(3, perturb=0.0, rotate=False, shuffle=False, L=3.0) if D == 3 else \
+        quad_box(3, perturb=0.0, rotate=False, shuffle=False, lo=0.0, hi=3.0)
+    o = Oracle(cfg, m)
+    vel = [0.3, 0.1, -0.05][:D]
+    o.initialize(lambda x, c: cases.ic_uniform(x, c, vel=vel, p=1.0 / 1.4) if equation == 1 else
+                 np.ones(x.shape[:-1] + (1,)))
+    if equation == 1:
+        c = np.sqrt(1.4 * (1 / 1.4) / 1.0)
+        speed = [abs(v) + c for v in vel]
+    else:
+        speed = [abs(v) for v in [1.0, 0.5, -0.25][:D]]
+    intw = sum(2 * speed[d] for d in range(D))  # h = 1: unit face areas
+    want = 0.5 * 0.1453 * 2 / intw
+    assert abs(o.calc_dt(1, 0.5) - want) < 1e-14 * want
+    assert abs(o.dt() - want) < 1e-14 * want
+    assert abs(o.calc_dt(2, 0.5) - want) < 1e-14 * want
+    assert all(abs(o.dt(e) - want) < 1e-14 * want for e in range(o.nEles))
+
+
+def test_local_time_step_update():
+    """dtType 2: forward Euler advances each element with its own dt,
+    U1 = U0 - divF / detJ * dt_e (solver.cpp:307-385 with eles[e]->dt)."""
+    cfg = euler_cfg(3, 2, timeType=0)
+    m = hex_box(3, perturb=0.2, seed=3)
+    o = Oracle(cfg, m)
+    U0 = perturbed_state(o.initialize(cases.ic_tgv), seed=2)
+    o.set_state(U0)
+    o.calc_dt(2, 0.8)
+    dts = np.array([o.dt(e) for e in range(o.nEles)])
+    assert dts.max() > 1.05 * dts.min()
+    assert abs(o.dt() - dts.min()) == 0
+    o.calc_residual(0)
+    R = o.residual(0)
+    want = U0 - R / o.geometry(1)[..., None] * dts[None, :, None]
+    o.set_state(U0)
+    o.update(1)
+    assert np.abs(o.get_state() - want).max() < 1e-13 * np.abs(U0).max()
+
+
 def test_residual_norms_quadrature():
     """Residual norms are Gauss quadratures of divF / detJ over the domain."""
     cfg = euler_cfg(3, 3)
