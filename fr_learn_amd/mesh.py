@@ -163,7 +163,7 @@ def write_gmsh(mesh, path, names=None):
     lines.append("$Nodes")
     lines.append(str(mesh.xv.shape[0]))
     for i, x in enumerate(mesh.xv):
-        lines.append(f"{i + 1} {x[0]!r} {x[1]!r} {x[2]!r}")
+        lines.append(f"{i + 1} {float(x[0])!r} {float(x[1])!r} {float(x[2])!r}")
     lines.append("$EndNodes")
     # boundary faces: element faces whose vertices all lie in one boundary set
     fv = ([[0, 1, 2, 3], [4, 5, 6, 7], [3, 0, 4, 7], [2, 1, 5, 6], [1, 0, 4, 5], [3, 2, 6, 7]] if D == 3
@@ -186,3 +186,83 @@ def write_gmsh(mesh, path, names=None):
     lines.append("$EndElements")
     with open(path, "w") as fh:
         fh.write("\n".join(lines) + "\n")
+
+
+# Boundary-condition names of the mesh file / input (bcStr2Num, src/global.cpp:11-25);
+# "periodic" and "overset" have no counterpart here.
+BC_NAMES = dict(BC, none=-1, fluid=-1)
+
+# vertices per boundary element, by Gmsh element type (geo.cpp:670-730)
+_GMSH_FACE_PTS = {1: 2, 2: 3, 3: 4, 10: 4, 16: 4, 36: 4, 37: 4, 38: 4, 8: 3, 26: 4, 27: 5, 28: 6, 62: 7,
+                  63: 8, 64: 9, 65: 10, 66: 11}
+
+
+def read_gmsh(path, mesh_bounds=None):
+    """Read a Gmsh 2.2 ASCII mesh as geo::readGmsh does (src/geo.cpp:448-770):
+    $PhysicalNames gives each physical group a name (quotes removed, lower
+    case) that `mesh_bounds` maps to a boundary condition (the input file's
+    meshBounds; by default the name itself); the group named "fluid" holds
+    the cells and fixes nDims.  Cells are linear hexes (type 5) or, in 2D,
+    linear quads (type 3); every other element is a boundary face whose
+    vertices join its group's vertex set.  Vertices are read in file order.
+    Errors raise ValueError with the reference's message."""
+    mesh_bounds = {k.lower(): v for k, v in (mesh_bounds or {}).items()}
+    with open(path) as fh:
+        lines = fh.read().split("\n")
+
+    def section(tag):
+        for i, ln in enumerate(lines):
+            if tag in ln:
+                return i + 1
+        raise ValueError(f"{tag} tag not found in Gmsh file!")
+
+    i = section("$PhysicalNames")
+    nPhys = int(lines[i].split()[0])
+    group = {}  # gmsh physical id -> bound index, -1 for the cells
+    bcs, nDims = [], None
+    for ln in lines[i + 1:i + 1 + nPhys]:
+        tok = ln.split(None, 2)
+        dim, pid, name = int(tok[0]), int(tok[1]), tok[2].strip().replace('"', "").lower()
+        if mesh_bounds and name not in mesh_bounds:
+            raise ValueError(f'Unrecognized mesh boundary: "{name}"\n'
+                             "Boundary names in input file must match those in mesh file.")
+        bc = mesh_bounds.get(name, name)
+        if bc not in BC_NAMES:
+            raise ValueError(f'Unrecognized boundary condition: "{bc}"')
+        if bc == "fluid":
+            nDims = dim
+            group[pid] = -1
+        else:
+            group[pid] = len(bcs)
+            bcs.append(BC[bc] if bc in BC else -1)
+    if nDims is None:
+        raise ValueError('no "fluid" physical group in the Gmsh file')
+    i = section("$Nodes")
+    nV = int(lines[i].split()[0])
+    xv = np.zeros((nV, 3))
+    for k in range(nV):
+        tok = lines[i + 1 + k].split()
+        xv[k] = [float(t) for t in tok[1:4]]
+    i = section("$Elements")
+    nEl = int(lines[i].split()[0])
+    cells, bnd = [], [set() for _ in bcs]
+    cell_type, cell_nodes = (5, 8) if nDims == 3 else (3, 4)
+    for k in range(nEl):
+        tok = [int(t) for t in lines[i + 1 + k].split()]
+        etype, ntags = tok[1], tok[2]
+        g = group.get(tok[3], None)
+        if g is None:
+            raise ValueError(f"Gmsh element ID {k}: physical group {tok[3]} not in $PhysicalNames")
+        nodes = tok[3 + ntags:]
+        if g == -1:
+            if etype != cell_type:
+                raise ValueError(f"Gmsh element ID {k}, Gmsh Element Type = {etype}: element type not recognized "
+                                 f"(linear {'hexahedra' if nDims == 3 else 'quadrilaterals'} only)")
+            cells.append([v - 1 for v in nodes[:cell_nodes]])
+        else:
+            if etype not in _GMSH_FACE_PTS:
+                raise ValueError(f"Gmsh element ID {k}, Gmsh Element Type = {etype}: "
+                                 "Boundary Element (Face) Type Not Recognized!")
+            bnd[g].update(v - 1 for v in nodes[:_GMSH_FACE_PTS[etype]])
+    bounds = [(bc, np.array(sorted(pts), dtype=np.int32)) for bc, pts in zip(bcs, bnd) if bc >= 0]
+    return Mesh(nDims, xv, np.array(cells, dtype=np.int32), None, bounds)

@@ -175,3 +175,20 @@ def test_cfl_time_step_and_local_update(dtType):
     assert err.max() <= TOL_FP64
     g.set_dt(1e-3)
     assert np.all(g.dt() == 1e-3)
+
+
+def test_residual_on_gmsh_mesh(tmp_path):
+    """A mesh read back from a Gmsh 2.2 file (geo::readGmsh path)."""
+    from fr_learn_amd import mesh as fmesh
+    m = hex_box(3, periodic=False, bc=WALL_BCS, L=1.0, seed=7)
+    inv = {v: k for k, v in fmesh.BC.items()}
+    names = [f"b{i}" for i in range(len(m.bounds))]
+    p = tmp_path / "walls.msh"
+    fmesh.write_gmsh(m, str(p), names=names)
+    r = fmesh.read_gmsh(str(p), mesh_bounds={nm: inv[bc] for nm, (bc, _) in zip(names, m.bounds)} |
+                        {"fluid": "fluid"})
+    cfg = euler_cfg(3, 2)
+    o, g, _ = _pair(cfg, r, lambda x, c: cases.ic_uniform(x, c, vel=[0.3, 0.1, -0.05], p=1.0 / 1.4), amp=2e-2)
+    o.calc_residual(0)
+    g.calc_residual(0)
+    assert rel_err(g.residual(0), o.residual(0)).max() <= TOL_FP64

@@ -71,3 +71,68 @@ def test_gmsh_writer_roundtrip(tmp_path):
     mesh.write_gmsh(m, str(p), names=["walls"] * len(m.bounds))
     text = p.read_text()
     assert "$Elements" in text and text.count("\n") > 27
+
+
+def test_gmsh_reader_roundtrip(tmp_path):
+    """write_gmsh -> read_gmsh gives back vertices, cells and boundary vertex
+    sets, in 3D (hexes) and 2D (quads), with the meshBounds name mapping."""
+    for D, spec in ((3, {"xmin": "char", "xmax": "sup_out", "ymin": "slip_wall", "ymax": "sub_out",
+                         "zmin": "sub_in", "zmax": "adiabatic_noslip"}),
+                    (2, {"xmin": "char", "xmax": "sub_out", "ymin": "slip_wall", "ymax": "sup_in"})):
+        n = (3,) * D
+        m = mesh.box(D, n, (0,) * D, (1,) * D, periodic=[False] * D, bc=spec, perturb=0.1, seed=2)
+        inv = {v: k for k, v in mesh.BC.items()}
+        names = [f"Surf_{i}" for i in range(len(m.bounds))]
+        p = tmp_path / f"box{D}.msh"
+        mesh.write_gmsh(m, str(p), names=names)
+        bmap = {nm.lower(): inv[bc] for nm, (bc, _) in zip(names, m.bounds)}
+        bmap["fluid"] = "fluid"
+        r = mesh.read_gmsh(str(p), mesh_bounds=bmap)
+        assert r.nDims == D
+        assert np.array_equal(r.xv, m.xv) and np.array_equal(r.c2v, m.c2v)
+        assert [bc for bc, _ in r.bounds] == [bc for bc, _ in m.bounds]
+        for (_, a), (_, b) in zip(r.bounds, m.bounds):
+            assert np.array_equal(a, b)
+
+
+def test_gmsh_reader_names_and_errors(tmp_path):
+    """Physical names are unquoted and lower-cased; unknown names and cell
+    types raise the reference's errors (geo.cpp:494-510, 636-641)."""
+    text = """$MeshFormat
+2.2 0 8
+$EndMeshFormat
+$PhysicalNames
+2
+1 1 "Wall"
+2 2 "FLUID"
+$EndPhysicalNames
+$Nodes
+6
+1 0 0 0
+2 1 0 0
+3 2 0 0
+4 0 1 0
+5 1 1 0
+6 2 1 0
+$EndNodes
+$Elements
+4
+1 1 2 1 1 1 2
+2 1 2 1 1 2 3
+3 3 2 2 2 1 2 5 4
+4 3 2 2 2 2 3 6 5
+$EndElements
+"""
+    p = tmp_path / "q.msh"
+    p.write_text(text)
+    r = mesh.read_gmsh(str(p), mesh_bounds={"wall": "slip_wall", "fluid": "fluid"})
+    assert r.nDims == 2 and r.nEles == 2
+    assert r.c2v.tolist() == [[0, 1, 4, 3], [1, 2, 5, 4]]
+    assert r.bounds[0][0] == mesh.BC["slip_wall"] and r.bounds[0][1].tolist() == [0, 1, 2]
+    with pytest.raises(ValueError, match="Unrecognized mesh boundary"):
+        mesh.read_gmsh(str(p), mesh_bounds={"inlet": "sup_in", "fluid": "fluid"})
+    with pytest.raises(ValueError, match="Unrecognized boundary condition"):
+        mesh.read_gmsh(str(p), mesh_bounds={"wall": "no_such_bc", "fluid": "fluid"})
+    p.write_text(text.replace("3 3 2 2 2 1 2 5 4", "3 2 2 2 2 1 2 5"))
+    with pytest.raises(ValueError, match="element type not recognized"):
+        mesh.read_gmsh(str(p), mesh_bounds={"wall": "slip_wall", "fluid": "fluid"})
