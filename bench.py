@@ -11,10 +11,16 @@ Prints ONE JSON line.  Besides the contract keys it carries:
                 its average launch time (HIP events on the launch stream),
                 against MI355X's 8 TB/s;
   cpu_baseline  the CPU oracle (same algorithm, dense operators like the
-                reference's cblas_dgemm calls) on one RK step of the same
-                workload, on the host's cores;
+                reference's cblas_dgemm calls) on about 10 s of RK steps of
+                the same workload, on the host's cores;
   fp32          the FP32 residual path (FP64 state / RK / reductions) and its
-                relative L2 distance from the FP64 path after the same steps.
+                relative L2 distance from the FP64 path after the same steps
+                (the FP64 path equals the oracle to 1e-12, tests/).
+  copy_GBps     a device-to-device copy of 1 GiB, read + write bytes over its
+                time: the streaming bandwidth this GPU reaches in practice.
+
+--copy-calibration runs only a 256 MiB copy, three times, for scaling the
+rocprofv3 FETCH_SIZE / WRITE_SIZE counters to bytes.
 """
 import argparse
 import json
@@ -63,16 +69,19 @@ def run_gpu(cfg, m, steps, warmup, precision):
     return out
 
 
-def run_cpu_baseline(cfg, m):
-    """One RK4 step of the same workload through the CPU oracle."""
+def run_cpu_baseline(cfg, m, budget_s=10.0):
+    """RK4 steps of the same workload through the CPU oracle, as many as fit
+    in about budget_s seconds (at least 2)."""
     from fr_learn_amd import cases
     from oracle import Oracle
     threads = min(16, os.cpu_count() or 1)
     o = Oracle(cfg, m, threads=threads)
     o.initialize(cases.ic_tgv)
-    nsteps = 2
+    nsteps = 0
     t0 = time.perf_counter()
-    o.update(nsteps)
+    while nsteps < 2 or time.perf_counter() - t0 < budget_s:
+        o.update(1)
+        nsteps += 1
     dt = time.perf_counter() - t0
     gdof = o.nEles * o.nSpts * 4 * nsteps / dt / 1e9
     o.close()
@@ -81,10 +90,55 @@ def run_cpu_baseline(cfg, m):
                        f"{dt:.1f} s, dense-operator oracle, {threads} OpenMP threads")
 
 
+def copy_bandwidth(nbytes=1 << 30, reps=10):
+    """Read + write bytes per second of torch's device-to-device copy.  Runs in
+    a child process of its own (--copy-bandwidth): torch brings its own HIP
+    runtime, which must not share a process with the one the library uses."""
+    import torch
+    a = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del a, b
+    torch.cuda.empty_cache()
+    return 2 * nbytes / (ms * 1e-3) / 1e9
+
+
+def copy_bandwidth_child():
+    import subprocess
+    try:
+        out = subprocess.run([sys.executable, os.path.abspath(__file__), "--copy-bandwidth"], capture_output=True,
+                             text=True, timeout=300, check=True).stdout
+        return json.loads(out.strip().splitlines()[-1])["copy_GBps"]
+    except (subprocess.SubprocessError, ValueError, KeyError, IndexError) as exc:
+        print(f"copy bandwidth not measured: {exc}", file=sys.stderr)
+        return None
+
+
+def copy_calibration():
+    """A 256 MiB device copy, three times: the known-size kernel the PMC
+    FETCH_SIZE / WRITE_SIZE readings are scaled by."""
+    import torch
+    n = (256 << 20) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": n * 8, "copies": 3}), flush=True)
+
+
 def traffic_from_profile(precision):
     """HBM bytes per launch of the residual kernel from the committed rocprofv3
     PMC summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor), if any."""
-    path = os.path.join(ROOT, "profiles", f"k_residual_tgv48_p3_{precision}_traffic.json")
+    path = os.path.join(ROOT, "profiles", f"k_stage_tgv48_p3_{precision}_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as fh:
@@ -94,18 +148,28 @@ def traffic_from_profile(precision):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=48, help="cells per direction")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 path")
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
+    ap.add_argument("--copy-calibration", action="store_true", help="only the 256 MiB copy for PMC scaling")
+    ap.add_argument("--copy-bandwidth", action="store_true", help="only measure the 1 GiB copy bandwidth")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU path is "
               "not part of this build", file=sys.stderr)
         sys.exit(2)
 
+    if args.copy_calibration:
+        copy_calibration()
+        return
+    if args.copy_bandwidth:
+        print(json.dumps({"copy_GBps": copy_bandwidth()}), flush=True)
+        return
+
+    copy_gbps = copy_bandwidth_child()  # before this process touches the GPU
     cfg, m = tgv_setup(args.n)
     main_prec = args.precision
     r = run_gpu(cfg, m, args.steps, args.warmup, main_prec)
@@ -134,9 +198,11 @@ def main():
         "roofline": {
             "bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
             "frac": achieved / HBM_PEAK, "traffic": traffic_from_profile(main_prec),
-            "kernel": "k_stage<3,3,5> (residual + RK stage update + next-stage face traces)",
+            "kernel": "k_stage<3,3,5,...> (residual + RK stage update + next-stage face traces; "
+                      "average over the RK4 stages, the final stage's variant included)",
             "algorithmic_bytes_per_launch": r["bytes"],
             "avg_launch_ms": r["res_ms"], "launches": r["res_n"],
+            "copy_GBps": copy_gbps,
         },
         "kernels_ms": {"stage": r["res_ms"], "trace_init": r["trace_ms"]},
     }

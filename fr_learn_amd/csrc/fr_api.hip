@@ -216,12 +216,23 @@ int launchStage(fr_ctx *c, int in, int mode, int step, double rk) {
       }
       if ((err = hipGetLastError()) != hipSuccess) return;
       TimeScope ts(c, 0);
-      if (bnd) hipLaunchKernelGGL((frdev::k_stage<D, P, NF, T, true, true>), dim3(nb), dim3(G::NTH), 0, c->stream, a);
-      else hipLaunchKernelGGL((frdev::k_stage<D, P, NF, T, false, true>), dim3(nb), dim3(G::NTH), 0, c->stream, a);
+      if (bnd) hipLaunchKernelGGL((frdev::k_stage<D, P, NF, T, true, true, false>), dim3(nb), dim3(G::NTH), 0,
+                                  c->stream, a);
+      else hipLaunchKernelGGL((frdev::k_stage<D, P, NF, T, false, true, false>), dim3(nb), dim3(G::NTH), 0,
+                              c->stream, a);
     } else {
       TimeScope ts(c, 0);
-      if (bnd) hipLaunchKernelGGL((frdev::k_stage<D, P, NF, T, true, false>), dim3(nb), dim3(G::NTH), 0, c->stream, a);
-      else hipLaunchKernelGGL((frdev::k_stage<D, P, NF, T, false, false>), dim3(nb), dim3(G::NTH), 0, c->stream, a);
+      if (mode == 2) {
+        if (bnd) hipLaunchKernelGGL((frdev::k_stage<D, P, NF, T, true, false, true>), dim3(nb), dim3(G::NTH), 0,
+                                    c->stream, a);
+        else hipLaunchKernelGGL((frdev::k_stage<D, P, NF, T, false, false, true>), dim3(nb), dim3(G::NTH), 0,
+                                c->stream, a);
+      } else {
+        if (bnd) hipLaunchKernelGGL((frdev::k_stage<D, P, NF, T, true, false, false>), dim3(nb), dim3(G::NTH), 0,
+                                    c->stream, a);
+        else hipLaunchKernelGGL((frdev::k_stage<D, P, NF, T, false, false, false>), dim3(nb), dim3(G::NTH), 0,
+                                c->stream, a);
+      }
     }
     err = hipGetLastError();
   });

@@ -320,15 +320,10 @@ __global__ __launch_bounds__((ElemGeo<D, P>::NTH)) void k_grad(GradArgs<T> a) {
 // terms (LDG): the corrected gradient at the solution points is recomputed
 // here from the state and the neighbours' traces exactly as k_grad forms it,
 // and the viscous common flux reads the gradient traces k_grad wrote.
-// Waves per SIMD the register allocation of k_stage is held to: 4 for the
-// inviscid FP64 kernel (at most 128 VGPRs), the compiler's choice otherwise.
-template <typename T, bool VIS>
-constexpr int stageWaves() {
-  return (!VIS && sizeof(T) == 8) ? 4 : 1;
-}
-
-template <int D, int P, int NF, typename T, bool BND, bool VIS>
-__global__ __launch_bounds__((ElemGeo<D, P>::NTH), (stageWaves<T, VIS>())) void k_stage(StageArgs<T> a) {
+// FIN2 = true is the variant for mode 2 (the final stage of the classic RK
+// schemes), which also prefetches the earlier stages' residuals.
+template <int D, int P, int NF, typename T, bool BND, bool VIS, bool FIN2>
+__global__ __launch_bounds__((ElemGeo<D, P>::NTH)) void k_stage(StageArgs<T> a) {
   using G = ElemGeo<D, P>;
   using O = OneD<P>;
   constexpr int NS = G::NS, NFP = G::NFP, NFPF = G::NFPF, P1 = G::P1, TPE = G::TPE, EPB = G::EPB;
@@ -406,6 +401,18 @@ __global__ __launch_bounds__((ElemGeo<D, P>::NTH), (stageWaves<T, VIS>())) void 
 #pragma unroll
     for (int k = 0; k < NF; k++) pu[r][k] = Ur[k * st];
     if (bndFace) pnb[r] = -1;
+  }
+  // earlier stages' residuals for the final update (issued last: they are
+  // needed last)
+  T pd[FIN2 ? 3 : 1][NF];
+  if constexpr (FIN2) {
+    const T *dp[3] = {a.dprev0, a.dprev1, a.dprev2};
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+      if (r < a.nprev) {
+#pragma unroll
+        for (int k = 0; k < NF; k++) pd[r][k] = dp[r][(ec * NF + k) * NS + tc];
+      }
   }
   if (threadIdx.x < O::SIZE) s1[threadIdx.x] = o1;
   if (act) {
@@ -622,9 +629,20 @@ __global__ __launch_bounds__((ElemGeo<D, P>::NTH), (stageWaves<T, VIS>())) void 
       const double dtJ = pdt / pdet;
       double u = pb[k];
       if (a.mode == 2) {
-        if (a.nprev > 0) u -= a.c0 * (double)a.dprev0[gi] * dtJ;
-        if (a.nprev > 1) u -= a.c1 * (double)a.dprev1[gi] * dtJ;
-        if (a.nprev > 2) u -= a.c2 * (double)a.dprev2[gi] * dtJ;
+        // earlier residuals: prefetched (FIN2) or read here (viscous variant)
+        T r0, r1, r2;
+        if constexpr (FIN2) {
+          r0 = pd[0][k];
+          r1 = pd[1][k];
+          r2 = pd[2][k];
+        } else {
+          r0 = (a.nprev > 0) ? a.dprev0[gi] : (T)0;
+          r1 = (a.nprev > 1) ? a.dprev1[gi] : (T)0;
+          r2 = (a.nprev > 2) ? a.dprev2[gi] : (T)0;
+        }
+        if (a.nprev > 0) u -= a.c0 * (double)r0 * dtJ;
+        if (a.nprev > 1) u -= a.c1 * (double)r1 * dtJ;
+        if (a.nprev > 2) u -= a.c2 * (double)r2 * dtJ;
         const double cl = (a.nprev == 0) ? a.c0 : (a.nprev == 1) ? a.c1 : (a.nprev == 2) ? a.c2 : a.c3;
         u -= cl * (double)dv * dtJ;
       } else {
