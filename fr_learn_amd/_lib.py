@@ -66,6 +66,10 @@ SIGNATURES = {
     "fr_set_dt": (ctypes.c_int, [_P, ctypes.c_double]),
     "fr_calc_dt": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_double, _DP]),
     "fr_get_dt": (ctypes.c_int, [_P, _DP]),
+    "fr_vjp_residual": (ctypes.c_int, [_P, _DP, _DP]),
+    "fr_vjp_step": (ctypes.c_int, [_P, _DP, _DP]),
+    "fr_vjp_steps": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, _DP, _DP]),
+    "fr_reduce_gradient": (ctypes.c_int, [_P, ctypes.c_int, _DP]),
     "fr_set_state": (ctypes.c_int, [_P, _DP, ctypes.c_int]),
     "fr_get_state": (ctypes.c_int, [_P, _DP, ctypes.c_int]),
     "fr_calc_residual": (ctypes.c_int, [_P, ctypes.c_int]),

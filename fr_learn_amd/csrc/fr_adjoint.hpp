@@ -1,0 +1,255 @@
+// fr_adjoint.hpp -- hand-written adjoint (vector-Jacobian product) of the
+// inviscid FR residual, FP64.
+//
+// For a cotangent lam of divF (layout [ele][field][spt]) the kernels return
+// lam^T d divF / dU in the state's layout.  They transpose the residual
+// kernel phase by phase, with the same index maps:
+//   correction  divF[t] += sum_f corr(t, q_f) J[q_f]  ->  lamJ[q] = sum over
+//               the solution points on q's line of corr * lam  (a g-line
+//               contraction, like an extrapolation with g_L / g_R weights);
+//   face flux   J[q] = Fn(U_L, U_R, n_L) * scale - disFn[q]: the cotangent of
+//               the common flux at a face point is g = lamJ * scale of this
+//               element plus that of the neighbour's paired point (both
+//               elements evaluate the same Fn); the derivative with respect
+//               to this element's trace comes from evaluating the common
+//               flux (and a boundary's ghost state) in Dual<NF>;
+//   volume      divF[t] = sum_d Dm F~_d and -disFn = -extrapolation(F~_d):
+//               transposed along the lines, then F~ = JGinv F transposed,
+//               then the flux Jacobian from inviscidFlux in Dual<NF>;
+//   traces      u_o[q] = extrapolation of U: transposed with corrEntries.
+// Element-parallel and gather-only: a first kernel writes every element's
+// scaled face cotangents (the adjoint of the trace exchange), the second
+// reads its neighbours' ones, so no atomics are needed.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "fr_dual.hpp"
+#include "fr_kernels.hpp"
+
+namespace frdev {
+
+// lamJ at flux point q: sum_m g_side[m] * lam[line_q(m)]
+template <int D, int P, int NF>
+__device__ __forceinline__ void lamJAt(const double *s1, const double *lamE, int q, double *out) {
+  using G = ElemGeo<D, P>;
+  using O = OneD<P>;
+  constexpr int NS = G::NS, P1 = G::P1;
+  int base, stride, side, dn;
+  fptLine<D, P>(q, base, stride, side, dn);
+  const double *g = s1 + (side ? O::GR : O::GL);
+#pragma unroll
+  for (int k = 0; k < NF; k++) {
+    double acc = 0;
+#pragma unroll
+    for (int m = 0; m < P1; m++) acc += g[m] * lamE[k * NS + base + m * stride];
+    out[k] = acc;
+  }
+}
+
+// (a) scaled face cotangents A[ele][field][fpt] = lamJ * (+dA left / boundary,
+// -dA right), one thread per (element, flux point)
+template <int D, int P, int NF>
+__global__ __launch_bounds__(256) void k_vjp_faces(const double *__restrict__ lam, const double *__restrict__ fgeo,
+                                                   const int8_t *__restrict__ finfo,
+                                                   const double *__restrict__ oned, int nEles,
+                                                   double *__restrict__ A) {
+  using G = ElemGeo<D, P>;
+  using O = OneD<P>;
+  constexpr int NS = G::NS, NFP = G::NFP, NFPF = G::NFPF, NFACE = G::NFACE;
+  __shared__ double s1[O::SIZE];
+  for (int i = threadIdx.x; i < O::SIZE; i += blockDim.x) s1[i] = oned[i];
+  __syncthreads();
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)nEles * NFP) return;
+  const int e = (int)(idx / NFP), q = (int)(idx - (long)e * NFP);
+  double lj[NF];
+  lamJAt<D, P, NF>(s1, lam + (size_t)e * NF * NS, q, lj);
+  const int fi = finfo[(size_t)e * NFACE + q / NFPF];
+  const double dA = fgeo[((size_t)e * (D + 1) + D) * NFP + q];
+  const bool right = fi < FINFO_BOUNDARY && ((fi >> 5) & 1);
+  const double scale = right ? -dA : dA;
+#pragma unroll
+  for (int k = 0; k < NF; k++) A[((size_t)e * NF + k) * NFP + q] = lj[k] * scale;
+}
+
+// (b) the state cotangent, one TPE-thread group per element
+template <int D, int P, int NF, bool BND>
+__global__ __launch_bounds__((ElemGeo<D, P>::NTH)) void k_vjp_state(
+    const double *__restrict__ U, const double *__restrict__ lam, const double *__restrict__ Uf,
+    const double *__restrict__ A, const double *__restrict__ JG, const double *__restrict__ fgeo,
+    const int32_t *__restrict__ fnbr, const int8_t *__restrict__ finfo, const double *__restrict__ oned, Phys ph,
+    int nEles, double *__restrict__ out) {
+  using G = ElemGeo<D, P>;
+  using O = OneD<P>;
+  using DN = Dual<NF>;
+  constexpr int NS = G::NS, NFP = G::NFP, NFPF = G::NFPF, P1 = G::P1, TPE = G::TPE, EPB = G::EPB;
+  constexpr int NFACE = G::NFACE;
+  __shared__ double sU[EPB][NF][NS];
+  __shared__ double sL[EPB][NF][NS];
+  __shared__ double sJ[EPB][NF][NFP];   // lamJ
+  __shared__ double sUo[EPB][NF][NFP];  // cotangent of this element's traces
+  __shared__ double s1[O::SIZE];
+  const int slot = threadIdx.x / TPE, t = threadIdx.x - slot * TPE;
+  const int e = blockIdx.x * EPB + slot;
+  const bool act = e < nEles;
+  for (int i = threadIdx.x; i < O::SIZE; i += blockDim.x) s1[i] = oned[i];
+  if (act)
+    for (int i = t; i < NF * NS; i += TPE) {
+      (&sU[slot][0][0])[i] = U[(size_t)e * NF * NS + i];
+      (&sL[slot][0][0])[i] = lam[(size_t)e * NF * NS + i];
+    }
+  __syncthreads();
+  // face points: lamJ, the flux cotangent g and its pull-back to this trace
+  if (act) {
+    for (int q = t; q < NFP; q += TPE) {
+      double lj[NF];
+      lamJAt<D, P, NF>(s1, &sL[slot][0][0], q, lj);
+      int base, stride, side, dn;
+      fptLine<D, P>(q, base, stride, side, dn);
+      const double *l1 = s1 + (side ? O::LR : O::LL);
+      DN uo[NF], un[NF], n[D], Fn[NF];
+#pragma unroll
+      for (int k = 0; k < NF; k++) {
+        double acc = 0;
+#pragma unroll
+        for (int m = 0; m < P1; m++) acc += l1[m] * sU[slot][k][base + m * stride];
+        uo[k] = DN::var(acc, k);
+        sJ[slot][k][q] = lj[k];
+      }
+#pragma unroll
+      for (int c = 0; c < D; c++) n[c] = DN(fgeo[((size_t)e * (D + 1) + c) * NFP + q]);
+      const double dA = fgeo[((size_t)e * (D + 1) + D) * NFP + q];
+      const int fi = finfo[(size_t)e * NFACE + q / NFPF];
+      const bool bnd = BND && fi >= FINFO_BOUNDARY;
+      double g[NF];
+      if (bnd) {
+        boundaryState<D, NF, DN>(ph, fi - FINFO_BOUNDARY, uo, n, un);
+        commonFlux<D, NF, DN>(ph, true, uo, un, n, Fn);
+#pragma unroll
+        for (int k = 0; k < NF; k++) g[k] = lj[k] * dA;
+      } else {
+        const int en = fnbr[(size_t)e * NFACE + q / NFPF];
+        const int qn = neighbourFpt<D, P>(q, fi);
+        const bool right = (fi >> 5) & 1;
+#pragma unroll
+        for (int k = 0; k < NF; k++) {
+          un[k] = DN(Uf[((size_t)en * NF + k) * NFP + qn]);
+          g[k] = lj[k] * (right ? -dA : dA) + A[((size_t)en * NF + k) * NFP + qn];
+        }
+        if (right) commonFlux<D, NF, DN>(ph, false, un, uo, n, Fn);
+        else commonFlux<D, NF, DN>(ph, false, uo, un, n, Fn);
+      }
+#pragma unroll
+      for (int j = 0; j < NF; j++) {
+        double acc = 0;
+#pragma unroll
+        for (int k = 0; k < NF; k++) acc += g[k] * Fn[k].d[j];
+        sUo[slot][j][q] = acc;
+      }
+    }
+  }
+  __syncthreads();
+  if (!act || t >= NS) return;
+  int si, sj, sk;
+  if (D == 3) {
+    sk = t / (P1 * P1);
+    sj = (t - P1 * P1 * sk) / P1;
+    si = t - P1 * sj - P1 * P1 * sk;
+  } else {
+    si = t % P1;
+    sj = t / P1;
+    sk = 0;
+  }
+  int cq[NFACE], cside[NFACE], cax[NFACE];
+  corrEntries<D, P>(t, cq, cside, cax);
+  // cotangent of the contravariant fluxes F~_d at this point
+  double lFt[D][NF];
+#pragma unroll
+  for (int d = 0; d < D; d++) {
+    const int id = (d == 0) ? si : (d == 1) ? sj : sk;
+#pragma unroll
+    for (int k = 0; k < NF; k++) {
+      double acc = 0;
+#pragma unroll
+      for (int m = 0; m < P1; m++) {
+        const int sp = (d == 0) ? m + P1 * (sj + P1 * sk) : (d == 1) ? si + P1 * (m + P1 * sk) : si + P1 * (sj + P1 * m);
+        acc += s1[O::DM + m * P1 + id] * sL[slot][k][sp];
+      }
+#pragma unroll
+      for (int f = 0; f < NFACE; f++)
+        if (faceAxis<D>(f) == d)
+          acc -= (cside[f] ? 1.0 : -1.0) * s1[(cside[f] ? O::LR : O::LL) + cax[f]] * sJ[slot][k][cq[f]];
+      lFt[d][k] = acc;
+    }
+  }
+  // physical fluxes: F~_d = sum_d2 JG(d, d2) F_d2
+  double lF[D][NF];
+#pragma unroll
+  for (int d2 = 0; d2 < D; d2++)
+#pragma unroll
+    for (int k = 0; k < NF; k++) {
+      double acc = 0;
+#pragma unroll
+      for (int d = 0; d < D; d++) acc += JG[((size_t)e * D * D + d * D + d2) * NS + t] * lFt[d][k];
+      lF[d2][k] = acc;
+    }
+  DN u[NF], F[D][NF];
+#pragma unroll
+  for (int k = 0; k < NF; k++) u[k] = DN::var(sU[slot][k][t], k);
+  inviscidFlux<D, NF, DN>(ph, u, F);
+#pragma unroll
+  for (int j = 0; j < NF; j++) {
+    double acc = 0;
+#pragma unroll
+    for (int d2 = 0; d2 < D; d2++)
+#pragma unroll
+      for (int k = 0; k < NF; k++) acc += lF[d2][k] * F[d2][k].d[j];
+    // trace extrapolation, transposed
+#pragma unroll
+    for (int f = 0; f < NFACE; f++) acc += s1[(cside[f] ? O::LR : O::LL) + cax[f]] * sUo[slot][j][cq[f]];
+    out[((size_t)e * NF + j) * NS + t] = acc;
+  }
+}
+
+// out = alpha * (dt / detJ) * x + beta * (dt / detJ) * y  (y may be null)
+__global__ void k_dtj_combine(const double *__restrict__ x, const double *__restrict__ y,
+                              const double *__restrict__ detJ, const double *__restrict__ dtE, int dtStride,
+                              double alpha, double beta, int NF, int NS, long n, double *__restrict__ out) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const long e = i / ((long)NF * NS);
+  const int s = (int)(i % NS);
+  const double dtJ = dtE[e * dtStride] / detJ[e * NS + s];
+  double v = alpha * x[i] * dtJ;
+  if (y) v += beta * y[i] * dtJ;
+  out[i] = v;
+}
+
+// acc += x
+__global__ void k_accumulate(double *__restrict__ acc, const double *__restrict__ x, long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) acc[i] += x[i];
+}
+
+// gradient of the kinetic energy sum_e sum_s w_s detJ |m|^2 / (2 rho) with
+// respect to U: (-|m|^2 / (2 rho^2), m / rho, 0) * w_s detJ
+template <int D>
+__global__ void k_ke_grad(const double *__restrict__ U, const double *__restrict__ detJ,
+                          const double *__restrict__ wts, int NS, long nPts, double *__restrict__ out) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nPts) return;
+  const long e = i / NS;
+  const int s = (int)(i % NS);
+  const double *Ue = U + (size_t)e * (D + 2) * NS;
+  double *Oe = out + (size_t)e * (D + 2) * NS;
+  const double w = wts[s] * detJ[i];
+  const double rho = Ue[s];
+  double m2 = 0;
+  for (int d = 0; d < D; d++) m2 += Ue[(d + 1) * NS + s] * Ue[(d + 1) * NS + s];
+  Oe[s] = -0.5 * m2 / (rho * rho) * w;
+  for (int d = 0; d < D; d++) Oe[(d + 1) * NS + s] = Ue[(d + 1) * NS + s] / rho * w;
+  Oe[(D + 1) * NS + s] = 0.0;
+}
+
+}  // namespace frdev

@@ -11,6 +11,7 @@
 
 #include "../../include/fr.h"
 #include "fr_host.hpp"
+#include "fr_adjoint.hpp"
 #include "fr_kernels.hpp"
 
 namespace {
@@ -69,6 +70,10 @@ struct fr_ctx {
   double *dtG = nullptr;              // the global time step (one value)
   double *dtE = nullptr;              // per-element time steps (local time stepping)
   int dtType = 0;                     // 0 fixed dt, 1 global CFL dt, 2 local CFL dt
+  // adjoint work buffers (FP64, allocated on first use)
+  double *adjA = nullptr, *adjTr = nullptr, *adjTr2 = nullptr;  // face cotangents, trace scratch
+  double *adjLam = nullptr, *adjOut = nullptr, *adjNu = nullptr, *adjW[2] = {nullptr, nullptr};
+  double *adjS[4] = {nullptr, nullptr, nullptr, nullptr};       // RK stage states
   int32_t *fnbr = nullptr;
   int8_t *finfo = nullptr;
   size_t partCap = 0;
@@ -92,7 +97,9 @@ void freeAll(fr_ctx *c) {
   void *bufs[] = {c->U,       c->U0,      c->detJ,    c->JG64,    c->fgeo64, c->ownN,  c->wts,
                   c->oned64,  c->part,    c->U32,     c->JG32,    c->fgeo32, c->oned32, c->Us,
                   c->Uf[0],   c->Uf[1],   c->divF[0], c->divF[1], c->divF[2], c->divF[3], c->fnbr,
-                  c->finfo,   c->dUf,     c->dtG,     c->dtE};
+                  c->finfo,   c->dUf,     c->dtG,     c->dtE,     c->adjA,   c->adjTr, c->adjTr2,
+                  c->adjLam,  c->adjOut,  c->adjNu,   c->adjW[0], c->adjW[1], c->adjS[0], c->adjS[1],
+                  c->adjS[2], c->adjS[3]};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   for (auto &v : c->ev)
@@ -163,6 +170,9 @@ int ensureTraces(fr_ctx *c, int which) {
   return 0;
 }
 
+template <typename T>
+int launchStageArgs(fr_ctx *c, const frdev::StageArgs<T> &a, int mode);
+
 // One launch of k_stage.  in: 0 = U_spts, 1 = Us.  mode as in k_stage.
 template <typename T>
 int launchStage(fr_ctx *c, int in, int mode, int step, double rk) {
@@ -199,6 +209,19 @@ int launchStage(fr_ctx *c, int in, int mode, int step, double rk) {
   a.nprev = step;
   a.nEles = c->nE;
   a.ph = c->ph;
+  int rc2 = launchStageArgs<T>(c, a, mode);
+  if (rc2) return rc2;
+  if (mode != 0) {
+    c->cur ^= 1;
+    c->traceOf = (mode == 1) ? 1 : 0;
+  }
+  return 0;
+}
+
+// Launch of the stage kernel variant for `mode` with arguments `a` (and the
+// gradient kernel before it on the viscous path).
+template <typename T>
+int launchStageArgs(fr_ctx *c, const frdev::StageArgs<T> &a, int mode) {
   const bool bnd = c->conn.nIntFaces < (int)c->conn.faces.size();
   const bool vis = c->cfg.viscous != 0;
   hipError_t err = hipSuccess;
@@ -238,10 +261,6 @@ int launchStage(fr_ctx *c, int in, int mode, int step, double rk) {
   });
   if (!ok) return fail(FR_EINVAL, "no kernel instance for this (nDims, order, nFields)");
   if (err != hipSuccess) return fail(FR_EHIP, std::string("stage launch: ") + hipGetErrorString(err));
-  if (mode != 0) {
-    c->cur ^= 1;
-    c->traceOf = (mode == 1) ? 1 : 0;
-  }
   return 0;
 }
 
@@ -303,6 +322,120 @@ int syncU32(fr_ctx *c) {
   long n = (long)c->nState();
   hipLaunchKernelGGL(frdev::k_to_float, dim3(nblk(n)), dim3(256), 0, c->stream, c->U, c->U32, n);
   HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// adjoint (FP64 contexts, inviscid and advection)
+// ---------------------------------------------------------------------------
+int adjCheck(fr_ctx *c) {
+  if (c->prec != FR_PREC_FP64) return fail(FR_EINVAL, "the adjoint needs an FP64 context");
+  if (c->cfg.viscous) return fail(FR_EINVAL, "the adjoint of the viscous residual is not available in this build");
+  const size_t nSt = c->nState(), nTr = (size_t)c->nE * c->NF * c->NFP;
+  int rc;
+  if (!c->adjA) {
+    if ((rc = dmalloc(&c->adjA, nTr)) || (rc = dmalloc(&c->adjTr, nTr)) || (rc = dmalloc(&c->adjTr2, nTr)) ||
+        (rc = dmalloc(&c->adjLam, nSt)) || (rc = dmalloc(&c->adjOut, nSt)) || (rc = dmalloc(&c->adjNu, nSt)) ||
+        (rc = dmalloc(&c->adjW[0], nSt)) || (rc = dmalloc(&c->adjW[1], nSt)))
+      return rc;
+    for (int s = 1; s < c->nRK; s++)
+      if ((rc = dmalloc(&c->adjS[s], nSt))) return rc;
+  }
+  return 0;
+}
+
+// out = (d divF / dU)^T lam at the state Us (device arrays, device layout)
+int vjpResidualDev(fr_ctx *c, const double *Us, const double *lam, double *out) {
+  const bool bnd = c->conn.nIntFaces < (int)c->conn.faces.size();
+  hipError_t err = hipSuccess;
+  bool ok = dispatch(c->D, c->P, c->NF, [&](auto Dc, auto Pc, auto NFc) {
+    constexpr int D = decltype(Dc)::value, P = decltype(Pc)::value, NF = decltype(NFc)::value;
+    using G = frdev::ElemGeo<D, P>;
+    const long nq = (long)c->nE * G::NFP;
+    hipLaunchKernelGGL((frdev::k_trace<D, P, NF, double>), dim3(nblk(nq)), dim3(256), 0, c->stream, Us, c->adjTr,
+                       c->oned64, c->nE);
+    hipLaunchKernelGGL((frdev::k_vjp_faces<D, P, NF>), dim3(nblk(nq)), dim3(256), 0, c->stream, lam, c->fgeo64,
+                       c->finfo, c->oned64, c->nE, c->adjA);
+    const unsigned nb = (unsigned)((c->nE + G::EPB - 1) / G::EPB);
+    if (bnd)
+      hipLaunchKernelGGL((frdev::k_vjp_state<D, P, NF, true>), dim3(nb), dim3(G::NTH), 0, c->stream, Us, lam,
+                         c->adjTr, c->adjA, c->JG64, c->fgeo64, c->fnbr, c->finfo, c->oned64, c->ph, c->nE, out);
+    else
+      hipLaunchKernelGGL((frdev::k_vjp_state<D, P, NF, false>), dim3(nb), dim3(G::NTH), 0, c->stream, Us, lam,
+                         c->adjTr, c->adjA, c->JG64, c->fgeo64, c->fnbr, c->finfo, c->oned64, c->ph, c->nE, out);
+    err = hipGetLastError();
+  });
+  if (!ok) return fail(FR_EINVAL, "no kernel instance for this (nDims, order, nFields)");
+  if (err != hipSuccess) return fail(FR_EHIP, std::string("adjoint launch: ") + hipGetErrorString(err));
+  return 0;
+}
+
+// The stage states of one RK step from U (timeStepA, solver.cpp:307-345):
+// S[0] = U, S[s+1] = U - RKa[s+1] R(S[s]) dt / detJ, with the stage kernel.
+int stageStates(fr_ctx *c, const double **S) {
+  S[0] = c->U;
+  for (int s = 0; s + 1 < c->nRK; s++) {
+    dispatch(c->D, c->P, c->NF, [&](auto Dc, auto Pc, auto NFc) {
+      constexpr int D = decltype(Dc)::value, P = decltype(Pc)::value, NF = decltype(NFc)::value;
+      const long nq = (long)c->nE * frdev::ElemGeo<D, P>::NFP;
+      hipLaunchKernelGGL((frdev::k_trace<D, P, NF, double>), dim3(nblk(nq)), dim3(256), 0, c->stream, S[s],
+                         c->adjTr, c->oned64, c->nE);
+    });
+    HIP_TRY(hipGetLastError());
+    frdev::StageArgs<double> a{};
+    a.Uin = S[s];
+    a.UfIn = c->adjTr;
+    a.UfOut = c->adjTr2;
+    a.JG = c->JG64;
+    a.fgeo = c->fgeo64;
+    a.fnbr = c->fnbr;
+    a.finfo = c->finfo;
+    a.oned = c->oned64;
+    a.divF = nullptr;
+    a.Ubase = c->U;
+    a.detJ = c->detJ;
+    a.Unext = c->adjS[s + 1];
+    a.Uout = nullptr;
+    a.Uout32 = nullptr;
+    a.dprev0 = a.dprev1 = a.dprev2 = c->adjOut;
+    a.rk = c->RKa[s + 1];
+    a.dtE = dtSource(c).first;
+    a.dtStride = dtSource(c).second;
+    a.mode = 1;
+    a.nprev = s;
+    a.nEles = c->nE;
+    a.ph = c->ph;
+    int rc = launchStageArgs<double>(c, a, 1);
+    if (rc) return rc;
+    S[s + 1] = c->adjS[s + 1];
+  }
+  return 0;
+}
+
+// lam0 = (d U_{n+1} / d U_n)^T lam1 for one update at the current U:
+// nu_s = -c_s (dt/detJ) lam1 - RKa[s+1] (dt/detJ) w_{s+1},
+// w_s = (dR/dU at S[s])^T nu_s, lam0 = lam1 + sum_s w_s, where c_s is the
+// weight of R_s in the final update (RKb, or RKa[nRK] on the last stage of
+// the Jameson schemes).
+int vjpStepDev(fr_ctx *c, const double *lam1, double *lam0) {
+  const double *S[4];
+  int rc = stageStates(c, S);
+  if (rc) return rc;
+  const long n = (long)c->nState();
+  HIP_TRY(hipMemcpyAsync(lam0, lam1, n * 8, hipMemcpyDeviceToDevice, c->stream));
+  const double *wNext = nullptr;
+  for (int s = c->nRK - 1; s >= 0; s--) {
+    const double cs = (c->cfg.timeType < 5) ? c->RKb[s] : (s == c->nRK - 1 ? c->RKa[c->nRK] : 0.0);
+    const double an = (s + 1 < c->nRK) ? c->RKa[s + 1] : 0.0;
+    hipLaunchKernelGGL(frdev::k_dtj_combine, dim3(nblk(n)), dim3(256), 0, c->stream, lam1, wNext, c->detJ,
+                       dtSource(c).first, dtSource(c).second, -cs, -an, c->NF, c->NS, n, c->adjNu);
+    HIP_TRY(hipGetLastError());
+    double *w = c->adjW[s & 1];
+    if ((rc = vjpResidualDev(c, S[s], c->adjNu, w))) return rc;
+    hipLaunchKernelGGL(frdev::k_accumulate, dim3(nblk(n)), dim3(256), 0, c->stream, lam0, w, n);
+    HIP_TRY(hipGetLastError());
+    wNext = w;
+  }
   return 0;
 }
 
@@ -838,6 +971,117 @@ int fr_get_dt(fr_ctx *c, double *dtEle) {
   std::vector<double> h(c->nE);
   HIP_TRY(hipMemcpy(h.data(), c->dtE, h.size() * 8, hipMemcpyDeviceToHost));
   for (int e = 0; e < c->nE; e++) dtEle[e] = h[c->iperm[e]];
+  return 0;
+}
+
+int fr_vjp_residual(fr_ctx *c, const double *lam, double *out) {
+  if (!c || !lam || !out) return fail(FR_EINVAL, "null argument");
+  int rc = adjCheck(c);
+  if (rc) return rc;
+  const size_t n = c->nState();
+  std::vector<double> h(n);
+  refToDev(c, lam, h.data(), c->NF, c->NS);
+  HIP_TRY(hipMemcpyAsync(c->adjLam, h.data(), n * 8, hipMemcpyHostToDevice, c->stream));
+  if ((rc = vjpResidualDev(c, c->U, c->adjLam, c->adjOut))) return rc;
+  HIP_TRY(hipMemcpyAsync(h.data(), c->adjOut, n * 8, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  devToRef(c, h.data(), out, c->NF, c->NS);
+  return 0;
+}
+
+int fr_vjp_step(fr_ctx *c, const double *lam1, double *lam0) {
+  if (!c || !lam1 || !lam0) return fail(FR_EINVAL, "null argument");
+  int rc = adjCheck(c);
+  if (rc) return rc;
+  const size_t n = c->nState();
+  std::vector<double> h(n);
+  refToDev(c, lam1, h.data(), c->NF, c->NS);
+  HIP_TRY(hipMemcpyAsync(c->adjLam, h.data(), n * 8, hipMemcpyHostToDevice, c->stream));
+  if ((rc = vjpStepDev(c, c->adjLam, c->adjOut))) return rc;
+  HIP_TRY(hipMemcpyAsync(h.data(), c->adjOut, n * 8, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  devToRef(c, h.data(), lam0, c->NF, c->NS);
+  return 0;
+}
+
+int fr_vjp_steps(fr_ctx *c, int nSteps, int every, const double *lamN, double *lam0) {
+  if (!c || !lamN || !lam0) return fail(FR_EINVAL, "null argument");
+  if (nSteps < 1 || every < 1) return fail(FR_EINVAL, "nSteps and the checkpoint interval must be positive");
+  int rc = adjCheck(c);
+  if (rc) return rc;
+  const size_t n = c->nState();
+  const int nCk = (nSteps + every - 1) / every;
+  std::vector<double *> ck(nCk, nullptr), seg(every, nullptr);
+  double *uN = nullptr;
+  auto cleanup = [&]() {
+    for (double *p : ck) if (p) (void)hipFree(p);
+    for (double *p : seg) if (p) (void)hipFree(p);
+    if (uN) (void)hipFree(uN);
+  };
+  auto copy = [&](double *dst, const double *src) {
+    return hipMemcpyAsync(dst, src, n * 8, hipMemcpyDeviceToDevice, c->stream);
+  };
+  for (int i = 0; i < nCk; i++)
+    if ((rc = dmalloc(&ck[i], n))) { cleanup(); return rc; }
+  for (int i = 0; i < every; i++)
+    if ((rc = dmalloc(&seg[i], n))) { cleanup(); return rc; }
+  if ((rc = dmalloc(&uN, n))) { cleanup(); return rc; }
+  // forward sweep, keeping the state at every `every`-th step
+  for (int i = 0; i < nSteps; i++) {
+    if (i % every == 0 && copy(ck[i / every], c->U) != hipSuccess) { cleanup(); return fail(FR_EHIP, "copy"); }
+    if ((rc = updateOnce(c))) { cleanup(); return rc; }
+  }
+  if (copy(uN, c->U) != hipSuccess) { cleanup(); return fail(FR_EHIP, "copy"); }
+  // reverse sweep, recomputing each segment's step states from its checkpoint
+  std::vector<double> h(n);
+  refToDev(c, lamN, h.data(), c->NF, c->NS);
+  HIP_TRY(hipMemcpyAsync(c->adjLam, h.data(), n * 8, hipMemcpyHostToDevice, c->stream));
+  double *lamA = c->adjLam, *lamB = c->adjOut;
+  for (int k = nCk - 1; k >= 0; k--) {
+    const int start = k * every, end = std::min(start + every, nSteps);
+    if (copy(c->U, ck[k]) != hipSuccess) { cleanup(); return fail(FR_EHIP, "copy"); }
+    c->traceOf = -1;
+    for (int i = start; i < end; i++) {
+      if (copy(seg[i - start], c->U) != hipSuccess) { cleanup(); return fail(FR_EHIP, "copy"); }
+      if (i + 1 < end && (rc = updateOnce(c))) { cleanup(); return rc; }
+    }
+    for (int i = end - 1; i >= start; i--) {
+      if (copy(c->U, seg[i - start]) != hipSuccess) { cleanup(); return fail(FR_EHIP, "copy"); }
+      if ((rc = vjpStepDev(c, lamA, lamB))) { cleanup(); return rc; }
+      std::swap(lamA, lamB);
+    }
+  }
+  HIP_TRY(hipMemcpyAsync(h.data(), lamA, n * 8, hipMemcpyDeviceToHost, c->stream));
+  if (copy(c->U, uN) != hipSuccess) { cleanup(); return fail(FR_EHIP, "copy"); }
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  c->traceOf = -1;
+  cleanup();
+  devToRef(c, h.data(), lam0, c->NF, c->NS);
+  return syncU32(c);
+}
+
+int fr_reduce_gradient(fr_ctx *c, int kind, double *out) {
+  if (!c || !out) return fail(FR_EINVAL, "null argument");
+  if (kind != 10) return fail(FR_EINVAL, "only the kinetic-energy reduction (10) has a gradient");
+  if (c->cfg.equation != 1) return fail(FR_EINVAL, "kinetic energy needs the Euler / Navier-Stokes equations");
+  const size_t n = c->nState();
+  double *g;
+  int rc = dmalloc(&g, n);
+  if (rc) return rc;
+  const long nPts = (long)c->nE * c->NS;
+  if (c->D == 3)
+    hipLaunchKernelGGL(frdev::k_ke_grad<3>, dim3(nblk(nPts)), dim3(256), 0, c->stream, c->U, c->detJ, c->wts, c->NS,
+                       nPts, g);
+  else
+    hipLaunchKernelGGL(frdev::k_ke_grad<2>, dim3(nblk(nPts)), dim3(256), 0, c->stream, c->U, c->detJ, c->wts, c->NS,
+                       nPts, g);
+  std::vector<double> h(n);
+  hipError_t e1 = hipGetLastError();
+  hipError_t e2 = hipMemcpyAsync(h.data(), g, n * 8, hipMemcpyDeviceToHost, c->stream);
+  hipError_t e3 = hipStreamSynchronize(c->stream);
+  (void)hipFree(g);
+  if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) return fail(FR_EHIP, "kinetic-energy gradient");
+  devToRef(c, h.data(), out, c->NF, c->NS);
   return 0;
 }
 

@@ -169,6 +169,34 @@ class Solver:
         _lib.check(self._lib.fr_get_dt(self._h, _lib.dptr(out)))
         return out
 
+    def vjp_residual(self, lam):
+        """(d divF / dU)^T lam at the current state (layout [spt][ele][field])."""
+        lam = np.ascontiguousarray(lam, dtype=np.float64).reshape(self.state_shape())
+        out = np.zeros(self.state_shape())
+        _lib.check(self._lib.fr_vjp_residual(self._h, _lib.dptr(lam), _lib.dptr(out)))
+        return out
+
+    def vjp_step(self, lam1):
+        """(d U_{n+1} / d U_n)^T lam1 for one update from the current state."""
+        lam1 = np.ascontiguousarray(lam1, dtype=np.float64).reshape(self.state_shape())
+        out = np.zeros(self.state_shape())
+        _lib.check(self._lib.fr_vjp_step(self._h, _lib.dptr(lam1), _lib.dptr(out)))
+        return out
+
+    def vjp_steps(self, n, every, lamN):
+        """Advance n updates (the state ends at U_n) and return
+        (d U_n / d U_0)^T lamN, with a checkpoint every `every` steps."""
+        lamN = np.ascontiguousarray(lamN, dtype=np.float64).reshape(self.state_shape())
+        out = np.zeros(self.state_shape())
+        _lib.check(self._lib.fr_vjp_steps(self._h, int(n), int(every), _lib.dptr(lamN), _lib.dptr(out)))
+        return out
+
+    def kinetic_energy_gradient(self):
+        """d(kinetic energy)/dU at the current state."""
+        out = np.zeros(self.state_shape())
+        _lib.check(self._lib.fr_reduce_gradient(self._h, 10, _lib.dptr(out)))
+        return out
+
     def element_order(self):
         """order[i] = mesh element at device position i (FR_LAYOUT_DEVICE order)."""
         out = np.zeros(self.nEles, dtype=np.int32)

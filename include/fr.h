@@ -158,6 +158,23 @@ int fr_set_dt(fr_ctx *ctx, double dt);
 int fr_calc_dt(fr_ctx *ctx, int dtType, double CFL, double *dtMin);
 int fr_get_dt(fr_ctx *ctx, double *dtEle /* nEles */);
 
+/* Adjoint (no reference counterpart: FlurryPP has no adjoint).  FP64,
+ * inviscid and advection residuals; FR_EINVAL otherwise.  Arrays in the
+ * reference layout.
+ * fr_vjp_residual: out = (d divF / dU)^T lam at the current U_spts
+ *   (the vector-Jacobian product of solver::calcResidual).
+ * fr_vjp_step: lam0 = (d U_{n+1} / d U_n)^T lam1 for one solver::update
+ *   from the current U_spts (stage states recomputed; U_spts unchanged).
+ * fr_vjp_steps: nSteps updates from the current U_spts (which ends at U_N,
+ *   as fr_update(nSteps)), then the reverse sweep lam0 = (dU_N/dU_0)^T lamN,
+ *   keeping a state every `every` steps and recomputing each segment.
+ * fr_reduce_gradient(kind 10): d(kinetic energy)/dU_spts, for objectives
+ *   J = KE(U_N). */
+int fr_vjp_residual(fr_ctx *ctx, const double *lam, double *out);
+int fr_vjp_step(fr_ctx *ctx, const double *lam1, double *lam0);
+int fr_vjp_steps(fr_ctx *ctx, int nSteps, int every, const double *lamN, double *lam0);
+int fr_reduce_gradient(fr_ctx *ctx, int kind, double *out);
+
 /* Stream control.  fr_set_stream(ctx, (void*)hipStream_t) makes the context
  * launch on the caller's stream (NULL = the context's own stream). */
 int fr_set_stream(fr_ctx *ctx, void *stream);
