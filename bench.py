@@ -61,10 +61,21 @@ def run_gpu(cfg, m, steps, warmup, precision):
     res_ms, res_n = s.kernel_times(0)
     tr_ms, tr_n = s.kernel_times(1)
     rk_ms, rk_n = s.kernel_times(2)
+    state = s.get_state()
+    # residual evaluations alone (solver::calcResidual): the stage kernel
+    # without the RK update and trace output
+    for _ in range(3):
+        s.calc_residual(0)
+    s.synchronize()
+    s.kernel_times(0)
+    for _ in range(20):
+        s.calc_residual(0)
+    s.synchronize()
+    r0_ms, r0_n = s.kernel_times(0)
     s.set_timing(False)
     out = dict(wall=wall, res_ms=res_ms / max(res_n, 1), res_n=res_n, trace_ms=tr_ms / max(tr_n, 1),
                rk_ms=rk_ms / max(rk_n, 1), bytes=s.residual_bytes(), nEles=s.nEles, nSpts=s.nSpts,
-               state=s.get_state())
+               state=state, res_only_ms=r0_ms / max(r0_n, 1), res_only_n=r0_n)
     s.close()
     return out
 
@@ -205,6 +216,13 @@ def main():
             "copy_GBps": copy_gbps,
         },
         "kernels_ms": {"stage": r["res_ms"], "trace_init": r["trace_ms"]},
+        "residual_only": {
+            "note": "calcResidual launches alone (no RK update, no trace output), same kernel, mode 0",
+            "avg_launch_ms": r["res_only_ms"], "launches": r["res_only_n"],
+            "achieved": r["bytes"] / (r["res_only_ms"] * 1e-3) / 1e9, "unit": "GB/s",
+            "frac": r["bytes"] / (r["res_only_ms"] * 1e-3) / HBM_PEAK,
+            "gdof_per_s": dof / (r["res_only_ms"] * 1e-3) / 1e9,
+        },
     }
     if not args.no_fp32 and main_prec == "fp64":
         r32 = run_gpu(cfg, m, args.steps, args.warmup, "fp32")

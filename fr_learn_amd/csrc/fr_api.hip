@@ -190,6 +190,7 @@ int launchStage(fr_ctx *c, int in, int mode, int step, double rk) {
   a.oned = f32 ? (const T *)c->oned32 : (const T *)c->oned64;
   a.divF = (T *)c->divF[step];
   a.Ubase = c->U;
+  a.UbaseT = f32 ? (const T *)c->U32 : (const T *)c->U;
   a.detJ = c->detJ;
   a.Unext = (T *)c->Us;
   a.Uout = c->U;
@@ -245,7 +246,7 @@ int launchStageArgs(fr_ctx *c, const frdev::StageArgs<T> &a, int mode) {
                               c->stream, a);
     } else {
       TimeScope ts(c, 0);
-      if (mode == 2) {
+      if (mode >= 2) {
         if (bnd) hipLaunchKernelGGL((frdev::k_stage<D, P, NF, T, true, false, true>), dim3(nb), dim3(G::NTH), 0,
                                     c->stream, a);
         else hipLaunchKernelGGL((frdev::k_stage<D, P, NF, T, false, false, true>), dim3(nb), dim3(G::NTH), 0,
@@ -393,6 +394,7 @@ int stageStates(fr_ctx *c, const double **S) {
     a.oned = c->oned64;
     a.divF = nullptr;
     a.Ubase = c->U;
+    a.UbaseT = c->U;
     a.detJ = c->detJ;
     a.Unext = c->adjS[s + 1];
     a.Uout = nullptr;

@@ -71,7 +71,8 @@ __global__ __launch_bounds__(256) void k_trace(const T *__restrict__ U, T *__res
 //   (timeStepA / timeStepB, solver.cpp:307-385) and the face traces of the
 //   updated state for the next stage (extrapolateU, solver.cpp:401-412).
 // mode 0: residual only (divF out).
-// mode 1: stage update Unext = Ubase - rk * divF / detJ * dt.
+// mode 1: stage update Unext = UbaseT - rk * divF / detJ * dt (the base in
+//         the residual precision: the FP32 path's stage states are FP32).
 // mode 2: final update U = Ubase - sum_r RKb[r] divF_r / detJ * dt, the
 //         subtractions in stage order as the timeStepB passes do.
 // mode 3: final Jameson-type update U = Ubase - rk * divF / detJ * dt.
@@ -89,7 +90,8 @@ struct StageArgs {
   const int8_t *__restrict__ finfo;  // [ele][face] packed face record
   const T *__restrict__ oned;
   T *divF;
-  const double *Ubase;
+  const double *Ubase;  // FP64 step base state (final stages)
+  const T *UbaseT;      // the same in the residual precision (intermediate stages)
   const double *__restrict__ detJ;
   T *Unext;
   double *Uout;
@@ -320,9 +322,10 @@ __global__ __launch_bounds__((ElemGeo<D, P>::NTH)) void k_grad(GradArgs<T> a) {
 // terms (LDG): the corrected gradient at the solution points is recomputed
 // here from the state and the neighbours' traces exactly as k_grad forms it,
 // and the viscous common flux reads the gradient traces k_grad wrote.
-// FIN2 = true is the variant for mode 2 (the final stage of the classic RK
-// schemes), which also prefetches the earlier stages' residuals.
-template <int D, int P, int NF, typename T, bool BND, bool VIS, bool FIN2>
+// FIN = true is the variant for the final stages (modes 2 and 3): the FP64
+// base state, and in mode 2 the earlier stages' residuals, are prefetched.
+// FIN = false serves modes 0 and 1 with the base in the residual precision.
+template <int D, int P, int NF, typename T, bool BND, bool VIS, bool FIN>
 __global__ __launch_bounds__((ElemGeo<D, P>::NTH)) void k_stage(StageArgs<T> a) {
   using G = ElemGeo<D, P>;
   using O = OneD<P>;
@@ -390,7 +393,9 @@ __global__ __launch_bounds__((ElemGeo<D, P>::NTH)) void k_stage(StageArgs<T> a) 
   const double pdet = a.detJ[ec * NS + tc];
   const double pdt = a.dtE[ec * a.dtStride];
 #pragma unroll
-  for (int k = 0; k < NF; k++) pb[k] = a.Ubase[(ec * NF + k) * NS + tc];
+  for (int k = 0; k < NF; k++)
+    pb[k] = (FIN || (VIS && a.mode >= 2)) ? a.Ubase[(ec * NF + k) * NS + tc]
+                                          : (double)a.UbaseT[(ec * NF + k) * NS + tc];
 #pragma unroll
   for (int r = 0; r < NQ; r++) {
     const int qc = min(t + r * TPE, NFP - 1);
@@ -404,12 +409,12 @@ __global__ __launch_bounds__((ElemGeo<D, P>::NTH)) void k_stage(StageArgs<T> a) 
   }
   // earlier stages' residuals for the final update (issued last: they are
   // needed last)
-  T pd[FIN2 ? 3 : 1][NF];
-  if constexpr (FIN2) {
+  T pd[FIN ? 3 : 1][NF];
+  if constexpr (FIN) {
     const T *dp[3] = {a.dprev0, a.dprev1, a.dprev2};
 #pragma unroll
     for (int r = 0; r < 3; r++)
-      if (r < a.nprev) {
+      if (a.mode == 2 && r < a.nprev) {
 #pragma unroll
         for (int k = 0; k < NF; k++) pd[r][k] = dp[r][(ec * NF + k) * NS + tc];
       }
@@ -629,9 +634,9 @@ __global__ __launch_bounds__((ElemGeo<D, P>::NTH)) void k_stage(StageArgs<T> a) 
       const double dtJ = pdt / pdet;
       double u = pb[k];
       if (a.mode == 2) {
-        // earlier residuals: prefetched (FIN2) or read here (viscous variant)
+        // earlier residuals: prefetched (FIN) or read here (viscous variant)
         T r0, r1, r2;
-        if constexpr (FIN2) {
+        if constexpr (FIN) {
           r0 = pd[0][k];
           r1 = pd[1][k];
           r2 = pd[2][k];
