@@ -12,3 +12,8 @@ from ._lib import FrError  # noqa: F401
 def Solver(*args, **kw):  # noqa: N802 - mirrors the reference class name
     from .solver import Solver as _S
     return _S(*args, **kw)
+
+
+def Batch(*args, **kw):  # noqa: N802
+    from .solver import Batch as _B
+    return _B(*args, **kw)

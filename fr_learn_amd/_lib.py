@@ -70,6 +70,10 @@ SIGNATURES = {
     "fr_vjp_step": (ctypes.c_int, [_P, _DP, _DP]),
     "fr_vjp_steps": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, _DP, _DP]),
     "fr_reduce_gradient": (ctypes.c_int, [_P, ctypes.c_int, _DP]),
+    "fr_batch_create": (ctypes.c_int, [ctypes.POINTER(_P), ctypes.c_int, ctypes.POINTER(_P)]),
+    "fr_batch_update": (ctypes.c_int, [_P, ctypes.c_int]),
+    "fr_batch_tuning": (ctypes.c_int, [_P, _IP, _IP, _DP]),
+    "fr_batch_destroy": (ctypes.c_int, [_P]),
     "fr_set_state": (ctypes.c_int, [_P, _DP, ctypes.c_int]),
     "fr_get_state": (ctypes.c_int, [_P, _DP, ctypes.c_int]),
     "fr_calc_residual": (ctypes.c_int, [_P, ctypes.c_int]),

@@ -12,6 +12,7 @@
 #include "../../include/fr.h"
 #include "fr_host.hpp"
 #include "fr_adjoint.hpp"
+#include "fr_batch.hpp"
 #include "fr_kernels.hpp"
 
 namespace {
@@ -89,6 +90,22 @@ struct fr_ctx {
 
   size_t nState() const { return (size_t)nE * NF * NS; }
   size_t tsize() const { return prec == FR_PREC_FP32 ? 4 : 8; }
+};
+
+// Many contexts advanced together: one persistent launch per RK stage over
+// all their element blocks, the launches of one step captured in a hipGraph.
+struct fr_batch {
+  std::vector<fr_ctx *> ctx;
+  int D = 2, NF = 4, nRK = 1, prec = FR_PREC_FP64;
+  hipStream_t stream = nullptr;
+  void *args[2] = {nullptr, nullptr};  // StageArgs per [trace parity][stage][instance]
+  void *argsTune = nullptr;            // residual-only (mode 0) StageArgs per instance, for tuning
+  int *order = nullptr;                // P per instance
+  frdev::BatchItem *items = nullptr;
+  int nItems = 0, grid = 0;
+  double tunedMs = 0;
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
 };
 
 namespace {
@@ -173,16 +190,15 @@ int ensureTraces(fr_ctx *c, int which) {
 template <typename T>
 int launchStageArgs(fr_ctx *c, const frdev::StageArgs<T> &a, int mode);
 
-// One launch of k_stage.  in: 0 = U_spts, 1 = Us.  mode as in k_stage.
+// Stage-kernel arguments: input state `in` (0 = U_spts, 1 = Us), traces in
+// Uf[cur], mode / stage / RK coefficient as in k_stage.
 template <typename T>
-int launchStage(fr_ctx *c, int in, int mode, int step, double rk) {
-  int rc = ensureTraces<T>(c, in);
-  if (rc) return rc;
+frdev::StageArgs<T> makeStageArgs(fr_ctx *c, int in, int cur, int mode, int step, double rk) {
   const bool f32 = std::is_same<T, float>::value;
   frdev::StageArgs<T> a{};
   a.Uin = stateIn<T>(c, in);
-  a.UfIn = (const T *)c->Uf[c->cur];
-  a.UfOut = (T *)c->Uf[c->cur ^ 1];
+  a.UfIn = (const T *)c->Uf[cur];
+  a.UfOut = (T *)c->Uf[cur ^ 1];
   a.JG = f32 ? (const T *)c->JG32 : (const T *)c->JG64;
   a.fgeo = f32 ? (const T *)c->fgeo32 : (const T *)c->fgeo64;
   a.fnbr = c->fnbr;
@@ -210,6 +226,15 @@ int launchStage(fr_ctx *c, int in, int mode, int step, double rk) {
   a.nprev = step;
   a.nEles = c->nE;
   a.ph = c->ph;
+  return a;
+}
+
+// One launch of k_stage.  in: 0 = U_spts, 1 = Us.  mode as in k_stage.
+template <typename T>
+int launchStage(fr_ctx *c, int in, int mode, int step, double rk) {
+  int rc = ensureTraces<T>(c, in);
+  if (rc) return rc;
+  const frdev::StageArgs<T> a = makeStageArgs<T>(c, in, c->cur, mode, step, rk);
   int rc2 = launchStageArgs<T>(c, a, mode);
   if (rc2) return rc2;
   if (mode != 0) {
@@ -324,6 +349,109 @@ int syncU32(fr_ctx *c) {
   hipLaunchKernelGGL(frdev::k_to_float, dim3(nblk(n)), dim3(256), 0, c->stream, c->U, c->U32, n);
   HIP_TRY(hipGetLastError());
   return 0;
+}
+
+// ---------------------------------------------------------------------------
+// batches
+// ---------------------------------------------------------------------------
+// The mode and RK coefficient of stage s of one update (solver::update).
+void stageMode(const fr_ctx *c, int s, int &mode, double &rk) {
+  if (s + 1 < c->nRK) {
+    mode = 1;
+    rk = c->RKa[s + 1];
+  } else if (c->cfg.timeType < 5) {
+    mode = 2;
+    rk = 0.0;
+  } else {
+    mode = 3;
+    rk = c->RKa[c->nRK];
+  }
+}
+
+// parity < 0: the residual-only tuning launch
+template <typename T>
+int batchLaunch(fr_batch *b, int parity, int s, int grid, hipStream_t st) {
+  const size_t n = b->ctx.size();
+  const frdev::StageArgs<T> *args = (parity < 0) ? (const frdev::StageArgs<T> *)b->argsTune
+                                                 : (const frdev::StageArgs<T> *)b->args[parity] + (size_t)s * n;
+  int mode = 0;
+  double rk;
+  if (parity >= 0) stageMode(b->ctx[0], s, mode, rk);
+  const bool fin = mode >= 2;
+#define FR_BATCH_LAUNCH(DD, NFF)                                                                         \
+  if (b->D == DD && b->NF == NFF) {                                                                    \
+    if (fin)                                                                                           \
+      hipLaunchKernelGGL((frdev::k_batch_stage<DD, NFF, T, true>), dim3(grid), dim3(256), 0, st, args, \
+                         b->order, b->items, b->nItems);                                               \
+    else                                                                                               \
+      hipLaunchKernelGGL((frdev::k_batch_stage<DD, NFF, T, false>), dim3(grid), dim3(256), 0, st,      \
+                         args, b->order, b->items, b->nItems);                                         \
+  }
+  FR_BATCH_LAUNCH(2, 4)
+  FR_BATCH_LAUNCH(2, 1)
+  FR_BATCH_LAUNCH(3, 5)
+  FR_BATCH_LAUNCH(3, 1)
+#undef FR_BATCH_LAUNCH
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+int batchLaunchStage(fr_batch *b, int parity, int s, int grid, hipStream_t st) {
+  return b->prec == FR_PREC_FP32 ? batchLaunch<float>(b, parity, s, grid, st)
+                                 : batchLaunch<double>(b, parity, s, grid, st);
+}
+
+template <typename T>
+int batchArgs(fr_batch *b) {
+  const size_t n = b->ctx.size();
+  for (int p = 0; p < 2; p++) {
+    std::vector<frdev::StageArgs<T>> h((size_t)b->nRK * n);
+    for (int s = 0; s < b->nRK; s++)
+      for (size_t i = 0; i < n; i++) {
+        int mode;
+        double rk;
+        stageMode(b->ctx[i], s, mode, rk);
+        h[(size_t)s * n + i] = makeStageArgs<T>(b->ctx[i], s == 0 ? 0 : 1, p ^ (s & 1), mode, s, rk);
+      }
+    int rc;
+    if ((rc = dmalloc((frdev::StageArgs<T> **)&b->args[p], h.size()))) return rc;
+    HIP_TRY(hipMemcpy(b->args[p], h.data(), h.size() * sizeof(frdev::StageArgs<T>), hipMemcpyHostToDevice));
+  }
+  std::vector<frdev::StageArgs<T>> h(n);
+  for (size_t i = 0; i < n; i++) h[i] = makeStageArgs<T>(b->ctx[i], 0, 0, 0, 0, 0.0);
+  int rc;
+  if ((rc = dmalloc((frdev::StageArgs<T> **)&b->argsTune, n))) return rc;
+  HIP_TRY(hipMemcpy(b->argsTune, h.data(), n * sizeof(frdev::StageArgs<T>), hipMemcpyHostToDevice));
+  return 0;
+}
+
+// Before a batched step: every instance holds the traces of U_spts in
+// Uf[parity] (the stage kernels leave only the cross-block traces in HBM,
+// which is what the next step reads).
+int batchPrepare(fr_batch *b, int parity) {
+  for (fr_ctx *c : b->ctx) {
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (c->traceOf != 0 || c->cur != parity) {
+      c->cur = parity;
+      c->traceOf = -1;
+      int rc = (c->prec == FR_PREC_FP32) ? ensureTraces<float>(c, 0) : ensureTraces<double>(c, 0);
+      if (rc) return rc;
+      HIP_TRY(hipStreamSynchronize(c->stream));
+    }
+  }
+  return 0;
+}
+
+void batchFree(fr_batch *b) {
+  for (int p = 0; p < 2; p++) {
+    if (b->exec[p]) (void)hipGraphExecDestroy(b->exec[p]);
+    if (b->graph[p]) (void)hipGraphDestroy(b->graph[p]);
+    if (b->args[p]) (void)hipFree(b->args[p]);
+  }
+  if (b->argsTune) (void)hipFree(b->argsTune);
+  if (b->order) (void)hipFree(b->order);
+  if (b->items) (void)hipFree(b->items);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
 }
 
 // ---------------------------------------------------------------------------
@@ -1084,6 +1212,135 @@ int fr_reduce_gradient(fr_ctx *c, int kind, double *out) {
   (void)hipFree(g);
   if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) return fail(FR_EHIP, "kinetic-energy gradient");
   devToRef(c, h.data(), out, c->NF, c->NS);
+  return 0;
+}
+
+int fr_batch_create(fr_ctx *const *ctxs, int n, fr_batch **out) {
+  if (!ctxs || !out || n < 1) return fail(FR_EINVAL, "bad argument");
+  *out = nullptr;
+  fr_batch *b = new fr_batch();
+  auto bad = [&](int code, const std::string &msg) {
+    batchFree(b);
+    delete b;
+    return fail(code, msg);
+  };
+  const fr_ctx *c0 = ctxs[0];
+  if (!c0) return bad(FR_EINVAL, "null context");
+  b->D = c0->D;
+  b->NF = c0->NF;
+  b->nRK = c0->nRK;
+  b->prec = c0->prec;
+  std::vector<int> order(n);
+  std::vector<frdev::BatchItem> items;
+  for (int i = 0; i < n; i++) {
+    fr_ctx *c = ctxs[i];
+    if (!c) return bad(FR_EINVAL, "null context");
+    if (c->D != b->D || c->NF != b->NF || c->prec != b->prec || c->cfg.timeType != c0->cfg.timeType)
+      return bad(FR_EINVAL, "batched meshes must share nDims, equation, precision and timeType");
+    if (c->cfg.viscous) return bad(FR_EINVAL, "the batched kernel is inviscid");
+    b->ctx.push_back(c);
+    order[i] = c->P;
+    const int nb = (c->nE + c->epb - 1) / c->epb;
+    for (int k = 0; k < nb; k++) items.push_back({i, k});
+  }
+  // costliest blocks first (solution points per block), so that the strided
+  // walk of the persistent workgroups balances
+  std::stable_sort(items.begin(), items.end(), [&](const frdev::BatchItem &x, const frdev::BatchItem &y) {
+    const fr_ctx *a = b->ctx[x.inst], *c = b->ctx[y.inst];
+    return a->epb * a->NS > c->epb * c->NS;
+  });
+  b->nItems = (int)items.size();
+  int rc;
+  if ((rc = dmalloc(&b->order, n)) || (rc = dmalloc(&b->items, items.size()))) return bad(rc, g_err);
+  HIP_TRY(hipMemcpy(b->order, order.data(), n * sizeof(int), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(b->items, items.data(), items.size() * sizeof(frdev::BatchItem), hipMemcpyHostToDevice));
+  rc = (b->prec == FR_PREC_FP32) ? batchArgs<float>(b) : batchArgs<double>(b);
+  if (rc) return bad(rc, g_err);
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) return bad(FR_EHIP, "stream");
+  // autotune the persistent grid on residual-only launches (they write only
+  // divF[0]), with the stage computation of the update
+  if ((rc = batchPrepare(b, 0))) return bad(rc, g_err);
+  int cu = 256;
+  {
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+      cu = prop.multiProcessorCount;
+  }
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return bad(FR_EHIP, "event");
+  double best = 1e300;
+  for (int mult : {1, 2, 4, 8, 16}) {
+    const int grid = std::max(1, std::min(cu * mult, b->nItems));
+    if ((rc = batchLaunchStage(b, -1, 0, grid, b->stream))) return bad(rc, g_err);  // warm-up
+    (void)hipEventRecord(e0, b->stream);
+    for (int r = 0; r < 3; r++)
+      if ((rc = batchLaunchStage(b, -1, 0, grid, b->stream))) return bad(rc, g_err);
+    (void)hipEventRecord(e1, b->stream);
+    if (hipEventSynchronize(e1) != hipSuccess) return bad(FR_EHIP, "tuning run");
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    if (ms / 3 < best) {
+      best = ms / 3;
+      b->grid = grid;
+    }
+    if (grid == b->nItems) break;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  b->tunedMs = best;
+  // one step's launches, captured per trace parity
+  for (int p = 0; p < ((b->nRK & 1) ? 2 : 1); p++) {
+    if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeGlobal) != hipSuccess) return bad(FR_EHIP, "capture");
+    for (int s = 0; s < b->nRK; s++) {
+      if ((rc = batchLaunchStage(b, p, s, b->grid, b->stream))) {
+        hipGraph_t g;
+        (void)hipStreamEndCapture(b->stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        return bad(rc, g_err);
+      }
+    }
+    if (hipStreamEndCapture(b->stream, &b->graph[p]) != hipSuccess) return bad(FR_EHIP, "end capture");
+    if (hipGraphInstantiate(&b->exec[p], b->graph[p], nullptr, nullptr, 0) != hipSuccess)
+      return bad(FR_EHIP, "graph instantiate");
+  }
+  // the tuning launches wrote only divF[0]; the states and traces are as before
+  *out = b;
+  return 0;
+}
+
+int fr_batch_update(fr_batch *b, int nSteps) {
+  if (!b) return fail(FR_EINVAL, "null argument");
+  // an even number of stages returns the traces to the buffer they started
+  // in, so one graph (parity 0) serves every step; odd counts alternate
+  int parity = (b->nRK & 1) ? (b->ctx[0]->cur & 1) : 0;
+  int rc = batchPrepare(b, parity);
+  if (rc) return rc;
+  for (int i = 0; i < nSteps; i++) {
+    HIP_TRY(hipGraphLaunch(b->exec[parity], b->stream));
+    parity ^= (b->nRK & 1);
+  }
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  for (fr_ctx *c : b->ctx) {
+    c->cur = parity;
+    c->traceOf = 0;
+  }
+  return 0;
+}
+
+int fr_batch_tuning(fr_batch *b, int *grid, int *nItems, double *msStage0) {
+  if (!b) return fail(FR_EINVAL, "null argument");
+  if (grid) *grid = b->grid;
+  if (nItems) *nItems = b->nItems;
+  if (msStage0) *msStage0 = b->tunedMs;
+  return 0;
+}
+
+int fr_batch_destroy(fr_batch *b) {
+  if (!b) return 0;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  batchFree(b);
+  delete b;
   return 0;
 }
 

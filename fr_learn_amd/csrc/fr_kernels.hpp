@@ -317,6 +317,16 @@ __global__ __launch_bounds__((ElemGeo<D, P>::NTH)) void k_grad(GradArgs<T> a) {
   }
 }
 
+// LDS of one stage-kernel workgroup.
+template <int D, int P, int NF, typename T>
+struct StageSmem {
+  using G = ElemGeo<D, P>;
+  T sU[G::EPB][NF][G::NS];    // stage state
+  T sFd[G::EPB][NF][G::NS];   // contravariant flux F~_d of one reference axis, later the updated state
+  T sJ[G::EPB][NF][G::NFP];   // normal-flux jump Fn*dA - disFn at the flux points
+  T s1[OneD<P>::SIZE];        // 1D factor table
+};
+
 // BND = false is the variant for meshes without boundary faces (no ghost
 // states, which keeps its register allocation small).  VIS adds the viscous
 // terms (LDG): the corrected gradient at the solution points is recomputed
@@ -325,21 +335,19 @@ __global__ __launch_bounds__((ElemGeo<D, P>::NTH)) void k_grad(GradArgs<T> a) {
 // FIN = true is the variant for the final stages (modes 2 and 3): the FP64
 // base state, and in mode 2 the earlier stages' residuals, are prefetched.
 // FIN = false serves modes 0 and 1 with the base in the residual precision.
+// The stage computation of element block b (the elements b*EPB ..), shared
+// by the per-mesh kernel and the batched persistent kernel.
 template <int D, int P, int NF, typename T, bool BND, bool VIS, bool FIN>
-__global__ __launch_bounds__((ElemGeo<D, P>::NTH)) void k_stage(StageArgs<T> a) {
+__device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, StageSmem<D, P, NF, T> &sm) {
   using G = ElemGeo<D, P>;
   using O = OneD<P>;
   constexpr int NS = G::NS, NFP = G::NFP, NFPF = G::NFPF, P1 = G::P1, TPE = G::TPE, EPB = G::EPB;
   constexpr int NFACE = G::NFACE;
   static_assert(TPE >= NS && TPE >= 2 * NFPF, "one solution point per thread");
-  __shared__ T sU[EPB][NF][NS];   // stage state, later the updated state
-  __shared__ T sFd[EPB][NF][NS];  // contravariant flux F~_d of one reference axis
-  __shared__ T sJ[EPB][NF][NFP];  // normal-flux jump Fn*dA - disFn at the flux points
-  __shared__ T s1[O::SIZE];
-  // Workgroups are dealt to the 8 XCDs in turn; remap so that each XCD walks a
-  // contiguous range of elements and face neighbours share its L2.  The host
-  // pads the grid to a multiple of 8.
-  const int b = (blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+  auto &sU = sm.sU;
+  auto &sFd = sm.sFd;
+  auto &sJ = sm.sJ;
+  auto &s1 = sm.s1;
   const int slot = threadIdx.x / TPE, t = threadIdx.x - slot * TPE;
   const int e = b * EPB + slot;
   const bool act = e < a.nEles;
@@ -686,6 +694,16 @@ __global__ __launch_bounds__((ElemGeo<D, P>::NTH)) void k_stage(StageArgs<T> a) 
       }
     }
   }
+}
+
+template <int D, int P, int NF, typename T, bool BND, bool VIS, bool FIN>
+__global__ __launch_bounds__((ElemGeo<D, P>::NTH)) void k_stage(StageArgs<T> a) {
+  __shared__ StageSmem<D, P, NF, T> sm;
+  // Workgroups are dealt to the 8 XCDs in turn; remap so that each XCD walks a
+  // contiguous range of elements and face neighbours share its L2.  The host
+  // pads the grid to a multiple of 8.
+  const int b = (blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+  stageBody<D, P, NF, T, BND, VIS, FIN>(a, b, sm);
 }
 
 // ---------------------------------------------------------------------------

@@ -216,3 +216,37 @@ class Solver:
         cnt = ctypes.c_long()
         _lib.check(self._lib.fr_kernel_times(self._h, kernel, ctypes.byref(ms), ctypes.byref(cnt)))
         return ms.value, cnt.value
+
+
+class Batch:
+    """Several Solvers advanced together (fr_batch_*): one persistent launch
+    per RK stage over all their element blocks, replayed from a hipGraph.
+    The solvers must share nDims, equation, precision and timeType."""
+
+    def __init__(self, solvers):
+        self.solvers = list(solvers)
+        self._lib = _lib.load()
+        arr = (ctypes.c_void_p * len(self.solvers))(*[s._h.value for s in self.solvers])
+        h = ctypes.c_void_p()
+        _lib.check(self._lib.fr_batch_create(arr, len(self.solvers), ctypes.byref(h)))
+        self._h = h
+
+    def update(self, n=1):
+        _lib.check(self._lib.fr_batch_update(self._h, int(n)))
+
+    def tuning(self):
+        """(persistent grid, work items, ms per stage-0 launch at that grid)."""
+        g, n, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        _lib.check(self._lib.fr_batch_tuning(self._h, ctypes.byref(g), ctypes.byref(n), ctypes.byref(ms)))
+        return g.value, n.value, ms.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.fr_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -175,6 +175,20 @@ int fr_vjp_step(fr_ctx *ctx, const double *lam1, double *lam0);
 int fr_vjp_steps(fr_ctx *ctx, int nSteps, int every, const double *lamN, double *lam0);
 int fr_reduce_gradient(fr_ctx *ctx, int kind, double *out);
 
+/* Batches (no reference counterpart): many contexts of one nDims, equation,
+ * precision and timeType (sizes, orders and boundaries free; inviscid)
+ * advanced together, one persistent launch per RK stage over all their
+ * element blocks, one step's launches replayed from a captured hipGraph.
+ * fr_batch_create autotunes the persistent grid on residual evaluations
+ * (which overwrite divF_spts[0] only; fr_batch_tuning reports the choice).
+ * fr_batch_update is fr_update(nSteps) on every member; the contexts stay
+ * usable on their own between batch calls. */
+typedef struct fr_batch fr_batch;
+int fr_batch_create(fr_ctx *const *ctxs, int n, fr_batch **out);
+int fr_batch_update(fr_batch *batch, int nSteps);
+int fr_batch_tuning(fr_batch *batch, int *grid, int *nItems, double *msStage0);
+int fr_batch_destroy(fr_batch *batch);
+
 /* Stream control.  fr_set_stream(ctx, (void*)hipStream_t) makes the context
  * launch on the caller's stream (NULL = the context's own stream). */
 int fr_set_stream(fr_ctx *ctx, void *stream);

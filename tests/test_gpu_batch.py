@@ -1,0 +1,95 @@
+"""Batched persistent stage kernel (fr_batch_*): meshes of different sizes,
+orders and boundaries advanced together must reproduce each mesh advanced on
+its own (same stage computation; bar 1e-14 relative), and the oracle."""
+import numpy as np
+import pytest
+
+from fr_learn_amd import Batch, Solver, cases, mesh
+from oracle import Oracle
+
+from helpers import QUAD_BCS, TOL_FP64, euler_cfg, hex_box, quad_box, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _quad_members(rng, n, timeType=4, precision="fp64"):
+    members = []
+    for i in range(n):
+        P = 1 + i % 5
+        cells = int(rng.integers(3, 9))
+        periodic = bool(i % 3)
+        m = quad_box(cells, periodic=periodic, bc=None if periodic else QUAD_BCS, seed=i, lo=-5.0, hi=5.0)
+        cfg = euler_cfg(2, P, riemannType=int(i % 2), timeType=timeType, dt=1e-3)
+        cx, cy = rng.uniform(-1.0, 1.0, 2)
+        beta, mach = rng.uniform(1.0, 5.0), rng.uniform(0.2, 0.6)
+        ic = (lambda cx, cy, beta, mach: lambda x, c: cases.ic_vortex(x, c, beta=beta, mach=mach,
+                                                                        center=(cx, cy)))(cx, cy, beta, mach)
+        members.append((cfg, m, ic))
+    print("batch of", n, "quad meshes, P 1..5", flush=True)
+    return members
+
+
+def _run(members, steps, precision="fp64"):
+    solo, batched, U0s = [], [], []
+    for cfg, m, ic in members:
+        a = Solver(cfg, m, precision=precision)
+        U0 = a.initialize(ic)
+        b = Solver(cfg, m, precision=precision)
+        b.set_state(U0)
+        solo.append(a)
+        batched.append(b)
+        U0s.append(U0)
+    for a in solo:
+        a.update(steps)
+    B = Batch(batched)
+    grid, items, ms = B.tuning()
+    print("autotuned grid", grid, "items", items, "ms", ms)
+    assert grid >= 1 and items >= len(members)
+    B.update(steps)
+    return solo, batched, U0s, B
+
+
+@pytest.mark.parametrize("timeType", [4, 0])
+def test_batch_matches_each_mesh(timeType):
+    rng = np.random.default_rng(7)
+    members = _quad_members(rng, 12, timeType=timeType)
+    solo, batched, _, B = _run(members, 3)
+    for i, (a, b) in enumerate(zip(solo, batched)):
+        err = rel_err(b.get_state(), a.get_state()).max()
+        print(i, "P", members[i][0]["order"], "err", err)
+        assert err <= 1e-14
+    # the members stay usable alone, and further batch steps continue from there
+    batched[0].update(1)
+    solo[0].update(1)
+    B.update(2)
+    for a in solo:
+        a.update(2)
+    for a, b in zip(solo, batched):
+        assert rel_err(b.get_state(), a.get_state()).max() <= 1e-14
+
+
+def test_batch_vs_oracle_and_hex():
+    rng = np.random.default_rng(9)
+    members = _quad_members(rng, 5)
+    _, batched, U0s, _ = _run(members, 2)
+    for (cfg, m, _), b, U0 in zip(members[:2], batched[:2], U0s[:2]):
+        o = Oracle(cfg, m)
+        o.set_state(U0)
+        o.update(2)
+        assert rel_err(b.get_state(), o.get_state()).max() <= TOL_FP64
+    # hexes, P1-P4, periodic
+    hm = []
+    for P in (1, 2, 3, 4):
+        cfg = euler_cfg(3, P, dt=1e-3)
+        hm.append((cfg, hex_box(3, seed=P), cases.ic_tgv))
+    solo, batched, _, _ = _run(hm, 2)
+    for a, b in zip(solo, batched):
+        assert rel_err(b.get_state(), a.get_state()).max() <= 1e-14
+
+
+def test_batch_fp32():
+    rng = np.random.default_rng(11)
+    members = _quad_members(rng, 6)
+    solo, batched, _, _ = _run(members, 3, precision="fp32")
+    for a, b in zip(solo, batched):
+        assert rel_err(b.get_state(), a.get_state()).max() <= 1e-7
