@@ -146,6 +146,56 @@ def copy_calibration():
     print(json.dumps({"copy_calibration_bytes": n * 8, "copies": 3}), flush=True)
 
 
+def run_batch(n_inst=256, steps=20, warmup=3, seed=0):
+    """The batched 2D Euler workload: n_inst seeded periodic quad meshes of 8^2
+    to 64^2 cells, P uniform in 1..5, isentropic vortices of random centre,
+    strength 1..5 and Mach 0.2..0.6, Roe, RK4.  One persistent launch per
+    stage for all meshes (replayed from a hipGraph) against one update call
+    per mesh."""
+    from fr_learn_amd import Batch, Solver, cases, mesh
+    rng = np.random.default_rng(seed)
+    solvers = []
+    dof = 0
+    for _ in range(n_inst):
+        cells = int(rng.integers(8, 65))
+        P = int(rng.integers(1, 6))
+        cx, cy = rng.uniform(-2.5, 2.5, 2)
+        beta, mach = rng.uniform(1.0, 5.0), rng.uniform(0.2, 0.6)
+        cfg = cases.config(nDims=2, order=P, equation=1, riemannType=1, timeType=4, dt=1e-3, gamma=1.4)
+        m = mesh.box(2, (cells, cells), (-5.0, -5.0), (5.0, 5.0))
+        s = Solver(cfg, m)
+        s.initialize(lambda x, c: cases.ic_vortex(x, c, beta=beta, mach=mach, center=(cx, cy)))
+        solvers.append(s)
+        dof += s.nEles * s.nSpts
+    b = Batch(solvers)
+    grid, items, _ = b.tuning()
+    b.update(warmup)
+    t0 = time.perf_counter()
+    b.update(steps)
+    wall_b = time.perf_counter() - t0
+    for s in solvers:
+        s.update(warmup)
+    for s in solvers:
+        s.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for s in solvers:
+            s.update(1)
+    for s in solvers:
+        s.synchronize()
+    wall_s = time.perf_counter() - t0
+    b.close()
+    for s in solvers:
+        s.close()
+    v_b = dof * 4 * steps / wall_b / 1e9
+    v_s = dof * 4 * steps / wall_s / 1e9
+    return {"workload": f"{n_inst} periodic quad meshes 8^2..64^2 cells, P 1..5, 2D Euler vortices, Roe, RK4 "
+                        "(quadrilaterals only: no triangle FR operators)",
+            "instances": n_inst, "dof_per_field": dof, "steps": steps, "value": v_b, "unit": "GDOF/s",
+            "ms_per_step": wall_b / steps * 1e3, "per_mesh_launches_GDOF_per_s": v_s,
+            "speedup_vs_per_mesh_launches": v_b / v_s, "autotuned_grid": grid, "work_items": items}
+
+
 def traffic_from_profile(precision):
     """HBM bytes per launch of the residual kernel from the committed rocprofv3
     PMC summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor), if any."""
@@ -164,6 +214,7 @@ def main():
     ap.add_argument("--n", type=int, default=48, help="cells per direction")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 path")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-mesh workload")
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
     ap.add_argument("--copy-calibration", action="store_true", help="only the 256 MiB copy for PMC scaling")
     ap.add_argument("--copy-bandwidth", action="store_true", help="only measure the 1 GiB copy bandwidth")
@@ -239,6 +290,8 @@ def main():
                          "algorithmic_bytes_per_launch": r32["bytes"]},
             "kernels_ms": {"stage": r32["res_ms"]},
         }
+    if not args.no_batch:
+        line["batched"] = run_batch()
     if not args.no_cpu:
         line["cpu_baseline"] = run_cpu_baseline(cfg, m)
     else:

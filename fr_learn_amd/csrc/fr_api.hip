@@ -100,12 +100,14 @@ struct fr_batch {
   hipStream_t stream = nullptr;
   void *args[2] = {nullptr, nullptr};  // StageArgs per [trace parity][stage][instance]
   void *argsTune = nullptr;            // residual-only (mode 0) StageArgs per instance, for tuning
-  int *order = nullptr;                // P per instance
-  frdev::BatchItem *items = nullptr;
-  int nItems = 0, grid = 0;
+  frdev::BatchItem *items = nullptr;  // grouped by order
+  int itemStart[6] = {0, 0, 0, 0, 0, 0}, itemCount[6] = {0, 0, 0, 0, 0, 0};
+  int nItems = 0, grid = 0;           // persistent workgroups per order's launch
   double tunedMs = 0;
   hipGraph_t graph[2] = {nullptr, nullptr};
   hipGraphExec_t exec[2] = {nullptr, nullptr};
+  hipStream_t ps[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};  // one per order (capture)
+  hipEvent_t fork = nullptr, join[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
 };
 
 namespace {
@@ -368,9 +370,10 @@ void stageMode(const fr_ctx *c, int s, int &mode, double &rk) {
   }
 }
 
-// parity < 0: the residual-only tuning launch
+// Stage s of the meshes of order P (P = 0: every order, one after the
+// other); parity < 0: the residual-only tuning launch.
 template <typename T>
-int batchLaunch(fr_batch *b, int parity, int s, int grid, hipStream_t st) {
+int batchLaunch(fr_batch *b, int parity, int s, int grid, hipStream_t st, int Ponly) {
   const size_t n = b->ctx.size();
   const frdev::StageArgs<T> *args = (parity < 0) ? (const frdev::StageArgs<T> *)b->argsTune
                                                  : (const frdev::StageArgs<T> *)b->args[parity] + (size_t)s * n;
@@ -378,27 +381,26 @@ int batchLaunch(fr_batch *b, int parity, int s, int grid, hipStream_t st) {
   double rk;
   if (parity >= 0) stageMode(b->ctx[0], s, mode, rk);
   const bool fin = mode >= 2;
-#define FR_BATCH_LAUNCH(DD, NFF)                                                                         \
-  if (b->D == DD && b->NF == NFF) {                                                                    \
-    if (fin)                                                                                           \
-      hipLaunchKernelGGL((frdev::k_batch_stage<DD, NFF, T, true>), dim3(grid), dim3(256), 0, st, args, \
-                         b->order, b->items, b->nItems);                                               \
-    else                                                                                               \
-      hipLaunchKernelGGL((frdev::k_batch_stage<DD, NFF, T, false>), dim3(grid), dim3(256), 0, st,      \
-                         args, b->order, b->items, b->nItems);                                         \
+  for (int P = 5; P >= 1; P--) {
+    const int cnt = b->itemCount[P];
+    if (!cnt || (Ponly && P != Ponly)) continue;
+    const frdev::BatchItem *it = b->items + b->itemStart[P];
+    const unsigned g = (unsigned)std::min(grid, cnt);
+    dispatch(b->D, P, b->NF, [&](auto Dc, auto Pc, auto NFc) {
+      constexpr int D = decltype(Dc)::value, PP = decltype(Pc)::value, NF = decltype(NFc)::value;
+      if (fin)
+        hipLaunchKernelGGL((frdev::k_batch_stage<D, PP, NF, T, true>), dim3(g), dim3(256), 0, st, args, it, cnt);
+      else
+        hipLaunchKernelGGL((frdev::k_batch_stage<D, PP, NF, T, false>), dim3(g), dim3(256), 0, st, args, it, cnt);
+    });
+    HIP_TRY(hipGetLastError());
   }
-  FR_BATCH_LAUNCH(2, 4)
-  FR_BATCH_LAUNCH(2, 1)
-  FR_BATCH_LAUNCH(3, 5)
-  FR_BATCH_LAUNCH(3, 1)
-#undef FR_BATCH_LAUNCH
-  HIP_TRY(hipGetLastError());
   return 0;
 }
 
-int batchLaunchStage(fr_batch *b, int parity, int s, int grid, hipStream_t st) {
-  return b->prec == FR_PREC_FP32 ? batchLaunch<float>(b, parity, s, grid, st)
-                                 : batchLaunch<double>(b, parity, s, grid, st);
+int batchLaunchStage(fr_batch *b, int parity, int s, int grid, hipStream_t st, int Ponly = 0) {
+  return b->prec == FR_PREC_FP32 ? batchLaunch<float>(b, parity, s, grid, st, Ponly)
+                                 : batchLaunch<double>(b, parity, s, grid, st, Ponly);
 }
 
 template <typename T>
@@ -449,7 +451,11 @@ void batchFree(fr_batch *b) {
     if (b->args[p]) (void)hipFree(b->args[p]);
   }
   if (b->argsTune) (void)hipFree(b->argsTune);
-  if (b->order) (void)hipFree(b->order);
+  for (int P = 0; P < 6; P++) {
+    if (b->ps[P]) (void)hipStreamDestroy(b->ps[P]);
+    if (b->join[P]) (void)hipEventDestroy(b->join[P]);
+  }
+  if (b->fork) (void)hipEventDestroy(b->fork);
   if (b->items) (void)hipFree(b->items);
   if (b->stream) (void)hipStreamDestroy(b->stream);
 }
@@ -1230,7 +1236,6 @@ int fr_batch_create(fr_ctx *const *ctxs, int n, fr_batch **out) {
   b->NF = c0->NF;
   b->nRK = c0->nRK;
   b->prec = c0->prec;
-  std::vector<int> order(n);
   std::vector<frdev::BatchItem> items;
   for (int i = 0; i < n; i++) {
     fr_ctx *c = ctxs[i];
@@ -1238,21 +1243,22 @@ int fr_batch_create(fr_ctx *const *ctxs, int n, fr_batch **out) {
     if (c->D != b->D || c->NF != b->NF || c->prec != b->prec || c->cfg.timeType != c0->cfg.timeType)
       return bad(FR_EINVAL, "batched meshes must share nDims, equation, precision and timeType");
     if (c->cfg.viscous) return bad(FR_EINVAL, "the batched kernel is inviscid");
+    if (c->P < 1 || c->P > 5) return bad(FR_EINVAL, "batched meshes need P in 1..5");
     b->ctx.push_back(c);
-    order[i] = c->P;
-    const int nb = (c->nE + c->epb - 1) / c->epb;
-    for (int k = 0; k < nb; k++) items.push_back({i, k});
   }
-  // costliest blocks first (solution points per block), so that the strided
-  // walk of the persistent workgroups balances
-  std::stable_sort(items.begin(), items.end(), [&](const frdev::BatchItem &x, const frdev::BatchItem &y) {
-    const fr_ctx *a = b->ctx[x.inst], *c = b->ctx[y.inst];
-    return a->epb * a->NS > c->epb * c->NS;
-  });
+  // work items grouped by order (one launch per order), each group in mesh order
+  for (int P = 1; P <= 5; P++) {
+    b->itemStart[P] = (int)items.size();
+    for (int i = 0; i < n; i++)
+      if (b->ctx[i]->P == P) {
+        const int nb = (b->ctx[i]->nE + b->ctx[i]->epb - 1) / b->ctx[i]->epb;
+        for (int k = 0; k < nb; k++) items.push_back({i, k});
+      }
+    b->itemCount[P] = (int)items.size() - b->itemStart[P];
+  }
   b->nItems = (int)items.size();
   int rc;
-  if ((rc = dmalloc(&b->order, n)) || (rc = dmalloc(&b->items, items.size()))) return bad(rc, g_err);
-  HIP_TRY(hipMemcpy(b->order, order.data(), n * sizeof(int), hipMemcpyHostToDevice));
+  if ((rc = dmalloc(&b->items, items.size()))) return bad(rc, g_err);
   HIP_TRY(hipMemcpy(b->items, items.data(), items.size() * sizeof(frdev::BatchItem), hipMemcpyHostToDevice));
   rc = (b->prec == FR_PREC_FP32) ? batchArgs<float>(b) : batchArgs<double>(b);
   if (rc) return bad(rc, g_err);
@@ -1270,7 +1276,7 @@ int fr_batch_create(fr_ctx *const *ctxs, int n, fr_batch **out) {
   hipEvent_t e0, e1;
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return bad(FR_EHIP, "event");
   double best = 1e300;
-  for (int mult : {1, 2, 4, 8, 16}) {
+  for (int mult : {1, 2, 4, 8, 16, 32}) {
     const int grid = std::max(1, std::min(cu * mult, b->nItems));
     if ((rc = batchLaunchStage(b, -1, 0, grid, b->stream))) return bad(rc, g_err);  // warm-up
     (void)hipEventRecord(e0, b->stream);
@@ -1289,16 +1295,30 @@ int fr_batch_create(fr_ctx *const *ctxs, int n, fr_batch **out) {
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   b->tunedMs = best;
-  // one step's launches, captured per trace parity
+  // One step's launches, captured per trace parity.  Meshes of different
+  // orders are independent, so each order's stages form their own branch of
+  // the graph (forked from and joined to the batch stream) and run
+  // concurrently with the other orders'.
+  if (hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess) return bad(FR_EHIP, "event");
+  for (int P = 1; P <= 5; P++)
+    if (b->itemCount[P] && (hipStreamCreateWithFlags(&b->ps[P], hipStreamNonBlocking) != hipSuccess ||
+                            hipEventCreateWithFlags(&b->join[P], hipEventDisableTiming) != hipSuccess))
+      return bad(FR_EHIP, "stream");
   for (int p = 0; p < ((b->nRK & 1) ? 2 : 1); p++) {
     if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeGlobal) != hipSuccess) return bad(FR_EHIP, "capture");
-    for (int s = 0; s < b->nRK; s++) {
-      if ((rc = batchLaunchStage(b, p, s, b->grid, b->stream))) {
-        hipGraph_t g;
-        (void)hipStreamEndCapture(b->stream, &g);
-        if (g) (void)hipGraphDestroy(g);
-        return bad(rc, g_err);
-      }
+    bool ok = hipEventRecord(b->fork, b->stream) == hipSuccess;
+    for (int P = 1; P <= 5 && ok && !rc; P++) {
+      if (!b->itemCount[P]) continue;
+      ok = hipStreamWaitEvent(b->ps[P], b->fork, 0) == hipSuccess;
+      for (int s = 0; s < b->nRK && ok && !rc; s++) rc = batchLaunchStage(b, p, s, b->grid, b->ps[P], P);
+      ok = ok && hipEventRecord(b->join[P], b->ps[P]) == hipSuccess &&
+           hipStreamWaitEvent(b->stream, b->join[P], 0) == hipSuccess;
+    }
+    if (!ok || rc) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(b->stream, &g);
+      if (g) (void)hipGraphDestroy(g);
+      return bad(rc ? rc : FR_EHIP, rc ? g_err : std::string("graph capture"));
     }
     if (hipStreamEndCapture(b->stream, &b->graph[p]) != hipSuccess) return bad(FR_EHIP, "end capture");
     if (hipGraphInstantiate(&b->exec[p], b->graph[p], nullptr, nullptr, 0) != hipSuccess)
