@@ -192,3 +192,32 @@ def test_residual_on_gmsh_mesh(tmp_path):
     o.calc_residual(0)
     g.calc_residual(0)
     assert rel_err(g.residual(0), o.residual(0)).max() <= TOL_FP64
+
+
+@pytest.mark.parametrize("scheme", [1, 2, 3])
+@pytest.mark.parametrize("D", [2, 3])
+def test_vcjh_schemes_residual(scheme, D):
+    """VCJH correction functions other than DG (SD, HU, C+; polynomials.cpp
+    392-475) through the same kernels."""
+    P = 3
+    cfg = euler_cfg(D, P, vcjhScheme=scheme)
+    m, ic = (hex_box(3), cases.ic_tgv) if D == 3 else (quad_box(4), lambda x, c: cases.ic_vortex(x, c))
+    o, g, _ = _pair(cfg, m, ic)
+    o.calc_residual(0)
+    g.calc_residual(0)
+    assert rel_err(g.residual(0), o.residual(0)).max() <= TOL_FP64
+
+
+@pytest.mark.parametrize("D", [2, 3])
+@pytest.mark.parametrize("order", [2, 4])
+def test_lobatto_points_residual_and_step(D, order):
+    """Gauss-Lobatto solution points (sptsType 1, points.cpp:254-330)."""
+    cfg = euler_cfg(D, order, sptsType=1, dt=1e-3)
+    m, ic = (hex_box(3), cases.ic_tgv) if D == 3 else (quad_box(4), lambda x, c: cases.ic_vortex(x, c))
+    o, g, _ = _pair(cfg, m, ic)
+    o.calc_residual(0)
+    g.calc_residual(0)
+    assert rel_err(g.residual(0), o.residual(0)).max() <= TOL_FP64
+    o.update(2)
+    g.update(2)
+    assert rel_err(g.get_state(), o.get_state()).max() <= TOL_FP64
