@@ -12,15 +12,18 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libfrhip.so")
 
-SOURCES = ["fr_api.hip", "fr_host.cpp"]
-HEADERS = ["fr_host.hpp", "fr_kernels.hpp", "fr_layout.hpp", "fr_physics.hpp"]
+# translation units, compiled in parallel (the stage kernels per precision,
+# the adjoint and the batch kernels each in their own)
+SOURCES = ["fr_adjoint_v3.hip", "fr_adjoint_v2.hip", "fr_stage64.hip", "fr_stage32.hip", "fr_adjoint.hip",
+           "fr_batch.hip", "fr_adjoint_v3ad.hip", "fr_api.hip", "fr_host.cpp"]
+HEADERS = ["fr_ctx.hpp", "fr_host.hpp", "fr_kernels.hpp", "fr_layout.hpp", "fr_physics.hpp", "fr_adjoint.hpp",
+           "fr_adjoint_vis.hpp", "fr_dual.hpp", "fr_batch.hpp", "fr_stage_impl.hpp"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
     "-O3",
     "-std=c++17",
     "-fPIC",
-    "-shared",
     # packed FP32 instructions are disabled for all device code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result",
@@ -34,17 +37,38 @@ def hipcc():
     raise RuntimeError("hipcc not found")
 
 
-def build(verbose=False):
-    """Compile libfrhip.so for gfx950.  Always rebuilds."""
-    srcs = [os.path.join(CSRC, s) for s in SOURCES]
+def build(verbose=False, jobs=None):
+    """Compile libfrhip.so for gfx950: every translation unit to an object in
+    parallel, then one link.  Always rebuilds."""
+    from concurrent.futures import ThreadPoolExecutor
+    objdir = os.path.join(HERE, "_obj")
+    os.makedirs(objdir, exist_ok=True)
+    inc = ["-I", os.path.join(ROOT, "include")]
+
+    def compile_one(src):
+        obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
+        cmd = [hipcc()] + HIPCC_FLAGS + inc + ["-c", os.path.join(CSRC, src), "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        return src, obj, r
+
+    jobs = jobs or min(len(SOURCES), max(1, (os.cpu_count() or 1)))
+    with ThreadPoolExecutor(max_workers=jobs) as pool:
+        results = list(pool.map(compile_one, SOURCES))
+    failed = [(src, r) for src, _, r in results if r.returncode != 0]
+    for src, r in failed:
+        sys.stderr.write(f"--- {src}\n" + r.stdout + r.stderr)
+    if failed:
+        raise RuntimeError("hipcc failed building libfrhip.so")
     tmp = LIB + ".tmp"
-    cmd = [hipcc()] + HIPCC_FLAGS + ["-I", os.path.join(ROOT, "include")] + srcs + ["-o", tmp]
+    cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC"] + [obj for _, obj, _ in results] + ["-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("hipcc failed building libfrhip.so")
+        raise RuntimeError("linking libfrhip.so failed")
     os.replace(tmp, LIB)
     return LIB
 

@@ -212,8 +212,344 @@ __global__ __launch_bounds__((ElemGeo<D, P>::NTH)) void k_vjp_state(
   }
 }
 
+// ---------------------------------------------------------------------------
+// Viscous (LDG) residual.  Besides the inviscid chain, divF depends on U
+// through the corrected gradient Q = JGinv^T (Dm U + lift(u^ - u)) / detJ at
+// the solution points (point viscous flux) and its traces Qf = E Q (face
+// viscous flux), and u^ couples neighbours.  Two more exchanges follow the
+// face-cotangent one: k_vjp_state_vis writes the cotangent of each flux
+// point's jump u^ - u (B), k_vjp_ldg gathers the neighbour's one to pull the
+// common solution u^ back to this element's traces.
+// One element per workgroup of TPE threads (the LDS of the viscous adjoint
+// does not fit several elements).
+// ---------------------------------------------------------------------------
+template <int D, int P, int NF, bool BND>
+__global__ __launch_bounds__((ElemGeo<D, P>::TPE)) void k_vjp_state_vis(
+    const double *__restrict__ U, const double *__restrict__ lam, const double *__restrict__ Uf,
+    const double *__restrict__ Qf, const double *__restrict__ A, const double *__restrict__ JG,
+    const double *__restrict__ detJ, const double *__restrict__ fgeo, const int32_t *__restrict__ fnbr,
+    const int8_t *__restrict__ finfo, const double *__restrict__ oned, Phys ph, int nEles,
+    double *__restrict__ out, double *__restrict__ B) {
+  using G = ElemGeo<D, P>;
+  using O = OneD<P>;
+  constexpr int NS = G::NS, NFP = G::NFP, NFPF = G::NFPF, P1 = G::P1, TPE = G::TPE, NFACE = G::NFACE;
+  constexpr int M = NF + D * NF;  // dual directions: the state, then the gradient [dim][field]
+  using DM = Dual<M>;
+  __shared__ double sU[NF][NS], sL[NF][NS];
+  __shared__ double sJ[NF][NFP];       // lamJ
+  __shared__ double sC[NF][NFP];       // forward LDG jumps u^ - u
+  __shared__ double sUo[NF][NFP];      // cotangent of this element's state traces
+  __shared__ double sQo[D][NF][NFP];   // cotangent of this element's gradient traces
+  __shared__ double sLQ[D][NF][NS];    // cotangent of the reference gradient
+  __shared__ double s1[O::SIZE];
+  const int e = blockIdx.x, t = threadIdx.x;
+  for (int i = t; i < O::SIZE; i += TPE) s1[i] = oned[i];
+  for (int i = t; i < NF * NS; i += TPE) {
+    (&sU[0][0])[i] = U[(size_t)e * NF * NS + i];
+    (&sL[0][0])[i] = lam[(size_t)e * NF * NS + i];
+  }
+  __syncthreads();
+  for (int q = t; q < NFP; q += TPE) {
+    double lj[NF];
+    lamJAt<D, P, NF>(s1, &sL[0][0], q, lj);
+    int base, stride, side, dn;
+    fptLine<D, P>(q, base, stride, side, dn);
+    const double *l1 = s1 + (side ? O::LR : O::LL);
+    double uov[NF];
+#pragma unroll
+    for (int k = 0; k < NF; k++) {
+      double acc = 0;
+#pragma unroll
+      for (int m = 0; m < P1; m++) acc += l1[m] * sU[k][base + m * stride];
+      uov[k] = acc;
+      sJ[k][q] = lj[k];
+    }
+    double nv[D];
+#pragma unroll
+    for (int c = 0; c < D; c++) nv[c] = fgeo[((size_t)e * (D + 1) + c) * NFP + q];
+    const double dA = fgeo[((size_t)e * (D + 1) + D) * NFP + q];
+    const int fi = finfo[(size_t)e * NFACE + q / NFPF];
+    const bool bnd = BND && fi >= FINFO_BOUNDARY;
+    const bool right = !bnd && ((fi >> 5) & 1);
+    DM uo[NF], un[NF], n[D], go[D][NF], gn[D][NF], Fn[NF];
+#pragma unroll
+    for (int k = 0; k < NF; k++) uo[k] = DM::var(uov[k], k);
+#pragma unroll
+    for (int c = 0; c < D; c++) n[c] = DM(nv[c]);
+#pragma unroll
+    for (int d = 0; d < D; d++)
+#pragma unroll
+      for (int k = 0; k < NF; k++)
+        go[d][k] = DM::var(Qf[(((size_t)e * D + d) * NF + k) * NFP + q], NF + d * NF + k);
+    double g[NF], unv[NF];
+    if (bnd) {
+      const int bc = fi - FINFO_BOUNDARY;
+      boundaryState<D, NF, DM>(ph, bc, uo, n, un);
+      if constexpr (NF > 1) {
+        boundaryGradient<D, NF, DM>(ph, bc, uo, un, go, gn);
+      } else {
+#pragma unroll
+        for (int d = 0; d < D; d++) gn[d][0] = DM(0.0);
+      }
+      commonFlux<D, NF, DM>(ph, true, uo, un, n, Fn);
+      ldgViscousFlux<D, NF, DM>(ph, (bc == 10) ? 2 : 1, uo, un, go, gn, n, Fn);
+#pragma unroll
+      for (int k = 0; k < NF; k++) {
+        g[k] = lj[k] * dA;
+        unv[k] = un[k].v;
+      }
+    } else {
+      const int en = fnbr[(size_t)e * NFACE + q / NFPF];
+      const int qn = neighbourFpt<D, P>(q, fi);
+#pragma unroll
+      for (int k = 0; k < NF; k++) {
+        unv[k] = Uf[((size_t)en * NF + k) * NFP + qn];
+        un[k] = DM(unv[k]);
+        g[k] = lj[k] * (right ? -dA : dA) + A[((size_t)en * NF + k) * NFP + qn];
+      }
+#pragma unroll
+      for (int d = 0; d < D; d++)
+#pragma unroll
+        for (int k = 0; k < NF; k++) gn[d][k] = DM(Qf[(((size_t)en * D + d) * NF + k) * NFP + qn]);
+      if (right) {
+        commonFlux<D, NF, DM>(ph, false, un, uo, n, Fn);
+        ldgViscousFlux<D, NF, DM>(ph, 0, un, uo, gn, go, n, Fn);
+      } else {
+        commonFlux<D, NF, DM>(ph, false, uo, un, n, Fn);
+        ldgViscousFlux<D, NF, DM>(ph, 0, uo, un, go, gn, n, Fn);
+      }
+    }
+    // forward LDG jump of this point (as k_grad forms it)
+    double jmp[NF];
+    ldgJump<D, NF, double>(ph, bnd, right, uov, unv, nv, jmp);
+#pragma unroll
+    for (int k = 0; k < NF; k++) sC[k][q] = jmp[k];
+#pragma unroll
+    for (int j = 0; j < M; j++) {
+      double acc = 0;
+#pragma unroll
+      for (int k = 0; k < NF; k++) acc += g[k] * Fn[k].d[j];
+      if (j < NF) sUo[j][q] = acc;
+      else sQo[(j - NF) / NF][(j - NF) % NF][q] = acc;
+    }
+  }
+  __syncthreads();
+  double lU[NF];
+  if (t < NS) {
+    int si, sj, sk;
+    if (D == 3) {
+      sk = t / (P1 * P1);
+      sj = (t - P1 * P1 * sk) / P1;
+      si = t - P1 * sj - P1 * P1 * sk;
+    } else {
+      si = t % P1;
+      sj = t / P1;
+      sk = 0;
+    }
+    int cq[NFACE], cside[NFACE], cax[NFACE];
+    corrEntries<D, P>(t, cq, cside, cax);
+    double pJ[D * D];
+#pragma unroll
+    for (int c = 0; c < D * D; c++) pJ[c] = JG[((size_t)e * D * D + c) * NS + t];
+    const double pdet = detJ[(size_t)e * NS + t];
+    double Q[D][NF];
+    correctedGradient<D, P, NF, double>(s1, &sU[0][0], &sC[0][0], t, si, sj, sk, pJ, pdet, Q);
+    // cotangent of the total point flux
+    double lFt[D][NF], lF[D][NF];
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+      const int id = (d == 0) ? si : (d == 1) ? sj : sk;
+#pragma unroll
+      for (int k = 0; k < NF; k++) {
+        double acc = 0;
+#pragma unroll
+        for (int m = 0; m < P1; m++) {
+          const int sp = (d == 0) ? m + P1 * (sj + P1 * sk) : (d == 1) ? si + P1 * (m + P1 * sk) : si + P1 * (sj + P1 * m);
+          acc += s1[O::DM + m * P1 + id] * sL[k][sp];
+        }
+#pragma unroll
+        for (int f = 0; f < NFACE; f++)
+          if (faceAxis<D>(f) == d)
+            acc -= (cside[f] ? 1.0 : -1.0) * s1[(cside[f] ? O::LR : O::LL) + cax[f]] * sJ[k][cq[f]];
+        lFt[d][k] = acc;
+      }
+    }
+#pragma unroll
+    for (int d2 = 0; d2 < D; d2++)
+#pragma unroll
+      for (int k = 0; k < NF; k++) {
+        double acc = 0;
+#pragma unroll
+        for (int d = 0; d < D; d++) acc += pJ[d * D + d2] * lFt[d][k];
+        lF[d2][k] = acc;
+      }
+    DM u[NF], q[D][NF], Fi[D][NF], Fv[D][NF];
+#pragma unroll
+    for (int k = 0; k < NF; k++) u[k] = DM::var(sU[k][t], k);
+#pragma unroll
+    for (int d = 0; d < D; d++)
+#pragma unroll
+      for (int k = 0; k < NF; k++) q[d][k] = DM::var(Q[d][k], NF + d * NF + k);
+    inviscidFlux<D, NF, DM>(ph, u, Fi);
+    viscousFlux<D, NF, DM>(ph, u, q, Fv);
+    double lQ[D][NF];
+#pragma unroll
+    for (int j = 0; j < M; j++) {
+      double acc = 0;
+#pragma unroll
+      for (int d2 = 0; d2 < D; d2++)
+#pragma unroll
+        for (int k = 0; k < NF; k++) acc += lF[d2][k] * (Fi[d2][k].d[j] + Fv[d2][k].d[j]);
+      if (j < NF) lU[j] = acc;
+      else lQ[(j - NF) / NF][(j - NF) % NF] = acc;
+    }
+    // gradient traces and state traces, transposed
+#pragma unroll
+    for (int f = 0; f < NFACE; f++) {
+      const double c = s1[(cside[f] ? O::LR : O::LL) + cax[f]];
+#pragma unroll
+      for (int k = 0; k < NF; k++) {
+        lU[k] += c * sUo[k][cq[f]];
+#pragma unroll
+        for (int d = 0; d < D; d++) lQ[d][k] += c * sQo[d][k][cq[f]];
+      }
+    }
+    // Q = JGinv^T Qref / detJ, transposed
+    const double invDet = 1.0 / pdet;
+#pragma unroll
+    for (int d = 0; d < D; d++)
+#pragma unroll
+      for (int k = 0; k < NF; k++) {
+        double acc = 0;
+#pragma unroll
+        for (int d1 = 0; d1 < D; d1++) acc += pJ[d * D + d1] * lQ[d1][k];
+        sLQ[d][k][t] = invDet * acc;
+      }
+  }
+  __syncthreads();
+  if (t < NS) {
+    int si, sj, sk;
+    if (D == 3) {
+      sk = t / (P1 * P1);
+      sj = (t - P1 * P1 * sk) / P1;
+      si = t - P1 * sj - P1 * P1 * sk;
+    } else {
+      si = t % P1;
+      sj = t / P1;
+      sk = 0;
+    }
+    // Qref = Dm U + ..., transposed along the lines
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+      const int id = (d == 0) ? si : (d == 1) ? sj : sk;
+#pragma unroll
+      for (int k = 0; k < NF; k++) {
+        double acc = 0;
+#pragma unroll
+        for (int m = 0; m < P1; m++) {
+          const int sp = (d == 0) ? m + P1 * (sj + P1 * sk) : (d == 1) ? si + P1 * (m + P1 * sk) : si + P1 * (sj + P1 * m);
+          acc += s1[O::DM + m * P1 + id] * sLQ[d][k][sp];
+        }
+        lU[k] += acc;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < NF; k++) out[((size_t)e * NF + k) * NS + t] = lU[k];
+  }
+  // cotangent of the jumps: the lift along each flux point's line
+  for (int q = t; q < NFP; q += TPE) {
+    int base, stride, side, dn;
+    fptLine<D, P>(q, base, stride, side, dn);
+    const double *gq = s1 + (side ? O::GR : O::GL);
+    const double sg = side ? 1.0 : -1.0;
+#pragma unroll
+    for (int k = 0; k < NF; k++) {
+      double acc = 0;
+#pragma unroll
+      for (int m = 0; m < P1; m++) acc += gq[m] * sLQ[dn][k][base + m * stride];
+      B[((size_t)e * NF + k) * NFP + q] = sg * acc;
+    }
+  }
+}
+
+// The common solution's pull-back: u^ = (uL+uR)/2 - pen (uL-uR) inside
+// (either side's jump u^ - u uses it), (u + ghost(u))/2 on boundaries.
+template <int D, int P, int NF, bool BND>
+__global__ __launch_bounds__((ElemGeo<D, P>::TPE)) void k_vjp_ldg(
+    const double *__restrict__ U, const double *__restrict__ B, const double *__restrict__ fgeo,
+    const int32_t *__restrict__ fnbr, const int8_t *__restrict__ finfo, const double *__restrict__ oned,
+    Phys ph, int nEles, double *__restrict__ out) {
+  using G = ElemGeo<D, P>;
+  using O = OneD<P>;
+  using DN = Dual<NF>;
+  constexpr int NS = G::NS, NFP = G::NFP, NFPF = G::NFPF, P1 = G::P1, TPE = G::TPE, NFACE = G::NFACE;
+  __shared__ double sU[NF][NS];
+  __shared__ double sUo[NF][NFP];
+  __shared__ double s1[O::SIZE];
+  const int e = blockIdx.x, t = threadIdx.x;
+  for (int i = t; i < O::SIZE; i += TPE) s1[i] = oned[i];
+  for (int i = t; i < NF * NS; i += TPE) (&sU[0][0])[i] = U[(size_t)e * NF * NS + i];
+  __syncthreads();
+  for (int q = t; q < NFP; q += TPE) {
+    const int fi = finfo[(size_t)e * NFACE + q / NFPF];
+    const bool bnd = BND && fi >= FINFO_BOUNDARY;
+    double bo[NF], lUC[NF];
+#pragma unroll
+    for (int k = 0; k < NF; k++) bo[k] = B[((size_t)e * NF + k) * NFP + q];
+    if (bnd) {
+      int base, stride, side, dn;
+      fptLine<D, P>(q, base, stride, side, dn);
+      const double *l1 = s1 + (side ? O::LR : O::LL);
+      DN uo[NF], ug[NF], n[D];
+#pragma unroll
+      for (int k = 0; k < NF; k++) {
+        double acc = 0;
+#pragma unroll
+        for (int m = 0; m < P1; m++) acc += l1[m] * sU[k][base + m * stride];
+        uo[k] = DN::var(acc, k);
+      }
+#pragma unroll
+      for (int c = 0; c < D; c++) n[c] = DN(fgeo[((size_t)e * (D + 1) + c) * NFP + q]);
+      boundaryState<D, NF, DN>(ph, fi - FINFO_BOUNDARY, uo, n, ug);
+#pragma unroll
+      for (int j = 0; j < NF; j++) {
+        double acc = 0;
+#pragma unroll
+        for (int k = 0; k < NF; k++) acc += bo[k] * 0.5 * ((j == k ? 1.0 : 0.0) + ug[k].d[j]);
+        sUo[j][q] = acc - bo[j];
+      }
+    } else {
+      const int en = fnbr[(size_t)e * NFACE + q / NFPF];
+      const int qn = neighbourFpt<D, P>(q, fi);
+      double nv[D];
+#pragma unroll
+      for (int c = 0; c < D; c++) nv[c] = fgeo[((size_t)e * (D + 1) + c) * NFP + q];
+      const double pen = ldgPenalty<D, double>(ph, nv);
+      const bool right = (fi >> 5) & 1;
+      const double coef = right ? 0.5 + pen : 0.5 - pen;
+#pragma unroll
+      for (int k = 0; k < NF; k++) {
+        lUC[k] = bo[k] + B[((size_t)en * NF + k) * NFP + qn];
+        sUo[k][q] = coef * lUC[k] - bo[k];
+      }
+    }
+  }
+  __syncthreads();
+  if (t < NS) {
+    int cq[NFACE], cside[NFACE], cax[NFACE];
+    corrEntries<D, P>(t, cq, cside, cax);
+#pragma unroll
+    for (int k = 0; k < NF; k++) {
+      double acc = 0;
+#pragma unroll
+      for (int f = 0; f < NFACE; f++) acc += s1[(cside[f] ? O::LR : O::LL) + cax[f]] * sUo[k][cq[f]];
+      out[((size_t)e * NF + k) * NS + t] += acc;
+    }
+  }
+}
+
 // out = alpha * (dt / detJ) * x + beta * (dt / detJ) * y  (y may be null)
-__global__ void k_dtj_combine(const double *__restrict__ x, const double *__restrict__ y,
+static __global__ void k_dtj_combine(const double *__restrict__ x, const double *__restrict__ y,
                               const double *__restrict__ detJ, const double *__restrict__ dtE, int dtStride,
                               double alpha, double beta, int NF, int NS, long n, double *__restrict__ out) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -227,7 +563,7 @@ __global__ void k_dtj_combine(const double *__restrict__ x, const double *__rest
 }
 
 // acc += x
-__global__ void k_accumulate(double *__restrict__ acc, const double *__restrict__ x, long n) {
+static __global__ void k_accumulate(double *__restrict__ acc, const double *__restrict__ x, long n) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) acc[i] += x[i];
 }

@@ -9,106 +9,11 @@
 #include <utility>
 #include <vector>
 
-#include "../../include/fr.h"
-#include "fr_host.hpp"
 #include "fr_adjoint.hpp"
-#include "fr_batch.hpp"
-#include "fr_kernels.hpp"
+#include "fr_ctx.hpp"
 
-namespace {
+using namespace frimpl;
 
-thread_local std::string g_err;
-
-int fail(int code, const std::string &msg) {
-  g_err = msg;
-  return code;
-}
-
-#define HIP_TRY(expr)                                                              \
-  do {                                                                             \
-    hipError_t e_ = (expr);                                                        \
-    if (e_ != hipSuccess) return fail(FR_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
-  } while (0)
-
-template <typename F>
-bool dispatch(int D, int P, int NF, F &&f) {
-#define FR_CASE(d, p, nf)                                                                          \
-  if (D == d && P == p && NF == nf) {                                                              \
-    f(std::integral_constant<int, d>{}, std::integral_constant<int, p>{}, std::integral_constant<int, nf>{}); \
-    return true;                                                                                   \
-  }
-  FR_CASE(3, 1, 5) FR_CASE(3, 2, 5) FR_CASE(3, 3, 5) FR_CASE(3, 4, 5) FR_CASE(3, 5, 5)
-  FR_CASE(3, 1, 1) FR_CASE(3, 2, 1) FR_CASE(3, 3, 1) FR_CASE(3, 4, 1) FR_CASE(3, 5, 1)
-  FR_CASE(2, 1, 4) FR_CASE(2, 2, 4) FR_CASE(2, 3, 4) FR_CASE(2, 4, 4) FR_CASE(2, 5, 4)
-  FR_CASE(2, 1, 1) FR_CASE(2, 2, 1) FR_CASE(2, 3, 1) FR_CASE(2, 4, 1) FR_CASE(2, 5, 1)
-#undef FR_CASE
-  return false;
-}
-
-inline unsigned nblk(long n, int b = 256) { return (unsigned)((n + b - 1) / b); }
-
-}  // namespace
-
-struct fr_ctx {
-  fr_config cfg{};
-  int prec = FR_PREC_FP64;
-  frhost::Operators ops;
-  frhost::Connectivity conn;
-  int D = 3, P = 3, NS = 0, NFP = 0, NFPF = 0, NFACE = 0, NF = 0, nE = 0, nRK = 0;
-  double RKa[5] = {0}, RKb[4] = {0};
-  frdev::Phys ph{};
-  hipStream_t own = nullptr, stream = nullptr;
-  // device buffers
-  double *U = nullptr, *U0 = nullptr, *detJ = nullptr, *JG64 = nullptr, *fgeo64 = nullptr, *ownN = nullptr;
-  double *wts = nullptr, *oned64 = nullptr, *part = nullptr;
-  float *U32 = nullptr, *JG32 = nullptr, *fgeo32 = nullptr, *oned32 = nullptr;
-  void *Us = nullptr;                 // RK stage state (residual precision)
-  void *Uf[2] = {nullptr, nullptr};   // face traces, double-buffered
-  int cur = 0;                        // Uf[cur] holds the traces of `traceOf`
-  int traceOf = -1;                   // 0: U_spts, 1: stage state Us, -1: stale
-  void *divF[4] = {nullptr, nullptr, nullptr, nullptr};
-  void *dUf = nullptr;                // viscous: gradient traces [ele][dim][field][fpt]
-  double *dtG = nullptr;              // the global time step (one value)
-  double *dtE = nullptr;              // per-element time steps (local time stepping)
-  int dtType = 0;                     // 0 fixed dt, 1 global CFL dt, 2 local CFL dt
-  // adjoint work buffers (FP64, allocated on first use)
-  double *adjA = nullptr, *adjTr = nullptr, *adjTr2 = nullptr;  // face cotangents, trace scratch
-  double *adjLam = nullptr, *adjOut = nullptr, *adjNu = nullptr, *adjW[2] = {nullptr, nullptr};
-  double *adjS[4] = {nullptr, nullptr, nullptr, nullptr};       // RK stage states
-  int32_t *fnbr = nullptr;
-  int8_t *finfo = nullptr;
-  size_t partCap = 0;
-  std::vector<double> X;  // host copy of element node coordinates [device ele][node][3]
-  std::vector<int> perm;  // device element -> mesh element (blocks of EPB face-connected elements)
-  std::vector<int> iperm; // mesh element -> device element
-  int epb = 1;
-  // timing
-  bool timing = false;
-  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[4];
-  double ms[4] = {0, 0, 0, 0};
-  long cnt[4] = {0, 0, 0, 0};
-
-  size_t nState() const { return (size_t)nE * NF * NS; }
-  size_t tsize() const { return prec == FR_PREC_FP32 ? 4 : 8; }
-};
-
-// Many contexts advanced together: one persistent launch per RK stage over
-// all their element blocks, the launches of one step captured in a hipGraph.
-struct fr_batch {
-  std::vector<fr_ctx *> ctx;
-  int D = 2, NF = 4, nRK = 1, prec = FR_PREC_FP64;
-  hipStream_t stream = nullptr;
-  void *args[2] = {nullptr, nullptr};  // StageArgs per [trace parity][stage][instance]
-  void *argsTune = nullptr;            // residual-only (mode 0) StageArgs per instance, for tuning
-  frdev::BatchItem *items = nullptr;  // grouped by order
-  int itemStart[6] = {0, 0, 0, 0, 0, 0}, itemCount[6] = {0, 0, 0, 0, 0, 0};
-  int nItems = 0, grid = 0;           // persistent workgroups per order's launch
-  double tunedMs = 0;
-  hipGraph_t graph[2] = {nullptr, nullptr};
-  hipGraphExec_t exec[2] = {nullptr, nullptr};
-  hipStream_t ps[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};  // one per order (capture)
-  hipEvent_t fork = nullptr, join[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
-};
 
 namespace {
 
@@ -118,7 +23,7 @@ void freeAll(fr_ctx *c) {
                   c->Uf[0],   c->Uf[1],   c->divF[0], c->divF[1], c->divF[2], c->divF[3], c->fnbr,
                   c->finfo,   c->dUf,     c->dtG,     c->dtE,     c->adjA,   c->adjTr, c->adjTr2,
                   c->adjLam,  c->adjOut,  c->adjNu,   c->adjW[0], c->adjW[1], c->adjS[0], c->adjS[1],
-                  c->adjS[2], c->adjS[3]};
+                  c->adjS[2], c->adjS[3], c->adjQf,   c->adjB};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   for (auto &v : c->ev)
@@ -127,39 +32,6 @@ void freeAll(fr_ctx *c) {
       (void)hipEventDestroy(pr.second);
     }
   if (c->own) (void)hipStreamDestroy(c->own);
-}
-
-template <typename T>
-int dmalloc(T **p, size_t n) {
-  hipError_t e = hipMalloc((void **)p, (n ? n : 1) * sizeof(T));
-  if (e != hipSuccess) return fail(FR_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
-  return 0;
-}
-
-// Kernel timing bracket on the launch stream.
-struct TimeScope {
-  fr_ctx *c;
-  int id;
-  hipEvent_t a = nullptr, b = nullptr;
-  TimeScope(fr_ctx *c_, int id_) : c(c_), id(id_) {
-    if (c->timing) {
-      (void)hipEventCreate(&a);
-      (void)hipEventCreate(&b);
-      (void)hipEventRecord(a, c->stream);
-    }
-  }
-  ~TimeScope() {
-    if (c->timing) {
-      (void)hipEventRecord(b, c->stream);
-      c->ev[id].push_back({a, b});
-    }
-  }
-};
-
-// Time step source of the update kernels: dtE[ele * stride].
-std::pair<const double *, int> dtSource(const fr_ctx *c) {
-  if (c->dtType == 2) return {c->dtE, 1};
-  return {c->dtG, 0};
 }
 
 template <typename T>
@@ -188,9 +60,6 @@ int ensureTraces(fr_ctx *c, int which) {
   c->traceOf = which;
   return 0;
 }
-
-template <typename T>
-int launchStageArgs(fr_ctx *c, const frdev::StageArgs<T> &a, int mode);
 
 // Stage-kernel arguments: input state `in` (0 = U_spts, 1 = Us), traces in
 // Uf[cur], mode / stage / RK coefficient as in k_stage.
@@ -243,52 +112,6 @@ int launchStage(fr_ctx *c, int in, int mode, int step, double rk) {
     c->cur ^= 1;
     c->traceOf = (mode == 1) ? 1 : 0;
   }
-  return 0;
-}
-
-// Launch of the stage kernel variant for `mode` with arguments `a` (and the
-// gradient kernel before it on the viscous path).
-template <typename T>
-int launchStageArgs(fr_ctx *c, const frdev::StageArgs<T> &a, int mode) {
-  const bool bnd = c->conn.nIntFaces < (int)c->conn.faces.size();
-  const bool vis = c->cfg.viscous != 0;
-  hipError_t err = hipSuccess;
-  bool ok = dispatch(c->D, c->P, c->NF, [&](auto Dc, auto Pc, auto NFc) {
-    constexpr int D = decltype(Dc)::value, P = decltype(Pc)::value, NF = decltype(NFc)::value;
-    using G = frdev::ElemGeo<D, P>;
-    // grid padded to a multiple of the 8 XCDs (the kernels remap block ids)
-    const unsigned nb = (unsigned)(((c->nE + G::EPB - 1) / G::EPB + 7) / 8 * 8);
-    if (vis) {
-      frdev::GradArgs<T> g{a.Uin, a.UfIn, a.JG, a.detJ, a.fgeo, a.fnbr, a.finfo, a.oned, (T *)c->dUf, a.nEles, a.ph};
-      {
-        TimeScope ts(c, 3);
-        if (bnd) hipLaunchKernelGGL((frdev::k_grad<D, P, NF, T, true>), dim3(nb), dim3(G::NTH), 0, c->stream, g);
-        else hipLaunchKernelGGL((frdev::k_grad<D, P, NF, T, false>), dim3(nb), dim3(G::NTH), 0, c->stream, g);
-      }
-      if ((err = hipGetLastError()) != hipSuccess) return;
-      TimeScope ts(c, 0);
-      if (bnd) hipLaunchKernelGGL((frdev::k_stage<D, P, NF, T, true, true, false>), dim3(nb), dim3(G::NTH), 0,
-                                  c->stream, a);
-      else hipLaunchKernelGGL((frdev::k_stage<D, P, NF, T, false, true, false>), dim3(nb), dim3(G::NTH), 0,
-                              c->stream, a);
-    } else {
-      TimeScope ts(c, 0);
-      if (mode >= 2) {
-        if (bnd) hipLaunchKernelGGL((frdev::k_stage<D, P, NF, T, true, false, true>), dim3(nb), dim3(G::NTH), 0,
-                                    c->stream, a);
-        else hipLaunchKernelGGL((frdev::k_stage<D, P, NF, T, false, false, true>), dim3(nb), dim3(G::NTH), 0,
-                                c->stream, a);
-      } else {
-        if (bnd) hipLaunchKernelGGL((frdev::k_stage<D, P, NF, T, true, false, false>), dim3(nb), dim3(G::NTH), 0,
-                                    c->stream, a);
-        else hipLaunchKernelGGL((frdev::k_stage<D, P, NF, T, false, false, false>), dim3(nb), dim3(G::NTH), 0,
-                                c->stream, a);
-      }
-    }
-    err = hipGetLastError();
-  });
-  if (!ok) return fail(FR_EINVAL, "no kernel instance for this (nDims, order, nFields)");
-  if (err != hipSuccess) return fail(FR_EHIP, std::string("stage launch: ") + hipGetErrorString(err));
   return 0;
 }
 
@@ -356,53 +179,6 @@ int syncU32(fr_ctx *c) {
 // ---------------------------------------------------------------------------
 // batches
 // ---------------------------------------------------------------------------
-// The mode and RK coefficient of stage s of one update (solver::update).
-void stageMode(const fr_ctx *c, int s, int &mode, double &rk) {
-  if (s + 1 < c->nRK) {
-    mode = 1;
-    rk = c->RKa[s + 1];
-  } else if (c->cfg.timeType < 5) {
-    mode = 2;
-    rk = 0.0;
-  } else {
-    mode = 3;
-    rk = c->RKa[c->nRK];
-  }
-}
-
-// Stage s of the meshes of order P (P = 0: every order, one after the
-// other); parity < 0: the residual-only tuning launch.
-template <typename T>
-int batchLaunch(fr_batch *b, int parity, int s, int grid, hipStream_t st, int Ponly) {
-  const size_t n = b->ctx.size();
-  const frdev::StageArgs<T> *args = (parity < 0) ? (const frdev::StageArgs<T> *)b->argsTune
-                                                 : (const frdev::StageArgs<T> *)b->args[parity] + (size_t)s * n;
-  int mode = 0;
-  double rk;
-  if (parity >= 0) stageMode(b->ctx[0], s, mode, rk);
-  const bool fin = mode >= 2;
-  for (int P = 5; P >= 1; P--) {
-    const int cnt = b->itemCount[P];
-    if (!cnt || (Ponly && P != Ponly)) continue;
-    const frdev::BatchItem *it = b->items + b->itemStart[P];
-    const unsigned g = (unsigned)std::min(grid, cnt);
-    dispatch(b->D, P, b->NF, [&](auto Dc, auto Pc, auto NFc) {
-      constexpr int D = decltype(Dc)::value, PP = decltype(Pc)::value, NF = decltype(NFc)::value;
-      if (fin)
-        hipLaunchKernelGGL((frdev::k_batch_stage<D, PP, NF, T, true>), dim3(g), dim3(256), 0, st, args, it, cnt);
-      else
-        hipLaunchKernelGGL((frdev::k_batch_stage<D, PP, NF, T, false>), dim3(g), dim3(256), 0, st, args, it, cnt);
-    });
-    HIP_TRY(hipGetLastError());
-  }
-  return 0;
-}
-
-int batchLaunchStage(fr_batch *b, int parity, int s, int grid, hipStream_t st, int Ponly = 0) {
-  return b->prec == FR_PREC_FP32 ? batchLaunch<float>(b, parity, s, grid, st, Ponly)
-                                 : batchLaunch<double>(b, parity, s, grid, st, Ponly);
-}
-
 template <typename T>
 int batchArgs(fr_batch *b) {
   const size_t n = b->ctx.size();
@@ -465,7 +241,6 @@ void batchFree(fr_batch *b) {
 // ---------------------------------------------------------------------------
 int adjCheck(fr_ctx *c) {
   if (c->prec != FR_PREC_FP64) return fail(FR_EINVAL, "the adjoint needs an FP64 context");
-  if (c->cfg.viscous) return fail(FR_EINVAL, "the adjoint of the viscous residual is not available in this build");
   const size_t nSt = c->nState(), nTr = (size_t)c->nE * c->NF * c->NFP;
   int rc;
   if (!c->adjA) {
@@ -475,33 +250,8 @@ int adjCheck(fr_ctx *c) {
       return rc;
     for (int s = 1; s < c->nRK; s++)
       if ((rc = dmalloc(&c->adjS[s], nSt))) return rc;
+    if (c->cfg.viscous && ((rc = dmalloc(&c->adjQf, nTr * c->D)) || (rc = dmalloc(&c->adjB, nTr)))) return rc;
   }
-  return 0;
-}
-
-// out = (d divF / dU)^T lam at the state Us (device arrays, device layout)
-int vjpResidualDev(fr_ctx *c, const double *Us, const double *lam, double *out) {
-  const bool bnd = c->conn.nIntFaces < (int)c->conn.faces.size();
-  hipError_t err = hipSuccess;
-  bool ok = dispatch(c->D, c->P, c->NF, [&](auto Dc, auto Pc, auto NFc) {
-    constexpr int D = decltype(Dc)::value, P = decltype(Pc)::value, NF = decltype(NFc)::value;
-    using G = frdev::ElemGeo<D, P>;
-    const long nq = (long)c->nE * G::NFP;
-    hipLaunchKernelGGL((frdev::k_trace<D, P, NF, double>), dim3(nblk(nq)), dim3(256), 0, c->stream, Us, c->adjTr,
-                       c->oned64, c->nE);
-    hipLaunchKernelGGL((frdev::k_vjp_faces<D, P, NF>), dim3(nblk(nq)), dim3(256), 0, c->stream, lam, c->fgeo64,
-                       c->finfo, c->oned64, c->nE, c->adjA);
-    const unsigned nb = (unsigned)((c->nE + G::EPB - 1) / G::EPB);
-    if (bnd)
-      hipLaunchKernelGGL((frdev::k_vjp_state<D, P, NF, true>), dim3(nb), dim3(G::NTH), 0, c->stream, Us, lam,
-                         c->adjTr, c->adjA, c->JG64, c->fgeo64, c->fnbr, c->finfo, c->oned64, c->ph, c->nE, out);
-    else
-      hipLaunchKernelGGL((frdev::k_vjp_state<D, P, NF, false>), dim3(nb), dim3(G::NTH), 0, c->stream, Us, lam,
-                         c->adjTr, c->adjA, c->JG64, c->fgeo64, c->fnbr, c->finfo, c->oned64, c->ph, c->nE, out);
-    err = hipGetLastError();
-  });
-  if (!ok) return fail(FR_EINVAL, "no kernel instance for this (nDims, order, nFields)");
-  if (err != hipSuccess) return fail(FR_EHIP, std::string("adjoint launch: ") + hipGetErrorString(err));
   return 0;
 }
 
@@ -517,30 +267,15 @@ int stageStates(fr_ctx *c, const double **S) {
                          c->adjTr, c->oned64, c->nE);
     });
     HIP_TRY(hipGetLastError());
-    frdev::StageArgs<double> a{};
+    // the update's own arguments, redirected to the adjoint's buffers
+    frdev::StageArgs<double> a = makeStageArgs<double>(c, 0, 0, 1, s, c->RKa[s + 1]);
     a.Uin = S[s];
     a.UfIn = c->adjTr;
     a.UfOut = c->adjTr2;
-    a.JG = c->JG64;
-    a.fgeo = c->fgeo64;
-    a.fnbr = c->fnbr;
-    a.finfo = c->finfo;
-    a.oned = c->oned64;
     a.divF = nullptr;
-    a.Ubase = c->U;
-    a.UbaseT = c->U;
-    a.detJ = c->detJ;
     a.Unext = c->adjS[s + 1];
     a.Uout = nullptr;
     a.Uout32 = nullptr;
-    a.dprev0 = a.dprev1 = a.dprev2 = c->adjOut;
-    a.rk = c->RKa[s + 1];
-    a.dtE = dtSource(c).first;
-    a.dtStride = dtSource(c).second;
-    a.mode = 1;
-    a.nprev = s;
-    a.nEles = c->nE;
-    a.ph = c->ph;
     int rc = launchStageArgs<double>(c, a, 1);
     if (rc) return rc;
     S[s + 1] = c->adjS[s + 1];

@@ -1,0 +1,175 @@
+// fr_ctx.hpp -- the library's internal context (one per fr_create) and batch,
+// shared by the translation units of libfrhip: the ABI and host logic
+// (fr_api.hip), the stage-kernel launches (fr_stage64.hip / fr_stage32.hip),
+// the adjoint (fr_adjoint.hip) and the batched launches (fr_batch.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <type_traits>
+#include <utility>
+#include <vector>
+
+#include "../../include/fr.h"
+#include "fr_batch.hpp"
+#include "fr_host.hpp"
+#include "fr_kernels.hpp"
+
+namespace frimpl {
+
+inline thread_local std::string g_err;
+
+inline int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                              \
+  do {                                                                             \
+    hipError_t e_ = (expr);                                                        \
+    if (e_ != hipSuccess) return fail(FR_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <typename F>
+inline bool dispatch(int D, int P, int NF, F &&f) {
+#define FR_CASE(d, p, nf)                                                                          \
+  if (D == d && P == p && NF == nf) {                                                              \
+    f(std::integral_constant<int, d>{}, std::integral_constant<int, p>{}, std::integral_constant<int, nf>{}); \
+    return true;                                                                                   \
+  }
+  FR_CASE(3, 1, 5) FR_CASE(3, 2, 5) FR_CASE(3, 3, 5) FR_CASE(3, 4, 5) FR_CASE(3, 5, 5)
+  FR_CASE(3, 1, 1) FR_CASE(3, 2, 1) FR_CASE(3, 3, 1) FR_CASE(3, 4, 1) FR_CASE(3, 5, 1)
+  FR_CASE(2, 1, 4) FR_CASE(2, 2, 4) FR_CASE(2, 3, 4) FR_CASE(2, 4, 4) FR_CASE(2, 5, 4)
+  FR_CASE(2, 1, 1) FR_CASE(2, 2, 1) FR_CASE(2, 3, 1) FR_CASE(2, 4, 1) FR_CASE(2, 5, 1)
+#undef FR_CASE
+  return false;
+}
+
+inline unsigned nblk(long n, int b = 256) { return (unsigned)((n + b - 1) / b); }
+
+}  // namespace frimpl
+
+struct fr_ctx {
+  fr_config cfg{};
+  int prec = FR_PREC_FP64;
+  frhost::Operators ops;
+  frhost::Connectivity conn;
+  int D = 3, P = 3, NS = 0, NFP = 0, NFPF = 0, NFACE = 0, NF = 0, nE = 0, nRK = 0;
+  double RKa[5] = {0}, RKb[4] = {0};
+  frdev::Phys ph{};
+  hipStream_t own = nullptr, stream = nullptr;
+  // device buffers
+  double *U = nullptr, *U0 = nullptr, *detJ = nullptr, *JG64 = nullptr, *fgeo64 = nullptr, *ownN = nullptr;
+  double *wts = nullptr, *oned64 = nullptr, *part = nullptr;
+  float *U32 = nullptr, *JG32 = nullptr, *fgeo32 = nullptr, *oned32 = nullptr;
+  void *Us = nullptr;                 // RK stage state (residual precision)
+  void *Uf[2] = {nullptr, nullptr};   // face traces, double-buffered
+  int cur = 0;                        // Uf[cur] holds the traces of `traceOf`
+  int traceOf = -1;                   // 0: U_spts, 1: stage state Us, -1: stale
+  void *divF[4] = {nullptr, nullptr, nullptr, nullptr};
+  void *dUf = nullptr;                // viscous: gradient traces [ele][dim][field][fpt]
+  double *dtG = nullptr;              // the global time step (one value)
+  double *dtE = nullptr;              // per-element time steps (local time stepping)
+  int dtType = 0;                     // 0 fixed dt, 1 global CFL dt, 2 local CFL dt
+  // adjoint work buffers (FP64, allocated on first use)
+  double *adjA = nullptr, *adjTr = nullptr, *adjTr2 = nullptr;  // face cotangents, trace scratch
+  double *adjLam = nullptr, *adjOut = nullptr, *adjNu = nullptr, *adjW[2] = {nullptr, nullptr};
+  double *adjQf = nullptr, *adjB = nullptr;                     // viscous: gradient traces, jump cotangents
+  double *adjS[4] = {nullptr, nullptr, nullptr, nullptr};       // RK stage states
+  int32_t *fnbr = nullptr;
+  int8_t *finfo = nullptr;
+  size_t partCap = 0;
+  std::vector<double> X;  // host copy of element node coordinates [device ele][node][3]
+  std::vector<int> perm;  // device element -> mesh element (blocks of EPB face-connected elements)
+  std::vector<int> iperm; // mesh element -> device element
+  int epb = 1;
+  // timing
+  bool timing = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[4];
+  double ms[4] = {0, 0, 0, 0};
+  long cnt[4] = {0, 0, 0, 0};
+
+  size_t nState() const { return (size_t)nE * NF * NS; }
+  size_t tsize() const { return prec == FR_PREC_FP32 ? 4 : 8; }
+};
+
+// Many contexts advanced together: one persistent launch per RK stage over
+// all their element blocks, the launches of one step captured in a hipGraph.
+struct fr_batch {
+  std::vector<fr_ctx *> ctx;
+  int D = 2, NF = 4, nRK = 1, prec = FR_PREC_FP64;
+  hipStream_t stream = nullptr;
+  void *args[2] = {nullptr, nullptr};  // StageArgs per [trace parity][stage][instance]
+  void *argsTune = nullptr;            // residual-only (mode 0) StageArgs per instance, for tuning
+  frdev::BatchItem *items = nullptr;  // grouped by order
+  int itemStart[6] = {0, 0, 0, 0, 0, 0}, itemCount[6] = {0, 0, 0, 0, 0, 0};
+  int nItems = 0, grid = 0;           // persistent workgroups per order's launch
+  double tunedMs = 0;
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
+  hipStream_t ps[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};  // one per order (capture)
+  hipEvent_t fork = nullptr, join[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+};
+
+namespace frimpl {
+
+template <typename T>
+inline int dmalloc(T **p, size_t n) {
+  hipError_t e = hipMalloc((void **)p, (n ? n : 1) * sizeof(T));
+  if (e != hipSuccess) return fail(FR_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  return 0;
+}
+
+// Kernel timing bracket on the launch stream.
+struct TimeScope {
+  fr_ctx *c;
+  int id;
+  hipEvent_t a = nullptr, b = nullptr;
+  TimeScope(fr_ctx *c_, int id_) : c(c_), id(id_) {
+    if (c->timing) {
+      (void)hipEventCreate(&a);
+      (void)hipEventCreate(&b);
+      (void)hipEventRecord(a, c->stream);
+    }
+  }
+  ~TimeScope() {
+    if (c->timing) {
+      (void)hipEventRecord(b, c->stream);
+      c->ev[id].push_back({a, b});
+    }
+  }
+};
+
+// Time step source of the update kernels: dtE[ele * stride].
+inline std::pair<const double *, int> dtSource(const fr_ctx *c) {
+  if (c->dtType == 2) return {c->dtE, 1};
+  return {c->dtG, 0};
+}
+
+// The mode and RK coefficient of stage s of one update (solver::update).
+inline void stageMode(const fr_ctx *c, int s, int &mode, double &rk) {
+  if (s + 1 < c->nRK) {
+    mode = 1;
+    rk = c->RKa[s + 1];
+  } else if (c->cfg.timeType < 5) {
+    mode = 2;
+    rk = 0.0;
+  } else {
+    mode = 3;
+    rk = c->RKa[c->nRK];
+  }
+}
+
+// Launch of the stage kernel variant for `mode` (fr_stage64.hip / fr_stage32.hip).
+template <typename T>
+int launchStageArgs(fr_ctx *c, const frdev::StageArgs<T> &a, int mode);
+// (d divF / dU)^T lam at the state Us, device arrays (fr_adjoint.hip).
+int vjpResidualDev(fr_ctx *c, const double *Us, const double *lam, double *out);
+// Stage s of one batched step (fr_batch.hip); parity < 0 is the tuning launch,
+// Ponly != 0 restricts it to the meshes of that order.
+int batchLaunchStage(fr_batch *b, int parity, int s, int grid, hipStream_t st, int Ponly = 0);
+
+}  // namespace frimpl

@@ -881,7 +881,7 @@ __global__ __launch_bounds__(256) void k_dt(const double *__restrict__ U, const 
   dtE[e] = factor * vol / intWave;
 }
 
-__global__ void k_to_float(const double *__restrict__ a, float *__restrict__ b, long n) {
+static __global__ void k_to_float(const double *__restrict__ a, float *__restrict__ b, long n) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) b[i] = (float)a[i];
 }

@@ -158,9 +158,9 @@ int fr_set_dt(fr_ctx *ctx, double dt);
 int fr_calc_dt(fr_ctx *ctx, int dtType, double CFL, double *dtMin);
 int fr_get_dt(fr_ctx *ctx, double *dtEle /* nEles */);
 
-/* Adjoint (no reference counterpart: FlurryPP has no adjoint).  FP64,
- * inviscid and advection residuals; FR_EINVAL otherwise.  Arrays in the
- * reference layout.
+/* Adjoint (no reference counterpart: FlurryPP has no adjoint).  FP64
+ * contexts (FR_EINVAL otherwise): Euler, Navier-Stokes (LDG), advection and
+ * advection-diffusion.  Arrays in the reference layout.
  * fr_vjp_residual: out = (d divF / dU)^T lam at the current U_spts
  *   (the vector-Jacobian product of solver::calcResidual).
  * fr_vjp_step: lam0 = (d U_{n+1} / d U_n)^T lam1 for one solver::update

@@ -176,3 +176,80 @@ def test_vjp_steps_checkpointing_and_ke_gradient():
     adj = float(np.sum(got * dU))
     print("fd", fd, "adjoint", adj)
     assert _rel(fd, adj) < 1e-5
+
+
+# ---------------------------------------------------------------------------
+# viscous (LDG) residual
+# ---------------------------------------------------------------------------
+def _ns_cfg(D, P, **kw):
+    base = dict(viscous=1, mu_inf=0.05, prandtl=0.72, penFact=0.3, tau=0.7, fixVis=0, c_sth=0.4, rt_inf=0.3)
+    base.update(kw)
+    return euler_cfg(D, P, **base)
+
+
+@pytest.mark.parametrize("D", [2, 3])
+@pytest.mark.parametrize("periodic", [True, False])
+def test_vjp_residual_linear_advection_diffusion(D, periodic):
+    """Advection-diffusion with LDG is linear in U: exact transposition."""
+    cfg = advection_cfg(D, 3, viscous=1, diffD=0.1, penFact=0.4)
+    if D == 3:
+        m = hex_box(3, periodic=periodic, bc=None if periodic else "sup_out", L=2.0)
+    else:
+        m = quad_box(4, periodic=periodic, bc=None if periodic else QUAD_BCS)
+    g, U = _solver(cfg, m, lambda x, c: cases.ic_gaussian(x - (1.0 if D == 3 else 0.0), c), amp=0)
+    rng = np.random.default_rng(12)
+    lam = rng.standard_normal(U.shape)
+    v = rng.standard_normal(U.shape)
+    lhs = float(np.sum(lam * _residual(g, v)))
+    g.set_state(U)
+    rhs = float(np.sum(g.vjp_residual(lam) * v))
+    print("dot test", lhs, rhs)
+    assert _rel(lhs, rhs) < 1e-12
+
+
+@pytest.mark.parametrize("case", ["hex_periodic", "hex_walls", "quad_bcs"])
+def test_vjp_viscous_residual_vs_cpu_reference_gradient(case):
+    if case == "hex_periodic":
+        cfg, m, ic = _ns_cfg(3, 1, viscFix=0), hex_box(3), cases.ic_tgv
+    elif case == "hex_walls":
+        cfg = _ns_cfg(3, 1, viscFix=1, fixVis=1)
+        m = hex_box(3, periodic=False, bc=WALL_BCS, L=1.0)
+        ic = lambda x, c: cases.ic_uniform(x, c, vel=[0.3, 0.1, -0.05], p=1.0 / 1.4)
+    else:
+        cfg = _ns_cfg(2, 2, riemannType=1)
+        m = quad_box(3, periodic=False, bc=dict(QUAD_BCS, ymin="adiabatic_noslip"))
+        ic = lambda x, c: cases.ic_uniform(x, c, vel=[0.3, 0.2], p=1.0 / 1.4)
+    g, U = _solver(cfg, m, ic, amp=2e-2)
+    lam = np.random.default_rng(13).standard_normal(U.shape)
+    got = g.vjp_residual(lam)
+    want = _dense_jacobian_T_lam(cfg, m, U, lam)
+    err = np.abs(got - want).max() / np.abs(want).max()
+    print("max rel err vs CPU reference gradient", err)
+    assert err < 1e-6
+
+
+@pytest.mark.parametrize("viscFix", [0, 1])
+def test_vjp_viscous_residual_and_step_dot_fd(viscFix):
+    cfg = _ns_cfg(3, 2, viscFix=viscFix, dt=1e-3)
+    g, U = _solver(cfg, hex_box(3, periodic=False, bc=MIXED_BCS, L=1.0),
+                   lambda x, c: cases.ic_uniform(x, c, vel=[0.3, 0.1, -0.05], p=1.0 / 1.4), amp=2e-2)
+    rng = np.random.default_rng(14)
+    lam = rng.standard_normal(U.shape)
+    dU = U * rng.standard_normal(U.shape)
+    h = 1e-6
+    fd = float(np.sum(lam * (_residual(g, U + h * dU) - _residual(g, U - h * dU)))) / (2 * h)
+    g.set_state(U)
+    adj = float(np.sum(g.vjp_residual(lam) * dU))
+    print("residual: fd", fd, "adjoint", adj)
+    assert _rel(fd, adj) < 1e-7
+
+    def step(V):
+        g.set_state(V)
+        g.update(1)
+        return g.get_state()
+
+    fd = float(np.sum(lam * (step(U + h * dU) - step(U - h * dU)))) / (2 * h)
+    g.set_state(U)
+    adj = float(np.sum(g.vjp_step(lam) * dU))
+    print("step: fd", fd, "adjoint", adj)
+    assert _rel(fd, adj) < 1e-7
