@@ -196,6 +196,53 @@ def run_batch(n_inst=256, steps=20, warmup=3, seed=0):
             "speedup_vs_per_mesh_launches": v_b / v_s, "autotuned_grid": grid, "work_items": items}
 
 
+def tgv_ns_adjoint(n=24, steps=200, every=20, eps=1e-6, seed=0):
+    """The adjoint workload: 3D Navier-Stokes Taylor-Green (Re = 1600, Pr = 0.71,
+    constant viscosity, LDG), n^3 hexes, P = 3, RK4, dt = 1e-3.  J = kinetic
+    energy after `steps` steps; its gradient with respect to the initial
+    velocity field by the checkpointed adjoint (a state every `every` steps),
+    checked against a central finite difference (eps) along a random
+    direction."""
+    from fr_learn_amd import Solver, cases, mesh
+    cfg = cases.config(nDims=3, order=3, equation=1, viscous=1, mu_inf=1.0 / 1600, prandtl=0.71, fixVis=1,
+                       penFact=0.0, tau=1.0, timeType=4, dt=1e-3, gamma=1.4)
+    m = mesh.box(3, (n, n, n), (0, 0, 0), (2 * np.pi,) * 3)
+    s = Solver(cfg, m)
+    U0 = s.initialize(cases.ic_tgv)
+    rho, vel = U0[..., 0], U0[..., 1:4] / U0[..., :1]
+    p = (cfg["gamma"] - 1) * (U0[..., 4] - 0.5 * rho * (vel ** 2).sum(-1))
+
+    def state(v):  # U0 from the velocity field at fixed rho and p
+        return cases.conservative(rho, [v[..., i] for i in range(3)], p, cfg["gamma"])
+
+    def J(v):
+        s.set_state(state(v))
+        s.update(steps)
+        return s.kinetic_energy()[0]
+
+    s.set_state(U0)
+    t0 = time.perf_counter()
+    s.update(steps)
+    s.synchronize()
+    t_fwd = time.perf_counter() - t0
+    lamN = s.kinetic_energy_gradient()
+    s.set_state(U0)
+    t0 = time.perf_counter()
+    lam0 = s.vjp_steps(steps, every, lamN)
+    t_adj = time.perf_counter() - t0
+    grad_v = rho[..., None] * lam0[..., 1:4] + (rho[..., None] * vel) * lam0[..., 4:5]
+    dv = np.random.default_rng(seed).standard_normal(vel.shape)
+    fd = (J(vel + eps * dv) - J(vel - eps * dv)) / (2 * eps)
+    adj = float(np.sum(grad_v * dv))
+    s.close()
+    return {"metric": "adjoint gradient of KE(T) w.r.t. the initial velocity, 3D NS TGV",
+            "workload": f"{n}^3 hexes P3, Navier-Stokes Re 1600 Pr 0.71 (LDG), RK4 dt 1e-3, {steps} steps, "
+                        f"checkpoint every {every}",
+            "forward_s": t_fwd, "adjoint_s": t_adj, "adjoint_over_forward": t_adj / t_fwd,
+            "fd_directional": fd, "adjoint_directional": adj, "rel_diff_vs_fd": abs(fd - adj) / abs(fd),
+            "eps": eps}
+
+
 def traffic_from_profile(precision):
     """HBM bytes per launch of the residual kernel from the committed rocprofv3
     PMC summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor), if any."""
@@ -218,6 +265,8 @@ def main():
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
     ap.add_argument("--copy-calibration", action="store_true", help="only the 256 MiB copy for PMC scaling")
     ap.add_argument("--copy-bandwidth", action="store_true", help="only measure the 1 GiB copy bandwidth")
+    ap.add_argument("--adjoint", action="store_true",
+                    help="only the Navier-Stokes TGV adjoint workload (24^3 P3, 200 steps), its own JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU path is "
@@ -229,6 +278,9 @@ def main():
         return
     if args.copy_bandwidth:
         print(json.dumps({"copy_GBps": copy_bandwidth()}), flush=True)
+        return
+    if args.adjoint:
+        print(json.dumps(tgv_ns_adjoint()), flush=True)
         return
 
     copy_gbps = copy_bandwidth_child()  # before this process touches the GPU

@@ -233,8 +233,10 @@ __global__ __launch_bounds__((ElemGeo<D, P>::TPE)) void k_vjp_state_vis(
   using G = ElemGeo<D, P>;
   using O = OneD<P>;
   constexpr int NS = G::NS, NFP = G::NFP, NFPF = G::NFPF, P1 = G::P1, TPE = G::TPE, NFACE = G::NFACE;
-  constexpr int M = NF + D * NF;  // dual directions: the state, then the gradient [dim][field]
-  using DM = Dual<M>;
+  // Derivatives in D + 1 passes of NF dual directions each: pass 0 seeds the
+  // state, pass d + 1 the gradient's dimension d (one pass over all of them
+  // would carry NF (D + 1) partials per value and spill).
+  using DM = Dual<NF>;
   __shared__ double sU[NF][NS], sL[NF][NS];
   __shared__ double sJ[NF][NFP];       // lamJ
   __shared__ double sC[NF][NFP];       // forward LDG jumps u^ - u
@@ -271,52 +273,65 @@ __global__ __launch_bounds__((ElemGeo<D, P>::TPE)) void k_vjp_state_vis(
     const int fi = finfo[(size_t)e * NFACE + q / NFPF];
     const bool bnd = BND && fi >= FINFO_BOUNDARY;
     const bool right = !bnd && ((fi >> 5) & 1);
-    DM uo[NF], un[NF], n[D], go[D][NF], gn[D][NF], Fn[NF];
-#pragma unroll
-    for (int k = 0; k < NF; k++) uo[k] = DM::var(uov[k], k);
-#pragma unroll
-    for (int c = 0; c < D; c++) n[c] = DM(nv[c]);
-#pragma unroll
-    for (int d = 0; d < D; d++)
-#pragma unroll
-      for (int k = 0; k < NF; k++)
-        go[d][k] = DM::var(Qf[(((size_t)e * D + d) * NF + k) * NFP + q], NF + d * NF + k);
+    const int en = bnd ? 0 : fnbr[(size_t)e * NFACE + q / NFPF];
+    const int qn = bnd ? 0 : neighbourFpt<D, P>(q, fi);
     double g[NF], unv[NF];
-    if (bnd) {
-      const int bc = fi - FINFO_BOUNDARY;
-      boundaryState<D, NF, DM>(ph, bc, uo, n, un);
-      if constexpr (NF > 1) {
-        boundaryGradient<D, NF, DM>(ph, bc, uo, un, go, gn);
-      } else {
+#pragma unroll 1
+    for (int pass = 0; pass <= D; pass++) {
+      DM uo[NF], un[NF], n[D], go[D][NF], gn[D][NF], Fn[NF];
 #pragma unroll
-        for (int d = 0; d < D; d++) gn[d][0] = DM(0.0);
-      }
-      commonFlux<D, NF, DM>(ph, true, uo, un, n, Fn);
-      ldgViscousFlux<D, NF, DM>(ph, (bc == 10) ? 2 : 1, uo, un, go, gn, n, Fn);
+      for (int k = 0; k < NF; k++) uo[k] = (pass == 0) ? DM::var(uov[k], k) : DM(uov[k]);
 #pragma unroll
-      for (int k = 0; k < NF; k++) {
-        g[k] = lj[k] * dA;
-        unv[k] = un[k].v;
-      }
-    } else {
-      const int en = fnbr[(size_t)e * NFACE + q / NFPF];
-      const int qn = neighbourFpt<D, P>(q, fi);
-#pragma unroll
-      for (int k = 0; k < NF; k++) {
-        unv[k] = Uf[((size_t)en * NF + k) * NFP + qn];
-        un[k] = DM(unv[k]);
-        g[k] = lj[k] * (right ? -dA : dA) + A[((size_t)en * NF + k) * NFP + qn];
-      }
+      for (int c = 0; c < D; c++) n[c] = DM(nv[c]);
 #pragma unroll
       for (int d = 0; d < D; d++)
 #pragma unroll
-        for (int k = 0; k < NF; k++) gn[d][k] = DM(Qf[(((size_t)en * D + d) * NF + k) * NFP + qn]);
-      if (right) {
-        commonFlux<D, NF, DM>(ph, false, un, uo, n, Fn);
-        ldgViscousFlux<D, NF, DM>(ph, 0, un, uo, gn, go, n, Fn);
+        for (int k = 0; k < NF; k++) {
+          const double v = Qf[(((size_t)e * D + d) * NF + k) * NFP + q];
+          go[d][k] = (pass == d + 1) ? DM::var(v, k) : DM(v);
+        }
+      if (bnd) {
+        const int bc = fi - FINFO_BOUNDARY;
+        boundaryState<D, NF, DM>(ph, bc, uo, n, un);
+        if constexpr (NF > 1) {
+          boundaryGradient<D, NF, DM>(ph, bc, uo, un, go, gn);
+        } else {
+#pragma unroll
+          for (int d = 0; d < D; d++) gn[d][0] = DM(0.0);
+        }
+        commonFlux<D, NF, DM>(ph, true, uo, un, n, Fn);
+        ldgViscousFlux<D, NF, DM>(ph, (bc == 10) ? 2 : 1, uo, un, go, gn, n, Fn);
+#pragma unroll
+        for (int k = 0; k < NF; k++) {
+          g[k] = lj[k] * dA;
+          unv[k] = un[k].v;
+        }
       } else {
-        commonFlux<D, NF, DM>(ph, false, uo, un, n, Fn);
-        ldgViscousFlux<D, NF, DM>(ph, 0, uo, un, go, gn, n, Fn);
+#pragma unroll
+        for (int k = 0; k < NF; k++) {
+          unv[k] = Uf[((size_t)en * NF + k) * NFP + qn];
+          un[k] = DM(unv[k]);
+          g[k] = lj[k] * (right ? -dA : dA) + A[((size_t)en * NF + k) * NFP + qn];
+        }
+#pragma unroll
+        for (int d = 0; d < D; d++)
+#pragma unroll
+          for (int k = 0; k < NF; k++) gn[d][k] = DM(Qf[(((size_t)en * D + d) * NF + k) * NFP + qn]);
+        if (right) {
+          commonFlux<D, NF, DM>(ph, false, un, uo, n, Fn);
+          ldgViscousFlux<D, NF, DM>(ph, 0, un, uo, gn, go, n, Fn);
+        } else {
+          commonFlux<D, NF, DM>(ph, false, uo, un, n, Fn);
+          ldgViscousFlux<D, NF, DM>(ph, 0, uo, un, go, gn, n, Fn);
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < NF; j++) {
+        double acc = 0;
+#pragma unroll
+        for (int k = 0; k < NF; k++) acc += g[k] * Fn[k].d[j];
+        if (pass == 0) sUo[j][q] = acc;
+        else sQo[pass - 1][j][q] = acc;
       }
     }
     // forward LDG jump of this point (as k_grad forms it)
@@ -324,14 +339,6 @@ __global__ __launch_bounds__((ElemGeo<D, P>::TPE)) void k_vjp_state_vis(
     ldgJump<D, NF, double>(ph, bnd, right, uov, unv, nv, jmp);
 #pragma unroll
     for (int k = 0; k < NF; k++) sC[k][q] = jmp[k];
-#pragma unroll
-    for (int j = 0; j < M; j++) {
-      double acc = 0;
-#pragma unroll
-      for (int k = 0; k < NF; k++) acc += g[k] * Fn[k].d[j];
-      if (j < NF) sUo[j][q] = acc;
-      else sQo[(j - NF) / NF][(j - NF) % NF][q] = acc;
-    }
   }
   __syncthreads();
   double lU[NF];
@@ -383,25 +390,28 @@ __global__ __launch_bounds__((ElemGeo<D, P>::TPE)) void k_vjp_state_vis(
         for (int d = 0; d < D; d++) acc += pJ[d * D + d2] * lFt[d][k];
         lF[d2][k] = acc;
       }
-    DM u[NF], q[D][NF], Fi[D][NF], Fv[D][NF];
-#pragma unroll
-    for (int k = 0; k < NF; k++) u[k] = DM::var(sU[k][t], k);
-#pragma unroll
-    for (int d = 0; d < D; d++)
-#pragma unroll
-      for (int k = 0; k < NF; k++) q[d][k] = DM::var(Q[d][k], NF + d * NF + k);
-    inviscidFlux<D, NF, DM>(ph, u, Fi);
-    viscousFlux<D, NF, DM>(ph, u, q, Fv);
     double lQ[D][NF];
+#pragma unroll 1
+    for (int pass = 0; pass <= D; pass++) {
+      DM u[NF], qd[D][NF], Fi[D][NF], Fv[D][NF];
 #pragma unroll
-    for (int j = 0; j < M; j++) {
-      double acc = 0;
+      for (int k = 0; k < NF; k++) u[k] = (pass == 0) ? DM::var(sU[k][t], k) : DM(sU[k][t]);
 #pragma unroll
-      for (int d2 = 0; d2 < D; d2++)
+      for (int d = 0; d < D; d++)
 #pragma unroll
-        for (int k = 0; k < NF; k++) acc += lF[d2][k] * (Fi[d2][k].d[j] + Fv[d2][k].d[j]);
-      if (j < NF) lU[j] = acc;
-      else lQ[(j - NF) / NF][(j - NF) % NF] = acc;
+        for (int k = 0; k < NF; k++) qd[d][k] = (pass == d + 1) ? DM::var(Q[d][k], k) : DM(Q[d][k]);
+      viscousFlux<D, NF, DM>(ph, u, qd, Fv);
+      if (pass == 0) inviscidFlux<D, NF, DM>(ph, u, Fi);
+#pragma unroll
+      for (int j = 0; j < NF; j++) {
+        double acc = 0;
+#pragma unroll
+        for (int d2 = 0; d2 < D; d2++)
+#pragma unroll
+          for (int k = 0; k < NF; k++) acc += lF[d2][k] * ((pass == 0 ? Fi[d2][k].d[j] : 0.0) + Fv[d2][k].d[j]);
+        if (pass == 0) lU[j] = acc;
+        else lQ[pass - 1][j] = acc;
+      }
     }
     // gradient traces and state traces, transposed
 #pragma unroll

@@ -253,3 +253,16 @@ def test_vjp_viscous_residual_and_step_dot_fd(viscFix):
     adj = float(np.sum(g.vjp_step(lam) * dU))
     print("step: fd", fd, "adjoint", adj)
     assert _rel(fd, adj) < 1e-7
+
+
+def test_ns_tgv_adjoint_workload_small():
+    """The Navier-Stokes TGV adjoint workload of bench.py --adjoint at 4^3 and
+    12 steps (checkpoint every 5): gradient of KE(T) with respect to the
+    initial velocity against a central difference (bar 1e-5)."""
+    import os
+    import sys
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    import bench
+    r = bench.tgv_ns_adjoint(n=4, steps=12, every=5)
+    print(r)
+    assert r["rel_diff_vs_fd"] < 1e-5
