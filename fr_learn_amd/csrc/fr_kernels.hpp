@@ -157,12 +157,14 @@ __device__ __forceinline__ void ldgJump(const Phys &ph, bool bnd, bool right, co
 // flux-point jumps dUc (only the two faces normal to the axis are nonzero, in
 // ascending flux-point order), then transformed with JGinv / detJ.
 // sUe: [NF][NS] state, sCe: [NF][NFP] jumps, J: JGinv(dimRef, dimPhys).
-template <int D, int P, int NF, typename T>
+// PAD: sUe uses the padded LDS rows (ElemGeo::NSP, padIdx).
+template <int D, int P, int NF, typename T, bool PAD = false>
 __device__ __forceinline__ void correctedGradient(const T *s1, const T *sUe, const T *sCe, int t, int si, int sj,
                                                   int sk, const T *J, T detJ, T g[D][NF]) {
   using G = ElemGeo<D, P>;
   using O = OneD<P>;
-  constexpr int NS = G::NS, NFP = G::NFP, P1 = G::P1, NFACE = G::NFACE;
+  constexpr int NFP = G::NFP, P1 = G::P1, NFACE = G::NFACE;
+  constexpr int NS = PAD ? G::NSP : G::NS;
   int cq[NFACE], cside[NFACE], cax[NFACE];
   corrEntries<D, P>(t, cq, cside, cax);
   T gr[D][NF];
@@ -175,7 +177,7 @@ __device__ __forceinline__ void correctedGradient(const T *s1, const T *sUe, con
 #pragma unroll
       for (int m = 0; m < P1; m++) {
         const int sp = (d == 0) ? m + P1 * (sj + P1 * sk) : (d == 1) ? si + P1 * (m + P1 * sk) : si + P1 * (sj + P1 * m);
-        acc += s1[O::DM + id * P1 + m] * sUe[k * NS + sp];
+        acc += s1[O::DM + id * P1 + m] * sUe[k * NS + (PAD ? padIdx<P>(sp) : sp)];
       }
       T corr = (T)0;
 #pragma unroll
@@ -321,8 +323,8 @@ __global__ __launch_bounds__((ElemGeo<D, P>::NTH)) void k_grad(GradArgs<T> a) {
 template <int D, int P, int NF, typename T>
 struct StageSmem {
   using G = ElemGeo<D, P>;
-  T sU[G::EPB][NF][G::NS];    // stage state
-  T sFd[G::EPB][NF][G::NS];   // contravariant flux F~_d of one reference axis, later the updated state
+  T sU[G::EPB][NF][G::NSP];   // stage state (padded rows, padIdx)
+  T sFd[G::EPB][NF][G::NSP];  // contravariant flux F~_d of one reference axis, later the updated state
   T sJ[G::EPB][NF][G::NFP];   // normal-flux jump Fn*dA - disFn at the flux points
   T s1[OneD<P>::SIZE];        // 1D factor table
 };
@@ -349,6 +351,7 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
   auto &sJ = sm.sJ;
   auto &s1 = sm.s1;
   const int slot = threadIdx.x / TPE, t = threadIdx.x - slot * TPE;
+  const int tP = padIdx<P>(t);  // padded LDS index of this thread's solution point
   const int e = b * EPB + slot;
   const bool act = e < a.nEles;
   const bool hasS = act && t < NS;
@@ -431,7 +434,10 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
   if (act) {
 #pragma unroll
     for (int j = 0; j < NU; j++)
-      if (t + j * TPE < NF * NS) (&sU[slot][0][0])[t + j * TPE] = ul[j];
+      if (t + j * TPE < NF * NS) {
+        const int f = t + j * TPE, k = f / NS;
+        sU[slot][k][padIdx<P>(f - k * NS)] = ul[j];
+      }
   }
   __syncthreads();
 
@@ -454,13 +460,14 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
       if (act && q < NFP) {
         int base, stride, side, dn;
         fptLine<D, P>(q, base, stride, side, dn);
+        const int lb = padIdx<P>(base), ls = padStride<P>(stride);
         const T *l1 = s1 + (side ? O::LR : O::LL);
         T uo[NF], un[NF], dUc[NF];
 #pragma unroll
         for (int k = 0; k < NF; k++) {
           T acc = (T)0;
 #pragma unroll
-          for (int m = 0; m < P1; m++) acc += l1[m] * sU[slot][k][base + m * stride];
+          for (int m = 0; m < P1; m++) acc += l1[m] * sU[slot][k][lb + m * ls];
           uo[k] = acc;
         }
         const int fi = pfi[r];
@@ -484,11 +491,11 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
   if (hasS) {
     T u[NF];
 #pragma unroll
-    for (int k = 0; k < NF; k++) u[k] = sU[slot][k][t];
+    for (int k = 0; k < NF; k++) u[k] = sU[slot][k][tP];
     inviscidFlux<D, NF, T>(a.ph, u, F);
     if constexpr (VIS) {
       T g[D][NF], Fv[D][NF];
-      correctedGradient<D, P, NF, T>(s1, &sU[slot][0][0], &sJ[slot][0][0], t, si, sj, sk, pJ, (T)pdet, g);
+      correctedGradient<D, P, NF, T, true>(s1, &sU[slot][0][0], &sJ[slot][0][0], t, si, sj, sk, pJ, (T)pdet, g);
       viscousFlux<D, NF, T>(a.ph, u, g, Fv);
 #pragma unroll
       for (int d = 0; d < D; d++)
@@ -507,7 +514,7 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
         T acc = (T)0;
 #pragma unroll
         for (int d2 = 0; d2 < D; d2++) acc += pJ[d * D + d2] * F[d2][k];
-        sFd[slot][k][t] = acc;
+        sFd[slot][k][tP] = acc;
       }
     }
     elementSync<TPE>();
@@ -518,7 +525,10 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
         T acc = (T)0;
 #pragma unroll
         for (int m = 0; m < P1; m++) {
-          const int sp = (d == 0) ? m + P1 * (sj + P1 * sk) : (d == 1) ? si + P1 * (m + P1 * sk) : si + P1 * (sj + P1 * m);
+          // padded index of the line point (padIdx of the unpadded one)
+          const int sp = (d == 0)   ? m + (P1 + 1) * (sj + P1 * sk)
+                         : (d == 1) ? si + (P1 + 1) * (m + P1 * sk)
+                                    : si + (P1 + 1) * (sj + P1 * m);
           acc += s1[O::DM + id * P1 + m] * sFd[slot][k][sp];
         }
         div[k] = (d == 0) ? acc : acc + div[k];
@@ -528,13 +538,14 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
       const int q = dirFace<D>(d, t / NFPF) * NFPF + t % NFPF;
       int base, stride, side, dn;
       fptLine<D, P>(q, base, stride, side, dn);
+      const int lb = padIdx<P>(base), ls = padStride<P>(stride);
       const T *l1 = s1 + (side ? O::LR : O::LL);
       const T sg = side ? (T)1 : (T)-1;
 #pragma unroll
       for (int k = 0; k < NF; k++) {
         T acc = (T)0;
 #pragma unroll
-        for (int m = 0; m < P1; m++) acc += (l1[m] * sg) * sFd[slot][k][base + m * stride];
+        for (int m = 0; m < P1; m++) acc += (l1[m] * sg) * sFd[slot][k][lb + m * ls];
         sJ[slot][k][q] = -acc;
       }
     }
@@ -549,13 +560,14 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
     if (act && q < NFP) {
       int base, stride, side, dn;
       fptLine<D, P>(q, base, stride, side, dn);
+      const int lb = padIdx<P>(base), ls = padStride<P>(stride);
       const T *l1 = s1 + (side ? O::LR : O::LL);
       T uo[NF];
 #pragma unroll
       for (int k = 0; k < NF; k++) {
         T acc = (T)0;
 #pragma unroll
-        for (int m = 0; m < P1; m++) acc += l1[m] * sU[slot][k][base + m * stride];
+        for (int m = 0; m < P1; m++) acc += l1[m] * sU[slot][k][lb + m * ls];
         uo[k] = acc;
       }
       const int fi = pfi[r];
@@ -570,13 +582,14 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
         // the neighbour's trace from its state in LDS, as it would extrapolate it
         int nbase, nstride, nside, ndn;
         fptLine<D, P>(neighbourFpt<D, P>(q, fi), nbase, nstride, nside, ndn);
+        const int npb = padIdx<P>(nbase), nps = padStride<P>(nstride);
         const T *ln = s1 + (nside ? O::LR : O::LL);
         const T *Un = &sU[pnb[r] - blockBase][0][0];
 #pragma unroll
         for (int k = 0; k < NF; k++) {
           T acc = (T)0;
 #pragma unroll
-          for (int m = 0; m < P1; m++) acc += ln[m] * Un[k * NS + nbase + m * nstride];
+          for (int m = 0; m < P1; m++) acc += ln[m] * Un[k * G::NSP + npb + m * nps];
           un[k] = acc;
         }
       } else {
@@ -667,7 +680,7 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
         a.Uout[gi] = u;
         if (a.Uout32) a.Uout32[gi] = (float)u;
       }
-      sFd[slot][k][t] = (T)u;
+      sFd[slot][k][tP] = (T)u;
     }
   }
   if (a.mode == 0) return;
@@ -684,12 +697,13 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
       if (!VIS && (pnb[r] < 0 || inBlock(pnb[r]))) continue;
       int base, stride, side, dn;
       fptLine<D, P>(q, base, stride, side, dn);
+      const int lb = padIdx<P>(base), ls = padStride<P>(stride);
       const T *l1 = s1 + (side ? O::LR : O::LL);
 #pragma unroll
       for (int k = 0; k < NF; k++) {
         T acc = (T)0;
 #pragma unroll
-        for (int m = 0; m < P1; m++) acc += l1[m] * sFd[slot][k][base + m * stride];
+        for (int m = 0; m < P1; m++) acc += l1[m] * sFd[slot][k][lb + m * ls];
         Fe[k * NFP + q] = acc;
       }
     }

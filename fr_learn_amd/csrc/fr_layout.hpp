@@ -32,7 +32,25 @@ struct ElemGeo {
   static constexpr int TPE = nextPow2(NS) < 8 ? 8 : nextPow2(NS);  // threads per element
   static constexpr int EPB = TPE >= 256 ? 1 : 256 / TPE;           // elements per block
   static constexpr int NTH = TPE * EPB;
+  // LDS rows of solution-point data are padded by one entry per x-line, so
+  // that lanes walking y- or x-face lines (strides P+1, 1 with bases P+1
+  // apart) fall in different banks
+  static constexpr int NSP = NS + NS / P1;
 };
+
+// Padded LDS index of solution point s (one pad entry per x-line).
+template <int P>
+__host__ __device__ __forceinline__ int padIdx(int s) {
+  return s + s / (P + 1);
+}
+
+// Padded stride of a solution-point line of stride `stride` (1, P+1 or
+// (P+1)^2): every line starts on an x-line boundary or moves along x, so
+// padIdx(base + m*stride) = padIdx(base) + m*padStride(stride).
+template <int P>
+__host__ __device__ __forceinline__ int padStride(int stride) {
+  return stride + stride / (P + 1);
+}
 
 // Solution-point line behind flux point q: spts base + m*stride (m = 0..P)
 // with 1D weights l_R (side 1) or l_L (side 0); dn = reference normal axis.
