@@ -87,6 +87,7 @@ SIGNATURES = {
     "fr_reduce": (ctypes.c_int, [_P, ctypes.c_int, _DP]),
     "fr_set_stream": (ctypes.c_int, [_P, _P]),
     "fr_synchronize": (ctypes.c_int, [_P]),
+    "fr_geometry_mode": (ctypes.c_int, [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
     "fr_set_timing": (ctypes.c_int, [_P, ctypes.c_int]),
     "fr_kernel_times": (ctypes.c_int, [_P, ctypes.c_int, _DP, ctypes.POINTER(ctypes.c_long)]),
     "fr_residual_bytes": (ctypes.c_int, [_P, _DP]),

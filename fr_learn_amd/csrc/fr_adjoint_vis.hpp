@@ -18,7 +18,7 @@ hipError_t vjpViscousLaunchP(fr_ctx *c, const double *Us, const double *lam, dou
   using G = frdev::ElemGeo<D, P>;
   const unsigned nbg = (unsigned)(((c->nE + G::EPB - 1) / G::EPB + 7) / 8 * 8);
   frdev::GradArgs<double> ga{Us, c->adjTr, c->JG64, c->detJ, c->fgeo64, c->fnbr, c->finfo, c->oned64, c->adjQf,
-                             c->nE, c->ph};
+                             c->nE, c->ph, /*affine=*/0};
   if (bnd) {
     hipLaunchKernelGGL((frdev::k_grad<D, P, NF, double, true>), dim3(nbg), dim3(G::NTH), 0, c->stream, ga);
     hipLaunchKernelGGL((frdev::k_vjp_state_vis<D, P, NF, true>), dim3(c->nE), dim3(G::TPE), 0, c->stream, Us, lam,

@@ -23,7 +23,8 @@ void freeAll(fr_ctx *c) {
                   c->Uf[0],   c->Uf[1],   c->divF[0], c->divF[1], c->divF[2], c->divF[3], c->fnbr,
                   c->finfo,   c->dUf,     c->dtG,     c->dtE,     c->adjA,   c->adjTr, c->adjTr2,
                   c->adjLam,  c->adjOut,  c->adjNu,   c->adjW[0], c->adjW[1], c->adjS[0], c->adjS[1],
-                  c->adjS[2], c->adjS[3], c->adjQf,   c->adjB};
+                  c->adjS[2], c->adjS[3], c->adjQf,   c->adjB,    c->JGc64,  c->detJc, c->fgeoc64,
+                  c->JGc32,   c->fgeoc32};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   for (auto &v : c->ev)
@@ -70,15 +71,22 @@ frdev::StageArgs<T> makeStageArgs(fr_ctx *c, int in, int cur, int mode, int step
   a.Uin = stateIn<T>(c, in);
   a.UfIn = (const T *)c->Uf[cur];
   a.UfOut = (T *)c->Uf[cur ^ 1];
-  a.JG = f32 ? (const T *)c->JG32 : (const T *)c->JG64;
-  a.fgeo = f32 ? (const T *)c->fgeo32 : (const T *)c->fgeo64;
+  const bool aff = c->affineMesh && c->affineUse;
+  a.affine = aff ? 1 : 0;
+  if (aff) {
+    a.JG = f32 ? (const T *)c->JGc32 : (const T *)c->JGc64;
+    a.fgeo = f32 ? (const T *)c->fgeoc32 : (const T *)c->fgeoc64;
+  } else {
+    a.JG = f32 ? (const T *)c->JG32 : (const T *)c->JG64;
+    a.fgeo = f32 ? (const T *)c->fgeo32 : (const T *)c->fgeo64;
+  }
   a.fnbr = c->fnbr;
   a.finfo = c->finfo;
   a.oned = f32 ? (const T *)c->oned32 : (const T *)c->oned64;
   a.divF = (T *)c->divF[step];
   a.Ubase = c->U;
   a.UbaseT = f32 ? (const T *)c->U32 : (const T *)c->U;
-  a.detJ = c->detJ;
+  a.detJ = aff ? c->detJc : c->detJ;
   a.Unext = (T *)c->Us;
   a.Uout = c->U;
   a.Uout32 = f32 ? c->U32 : nullptr;
@@ -476,6 +484,36 @@ int createImpl(const fr_config *cfg, const fr_mesh *m, int precision, fr_ctx *c)
     (void)hipFree(dX); (void)hipFree(dS); (void)hipFree(dF); (void)hipFree(dT); (void)hipFree(dNeg);
     (void)hipFree(pairEF);
     if (neg) return fail(FR_EJACOBIAN, "Negative Jacobian at solution points.");
+  }
+  // affine meshes: compact per-element / per-face geometry for the stage kernels
+  {
+    const size_t nJ = (size_t)nE * D * D, nG = (size_t)nE * (D + 1) * c->NFACE;
+    int8_t *dOk;
+    if ((rc = dmalloc(&c->JGc64, nJ)) || (rc = dmalloc(&c->detJc, nE)) || (rc = dmalloc(&c->fgeoc64, nG)) ||
+        (rc = dmalloc(&dOk, nE)))
+      return rc;
+    const double tol = 1e-13;
+    if (D == 3)
+      hipLaunchKernelGGL(frdev::k_affine<3>, dim3(nblk(nE)), dim3(256), 0, c->stream, c->JG64, c->detJ, c->fgeo64,
+                         NS, NFP, c->NFPF, nE, tol, c->JGc64, c->detJc, c->fgeoc64, dOk);
+    else
+      hipLaunchKernelGGL(frdev::k_affine<2>, dim3(nblk(nE)), dim3(256), 0, c->stream, c->JG64, c->detJ, c->fgeo64,
+                         NS, NFP, c->NFPF, nE, tol, c->JGc64, c->detJc, c->fgeoc64, dOk);
+    HIP_TRY(hipGetLastError());
+    std::vector<int8_t> ok(nE);
+    HIP_TRY(hipMemcpyAsync(ok.data(), dOk, nE, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    (void)hipFree(dOk);
+    c->affineMesh = true;
+    for (int e = 0; e < nE; e++) c->affineMesh = c->affineMesh && ok[e];
+    if (c->affineMesh && c->prec == FR_PREC_FP32) {
+      if ((rc = dmalloc(&c->JGc32, nJ)) || (rc = dmalloc(&c->fgeoc32, nG))) return rc;
+      hipLaunchKernelGGL(frdev::k_to_float, dim3(nblk((long)nJ)), dim3(256), 0, c->stream, c->JGc64, c->JGc32,
+                         (long)nJ);
+      hipLaunchKernelGGL(frdev::k_to_float, dim3(nblk((long)nG)), dim3(256), 0, c->stream, c->fgeoc64, c->fgeoc32,
+                         (long)nG);
+      HIP_TRY(hipGetLastError());
+    }
   }
   HIP_TRY(hipMemcpy(c->wts, c->ops.wts.data(), NS * 8, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(c->oned64, c->ops.oned.data(), c->ops.oned.size() * 8, hipMemcpyHostToDevice));
@@ -1109,6 +1147,13 @@ int fr_set_stream(fr_ctx *c, void *stream) {
 int fr_synchronize(fr_ctx *c) {
   if (!c) return fail(FR_EINVAL, "null argument");
   HIP_TRY(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int fr_geometry_mode(fr_ctx *c, int request, int *active) {
+  if (!c || request < -1 || request > 1) return fail(FR_EINVAL, "bad argument");
+  if (request >= 0) c->affineUse = request == 1;
+  if (active) *active = (c->affineMesh && c->affineUse) ? 1 : 0;
   return 0;
 }
 

@@ -65,6 +65,11 @@ struct fr_ctx {
   double *U = nullptr, *U0 = nullptr, *detJ = nullptr, *JG64 = nullptr, *fgeo64 = nullptr, *ownN = nullptr;
   double *wts = nullptr, *oned64 = nullptr, *part = nullptr;
   float *U32 = nullptr, *JG32 = nullptr, *fgeo32 = nullptr, *oned32 = nullptr;
+  // compact geometry of affine meshes (frdev::GeoAddr): per element / face
+  double *JGc64 = nullptr, *detJc = nullptr, *fgeoc64 = nullptr;
+  float *JGc32 = nullptr, *fgeoc32 = nullptr;
+  bool affineMesh = false;  // every element affine (compact arrays built)
+  bool affineUse = true;    // the stage kernels read the compact arrays when affineMesh
   void *Us = nullptr;                 // RK stage state (residual precision)
   void *Uf[2] = {nullptr, nullptr};   // face traces, double-buffered
   int cur = 0;                        // Uf[cur] holds the traces of `traceOf`

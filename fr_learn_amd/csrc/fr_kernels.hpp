@@ -79,6 +79,28 @@ __global__ __launch_bounds__(256) void k_trace(const T *__restrict__ U, T *__res
 // The state an element reads (Uin / Ubase) is read only by that element, so
 // Unext / Uout may alias them; traces are double-buffered (UfIn != UfOut).
 // ---------------------------------------------------------------------------
+// Geometry addressing.  Per point: JG[ele][c][spt], detJ[ele][spt],
+// fgeo[ele][c][fpt].  Affine meshes (every element's metrics and detJ the
+// same at all its solution points, every face's normal and dA the same at all
+// its flux points): JG[ele][c], detJ[ele], fgeo[ele][c][face] -- one value
+// per element or face (34 instead of 1024 values per hex element at P = 3).
+template <int D, int P>
+struct GeoAddr {
+  int gS, gP, fS;  // (ele, c) stride and point offset of JG/detJ; (ele, c) stride of fgeo
+  __device__ __forceinline__ GeoAddr(int affine, int spt) {
+    gS = affine ? 1 : ElemGeo<D, P>::NS;
+    gP = affine ? 0 : spt;
+    fS = affine ? ElemGeo<D, P>::NFACE : ElemGeo<D, P>::NFP;
+    aff = affine;
+  }
+  __device__ __forceinline__ size_t jg(size_t e, int c) const { return (e * D * D + c) * gS + gP; }
+  __device__ __forceinline__ size_t det(size_t e) const { return e * gS + gP; }
+  __device__ __forceinline__ size_t fg(size_t e, int c, int q) const {
+    return (e * (D + 1) + c) * fS + (aff ? q / ElemGeo<D, P>::NFPF : q);
+  }
+  int aff;
+};
+
 template <typename T>
 struct StageArgs {
   const T *Uin;
@@ -103,6 +125,7 @@ struct StageArgs {
   double c0, c1, c2, c3, rk;
   int mode, nprev, nEles;
   Phys ph;
+  int affine;  // geometry layout (GeoAddr)
 };
 
 // Face f of the two faces normal to reference axis d (hex: x 2,3; y 4,5;
@@ -217,6 +240,7 @@ struct GradArgs {
   T *__restrict__ dUf;
   int nEles;
   Phys ph;
+  int affine;  // geometry layout (GeoAddr)
 };
 
 template <int D, int P, int NF, typename T, bool BND>
@@ -241,10 +265,11 @@ __global__ __launch_bounds__((ElemGeo<D, P>::NTH)) void k_grad(GradArgs<T> a) {
   }
   T pJ[D * D];
   T pdet = (T)1;
+  const GeoAddr<D, P> ga(a.affine, t);
   if (hasS) {
 #pragma unroll
-    for (int c = 0; c < D * D; c++) pJ[c] = a.JG[((size_t)e * D * D + c) * NS + t];
-    pdet = (T)a.detJ[(size_t)e * NS + t];
+    for (int c = 0; c < D * D; c++) pJ[c] = a.JG[ga.jg(e, c)];
+    pdet = (T)a.detJ[ga.det(e)];
   }
   __syncthreads();
   // LDG jump at every flux point
@@ -264,7 +289,7 @@ __global__ __launch_bounds__((ElemGeo<D, P>::NTH)) void k_grad(GradArgs<T> a) {
         uo[k] = acc;
       }
 #pragma unroll
-      for (int c = 0; c < D; c++) n[c] = a.fgeo[((size_t)e * (D + 1) + c) * NFP + q];
+      for (int c = 0; c < D; c++) n[c] = a.fgeo[ga.fg(e, c, q)];
       const int fi = a.finfo[(size_t)e * NFACE + q / NFPF];
       const bool bnd = BND && fi >= FINFO_BOUNDARY;
       if (bnd) {
@@ -388,20 +413,21 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
 #pragma unroll
     for (int j = 0; j < NU; j++) ul[j] = Ue[min(t + j * TPE, NF * NS - 1)];
   }
+  const GeoAddr<D, P> ga(a.affine, tc);
   T pJ[D * D];  // adjugate metrics of this solution point
 #pragma unroll
-  for (int c = 0; c < D * D; c++) pJ[c] = a.JG[(ec * D * D + c) * NS + tc];
+  for (int c = 0; c < D * D; c++) pJ[c] = a.JG[ga.jg(ec, c)];
   T pn[NQ][D + 1], pu[NQ][NF];  // face normal + dA, neighbour trace
 #pragma unroll
   for (int r = 0; r < NQ; r++) {
     const int qc = min(t + r * TPE, NFP - 1);
 #pragma unroll
-    for (int c = 0; c <= D; c++) pn[r][c] = a.fgeo[(ec * (D + 1) + c) * NFP + qc];
+    for (int c = 0; c <= D; c++) pn[r][c] = a.fgeo[ga.fg(ec, c, qc)];
   }
   // stage base state, detJ and time step for the update (read in every mode:
   // in mode 0 the base is the input state, whose lines are already in cache)
   double pb[NF];
-  const double pdet = a.detJ[ec * NS + tc];
+  const double pdet = a.detJ[ga.det(ec)];
   const double pdt = a.dtE[ec * a.dtStride];
 #pragma unroll
   for (int k = 0; k < NF; k++)
@@ -893,6 +919,39 @@ __global__ __launch_bounds__(256) void k_dt(const double *__restrict__ U, const 
     intWave += ws * wsurf[q % NFPF] * fgeo[((size_t)e * (D + 1) + D) * NFP + q];
   }
   dtE[e] = factor * vol / intWave;
+}
+
+// Affine-element test and compact geometry (GeoAddr): element e passes when
+// its metrics and detJ agree with those of its first solution point, and each
+// face's normal and dA with those of the face's first flux point, to `tol`
+// relative to the element's largest metric entry, 1 (unit normals) and the
+// first value (detJ, dA).  The compact arrays take the first point's values.
+template <int D>
+__global__ void k_affine(const double *__restrict__ JG, const double *__restrict__ detJ,
+                         const double *__restrict__ fgeo, int NS, int NFP, int NFPF, int nEles, double tol,
+                         double *__restrict__ JGc, double *__restrict__ detJc, double *__restrict__ fgeoc,
+                         int8_t *__restrict__ ok) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nEles) return;
+  bool good = true;
+  double sc = 0.0;
+  for (int c = 0; c < D * D; c++) sc = fmax(sc, fabs(JG[(e * D * D + c) * NS]));
+  for (int c = 0; c < D * D; c++) {
+    const double *g = JG + (e * D * D + c) * NS;
+    JGc[e * D * D + c] = g[0];
+    for (int p = 1; p < NS; p++) good = good && fabs(g[p] - g[0]) <= tol * sc;
+  }
+  const double *dj = detJ + e * NS;
+  detJc[e] = dj[0];
+  for (int p = 1; p < NS; p++) good = good && fabs(dj[p] - dj[0]) <= tol * fabs(dj[0]);
+  for (int f = 0; f < 2 * D; f++)
+    for (int c = 0; c <= D; c++) {
+      const double *v = fgeo + (e * (D + 1) + c) * NFP + f * NFPF;
+      fgeoc[(e * (D + 1) + c) * 2 * D + f] = v[0];
+      const double s = (c < D) ? 1.0 : fabs(v[0]);
+      for (int j = 1; j < NFPF; j++) good = good && fabs(v[j] - v[0]) <= tol * s;
+    }
+  ok[e] = good ? 1 : 0;
 }
 
 static __global__ void k_to_float(const double *__restrict__ a, float *__restrict__ b, long n) {

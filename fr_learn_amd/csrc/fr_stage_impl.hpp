@@ -20,7 +20,8 @@ int launchStageArgs(fr_ctx *c, const frdev::StageArgs<T> &a, int mode) {
     // grid padded to a multiple of the 8 XCDs (the kernels remap block ids)
     const unsigned nb = (unsigned)(((c->nE + G::EPB - 1) / G::EPB + 7) / 8 * 8);
     if (vis) {
-      frdev::GradArgs<T> g{a.Uin, a.UfIn, a.JG, a.detJ, a.fgeo, a.fnbr, a.finfo, a.oned, (T *)c->dUf, a.nEles, a.ph};
+      frdev::GradArgs<T> g{a.Uin,  a.UfIn,  a.JG,   a.detJ,  a.fgeo, a.fnbr,
+                            a.finfo, a.oned, (T *)c->dUf, a.nEles, a.ph, a.affine};
       {
         TimeScope ts(c, 3);
         if (bnd) hipLaunchKernelGGL((frdev::k_grad<D, P, NF, T, true>), dim3(nb), dim3(G::NTH), 0, c->stream, g);

@@ -203,6 +203,14 @@ class Solver:
         _lib.check(self._lib.fr_get_element_order(self._h, _lib.iptr(out)))
         return out
 
+    def geometry_mode(self, request=-1):
+        """Stage-kernel geometry layout: request 1 compact per element/face
+        when the mesh is affine (default), 0 per point, -1 query.  Returns
+        True when the compact arrays are in use."""
+        act = ctypes.c_int()
+        _lib.check(self._lib.fr_geometry_mode(self._h, int(request), ctypes.byref(act)))
+        return bool(act.value)
+
     def residual_bytes(self):
         b = ctypes.c_double()
         _lib.check(self._lib.fr_residual_bytes(self._h, ctypes.byref(b)))

@@ -194,6 +194,17 @@ int fr_batch_destroy(fr_batch *batch);
 int fr_set_stream(fr_ctx *ctx, void *stream);
 int fr_synchronize(fr_ctx *ctx);
 
+/* Geometry layout of the stage kernels.  On a mesh whose elements are all
+ * affine (metrics and detJ constant over each element, normal and dA
+ * constant over each face, to 1e-13 relative) the stage kernels read one
+ * metric set per element and one normal per face instead of per-point
+ * copies; otherwise, or with request 0, they read the per-point arrays.
+ * request: -1 query, 0 per-point, 1 compact when the mesh is affine (the
+ * default).  *active (may be NULL) = 1 when the compact arrays are in use.
+ * No reference counterpart: FlurryPP always stores metrics per point
+ * (ele::calcTransforms, ele.cpp:202-342); the results agree to rounding. */
+int fr_geometry_mode(fr_ctx *ctx, int request, int *active);
+
 /* Kernel timing: with timing on, every launch of the residual kernel is
  * bracketed by HIP events on the launch stream; fr_kernel_times returns the
  * accumulated milliseconds and launch count since the last reset for

@@ -1,0 +1,125 @@
+"""GPU stage kernels with the compact geometry of affine meshes (one metric
+set per element, one normal and dA per face) vs the CPU oracle, which always
+uses per-point metrics, and vs the same GPU path with per-point geometry.
+
+Bar: FP64 per-field max|gpu - oracle| / max|oracle| <= 1e-12
+(helpers.TOL_FP64); compact vs per-point geometry on the GPU within 1e-13."""
+import numpy as np
+import pytest
+
+from fr_learn_amd import Solver, cases, mesh
+from oracle import Oracle
+
+from helpers import (MIXED_BCS, QUAD_BCS, TOL_FP64, WALL_BCS, euler_cfg, hex_box, perturbed_state, quad_box,
+                     rel_err)
+from test_gpu_viscous import ns_cfg
+
+pytestmark = pytest.mark.gpu
+
+
+def sheared(m, A):
+    """The mesh under the linear map x -> A x (elements stay affine)."""
+    A = np.asarray(A, dtype=np.float64)
+    xv = m.xv.copy()
+    D = m.nDims
+    xv[:, :D] = xv[:, :D] @ A.T
+    return mesh.Mesh(D, xv, m.c2v, m.vmap, m.bounds)
+
+
+SHEAR3 = [[1.0, 0.3, -0.1], [0.05, 0.9, 0.2], [0.0, -0.15, 1.1]]
+SHEAR2 = [[1.0, 0.35], [-0.2, 0.8]]
+
+
+def _pair(cfg, m, ic, amp=1e-2):
+    print(f"affine case nDims={cfg['nDims']} P={cfg['order']} eq={cfg['equation']} viscous={cfg['viscous']} "
+          f"nEles={m.nEles}", flush=True)
+    o = Oracle(cfg, m)
+    g = Solver(cfg, m)
+    U = perturbed_state(o.initialize(ic), seed=0, amp=amp)
+    o.set_state(U)
+    g.set_state(U)
+    return o, g, U
+
+
+def _check(o, g, U, steps=2):
+    """Residual and RK steps: compact geometry vs the oracle and vs per-point."""
+    assert g.geometry_mode() is True
+    o.calc_residual(0)
+    g.calc_residual(0)
+    r_aff = g.residual(0)
+    err = rel_err(r_aff, o.residual(0))
+    print("residual rel err", err)
+    assert err.max() <= TOL_FP64
+    assert g.geometry_mode(0) is False
+    g.calc_residual(0)
+    assert rel_err(r_aff, g.residual(0)).max() <= 1e-13
+    assert g.geometry_mode(1) is True
+    o.update(steps)
+    g.update(steps)
+    Ua = g.get_state()
+    err = rel_err(Ua, o.get_state())
+    print("state rel err", err)
+    assert err.max() <= TOL_FP64
+    g.set_state(U)
+    g.geometry_mode(0)
+    g.update(steps)
+    assert rel_err(Ua, g.get_state()).max() <= 1e-13
+
+
+@pytest.mark.parametrize("order", [1, 2, 3, 4, 5])
+def test_hex_euler_sheared_periodic(order):
+    cfg = euler_cfg(3, order, dt=1e-3)
+    m = sheared(hex_box(3, perturb=0.0), SHEAR3)
+    _check(*_pair(cfg, m, cases.ic_tgv))
+
+
+@pytest.mark.parametrize("bcs", [MIXED_BCS, WALL_BCS])
+def test_hex_euler_boundaries(bcs):
+    cfg = euler_cfg(3, 3, dt=1e-3)
+    m = sheared(hex_box(3, periodic=False, bc=bcs, L=1.0, perturb=0.0), SHEAR3)
+    _check(*_pair(cfg, m, lambda x, c: cases.ic_uniform(x, c, vel=[0.3, 0.1, -0.05], p=1.0 / 1.4), amp=2e-2))
+
+
+@pytest.mark.parametrize("order", [1, 3, 5])
+def test_quad_euler_roe_walls(order):
+    cfg = euler_cfg(2, order, riemannType=1, dt=1e-3)
+    m = sheared(quad_box(4, periodic=False, bc=QUAD_BCS, perturb=0.0), SHEAR2)
+    _check(*_pair(cfg, m, lambda x, c: cases.ic_vortex(x, c, beta=2.0, mach=0.4)))
+
+
+@pytest.mark.parametrize("D", [2, 3])
+def test_navier_stokes_walls(D):
+    """Viscous: the gradient kernel reads the compact geometry too."""
+    if D == 3:
+        cfg = ns_cfg(3, 3, dt=1e-4)
+        m = sheared(hex_box(3, periodic=False, bc=WALL_BCS, L=1.0, perturb=0.0), SHEAR3)
+        ic = lambda x, c: cases.ic_uniform(x, c, vel=[0.3, 0.1, -0.05], p=1.0 / 1.4)
+        _check(*_pair(cfg, m, ic, amp=2e-2))
+    else:
+        cfg = ns_cfg(2, 3, riemannType=1, dt=1e-4)
+        bcs = dict(QUAD_BCS, ymin="adiabatic_noslip", ymax="isothermal_noslip_moving")
+        m = sheared(quad_box(4, periodic=False, bc=bcs, perturb=0.0), SHEAR2)
+        _check(*_pair(cfg, m, lambda x, c: cases.ic_vortex(x, c, beta=2.0, mach=0.4)))
+
+
+def test_fp32_compact_geometry():
+    cfg = euler_cfg(3, 3, dt=2e-3)
+    m = sheared(hex_box(4, perturb=0.0), SHEAR3)
+    o = Oracle(cfg, m)
+    g = Solver(cfg, m, precision="fp32")
+    assert g.geometry_mode() is True
+    U = perturbed_state(o.initialize(cases.ic_tgv), seed=0)
+    o.set_state(U)
+    g.set_state(U)
+    o.update(5)
+    g.update(5)
+    Ug, Uo = g.get_state(), o.get_state()
+    l2 = np.linalg.norm(Ug - Uo) / np.linalg.norm(Uo)
+    print("fp32 state rel L2", l2)
+    assert l2 < 1e-6
+
+
+def test_non_affine_mesh_keeps_per_point_geometry():
+    g = Solver(euler_cfg(3, 2), hex_box(3, perturb=0.15))
+    assert g.geometry_mode() is False
+    assert g.geometry_mode(1) is False
