@@ -88,6 +88,10 @@ SIGNATURES = {
     "fr_set_stream": (ctypes.c_int, [_P, _P]),
     "fr_synchronize": (ctypes.c_int, [_P]),
     "fr_geometry_mode": (ctypes.c_int, [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
+    "fr_write_data": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_double]),
+    "fr_read_restart": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.c_int, _DP]),
+    "fr_write_checkpoint": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.c_int, ctypes.c_double]),
+    "fr_read_checkpoint": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int), _DP]),
     "fr_set_timing": (ctypes.c_int, [_P, ctypes.c_int]),
     "fr_kernel_times": (ctypes.c_int, [_P, ctypes.c_int, _DP, ctypes.POINTER(ctypes.c_long)]),
     "fr_residual_bytes": (ctypes.c_int, [_P, _DP]),

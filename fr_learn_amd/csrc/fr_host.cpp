@@ -426,16 +426,30 @@ std::string buildOperators(const fr_config &cfg, Operators &o) {
   };
   fillDshape(o.locSpts, nS, o.dshapeSpts);
   o.shapeSpts.assign((size_t)nS * nN, 0.0);
-  for (int p = 0; p < nS; p++) {
-    const double *r = &o.locSpts[3 * p];
-    for (int n = 0; n < nN; n++) {
-      double v = lag(xl, r[0], CI[n]) * lag(xl, r[1], CJ[n]);
-      if (D == 3) v = v * lag(xl, r[2], CK[n]);
-      o.shapeSpts[(size_t)p * nN + n] = v;
-    }
-  }
+  for (int p = 0; p < nS; p++) shapeLinear(D, &o.locSpts[3 * p], &o.shapeSpts[(size_t)p * nN]);
   fillDshape(o.locFpts, nF, o.dshapeFpts);
   return "";
+}
+
+std::vector<double> pts1D(int sptsType, int order) {
+  if (order < 0 || order > 6 || (sptsType != 0 && sptsType != 1)) return {};
+  std::vector<double> x(order + 1);
+  for (int i = 0; i <= order; i++) x[i] = (sptsType == 0) ? kGL[order][i] : kGLL[order][i];
+  return x;
+}
+
+double lagrange(const std::vector<double> &x, double y, int mode) { return lag(x, y, mode); }
+
+void shapeLinear(int nDims, const double *r, double *out) {
+  static const int CI[8] = {0, 1, 1, 0, 0, 1, 1, 0}, CJ[8] = {0, 0, 1, 1, 0, 0, 1, 1},
+                   CK[8] = {0, 0, 0, 0, 1, 1, 1, 1};
+  const std::vector<double> xl = {-1., 1.};
+  const int nN = (nDims == 3) ? 8 : 4;
+  for (int n = 0; n < nN; n++) {
+    double v = lag(xl, r[0], CI[n]) * lag(xl, r[1], CJ[n]);
+    if (nDims == 3) v = v * lag(xl, r[2], CK[n]);
+    out[n] = v;
+  }
 }
 
 std::string buildConnectivity(const fr_config &cfg, const fr_mesh &m, const Operators &o, Connectivity &c) {

@@ -62,4 +62,15 @@ std::string buildConnectivity(const fr_config &cfg, const fr_mesh &mesh, const O
 std::vector<int> blockOrder(const Connectivity &conn, const std::vector<double> &centroid, int nE, int nFaceEle,
                             int epb);
 
+// 1D solution-point abscissae (getPts1D, points.cpp:162-330): sptsType 0
+// Gauss-Legendre, 1 Gauss-Lobatto; order 0..6.  Empty for other orders.
+std::vector<double> pts1D(int sptsType, int order);
+
+// Lagrange basis l_mode(y) on the nodes x (polynomials.cpp:9-19).
+double lagrange(const std::vector<double> &x, double y, int mode);
+
+// Shape functions of the linear element (Gmsh corner order; shape_quad,
+// funcs.cpp:121-176, shape_hex 178-260) at the reference point r[3]: out[4 or 8].
+void shapeLinear(int nDims, const double *r, double *out);
+
 }  // namespace frhost

@@ -203,6 +203,33 @@ class Solver:
         _lib.check(self._lib.fr_get_element_order(self._h, _lib.iptr(out)))
         return out
 
+    # -- files ----------------------------------------------------------------
+    def write_data(self, data_file_name, iter=0, time=0.0, plot_type=1):
+        """writeData (output.cpp:11-24): plot_type 1 writes the ParaView file
+        <data_file_name>_<iter %09d>.vtu (3D), 0 the CSV file
+        <data_file_name>.csv.<iter %09d>."""
+        _lib.check(self._lib.fr_write_data(self._h, str(data_file_name).encode(), int(plot_type), int(iter),
+                                           float(time)))
+
+    def read_restart(self, data_file_name, restart_iter, time=0.0):
+        """solver::readRestartFile (solver.cpp:1420-1482): sets the state from
+        <data_file_name>_<restart_iter %09d>.vtu; returns the file's TIME
+        (`time` when the file has none)."""
+        t = ctypes.c_double(float(time))
+        _lib.check(self._lib.fr_read_restart(self._h, str(data_file_name).encode(), int(restart_iter),
+                                             ctypes.byref(t)))
+        return t.value
+
+    def write_checkpoint(self, path, iter=0, time=0.0):
+        """Exact binary copy of the FP64 state with iter and time."""
+        _lib.check(self._lib.fr_write_checkpoint(self._h, str(path).encode(), int(iter), float(time)))
+
+    def read_checkpoint(self, path):
+        """Sets the state from a checkpoint; returns (iter, time)."""
+        it, t = ctypes.c_int(), ctypes.c_double()
+        _lib.check(self._lib.fr_read_checkpoint(self._h, str(path).encode(), ctypes.byref(it), ctypes.byref(t)))
+        return it.value, t.value
+
     def geometry_mode(self, request=-1):
         """Stage-kernel geometry layout: request 1 compact per element/face
         when the mesh is affine (default), 0 per point, -1 query.  Returns

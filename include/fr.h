@@ -35,7 +35,8 @@ enum {
   FR_EJACOBIAN = -3,/* negative Jacobian (solver.cpp:1283-1284) */
   FR_EHIP = -4,     /* HIP runtime error or missing device */
   FR_ENAN = -5,     /* NaN in residual (output.cpp:833-839) */
-  FR_ENOMEM = -6
+  FR_ENOMEM = -6,
+  FR_EIO = -7       /* file cannot be opened, read or written (solution files, restart) */
 };
 
 /* FR_LAYOUT_DEVICE: the library's own [ele][field][pt] layout with the
@@ -193,6 +194,32 @@ int fr_batch_destroy(fr_batch *batch);
  * launch on the caller's stream (NULL = the context's own stream). */
 int fr_set_stream(fr_ctx *ctx, void *stream);
 int fr_synchronize(fr_ctx *ctx);
+
+/* Solution files (writeData, output.cpp:11-24).  plotType 1: ParaView
+ * <dataFileName>_<iter %09d>.vtu (writeParaview, output.cpp:103-391; 3D
+ * only, FR_EINVAL otherwise, as there): per element the density, velocity
+ * and pressure at the (P+3)^3 plot points (the tensor grid of -1, the
+ * solution points and +1; extrapolateUPpts, solver.cpp:449-484), their
+ * positions and the (P+2)^3 linear sub-hexes, with the TIME and ITER
+ * comments.  plotType 0: <dataFileName>.csv.<iter %09d> (writeCSV,
+ * output.cpp:26-100): position and primitives at every solution point,
+ * then every flux point.  Elements in mesh order. */
+int fr_write_data(fr_ctx *ctx, const char *dataFileName, int plotType, int iter, double time);
+
+/* Restart from <dataFileName>_<restartIter %09d>.vtu (solver::readRestartFile,
+ * solver.cpp:1420-1482; ele::restart, ele.cpp:1451-1880): density, velocity
+ * and pressure at the solution points -> U_spts, interpolated when the
+ * file's order differs from this context's.  *time (may be NULL) receives
+ * the file's TIME comment and is left unchanged when there is none.
+ * Errors: FR_EIO for a missing file, no <UnstructuredGrid> tag, or a
+ * missing / short element (the reference fills such an element with 100). */
+int fr_read_restart(fr_ctx *ctx, const char *dataFileName, int restartIter, double *time);
+
+/* Exact binary checkpoint of U_spts (FP64, reference layout) with the
+ * iteration and time; reading checks nDims, order, nFields and nEles.  No
+ * reference counterpart (its restart goes through the 16-digit .vtu). */
+int fr_write_checkpoint(fr_ctx *ctx, const char *path, int iter, double time);
+int fr_read_checkpoint(fr_ctx *ctx, const char *path, int *iter, double *time);
 
 /* Geometry layout of the stage kernels.  On a mesh whose elements are all
  * affine (metrics and detJ constant over each element, normal and dA
