@@ -47,6 +47,12 @@ int ensureTraces(fr_ctx *c, int which) {
   if (c->traceOf == which) return 0;
   const T *U = stateIn<T>(c, which);
   const T *o1 = std::is_same<T, float>::value ? (const T *)c->oned32 : (const T *)c->oned64;
+  if (c->tri) {
+    int rc = launchTriTrace<T>(c, U, (T *)c->Uf[c->cur], o1);
+    if (rc) return rc;
+    c->traceOf = which;
+    return 0;
+  }
   hipError_t err = hipSuccess;
   bool ok = dispatch(c->D, c->P, c->NF, [&](auto Dc, auto Pc, auto NFc) {
     constexpr int D = decltype(Dc)::value, P = decltype(Pc)::value, NF = decltype(NFc)::value;
@@ -249,6 +255,7 @@ void batchFree(fr_batch *b) {
 // ---------------------------------------------------------------------------
 int adjCheck(fr_ctx *c) {
   if (c->prec != FR_PREC_FP64) return fail(FR_EINVAL, "the adjoint needs an FP64 context");
+  if (c->tri) return fail(FR_EINVAL, "the adjoint is not available for triangles");
   const size_t nSt = c->nState(), nTr = (size_t)c->nE * c->NF * c->NFP;
   int rc;
   if (!c->adjA) {
@@ -328,7 +335,9 @@ int createImpl(const fr_config *cfg, const fr_mesh *m, int precision, fr_ctx *c)
   if (cfg->riemannType != 0 && cfg->riemannType != 1) return fail(FR_EINVAL, "riemannType must be 0 or 1");
   if (cfg->viscous != 0 && cfg->viscous != 1) return fail(FR_EINVAL, "viscous must be 0 or 1");
   if (!m || m->nEles <= 0) return fail(FR_EMESH, "Cannot setup elements array - nEles = 0");
-  if (m->nNodes != (cfg->nDims == 3 ? 8 : 4)) return fail(FR_EINVAL, "only linear hexes (8 nodes) or quads (4)");
+  if (m->nNodes != (cfg->nDims == 3 ? 8 : 4) && !(cfg->nDims == 2 && m->nNodes == 3))
+    return fail(FR_EINVAL, "only linear hexes (8 nodes), quads (4) or triangles (3)");
+  c->tri = (m->nNodes == 3);
   for (int i = 0; i < m->nEles * m->nNodes; i++)
     if (m->c2v[i] < 0 || m->c2v[i] >= m->nVerts) return fail(FR_EMESH, "c2v vertex id out of range");
   // RK tables (input.cpp:424-461)
@@ -348,7 +357,7 @@ int createImpl(const fr_config *cfg, const fr_mesh *m, int precision, fr_ctx *c)
       break;
     default: return fail(FR_EINVAL, "Time-Stepping type not supported.");
   }
-  std::string err = frhost::buildOperators(*cfg, c->ops);
+  std::string err = c->tri ? frhost::buildTriOperators(*cfg, c->ops) : frhost::buildOperators(*cfg, c->ops);
   if (!err.empty()) return fail(FR_EINVAL, err);
   err = frhost::buildConnectivity(*cfg, *m, c->ops, c->conn);
   if (!err.empty()) return fail(FR_EMESH, err);
@@ -408,6 +417,7 @@ int createImpl(const fr_config *cfg, const fr_mesh *m, int precision, fr_ctx *c)
   dispatch(D, c->P, c->NF, [&](auto Dc, auto Pc, auto) {
     c->epb = frdev::ElemGeo<decltype(Dc)::value, decltype(Pc)::value>::EPB;
   });
+  if (c->tri) c->epb = 8;  // frdev::TriGeo<P>::EPB
   {
     std::vector<double> cen((size_t)nE * 3, 0.0);
     for (int e = 0; e < nE; e++)
@@ -485,8 +495,9 @@ int createImpl(const fr_config *cfg, const fr_mesh *m, int precision, fr_ctx *c)
     (void)hipFree(pairEF);
     if (neg) return fail(FR_EJACOBIAN, "Negative Jacobian at solution points.");
   }
-  // affine meshes: compact per-element / per-face geometry for the stage kernels
-  {
+  // affine meshes: compact per-element / per-face geometry for the stage
+  // kernels (quads and hexes; the triangle kernel reads the per-point arrays)
+  if (!c->tri) {
     const size_t nJ = (size_t)nE * D * D, nG = (size_t)nE * (D + 1) * c->NFACE;
     int8_t *dOk;
     if ((rc = dmalloc(&c->JGc64, nJ)) || (rc = dmalloc(&c->detJc, nE)) || (rc = dmalloc(&c->fgeoc64, nG)) ||
@@ -837,6 +848,7 @@ int fr_set_dt(fr_ctx *c, double dt) {
 int fr_calc_dt(fr_ctx *c, int dtType, double CFL, double *dtMin) {
   if (!c) return fail(FR_EINVAL, "null argument");
   if (dtType != 1 && dtType != 2) return fail(FR_EINVAL, "dtType must be 1 (global) or 2 (local)");
+  if (c->tri) return fail(FR_EINVAL, "the CFL time step is defined for quads and hexes");
   // getCFLLimit (global.cpp:45-68)
   static const double lim[6] = {1.393, 0.464, 0.235, 0.1453, 0.100, 0.0736};
   if (c->P < 0 || c->P > 5) return fail(FR_EINVAL, "CFL limit not available for this order!");
@@ -1016,6 +1028,7 @@ int fr_batch_create(fr_ctx *const *ctxs, int n, fr_batch **out) {
     if (c->D != b->D || c->NF != b->NF || c->prec != b->prec || c->cfg.timeType != c0->cfg.timeType)
       return bad(FR_EINVAL, "batched meshes must share nDims, equation, precision and timeType");
     if (c->cfg.viscous) return bad(FR_EINVAL, "the batched kernel is inviscid");
+    if (c->tri) return bad(FR_EINVAL, "the batched kernel takes quads and hexes");
     if (c->P < 1 || c->P > 5) return bad(FR_EINVAL, "batched meshes need P in 1..5");
     b->ctx.push_back(c);
   }

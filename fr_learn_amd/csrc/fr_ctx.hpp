@@ -91,6 +91,7 @@ struct fr_ctx {
   std::vector<int> perm;  // device element -> mesh element (blocks of EPB face-connected elements)
   std::vector<int> iperm; // mesh element -> device element
   int epb = 1;
+  bool tri = false;  // linear triangles (fr_tri_ops.cpp / fr_tri.hpp)
   // timing
   bool timing = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[4];
@@ -173,6 +174,11 @@ template <typename T>
 int launchStageArgs(fr_ctx *c, const frdev::StageArgs<T> &a, int mode);
 // (d divF / dU)^T lam at the state Us, device arrays (fr_adjoint.hip).
 int vjpResidualDev(fr_ctx *c, const double *Us, const double *lam, double *out);
+// triangle kernels (fr_tri.hip)
+template <typename T>
+int launchTriStage(fr_ctx *c, const frdev::StageArgs<T> &a, int mode);
+template <typename T>
+int launchTriTrace(fr_ctx *c, const T *U, T *Uf, const T *tab);
 // Stage s of one batched step (fr_batch.hip); parity < 0 is the tuning launch,
 // Ponly != 0 restricts it to the meshes of that order.
 int batchLaunchStage(fr_batch *b, int parity, int s, int grid, hipStream_t st, int Ponly = 0);

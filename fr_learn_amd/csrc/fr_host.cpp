@@ -440,7 +440,13 @@ std::vector<double> pts1D(int sptsType, int order) {
 
 double lagrange(const std::vector<double> &x, double y, int mode) { return lag(x, y, mode); }
 
-void shapeLinear(int nDims, const double *r, double *out) {
+void shapeLinear(int nDims, const double *r, double *out, int nNodes) {
+  if (nNodes == 3) {
+    out[0] = -(r[0] + r[1]) / 2;
+    out[1] = (1 + r[0]) / 2;
+    out[2] = (1 + r[1]) / 2;
+    return;
+  }
   static const int CI[8] = {0, 1, 1, 0, 0, 1, 1, 0}, CJ[8] = {0, 0, 1, 1, 0, 0, 1, 1},
                    CK[8] = {0, 0, 0, 0, 1, 1, 1, 1};
   const std::vector<double> xl = {-1., 1.};
@@ -460,6 +466,8 @@ std::string buildConnectivity(const fr_config &cfg, const fr_mesh &m, const Oper
   static const int HFV[6][4] = {{0, 1, 2, 3}, {4, 5, 6, 7}, {3, 0, 4, 7}, {2, 1, 5, 6}, {1, 0, 4, 5}, {3, 2, 6, 7}};
   static const int HOR[6][4] = {{0, 1, 2, 3}, {5, 4, 7, 6}, {0, 3, 7, 4}, {2, 1, 5, 6}, {1, 0, 4, 5}, {3, 2, 6, 7}};
   static const int QFV[4][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}};
+  static const int TFV[3][2] = {{0, 1}, {1, 2}, {2, 0}};  // triangle edges (fr_tri_ops.cpp)
+  const bool triEle = (nN == 3);
   auto cv = [&](int e, int n) -> int {
     int v = m.c2v[(size_t)e * nN + n];
     return m.vmap ? m.vmap[v] : v;
@@ -473,7 +481,7 @@ std::string buildConnectivity(const fr_config &cfg, const fr_mesh &m, const Oper
     for (int f = 0; f < nfe; f++) {
       Key &k = keys[(size_t)e * nfe + f];
       k.v = {INT32_MAX, INT32_MAX, INT32_MAX, INT32_MAX};
-      for (int i = 0; i < nfv; i++) k.v[i] = (D == 3) ? cv(e, HFV[f][i]) : cv(e, QFV[f][i]);
+      for (int i = 0; i < nfv; i++) k.v[i] = (D == 3) ? cv(e, HFV[f][i]) : triEle ? cv(e, TFV[f][i]) : cv(e, QFV[f][i]);
       std::sort(k.v.begin(), k.v.begin() + nfv);
       k.e = e;
       k.f = f;

@@ -70,7 +70,11 @@ std::vector<double> pts1D(int sptsType, int order);
 double lagrange(const std::vector<double> &x, double y, int mode);
 
 // Shape functions of the linear element (Gmsh corner order; shape_quad,
-// funcs.cpp:121-176, shape_hex 178-260) at the reference point r[3]: out[4 or 8].
-void shapeLinear(int nDims, const double *r, double *out);
+// funcs.cpp:121-176, shape_hex 178-260) at the reference point r[3]: out[4 or 8];
+// nNodes 3: the linear triangle (-1,-1), (1,-1), (-1,1).
+void shapeLinear(int nDims, const double *r, double *out, int nNodes = 0);
+
+// FR operators of the linear triangle (fr_tri_ops.cpp); "" or an error message.
+std::string buildTriOperators(const fr_config &cfg, Operators &ops);
 
 }  // namespace frhost

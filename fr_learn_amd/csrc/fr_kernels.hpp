@@ -820,7 +820,7 @@ __global__ void k_fgeo(const double *__restrict__ ownN, const int32_t *__restric
   const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (long)nEles * NFP) return;
   const int e = (int)(idx / NFP), q = (int)(idx - (long)e * NFP);
-  const int fi = finfo[(size_t)e * 2 * D + q / NFPF];
+  const int fi = finfo[(size_t)e * (NFP / NFPF) + q / NFPF];  // NFP / NFPF faces per element
   int se = e, sq = q;
   if (fi < FINFO_BOUNDARY && ((fi >> 5) & 1)) {
     const int32_t g = pairEF[(size_t)e * NFP + q];

@@ -256,8 +256,8 @@ int writeCSV(fr_ctx *c, const char *name, int iter) {
   if (int rc = hostState(c, Ue)) return rc;
   const int D = c->D, NF = c->NF, NS = c->NS, NFP = c->NFP, nN = c->ops.nNodes;
   std::vector<double> shS((size_t)NS * nN), shF((size_t)NFP * nN);
-  for (int p = 0; p < NS; p++) frhost::shapeLinear(D, &c->ops.locSpts[3 * p], &shS[(size_t)p * nN]);
-  for (int p = 0; p < NFP; p++) frhost::shapeLinear(D, &c->ops.locFpts[3 * p], &shF[(size_t)p * nN]);
+  for (int p = 0; p < NS; p++) frhost::shapeLinear(D, &c->ops.locSpts[3 * p], &shS[(size_t)p * nN], nN);
+  for (int p = 0; p < NFP; p++) frhost::shapeLinear(D, &c->ops.locFpts[3 * p], &shF[(size_t)p * nN], nN);
   const std::vector<double> &E = c->ops.spts_to_fpts;  // nFpts x nSpts
 
   std::ofstream out(iterName(name, "%s.csv.%.09d", iter), std::ios::binary);

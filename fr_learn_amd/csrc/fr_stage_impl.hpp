@@ -11,6 +11,7 @@ namespace frimpl {
 // gradient kernel before it on the viscous path).
 template <typename T>
 int launchStageArgs(fr_ctx *c, const frdev::StageArgs<T> &a, int mode) {
+  if (c->tri) return launchTriStage<T>(c, a, mode);
   const bool bnd = c->conn.nIntFaces < (int)c->conn.faces.size();
   const bool vis = c->cfg.viscous != 0;
   hipError_t err = hipSuccess;

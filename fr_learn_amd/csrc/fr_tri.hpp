@@ -1,0 +1,207 @@
+// fr_tri.hpp -- the stage kernel of the linear triangle (2D), dense operators.
+//
+// The triangle's FR operators (fr_tri_ops.cpp) are dense: (P+1)(P+2)/2 solution
+// points with no tensor lines.  A workgroup of 256 threads holds 8 elements,
+// 32 threads each (one solution point and one flux point per thread for
+// P <= 5: at most 21 and 18), two elements per wavefront.  The table
+// Dr | Ds | E | L (at most 1638 values) is staged in LDS once per workgroup.
+// The phases follow the quad kernel's (k_stage): physical and reference flux
+// at the solution points and the common flux at the flux points; the jump
+// Fn dA - disFn; divergence plus correction; the stage update; the traces of
+// the new state.  The stage arguments and modes are those of k_stage; every
+// face's traces go through HBM (no in-block exchange).
+#pragma once
+
+#include "fr_kernels.hpp"
+
+namespace frdev {
+
+template <int P>
+struct TriGeo {
+  static constexpr int P1 = P + 1, NS = P1 * (P + 2) / 2, NFPF = P1, NFACE = 3, NFP = 3 * P1;
+  static constexpr int TPE = 32, NTH = 256, EPB = NTH / TPE;
+  static constexpr int DR = 0, DS = NS * NS, E = 2 * NS * NS, L = 2 * NS * NS + NFP * NS;
+  static constexpr int TAB = 2 * NS * NS + 2 * NFP * NS;
+  static_assert(NS <= TPE && NFP <= TPE, "one point per thread");
+};
+
+template <int P, int NF, typename T>
+struct TriSmem {
+  using G = TriGeo<P>;
+  T tab[G::TAB];
+  T sU[G::EPB][NF][G::NS];     // stage state
+  T sF[G::EPB][2][NF][G::NS];  // reference flux F~_r, F~_s
+  T sJ[G::EPB][NF][G::NFP];    // jump Fn dA - disFn
+  T sN[G::EPB][NF][G::NS];     // updated state
+};
+
+// Reference unit normal of the edge of flux point q (edges v0v1, v1v2, v2v0).
+template <typename T>
+__device__ __forceinline__ void triNormal(int f, T &nx, T &ny) {
+  const T h = (T)0.70710678118654752440;
+  nx = (f == 0) ? (T)0 : (f == 1) ? h : (T)-1;
+  ny = (f == 0) ? (T)-1 : (f == 1) ? h : (T)0;
+}
+
+template <int P, int NF, typename T, bool BND, bool FIN>
+__global__ __launch_bounds__(256) void k_tri_stage(StageArgs<T> a) {
+  using G = TriGeo<P>;
+  constexpr int NS = G::NS, NFP = G::NFP, NFPF = G::NFPF, TPE = G::TPE, EPB = G::EPB, NFACE = G::NFACE;
+  __shared__ TriSmem<P, NF, T> sm;
+  const int b = (blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);  // XCD-contiguous blocks
+  const int slot = threadIdx.x / TPE, t = threadIdx.x - slot * TPE;
+  const int e = b * EPB + slot;
+  const bool act = e < a.nEles;
+  const bool hasS = act && t < NS, hasQ = act && t < NFP;
+  const size_t ec = (size_t)(act ? e : a.nEles - 1);
+  const int tc = t < NS ? t : NS - 1, qc = t < NFP ? t : NFP - 1;
+
+  for (int i = threadIdx.x; i < G::TAB; i += blockDim.x) sm.tab[i] = a.oned[i];
+  if (act)
+    for (int i = t; i < NF * NS; i += TPE) (&sm.sU[slot][0][0])[i] = a.Uin[ec * NF * NS + i];
+  // this thread's solution point: metrics, detJ, dt, base state
+  T pJ[4];
+#pragma unroll
+  for (int c = 0; c < 4; c++) pJ[c] = a.JG[(ec * 4 + c) * NS + tc];
+  const double pdet = a.detJ[ec * NS + tc];
+  const double pdt = a.dtE[ec * a.dtStride];
+  double pb[NF];
+#pragma unroll
+  for (int k = 0; k < NF; k++)
+    pb[k] = FIN ? a.Ubase[(ec * NF + k) * NS + tc] : (double)a.UbaseT[(ec * NF + k) * NS + tc];
+  // this thread's flux point: face record, normal, dA, neighbour trace
+  const int fi = a.finfo[ec * NFACE + qc / NFPF];
+  const int nbe = a.fnbr[ec * NFACE + qc / NFPF];
+  const bool bnd = BND && fi >= FINFO_BOUNDARY;
+  T n[2], un[NF];
+  n[0] = a.fgeo[(ec * 3 + 0) * NFP + qc];
+  n[1] = a.fgeo[(ec * 3 + 1) * NFP + qc];
+  const T dA = a.fgeo[(ec * 3 + 2) * NFP + qc];
+  if (!bnd && nbe >= 0) {
+    const T *Ur = a.UfIn + (size_t)nbe * NF * NFP + neighbourFpt<2, P>(qc, fi);
+#pragma unroll
+    for (int k = 0; k < NF; k++) un[k] = Ur[k * NFP];
+  } else {
+#pragma unroll
+    for (int k = 0; k < NF; k++) un[k] = (T)0;
+  }
+  __syncthreads();
+
+  const T *Dr = sm.tab + G::DR, *Ds = sm.tab + G::DS, *Em = sm.tab + G::E, *Lm = sm.tab + G::L;
+  // (1) flux at the solution point, common flux at the flux point
+  if (hasS) {
+    T u[NF], F[2][NF];
+#pragma unroll
+    for (int k = 0; k < NF; k++) u[k] = sm.sU[slot][k][t];
+    inviscidFlux<2, NF, T>(a.ph, u, F);
+#pragma unroll
+    for (int d = 0; d < 2; d++)
+#pragma unroll
+      for (int k = 0; k < NF; k++) sm.sF[slot][d][k][t] = pJ[d * 2 + 0] * F[0][k] + pJ[d * 2 + 1] * F[1][k];
+  }
+  T Fn[NF];
+  if (hasQ) {
+    T uo[NF], ub[NF], Fc[NF];
+#pragma unroll
+    for (int k = 0; k < NF; k++) {
+      T acc = (T)0;
+      for (int s = 0; s < NS; s++) acc += Em[t * NS + s] * sm.sU[slot][k][s];
+      uo[k] = acc;
+    }
+    const bool right = !bnd && ((fi >> 5) & 1);
+    if (bnd) {
+      boundaryState<2, NF, T>(a.ph, fi - FINFO_BOUNDARY, uo, n, ub);
+    } else {
+#pragma unroll
+      for (int k = 0; k < NF; k++) ub[k] = un[k];
+    }
+    if (right) commonFlux<2, NF, T>(a.ph, false, ub, uo, n, Fc);
+    else commonFlux<2, NF, T>(a.ph, bnd, uo, ub, n, Fc);
+    const T scale = right ? -dA : dA;
+#pragma unroll
+    for (int k = 0; k < NF; k++) Fn[k] = Fc[k] * scale;
+  }
+  elementSync<TPE>();
+  // (2) jump at the flux point: Fn dA - (E F~) . tNorm
+  if (hasQ) {
+    T tx, ty;
+    triNormal<T>(t / NFPF, tx, ty);
+#pragma unroll
+    for (int k = 0; k < NF; k++) {
+      T fr = (T)0, fs = (T)0;
+      for (int s = 0; s < NS; s++) {
+        fr += Em[t * NS + s] * sm.sF[slot][0][k][s];
+        fs += Em[t * NS + s] * sm.sF[slot][1][k][s];
+      }
+      sm.sJ[slot][k][t] = Fn[k] - (tx * fr + ty * fs);
+    }
+  }
+  elementSync<TPE>();
+  // (3) divergence + correction, stage update (as k_stage phase 4)
+  if (hasS) {
+    const size_t eo = ec * NF * NS;
+#pragma unroll
+    for (int k = 0; k < NF; k++) {
+      T dv = (T)0;
+      for (int s = 0; s < NS; s++) dv += Dr[t * NS + s] * sm.sF[slot][0][k][s] + Ds[t * NS + s] * sm.sF[slot][1][k][s];
+      T corr = (T)0;
+      for (int q = 0; q < NFP; q++) corr += Lm[t * NFP + q] * sm.sJ[slot][k][q];
+      dv = dv + corr;
+      const size_t gi = eo + (size_t)k * NS + t;
+      if (a.divF) a.divF[gi] = dv;
+      if (a.mode == 0) continue;
+      const double dtJ = pdt / pdet;
+      double u = pb[k];
+      if (a.mode == 2) {
+        const T r0 = (a.nprev > 0) ? a.dprev0[gi] : (T)0;
+        const T r1 = (a.nprev > 1) ? a.dprev1[gi] : (T)0;
+        const T r2 = (a.nprev > 2) ? a.dprev2[gi] : (T)0;
+        if (a.nprev > 0) u -= a.c0 * (double)r0 * dtJ;
+        if (a.nprev > 1) u -= a.c1 * (double)r1 * dtJ;
+        if (a.nprev > 2) u -= a.c2 * (double)r2 * dtJ;
+        const double cl = (a.nprev == 0) ? a.c0 : (a.nprev == 1) ? a.c1 : (a.nprev == 2) ? a.c2 : a.c3;
+        u -= cl * (double)dv * dtJ;
+      } else {
+        u = u - a.rk * (double)dv * dtJ;
+      }
+      if (a.mode == 1) {
+        a.Unext[gi] = (T)u;
+      } else {
+        a.Uout[gi] = u;
+        if (a.Uout32) a.Uout32[gi] = (float)u;
+      }
+      sm.sN[slot][k][t] = (T)u;
+    }
+  }
+  if (a.mode == 0) return;
+  elementSync<TPE>();
+  // (4) traces of the updated state
+  if (hasQ) {
+    T *Fe = a.UfOut + ec * NF * NFP;
+#pragma unroll
+    for (int k = 0; k < NF; k++) {
+      T acc = (T)0;
+      for (int s = 0; s < NS; s++) acc += Em[t * NS + s] * sm.sN[slot][k][s];
+      Fe[k * NFP + t] = acc;
+    }
+  }
+}
+
+// Face traces U_fpts = E U_spts, one thread per (element, flux point).
+template <int P, int NF, typename T>
+__global__ void k_tri_trace(const T *__restrict__ U, T *__restrict__ Uf, const T *__restrict__ tab, int nEles) {
+  using G = TriGeo<P>;
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)nEles * G::NFP) return;
+  const long e = idx / G::NFP;
+  const int q = (int)(idx - e * G::NFP);
+  const T *Em = tab + G::E + q * G::NS;
+#pragma unroll
+  for (int k = 0; k < NF; k++) {
+    T acc = (T)0;
+    for (int s = 0; s < G::NS; s++) acc += Em[s] * U[(e * NF + k) * G::NS + s];
+    Uf[(e * NF + k) * G::NFP + q] = acc;
+  }
+}
+
+}  // namespace frdev

@@ -149,6 +149,22 @@ def box(nDims, n, lo, hi, periodic=None, bc=None, perturb=0.0, seed=0, rotate=Fa
     return Mesh(D, xv, c2v, vmap, bounds)
 
 
+def tri_box(n, lo, hi, periodic=None, bc=None, perturb=0.0, seed=0, rotate=False, shuffle=False):
+    """Structured box of linear triangles: each cell of box(2, ...) split
+    along its first corner's diagonal into two counter-clockwise triangles.
+    Arguments as box(); rotate re-orients each triangle's corner order
+    cyclically, shuffle permutes the element order."""
+    q = box(2, n, lo, hi, periodic=periodic, bc=bc, perturb=perturb, seed=seed)
+    c2v = np.concatenate([q.c2v[:, [0, 1, 2]], q.c2v[:, [0, 2, 3]]], axis=0)
+    rng = np.random.default_rng(seed + 1)
+    if rotate:
+        r = rng.integers(0, 3, size=c2v.shape[0])
+        c2v = np.stack([np.roll(c2v[e], -r[e]) for e in range(c2v.shape[0])])
+    if shuffle:
+        c2v = c2v[rng.permutation(c2v.shape[0])]
+    return Mesh(2, q.xv, c2v.astype(np.int32), q.vmap, q.bounds)
+
+
 def write_gmsh(mesh, path, names=None):
     """Write a Gmsh 2.2 ASCII mesh with one physical volume "fluid" and one
     physical surface per boundary (the format read by geo::readGmsh,
