@@ -151,11 +151,12 @@ def copy_calibration():
 
 
 def run_batch(n_inst=256, steps=20, warmup=3, seed=0):
-    """The batched 2D Euler workload: n_inst seeded periodic quad meshes of 8^2
-    to 64^2 cells, P uniform in 1..5, isentropic vortices of random centre,
-    strength 1..5 and Mach 0.2..0.6, Roe, RK4.  One persistent launch per
-    stage for all meshes (replayed from a hipGraph) against one update call
-    per mesh."""
+    """The batched 2D Euler workload: n_inst seeded periodic meshes of 8^2 to
+    64^2 cells, quadrilaterals or triangles (two per cell) with probability
+    1/2, P uniform in 1..5, isentropic vortices of random centre, strength
+    1..5 and Mach 0.2..0.6, Roe, RK4.  One persistent launch per stage,
+    element type and order for all meshes (replayed from a hipGraph) against
+    one update call per mesh."""
     from fr_learn_amd import Batch, Solver, cases, mesh
     rng = np.random.default_rng(seed)
     solvers = []
@@ -166,7 +167,10 @@ def run_batch(n_inst=256, steps=20, warmup=3, seed=0):
         cx, cy = rng.uniform(-2.5, 2.5, 2)
         beta, mach = rng.uniform(1.0, 5.0), rng.uniform(0.2, 0.6)
         cfg = cases.config(nDims=2, order=P, equation=1, riemannType=1, timeType=4, dt=1e-3, gamma=1.4)
-        m = mesh.box(2, (cells, cells), (-5.0, -5.0), (5.0, 5.0))
+        if rng.uniform() < 0.5:
+            m = mesh.tri_box((cells, cells), (-5.0, -5.0), (5.0, 5.0))
+        else:
+            m = mesh.box(2, (cells, cells), (-5.0, -5.0), (5.0, 5.0))
         s = Solver(cfg, m)
         s.geometry_mode(0)  # per-point metrics, as on the main workload
         s.initialize(lambda x, c: cases.ic_vortex(x, c, beta=beta, mach=mach, center=(cx, cy)))
@@ -194,8 +198,8 @@ def run_batch(n_inst=256, steps=20, warmup=3, seed=0):
         s.close()
     v_b = dof * 4 * steps / wall_b / 1e9
     v_s = dof * 4 * steps / wall_s / 1e9
-    return {"workload": f"{n_inst} periodic quad meshes 8^2..64^2 cells, P 1..5, 2D Euler vortices, Roe, RK4 "
-                        "(quadrilaterals only: no triangle FR operators)",
+    return {"workload": f"{n_inst} periodic meshes 8^2..64^2 cells, quads or triangles (p = 0.5), P 1..5, "
+                        "2D Euler vortices, Roe, RK4",
             "instances": n_inst, "dof_per_field": dof, "steps": steps, "value": v_b, "unit": "GDOF/s",
             "ms_per_step": wall_b / steps * 1e3, "per_mesh_launches_GDOF_per_s": v_s,
             "speedup_vs_per_mesh_launches": v_b / v_s, "autotuned_grid": grid, "work_items": items}

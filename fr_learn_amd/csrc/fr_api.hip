@@ -241,9 +241,9 @@ void batchFree(fr_batch *b) {
     if (b->args[p]) (void)hipFree(b->args[p]);
   }
   if (b->argsTune) (void)hipFree(b->argsTune);
-  for (int P = 0; P < 6; P++) {
-    if (b->ps[P]) (void)hipStreamDestroy(b->ps[P]);
-    if (b->join[P]) (void)hipEventDestroy(b->join[P]);
+  for (int grp = 0; grp < 11; grp++) {
+    if (b->ps[grp]) (void)hipStreamDestroy(b->ps[grp]);
+    if (b->join[grp]) (void)hipEventDestroy(b->join[grp]);
   }
   if (b->fork) (void)hipEventDestroy(b->fork);
   if (b->items) (void)hipFree(b->items);
@@ -1028,19 +1028,19 @@ int fr_batch_create(fr_ctx *const *ctxs, int n, fr_batch **out) {
     if (c->D != b->D || c->NF != b->NF || c->prec != b->prec || c->cfg.timeType != c0->cfg.timeType)
       return bad(FR_EINVAL, "batched meshes must share nDims, equation, precision and timeType");
     if (c->cfg.viscous) return bad(FR_EINVAL, "the batched kernel is inviscid");
-    if (c->tri) return bad(FR_EINVAL, "the batched kernel takes quads and hexes");
     if (c->P < 1 || c->P > 5) return bad(FR_EINVAL, "batched meshes need P in 1..5");
     b->ctx.push_back(c);
   }
-  // work items grouped by order (one launch per order), each group in mesh order
-  for (int P = 1; P <= 5; P++) {
-    b->itemStart[P] = (int)items.size();
+  // work items grouped by kernel (one launch per element type and order),
+  // each group in mesh order
+  for (int grp = 1; grp <= 10; grp++) {
+    b->itemStart[grp] = (int)items.size();
     for (int i = 0; i < n; i++)
-      if (b->ctx[i]->P == P) {
+      if (b->ctx[i]->P + (b->ctx[i]->tri ? 5 : 0) == grp) {
         const int nb = (b->ctx[i]->nE + b->ctx[i]->epb - 1) / b->ctx[i]->epb;
         for (int k = 0; k < nb; k++) items.push_back({i, k});
       }
-    b->itemCount[P] = (int)items.size() - b->itemStart[P];
+    b->itemCount[grp] = (int)items.size() - b->itemStart[grp];
   }
   b->nItems = (int)items.size();
   int rc;
@@ -1082,23 +1082,23 @@ int fr_batch_create(fr_ctx *const *ctxs, int n, fr_batch **out) {
   (void)hipEventDestroy(e1);
   b->tunedMs = best;
   // One step's launches, captured per trace parity.  Meshes of different
-  // orders are independent, so each order's stages form their own branch of
-  // the graph (forked from and joined to the batch stream) and run
-  // concurrently with the other orders'.
+  // element types and orders are independent, so each group's stages form
+  // their own branch of the graph (forked from and joined to the batch
+  // stream) and run concurrently with the other groups'.
   if (hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess) return bad(FR_EHIP, "event");
-  for (int P = 1; P <= 5; P++)
-    if (b->itemCount[P] && (hipStreamCreateWithFlags(&b->ps[P], hipStreamNonBlocking) != hipSuccess ||
-                            hipEventCreateWithFlags(&b->join[P], hipEventDisableTiming) != hipSuccess))
+  for (int grp = 1; grp <= 10; grp++)
+    if (b->itemCount[grp] && (hipStreamCreateWithFlags(&b->ps[grp], hipStreamNonBlocking) != hipSuccess ||
+                              hipEventCreateWithFlags(&b->join[grp], hipEventDisableTiming) != hipSuccess))
       return bad(FR_EHIP, "stream");
   for (int p = 0; p < ((b->nRK & 1) ? 2 : 1); p++) {
     if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeGlobal) != hipSuccess) return bad(FR_EHIP, "capture");
     bool ok = hipEventRecord(b->fork, b->stream) == hipSuccess;
-    for (int P = 1; P <= 5 && ok && !rc; P++) {
-      if (!b->itemCount[P]) continue;
-      ok = hipStreamWaitEvent(b->ps[P], b->fork, 0) == hipSuccess;
-      for (int s = 0; s < b->nRK && ok && !rc; s++) rc = batchLaunchStage(b, p, s, b->grid, b->ps[P], P);
-      ok = ok && hipEventRecord(b->join[P], b->ps[P]) == hipSuccess &&
-           hipStreamWaitEvent(b->stream, b->join[P], 0) == hipSuccess;
+    for (int grp = 1; grp <= 10 && ok && !rc; grp++) {
+      if (!b->itemCount[grp]) continue;
+      ok = hipStreamWaitEvent(b->ps[grp], b->fork, 0) == hipSuccess;
+      for (int s = 0; s < b->nRK && ok && !rc; s++) rc = batchLaunchStage(b, p, s, b->grid, b->ps[grp], grp);
+      ok = ok && hipEventRecord(b->join[grp], b->ps[grp]) == hipSuccess &&
+           hipStreamWaitEvent(b->stream, b->join[grp], 0) == hipSuccess;
     }
     if (!ok || rc) {
       hipGraph_t g = nullptr;

@@ -3,7 +3,8 @@
 // Each work item is one element block of one mesh (instance); the meshes may
 // differ in size, order P (1..5) and boundaries, and share the dimension, the
 // equation and the RK scheme.  Per RK stage there is one persistent launch
-// per order present: a workgroup walks that order's item list with a stride
+// per order present (and, in fr_tri.hpp, per triangle order): a workgroup
+// walks that group's item list with a stride
 // of the grid size and runs the same stage computation as the per-mesh
 // kernel (stageBody) for each item, with that mesh's arguments.
 #pragma once
@@ -17,6 +18,17 @@ struct BatchItem {
   int block;  // element block of that mesh
 };
 
+// The thread index, made opaque to the compiler for one item of a persistent
+// loop: index arithmetic derived from it is then recomputed per item instead
+// of being hoisted out of the loop and kept live across it (which doubles the
+// register allocation of the stage computation and halves occupancy).
+__device__ __forceinline__ int itemThreadIdx() {
+  int t = (int)threadIdx.x;
+  asm volatile("" : "+v"(t));
+  return t;
+}
+
+
 // Every order's workgroup is 256 threads (ElemGeo NTH).  One kernel per order
 // keeps each order's register allocation (a single kernel switching over the
 // orders is allocated for their union and spills).
@@ -27,7 +39,7 @@ __global__ __launch_bounds__(256) void k_batch_stage(const StageArgs<T> *__restr
   __shared__ StageSmem<D, P, NF, T> sm;
   for (int it = blockIdx.x; it < nItems; it += gridDim.x) {
     const BatchItem w = items[it];
-    stageBody<D, P, NF, T, true, false, FIN>(args[w.inst], w.block, sm);
+    stageBody<D, P, NF, T, true, false, FIN>(args[w.inst], w.block, sm, itemThreadIdx());
     __syncthreads();  // the next item reuses the LDS
   }
 }

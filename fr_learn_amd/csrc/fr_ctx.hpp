@@ -110,14 +110,16 @@ struct fr_batch {
   hipStream_t stream = nullptr;
   void *args[2] = {nullptr, nullptr};  // StageArgs per [trace parity][stage][instance]
   void *argsTune = nullptr;            // residual-only (mode 0) StageArgs per instance, for tuning
-  frdev::BatchItem *items = nullptr;  // grouped by order
-  int itemStart[6] = {0, 0, 0, 0, 0, 0}, itemCount[6] = {0, 0, 0, 0, 0, 0};
+  // work items grouped by kernel: group g = P (quads / hexes of order P) or
+  // 5 + P (triangles of order P), g in 1..10
+  frdev::BatchItem *items = nullptr;
+  int itemStart[11] = {0}, itemCount[11] = {0};
   int nItems = 0, grid = 0;           // persistent workgroups per order's launch
   double tunedMs = 0;
   hipGraph_t graph[2] = {nullptr, nullptr};
   hipGraphExec_t exec[2] = {nullptr, nullptr};
-  hipStream_t ps[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};  // one per order (capture)
-  hipEvent_t fork = nullptr, join[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  hipStream_t ps[11] = {nullptr};  // one per group (capture)
+  hipEvent_t fork = nullptr, join[11] = {nullptr};
 };
 
 namespace frimpl {
@@ -181,6 +183,6 @@ template <typename T>
 int launchTriTrace(fr_ctx *c, const T *U, T *Uf, const T *tab);
 // Stage s of one batched step (fr_batch.hip); parity < 0 is the tuning launch,
 // Ponly != 0 restricts it to the meshes of that order.
-int batchLaunchStage(fr_batch *b, int parity, int s, int grid, hipStream_t st, int Ponly = 0);
+int batchLaunchStage(fr_batch *b, int parity, int s, int grid, hipStream_t st, int gOnly = 0);
 
 }  // namespace frimpl

@@ -365,7 +365,8 @@ struct StageSmem {
 // The stage computation of element block b (the elements b*EPB ..), shared
 // by the per-mesh kernel and the batched persistent kernel.
 template <int D, int P, int NF, typename T, bool BND, bool VIS, bool FIN>
-__device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, StageSmem<D, P, NF, T> &sm) {
+__device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, StageSmem<D, P, NF, T> &sm,
+                                          const int tid) {
   using G = ElemGeo<D, P>;
   using O = OneD<P>;
   constexpr int NS = G::NS, NFP = G::NFP, NFPF = G::NFPF, P1 = G::P1, TPE = G::TPE, EPB = G::EPB;
@@ -375,7 +376,7 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
   auto &sFd = sm.sFd;
   auto &sJ = sm.sJ;
   auto &s1 = sm.s1;
-  const int slot = threadIdx.x / TPE, t = threadIdx.x - slot * TPE;
+  const int slot = tid / TPE, t = tid - slot * TPE;
   const int tP = padIdx<P>(t);  // padded LDS index of this thread's solution point
   const int e = b * EPB + slot;
   const bool act = e < a.nEles;
@@ -406,7 +407,7 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
     pfi[r] = a.finfo[ec * NFACE + qc / NFPF];
     pnb[r] = a.fnbr[ec * NFACE + qc / NFPF];
   }
-  const T o1 = a.oned[min((int)threadIdx.x, O::SIZE - 1)];
+  const T o1 = a.oned[min(tid, O::SIZE - 1)];
   T ul[NU];
   {
     const T *Ue = a.Uin + ec * NF * NS;
@@ -456,7 +457,7 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
         for (int k = 0; k < NF; k++) pd[r][k] = dp[r][(ec * NF + k) * NS + tc];
       }
   }
-  if (threadIdx.x < O::SIZE) s1[threadIdx.x] = o1;
+  if (tid < O::SIZE) s1[tid] = o1;
   if (act) {
 #pragma unroll
     for (int j = 0; j < NU; j++)
@@ -743,7 +744,7 @@ __global__ __launch_bounds__((ElemGeo<D, P>::NTH)) void k_stage(StageArgs<T> a) 
   // contiguous range of elements and face neighbours share its L2.  The host
   // pads the grid to a multiple of 8.
   const int b = (blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);
-  stageBody<D, P, NF, T, BND, VIS, FIN>(a, b, sm);
+  stageBody<D, P, NF, T, BND, VIS, FIN>(a, b, sm, (int)threadIdx.x);
 }
 
 // ---------------------------------------------------------------------------

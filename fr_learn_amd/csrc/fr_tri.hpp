@@ -12,6 +12,7 @@
 // face's traces go through HBM (no in-block exchange).
 #pragma once
 
+#include "fr_batch.hpp"
 #include "fr_kernels.hpp"
 
 namespace frdev {
@@ -43,20 +44,20 @@ __device__ __forceinline__ void triNormal(int f, T &nx, T &ny) {
   ny = (f == 0) ? (T)-1 : (f == 1) ? h : (T)0;
 }
 
+// One RK stage of element block b (EPB elements) with the arguments a.
 template <int P, int NF, typename T, bool BND, bool FIN>
-__global__ __launch_bounds__(256) void k_tri_stage(StageArgs<T> a) {
+__device__ __forceinline__ void triStageBody(const StageArgs<T> &a, const int b, TriSmem<P, NF, T> &sm,
+                                             const int tid) {
   using G = TriGeo<P>;
   constexpr int NS = G::NS, NFP = G::NFP, NFPF = G::NFPF, TPE = G::TPE, EPB = G::EPB, NFACE = G::NFACE;
-  __shared__ TriSmem<P, NF, T> sm;
-  const int b = (blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);  // XCD-contiguous blocks
-  const int slot = threadIdx.x / TPE, t = threadIdx.x - slot * TPE;
+  const int slot = tid / TPE, t = tid - slot * TPE;
   const int e = b * EPB + slot;
   const bool act = e < a.nEles;
   const bool hasS = act && t < NS, hasQ = act && t < NFP;
   const size_t ec = (size_t)(act ? e : a.nEles - 1);
   const int tc = t < NS ? t : NS - 1, qc = t < NFP ? t : NFP - 1;
 
-  for (int i = threadIdx.x; i < G::TAB; i += blockDim.x) sm.tab[i] = a.oned[i];
+  for (int i = tid; i < G::TAB; i += G::NTH) sm.tab[i] = a.oned[i];
   if (act)
     for (int i = t; i < NF * NS; i += TPE) (&sm.sU[slot][0][0])[i] = a.Uin[ec * NF * NS + i];
   // this thread's solution point: metrics, detJ, dt, base state
@@ -102,11 +103,14 @@ __global__ __launch_bounds__(256) void k_tri_stage(StageArgs<T> a) {
   T Fn[NF];
   if (hasQ) {
     T uo[NF], ub[NF], Fc[NF];
+    // operator entries read once for all fields (the LDS reads bound these loops)
 #pragma unroll
-    for (int k = 0; k < NF; k++) {
-      T acc = (T)0;
-      for (int s = 0; s < NS; s++) acc += Em[t * NS + s] * sm.sU[slot][k][s];
-      uo[k] = acc;
+    for (int k = 0; k < NF; k++) uo[k] = (T)0;
+#pragma unroll 3
+    for (int s = 0; s < NS; s++) {
+      const T w = Em[t * NS + s];
+#pragma unroll
+      for (int k = 0; k < NF; k++) uo[k] += w * sm.sU[slot][k][s];
     }
     const bool right = !bnd && ((fi >> 5) & 1);
     if (bnd) {
@@ -126,27 +130,43 @@ __global__ __launch_bounds__(256) void k_tri_stage(StageArgs<T> a) {
   if (hasQ) {
     T tx, ty;
     triNormal<T>(t / NFPF, tx, ty);
+    T fr[NF], fs[NF];
 #pragma unroll
-    for (int k = 0; k < NF; k++) {
-      T fr = (T)0, fs = (T)0;
-      for (int s = 0; s < NS; s++) {
-        fr += Em[t * NS + s] * sm.sF[slot][0][k][s];
-        fs += Em[t * NS + s] * sm.sF[slot][1][k][s];
+    for (int k = 0; k < NF; k++) fr[k] = fs[k] = (T)0;
+#pragma unroll 3
+    for (int s = 0; s < NS; s++) {
+      const T w = Em[t * NS + s];
+#pragma unroll
+      for (int k = 0; k < NF; k++) {
+        fr[k] += w * sm.sF[slot][0][k][s];
+        fs[k] += w * sm.sF[slot][1][k][s];
       }
-      sm.sJ[slot][k][t] = Fn[k] - (tx * fr + ty * fs);
     }
+#pragma unroll
+    for (int k = 0; k < NF; k++) sm.sJ[slot][k][t] = Fn[k] - (tx * fr[k] + ty * fs[k]);
   }
   elementSync<TPE>();
   // (3) divergence + correction, stage update (as k_stage phase 4)
   if (hasS) {
     const size_t eo = ec * NF * NS;
+    T dvs[NF], cor[NF];
+#pragma unroll
+    for (int k = 0; k < NF; k++) dvs[k] = cor[k] = (T)0;
+#pragma unroll 3
+    for (int s = 0; s < NS; s++) {
+      const T wr = Dr[t * NS + s], ws = Ds[t * NS + s];
+#pragma unroll
+      for (int k = 0; k < NF; k++) dvs[k] += wr * sm.sF[slot][0][k][s] + ws * sm.sF[slot][1][k][s];
+    }
+#pragma unroll 3
+    for (int q = 0; q < NFP; q++) {
+      const T wl = Lm[t * NFP + q];
+#pragma unroll
+      for (int k = 0; k < NF; k++) cor[k] += wl * sm.sJ[slot][k][q];
+    }
 #pragma unroll
     for (int k = 0; k < NF; k++) {
-      T dv = (T)0;
-      for (int s = 0; s < NS; s++) dv += Dr[t * NS + s] * sm.sF[slot][0][k][s] + Ds[t * NS + s] * sm.sF[slot][1][k][s];
-      T corr = (T)0;
-      for (int q = 0; q < NFP; q++) corr += Lm[t * NFP + q] * sm.sJ[slot][k][q];
-      dv = dv + corr;
+      const T dv = dvs[k] + cor[k];
       const size_t gi = eo + (size_t)k * NS + t;
       if (a.divF) a.divF[gi] = dv;
       if (a.mode == 0) continue;
@@ -178,12 +198,37 @@ __global__ __launch_bounds__(256) void k_tri_stage(StageArgs<T> a) {
   // (4) traces of the updated state
   if (hasQ) {
     T *Fe = a.UfOut + ec * NF * NFP;
+    T acc[NF];
 #pragma unroll
-    for (int k = 0; k < NF; k++) {
-      T acc = (T)0;
-      for (int s = 0; s < NS; s++) acc += Em[t * NS + s] * sm.sN[slot][k][s];
-      Fe[k * NFP + t] = acc;
+    for (int k = 0; k < NF; k++) acc[k] = (T)0;
+#pragma unroll 3
+    for (int s = 0; s < NS; s++) {
+      const T w = Em[t * NS + s];
+#pragma unroll
+      for (int k = 0; k < NF; k++) acc[k] += w * sm.sN[slot][k][s];
     }
+#pragma unroll
+    for (int k = 0; k < NF; k++) Fe[k * NFP + t] = acc[k];
+  }
+}
+
+template <int P, int NF, typename T, bool BND, bool FIN>
+__global__ __launch_bounds__(256) void k_tri_stage(StageArgs<T> a) {
+  __shared__ TriSmem<P, NF, T> sm;
+  const int b = (blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);  // XCD-contiguous blocks
+  triStageBody<P, NF, T, BND, FIN>(a, b, sm, (int)threadIdx.x);
+}
+
+// Batched: a persistent workgroup walks the triangle meshes' element blocks of
+// order P (see k_batch_stage), with each mesh's arguments.
+template <int P, int NF, typename T, bool FIN>
+__global__ __launch_bounds__(256) void k_batch_tri_stage(const StageArgs<T> *__restrict__ args,
+                                                         const BatchItem *__restrict__ items, int nItems) {
+  __shared__ TriSmem<P, NF, T> sm;
+  for (int it = blockIdx.x; it < nItems; it += gridDim.x) {
+    const BatchItem w = items[it];
+    triStageBody<P, NF, T, true, FIN>(args[w.inst], w.block, sm, itemThreadIdx());
+    __syncthreads();  // the next item reuses the LDS
   }
 }
 

@@ -177,9 +177,10 @@ int fr_vjp_steps(fr_ctx *ctx, int nSteps, int every, const double *lamN, double 
 int fr_reduce_gradient(fr_ctx *ctx, int kind, double *out);
 
 /* Batches (no reference counterpart): many contexts of one nDims, equation,
- * precision and timeType (sizes, orders and boundaries free; inviscid)
- * advanced together, one persistent launch per RK stage over all their
- * element blocks, one step's launches replayed from a captured hipGraph.
+ * precision and timeType (sizes, orders, boundaries and, in 2D, quads and
+ * triangles mixed; inviscid) advanced together, one persistent launch per
+ * RK stage, element type and order over all their element blocks, one
+ * step's launches replayed from a captured hipGraph.
  * fr_batch_create autotunes the persistent grid on residual evaluations
  * (which overwrite divF_spts[0] only; fr_batch_tuning reports the choice).
  * fr_batch_update is fr_update(nSteps) on every member; the contexts stay

@@ -49,6 +49,36 @@ def _run(members, steps, precision="fp64"):
     return solo, batched, U0s, B
 
 
+def _mixed_members(rng, n):
+    """Quads and triangles (every second member), P 1..5."""
+    members = []
+    for i in range(n):
+        P = 1 + (i // 2) % 5
+        cells = int(rng.integers(3, 8))
+        periodic = bool(i % 3)
+        bc = None if periodic else QUAD_BCS
+        if i % 2:
+            m = mesh.tri_box((cells, cells), (-5.0, -5.0), (5.0, 5.0), periodic=[periodic] * 2, bc=bc,
+                             perturb=0.15, rotate=True, shuffle=True, seed=i)
+        else:
+            m = quad_box(cells, periodic=periodic, bc=bc, seed=i, lo=-5.0, hi=5.0)
+        cfg = euler_cfg(2, P, riemannType=int((i // 2) % 2), timeType=4, dt=1e-3)
+        ic = (lambda b: lambda x, c: cases.ic_vortex(x, c, beta=b, mach=0.4))(rng.uniform(1.0, 5.0))
+        members.append((cfg, m, ic))
+    print("batch of", n, "quad and triangle meshes", flush=True)
+    return members
+
+
+def test_batch_mixes_quads_and_triangles():
+    rng = np.random.default_rng(11)
+    members = _mixed_members(rng, 14)
+    solo, batched, _, B = _run(members, 3)
+    for i, (a, b) in enumerate(zip(solo, batched)):
+        err = rel_err(b.get_state(), a.get_state()).max()
+        print(i, "tri" if i % 2 else "quad", "P", members[i][0]["order"], "err", err)
+        assert err <= 1e-14
+
+
 @pytest.mark.parametrize("timeType", [4, 0])
 def test_batch_matches_each_mesh(timeType):
     rng = np.random.default_rng(7)

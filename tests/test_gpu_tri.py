@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 from numpy.polynomial.legendre import leggauss
 
-from fr_learn_amd import Batch, Solver, cases, mesh
+from fr_learn_amd import Solver, cases, mesh
 from fr_learn_amd._lib import FrError
 from oracle import tri
 
@@ -155,5 +155,3 @@ def test_unsupported_verbs_raise():
         g.calc_dt(1, 0.5)
     with pytest.raises(FrError):
         g.vjp_residual(np.zeros(g.state_shape()))
-    with pytest.raises(FrError):
-        Batch([g])
