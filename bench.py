@@ -99,10 +99,24 @@ def run_cpu_baseline(cfg, m, budget_s=10.0):
         nsteps += 1
     dt = time.perf_counter() - t0
     gdof = o.nEles * o.nSpts * 4 * nsteps / dt / 1e9
+    # one step on one core, continuing from the same state
+    o._lib.or_set_threads(1)
+    t1 = time.perf_counter()
+    o.update(1)
+    dt1 = time.perf_counter() - t1
     o.close()
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        pass
     return dict(value=gdof, unit="GDOF/s", cores=threads, kind="port",
                 sample=f"{nsteps} RK4 steps ({4 * nsteps} residuals) of the same 48^3 P3 TGV workload, "
-                       f"{dt:.1f} s, dense-operator oracle, {threads} OpenMP threads")
+                       f"{dt:.1f} s, dense-operator oracle, {threads} OpenMP threads",
+                single_core={"value": o.nEles * o.nSpts * 4 / dt1 / 1e9, "unit": "GDOF/s",
+                             "sample": f"1 RK4 step of the same workload, {dt1:.1f} s, 1 thread"},
+                host={"cpu": model, "logical_cpus": os.cpu_count()})
 
 
 def copy_bandwidth(nbytes=1 << 30, reps=10):
