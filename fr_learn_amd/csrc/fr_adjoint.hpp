@@ -233,10 +233,12 @@ __global__ __launch_bounds__((ElemGeo<D, P>::TPE)) void k_vjp_state_vis(
   using G = ElemGeo<D, P>;
   using O = OneD<P>;
   constexpr int NS = G::NS, NFP = G::NFP, NFPF = G::NFPF, P1 = G::P1, TPE = G::TPE, NFACE = G::NFACE;
-  // Derivatives in D + 1 passes of NF dual directions each: pass 0 seeds the
-  // state, pass d + 1 the gradient's dimension d (one pass over all of them
-  // would carry NF (D + 1) partials per value and spill).
-  using DM = Dual<NF>;
+  // Derivatives one direction at a time: pass p = grp * NF + j seeds field j
+  // of the state (grp 0) or of the gradient's dimension grp - 1.  Every value
+  // of a pass, seeded or not, carries as many partials as the dual type has,
+  // so wider duals (NF or NF (D + 1) directions per pass) hold several times
+  // the registers for the many unseeded values and spill heavily.
+  using DM = Dual<1>;
   __shared__ double sU[NF][NS], sL[NF][NS];
   __shared__ double sJ[NF][NFP];       // lamJ
   __shared__ double sC[NF][NFP];       // forward LDG jumps u^ - u
@@ -277,10 +279,11 @@ __global__ __launch_bounds__((ElemGeo<D, P>::TPE)) void k_vjp_state_vis(
     const int qn = bnd ? 0 : neighbourFpt<D, P>(q, fi);
     double g[NF], unv[NF];
 #pragma unroll 1
-    for (int pass = 0; pass <= D; pass++) {
+    for (int pass = 0; pass < (D + 1) * NF; pass++) {
+      const int grp = pass / NF, sj = pass - grp * NF;  // seeded group and field
       DM uo[NF], un[NF], n[D], go[D][NF], gn[D][NF], Fn[NF];
 #pragma unroll
-      for (int k = 0; k < NF; k++) uo[k] = (pass == 0) ? DM::var(uov[k], k) : DM(uov[k]);
+      for (int k = 0; k < NF; k++) uo[k] = (grp == 0 && k == sj) ? DM::var(uov[k], 0) : DM(uov[k]);
 #pragma unroll
       for (int c = 0; c < D; c++) n[c] = DM(nv[c]);
 #pragma unroll
@@ -288,7 +291,7 @@ __global__ __launch_bounds__((ElemGeo<D, P>::TPE)) void k_vjp_state_vis(
 #pragma unroll
         for (int k = 0; k < NF; k++) {
           const double v = Qf[(((size_t)e * D + d) * NF + k) * NFP + q];
-          go[d][k] = (pass == d + 1) ? DM::var(v, k) : DM(v);
+          go[d][k] = (grp == d + 1 && k == sj) ? DM::var(v, 0) : DM(v);
         }
       if (bnd) {
         const int bc = fi - FINFO_BOUNDARY;
@@ -325,14 +328,11 @@ __global__ __launch_bounds__((ElemGeo<D, P>::TPE)) void k_vjp_state_vis(
           ldgViscousFlux<D, NF, DM>(ph, 0, uo, un, go, gn, n, Fn);
         }
       }
+      double acc = 0;
 #pragma unroll
-      for (int j = 0; j < NF; j++) {
-        double acc = 0;
-#pragma unroll
-        for (int k = 0; k < NF; k++) acc += g[k] * Fn[k].d[j];
-        if (pass == 0) sUo[j][q] = acc;
-        else sQo[pass - 1][j][q] = acc;
-      }
+      for (int k = 0; k < NF; k++) acc += g[k] * Fn[k].d[0];
+      if (grp == 0) sUo[sj][q] = acc;
+      else sQo[grp - 1][sj][q] = acc;
     }
     // forward LDG jump of this point (as k_grad forms it)
     double jmp[NF];
@@ -392,26 +392,24 @@ __global__ __launch_bounds__((ElemGeo<D, P>::TPE)) void k_vjp_state_vis(
       }
     double lQ[D][NF];
 #pragma unroll 1
-    for (int pass = 0; pass <= D; pass++) {
+    for (int pass = 0; pass < (D + 1) * NF; pass++) {
+      const int grp = pass / NF, sj = pass - grp * NF;
       DM u[NF], qd[D][NF], Fi[D][NF], Fv[D][NF];
 #pragma unroll
-      for (int k = 0; k < NF; k++) u[k] = (pass == 0) ? DM::var(sU[k][t], k) : DM(sU[k][t]);
+      for (int k = 0; k < NF; k++) u[k] = (grp == 0 && k == sj) ? DM::var(sU[k][t], 0) : DM(sU[k][t]);
 #pragma unroll
       for (int d = 0; d < D; d++)
 #pragma unroll
-        for (int k = 0; k < NF; k++) qd[d][k] = (pass == d + 1) ? DM::var(Q[d][k], k) : DM(Q[d][k]);
+        for (int k = 0; k < NF; k++) qd[d][k] = (grp == d + 1 && k == sj) ? DM::var(Q[d][k], 0) : DM(Q[d][k]);
       viscousFlux<D, NF, DM>(ph, u, qd, Fv);
-      if (pass == 0) inviscidFlux<D, NF, DM>(ph, u, Fi);
+      if (grp == 0) inviscidFlux<D, NF, DM>(ph, u, Fi);
+      double acc = 0;
 #pragma unroll
-      for (int j = 0; j < NF; j++) {
-        double acc = 0;
+      for (int d2 = 0; d2 < D; d2++)
 #pragma unroll
-        for (int d2 = 0; d2 < D; d2++)
-#pragma unroll
-          for (int k = 0; k < NF; k++) acc += lF[d2][k] * ((pass == 0 ? Fi[d2][k].d[j] : 0.0) + Fv[d2][k].d[j]);
-        if (pass == 0) lU[j] = acc;
-        else lQ[pass - 1][j] = acc;
-      }
+        for (int k = 0; k < NF; k++) acc += lF[d2][k] * ((grp == 0 ? Fi[d2][k].d[0] : 0.0) + Fv[d2][k].d[0]);
+      if (grp == 0) lU[sj] = acc;
+      else lQ[grp - 1][sj] = acc;
     }
     // gradient traces and state traces, transposed
 #pragma unroll
