@@ -417,13 +417,13 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
   const GeoAddr<D, P> ga(a.affine, tc);
   T pJ[D * D];  // adjugate metrics of this solution point
 #pragma unroll
-  for (int c = 0; c < D * D; c++) pJ[c] = a.JG[ga.jg(ec, c)];
+  for (int c = 0; c < D * D; c++) pJ[c] = __builtin_nontemporal_load(a.JG + ga.jg(ec, c));
   T pn[NQ][D + 1], pu[NQ][NF];  // face normal + dA, neighbour trace
 #pragma unroll
   for (int r = 0; r < NQ; r++) {
     const int qc = min(t + r * TPE, NFP - 1);
 #pragma unroll
-    for (int c = 0; c <= D; c++) pn[r][c] = a.fgeo[ga.fg(ec, c, qc)];
+    for (int c = 0; c <= D; c++) pn[r][c] = __builtin_nontemporal_load(a.fgeo + ga.fg(ec, c, qc));
   }
   // stage base state, detJ and time step for the update (read in every mode:
   // in mode 0 the base is the input state, whose lines are already in cache)
@@ -442,7 +442,7 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
     const T *Ur = fetch ? a.UfIn + (size_t)pnb[r] * NF * NFP + neighbourFpt<D, P>(qc, pfi[r]) : a.oned;
     const int st = fetch ? NFP : 0;
 #pragma unroll
-    for (int k = 0; k < NF; k++) pu[r][k] = Ur[k * st];
+    for (int k = 0; k < NF; k++) pu[r][k] = __builtin_nontemporal_load(Ur + k * st);
     if (bndFace) pnb[r] = -1;
   }
   // earlier stages' residuals for the final update (issued last: they are
@@ -454,7 +454,7 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
     for (int r = 0; r < 3; r++)
       if (a.mode == 2 && r < a.nprev) {
 #pragma unroll
-        for (int k = 0; k < NF; k++) pd[r][k] = dp[r][(ec * NF + k) * NS + tc];
+        for (int k = 0; k < NF; k++) pd[r][k] = __builtin_nontemporal_load(dp[r] + (ec * NF + k) * NS + tc);
       }
   }
   if (tid < O::SIZE) s1[tid] = o1;
@@ -676,7 +676,7 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
       for (int f = 0; f < NFACE; f++) corr += s1[(cside[f] ? O::GR : O::GL) + cax[f]] * sJ[slot][k][cq[f]];
       const T dv = corr + div[k];
       const size_t gi = eo + (size_t)k * NS + t;
-      if (a.divF) a.divF[gi] = dv;
+      if (a.divF) __builtin_nontemporal_store(dv, a.divF + gi);
       if (a.mode == 0) continue;
       // stage terms RK * divF / detJ * dt, with dt / detJ formed once per point
       const double dtJ = pdt / pdet;
@@ -702,10 +702,10 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
         u = u - a.rk * (double)dv * dtJ;
       }
       if (a.mode == 1) {
-        a.Unext[gi] = (T)u;
+        __builtin_nontemporal_store((T)u, a.Unext + gi);
       } else {
-        a.Uout[gi] = u;
-        if (a.Uout32) a.Uout32[gi] = (float)u;
+        __builtin_nontemporal_store(u, a.Uout + gi);
+        if (a.Uout32) __builtin_nontemporal_store((float)u, a.Uout32 + gi);
       }
       sFd[slot][k][tP] = (T)u;
     }
@@ -731,7 +731,7 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
         T acc = (T)0;
 #pragma unroll
         for (int m = 0; m < P1; m++) acc += l1[m] * sFd[slot][k][lb + m * ls];
-        Fe[k * NFP + q] = acc;
+        __builtin_nontemporal_store(acc, Fe + k * NFP + q);
       }
     }
   }
