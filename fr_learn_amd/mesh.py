@@ -168,7 +168,8 @@ def tri_box(n, lo, hi, periodic=None, bc=None, perturb=0.0, seed=0, rotate=False
 def write_gmsh(mesh, path, names=None):
     """Write a Gmsh 2.2 ASCII mesh with one physical volume "fluid" and one
     physical surface per boundary (the format read by geo::readGmsh,
-    src/geo.cpp:448-770).  Boundary faces are emitted as linear quads/lines."""
+    src/geo.cpp:448-770).  Boundary faces are emitted as linear quads/lines;
+    cells as hexes, quads or triangles (Gmsh types 5, 3, 2)."""
     D = mesh.nDims
     names = names or [f"bnd{i}" for i in range(len(mesh.bounds))]
     lines = ["$MeshFormat", "2.2 0 8", "$EndMeshFormat", "$PhysicalNames", str(len(mesh.bounds) + 1)]
@@ -182,8 +183,9 @@ def write_gmsh(mesh, path, names=None):
         lines.append(f"{i + 1} {float(x[0])!r} {float(x[1])!r} {float(x[2])!r}")
     lines.append("$EndNodes")
     # boundary faces: element faces whose vertices all lie in one boundary set
+    tri = D == 2 and mesh.nNodes == 3
     fv = ([[0, 1, 2, 3], [4, 5, 6, 7], [3, 0, 4, 7], [2, 1, 5, 6], [1, 0, 4, 5], [3, 2, 6, 7]] if D == 3
-          else [[0, 1], [1, 2], [2, 3], [3, 0]])
+          else [[0, 1], [1, 2], [2, 0]] if tri else [[0, 1], [1, 2], [2, 3], [3, 0]])
     elems = []
     sets = [set(v.tolist()) for _, v in mesh.bounds]
     for e in range(mesh.nEles):
@@ -194,7 +196,7 @@ def write_gmsh(mesh, path, names=None):
                     elems.append((3 if D == 3 else 1, b + 1, vs))
                     break
     for e in range(mesh.nEles):
-        elems.append((5 if D == 3 else 3, len(mesh.bounds) + 1, mesh.c2v[e]))
+        elems.append((5 if D == 3 else 2 if tri else 3, len(mesh.bounds) + 1, mesh.c2v[e]))
     lines.append("$Elements")
     lines.append(str(len(elems)))
     for i, (t, tag, vs) in enumerate(elems):
@@ -219,9 +221,11 @@ def read_gmsh(path, mesh_bounds=None):
     case) that `mesh_bounds` maps to a boundary condition (the input file's
     meshBounds; by default the name itself); the group named "fluid" holds
     the cells and fixes nDims.  Cells are linear hexes (type 5) or, in 2D,
-    linear quads (type 3); every other element is a boundary face whose
-    vertices join its group's vertex set.  Vertices are read in file order.
-    Errors raise ValueError with the reference's message."""
+    linear quads (type 3) or linear triangles (type 2; one type per mesh;
+    triangles are turned counter-clockwise, the library's corner order --
+    FlurryPP itself reads no triangles); every other element is a boundary
+    face whose vertices join its group's vertex set.  Vertices are read in
+    file order.  Errors raise ValueError with the reference's message."""
     mesh_bounds = {k.lower(): v for k, v in (mesh_bounds or {}).items()}
     with open(path) as fh:
         lines = fh.read().split("\n")
@@ -262,7 +266,8 @@ def read_gmsh(path, mesh_bounds=None):
     i = section("$Elements")
     nEl = int(lines[i].split()[0])
     cells, bnd = [], [set() for _ in bcs]
-    cell_type, cell_nodes = (5, 8) if nDims == 3 else (3, 4)
+    cell_types = {5: 8} if nDims == 3 else {3: 4, 2: 3}
+    cell_type = None
     for k in range(nEl):
         tok = [int(t) for t in lines[i + 1 + k].split()]
         etype, ntags = tok[1], tok[2]
@@ -271,14 +276,25 @@ def read_gmsh(path, mesh_bounds=None):
             raise ValueError(f"Gmsh element ID {k}: physical group {tok[3]} not in $PhysicalNames")
         nodes = tok[3 + ntags:]
         if g == -1:
-            if etype != cell_type:
+            if etype not in cell_types:
                 raise ValueError(f"Gmsh element ID {k}, Gmsh Element Type = {etype}: element type not recognized "
-                                 f"(linear {'hexahedra' if nDims == 3 else 'quadrilaterals'} only)")
-            cells.append([v - 1 for v in nodes[:cell_nodes]])
+                                 f"(linear {'hexahedra' if nDims == 3 else 'quadrilaterals or triangles'} only)")
+            if cell_type is None:
+                cell_type = etype
+            elif etype != cell_type:
+                raise ValueError(f"Gmsh element ID {k}: mixed cell types in one mesh")
+            cells.append([v - 1 for v in nodes[:cell_types[etype]]])
         else:
             if etype not in _GMSH_FACE_PTS:
                 raise ValueError(f"Gmsh element ID {k}, Gmsh Element Type = {etype}: "
                                  "Boundary Element (Face) Type Not Recognized!")
             bnd[g].update(v - 1 for v in nodes[:_GMSH_FACE_PTS[etype]])
     bounds = [(bc, np.array(sorted(pts), dtype=np.int32)) for bc, pts in zip(bcs, bnd) if bc >= 0]
-    return Mesh(nDims, xv, np.array(cells, dtype=np.int32), None, bounds)
+    c2v = np.array(cells, dtype=np.int32)
+    if cell_type == 2:
+        X = xv[c2v]
+        area = ((X[:, 1, 0] - X[:, 0, 0]) * (X[:, 2, 1] - X[:, 0, 1]) -
+                (X[:, 1, 1] - X[:, 0, 1]) * (X[:, 2, 0] - X[:, 0, 0]))
+        cw = area < 0
+        c2v[cw] = c2v[cw][:, [0, 2, 1]]
+    return Mesh(nDims, xv, c2v, None, bounds)

@@ -95,6 +95,26 @@ def test_gmsh_reader_roundtrip(tmp_path):
             assert np.array_equal(a, b)
 
 
+def test_gmsh_triangles(tmp_path):
+    """Triangle cells (Gmsh type 2) round-trip; clockwise triangles come back
+    counter-clockwise (the library's corner order)."""
+    spec = {"xmin": "char", "xmax": "sub_out", "ymin": "slip_wall", "ymax": "sup_in"}
+    m = mesh.tri_box((3, 3), (0, 0), (1, 1), periodic=[False, False], bc=spec, perturb=0.1, seed=3)
+    inv = {v: k for k, v in mesh.BC.items()}
+    names = [f"S{i}" for i in range(len(m.bounds))]
+    p = tmp_path / "tri.msh"
+    mesh.write_gmsh(m, str(p), names=names)
+    bmap = {nm.lower(): inv[bc] for nm, (bc, _) in zip(names, m.bounds)} | {"fluid": "fluid"}
+    r = mesh.read_gmsh(str(p), mesh_bounds=bmap)
+    assert r.nNodes == 3 and np.array_equal(r.c2v, m.c2v) and np.array_equal(r.xv, m.xv)
+    cw = mesh.Mesh(2, m.xv, m.c2v[:, [0, 2, 1]], None, m.bounds)
+    mesh.write_gmsh(cw, str(p), names=names)
+    r = mesh.read_gmsh(str(p), mesh_bounds=bmap)
+    X = r.xv[r.c2v]
+    area = (X[:, 1, 0] - X[:, 0, 0]) * (X[:, 2, 1] - X[:, 0, 1]) - (X[:, 1, 1] - X[:, 0, 1]) * (X[:, 2, 0] - X[:, 0, 0])
+    assert (area > 0).all()
+
+
 def test_gmsh_reader_names_and_errors(tmp_path):
     """Physical names are unquoted and lower-cased; unknown names and cell
     types raise the reference's errors (geo.cpp:494-510, 636-641)."""
@@ -133,6 +153,9 @@ $EndElements
         mesh.read_gmsh(str(p), mesh_bounds={"inlet": "sup_in", "fluid": "fluid"})
     with pytest.raises(ValueError, match="Unrecognized boundary condition"):
         mesh.read_gmsh(str(p), mesh_bounds={"wall": "no_such_bc", "fluid": "fluid"})
-    p.write_text(text.replace("3 3 2 2 2 1 2 5 4", "3 2 2 2 2 1 2 5"))
+    p.write_text(text.replace("3 3 2 2 2 1 2 5 4", "3 4 2 2 2 1 2 5 4"))
     with pytest.raises(ValueError, match="element type not recognized"):
+        mesh.read_gmsh(str(p), mesh_bounds={"wall": "slip_wall", "fluid": "fluid"})
+    p.write_text(text.replace("3 3 2 2 2 1 2 5 4", "3 2 2 2 2 1 2 5"))
+    with pytest.raises(ValueError, match="mixed cell types"):
         mesh.read_gmsh(str(p), mesh_bounds={"wall": "slip_wall", "fluid": "fluid"})
