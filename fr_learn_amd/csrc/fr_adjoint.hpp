@@ -217,57 +217,54 @@ __global__ __launch_bounds__((ElemGeo<D, P>::NTH)) void k_vjp_state(
 // through the corrected gradient Q = JGinv^T (Dm U + lift(u^ - u)) / detJ at
 // the solution points (point viscous flux) and its traces Qf = E Q (face
 // viscous flux), and u^ couples neighbours.  Two more exchanges follow the
-// face-cotangent one: k_vjp_state_vis writes the cotangent of each flux
-// point's jump u^ - u (B), k_vjp_ldg gathers the neighbour's one to pull the
-// common solution u^ back to this element's traces.
-// One element per workgroup of TPE threads (the LDS of the viscous adjoint
-// does not fit several elements).
+// face-cotangent one: k_vjp_face_vis pulls the common fluxes back to each flux
+// point's traces (one thread per point), k_vjp_state_vis (one element per
+// workgroup of TPE threads) the point fluxes and the gradient back to the
+// state and writes the cotangent of each flux point's jump u^ - u (B), and
+// k_vjp_ldg gathers the neighbour's one to pull the common solution u^ back
+// to this element's traces.
 // ---------------------------------------------------------------------------
+// Face part of the viscous state VJP, one thread per (element, flux point):
+// the common flux (inviscid + LDG viscous) pulled back to this element's
+// state trace and gradient traces, Uo[ele][field][fpt] and
+// Qo[ele][dim][field][fpt], contracted with the point's face cotangent
+// (lamJ * dA plus the paired point's A); and the forward LDG jump u^ - u,
+// Cj[ele][field][fpt].  Uf are the state traces, Qf the gradient traces.
 template <int D, int P, int NF, bool BND>
-__global__ __launch_bounds__((ElemGeo<D, P>::TPE)) void k_vjp_state_vis(
-    const double *__restrict__ U, const double *__restrict__ lam, const double *__restrict__ Uf,
-    const double *__restrict__ Qf, const double *__restrict__ A, const double *__restrict__ JG,
-    const double *__restrict__ detJ, const double *__restrict__ fgeo, const int32_t *__restrict__ fnbr,
+__global__ __launch_bounds__(256) void k_vjp_face_vis(
+    const double *__restrict__ lam, const double *__restrict__ Uf, const double *__restrict__ Qf,
+    const double *__restrict__ A, const double *__restrict__ fgeo, const int32_t *__restrict__ fnbr,
     const int8_t *__restrict__ finfo, const double *__restrict__ oned, Phys ph, int nEles,
-    double *__restrict__ out, double *__restrict__ B) {
+    double *__restrict__ Uo, double *__restrict__ Qo, double *__restrict__ Cj) {
   using G = ElemGeo<D, P>;
   using O = OneD<P>;
-  constexpr int NS = G::NS, NFP = G::NFP, NFPF = G::NFPF, P1 = G::P1, TPE = G::TPE, NFACE = G::NFACE;
-  // Derivatives one direction at a time: pass p = grp * NF + j seeds field j
-  // of the state (grp 0) or of the gradient's dimension grp - 1.  Every value
-  // of a pass, seeded or not, carries as many partials as the dual type has,
-  // so wider duals (NF or NF (D + 1) directions per pass) hold several times
-  // the registers for the many unseeded values and spill heavily.
-  using DM = Dual<1>;
-  __shared__ double sU[NF][NS], sL[NF][NS];
-  __shared__ double sJ[NF][NFP];       // lamJ
-  __shared__ double sC[NF][NFP];       // forward LDG jumps u^ - u
-  __shared__ double sUo[NF][NFP];      // cotangent of this element's state traces
-  __shared__ double sQo[D][NF][NFP];   // cotangent of this element's gradient traces
-  __shared__ double sLQ[D][NF][NS];    // cotangent of the reference gradient
+  constexpr int NS = G::NS, NFP = G::NFP, NFPF = G::NFPF, P1 = G::P1, NFACE = G::NFACE;
+  using DM = Dual<1>;  // one direction per pass (see k_vjp_state_vis)
   __shared__ double s1[O::SIZE];
-  const int e = blockIdx.x, t = threadIdx.x;
-  for (int i = t; i < O::SIZE; i += TPE) s1[i] = oned[i];
-  for (int i = t; i < NF * NS; i += TPE) {
-    (&sU[0][0])[i] = U[(size_t)e * NF * NS + i];
-    (&sL[0][0])[i] = lam[(size_t)e * NF * NS + i];
-  }
+  for (int i = threadIdx.x; i < O::SIZE; i += blockDim.x) s1[i] = oned[i];
   __syncthreads();
-  for (int q = t; q < NFP; q += TPE) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)nEles * NFP) return;
+  const int e = (int)(idx / NFP), q = (int)(idx - (long)e * NFP);
+  {
     double lj[NF];
-    lamJAt<D, P, NF>(s1, &sL[0][0], q, lj);
-    int base, stride, side, dn;
-    fptLine<D, P>(q, base, stride, side, dn);
-    const double *l1 = s1 + (side ? O::LR : O::LL);
+    {
+      // lamJ at this point from the element's cotangent along the point's line
+      int base, stride, side, dn;
+      fptLine<D, P>(q, base, stride, side, dn);
+      const double *g = s1 + (side ? O::GR : O::GL);
+      const double *le = lam + (size_t)e * NF * NS;
+#pragma unroll
+      for (int k = 0; k < NF; k++) {
+        double acc = 0;
+#pragma unroll
+        for (int m = 0; m < P1; m++) acc += g[m] * le[k * NS + base + m * stride];
+        lj[k] = acc;
+      }
+    }
     double uov[NF];
 #pragma unroll
-    for (int k = 0; k < NF; k++) {
-      double acc = 0;
-#pragma unroll
-      for (int m = 0; m < P1; m++) acc += l1[m] * sU[k][base + m * stride];
-      uov[k] = acc;
-      sJ[k][q] = lj[k];
-    }
+    for (int k = 0; k < NF; k++) uov[k] = Uf[((size_t)e * NF + k) * NFP + q];
     double nv[D];
 #pragma unroll
     for (int c = 0; c < D; c++) nv[c] = fgeo[((size_t)e * (D + 1) + c) * NFP + q];
@@ -331,14 +328,57 @@ __global__ __launch_bounds__((ElemGeo<D, P>::TPE)) void k_vjp_state_vis(
       double acc = 0;
 #pragma unroll
       for (int k = 0; k < NF; k++) acc += g[k] * Fn[k].d[0];
-      if (grp == 0) sUo[sj][q] = acc;
-      else sQo[grp - 1][sj][q] = acc;
+      if (grp == 0) Uo[((size_t)e * NF + sj) * NFP + q] = acc;
+      else Qo[(((size_t)e * D + grp - 1) * NF + sj) * NFP + q] = acc;
     }
     // forward LDG jump of this point (as k_grad forms it)
     double jmp[NF];
     ldgJump<D, NF, double>(ph, bnd, right, uov, unv, nv, jmp);
 #pragma unroll
-    for (int k = 0; k < NF; k++) sC[k][q] = jmp[k];
+    for (int k = 0; k < NF; k++) Cj[((size_t)e * NF + k) * NFP + q] = jmp[k];
+  }
+}
+
+template <int D, int P, int NF, bool BND>
+__global__ __launch_bounds__((ElemGeo<D, P>::TPE)) void k_vjp_state_vis(
+    const double *__restrict__ U, const double *__restrict__ lam, const double *__restrict__ Uo,
+    const double *__restrict__ Qo, const double *__restrict__ Cj, const double *__restrict__ JG,
+    const double *__restrict__ detJ, const int8_t *__restrict__ finfo, const double *__restrict__ oned, Phys ph,
+    int nEles, double *__restrict__ out, double *__restrict__ B) {
+  using G = ElemGeo<D, P>;
+  using O = OneD<P>;
+  constexpr int NS = G::NS, NFP = G::NFP, NFPF = G::NFPF, P1 = G::P1, TPE = G::TPE, NFACE = G::NFACE;
+  // Derivatives one direction at a time: pass p = grp * NF + j seeds field j
+  // of the state (grp 0) or of the gradient's dimension grp - 1.  Every value
+  // of a pass, seeded or not, carries as many partials as the dual type has,
+  // so wider duals (NF or NF (D + 1) directions per pass) hold several times
+  // the registers for the many unseeded values and spill heavily.
+  using DM = Dual<1>;
+  __shared__ double sU[NF][NS], sL[NF][NS];
+  __shared__ double sJ[NF][NFP];       // lamJ
+  __shared__ double sC[NF][NFP];       // forward LDG jumps u^ - u
+  __shared__ double sUo[NF][NFP];      // cotangent of this element's state traces
+  __shared__ double sQo[D][NF][NFP];   // cotangent of this element's gradient traces
+  __shared__ double sLQ[D][NF][NS];    // cotangent of the reference gradient
+  __shared__ double s1[O::SIZE];
+  const int e = blockIdx.x, t = threadIdx.x;
+  for (int i = t; i < O::SIZE; i += TPE) s1[i] = oned[i];
+  for (int i = t; i < NF * NS; i += TPE) {
+    (&sU[0][0])[i] = U[(size_t)e * NF * NS + i];
+    (&sL[0][0])[i] = lam[(size_t)e * NF * NS + i];
+  }
+  __syncthreads();
+  for (int q = t; q < NFP; q += TPE) {
+    double lj[NF];
+    lamJAt<D, P, NF>(s1, &sL[0][0], q, lj);
+#pragma unroll
+    for (int k = 0; k < NF; k++) {
+      sJ[k][q] = lj[k];
+      sUo[k][q] = Uo[((size_t)e * NF + k) * NFP + q];
+      sC[k][q] = Cj[((size_t)e * NF + k) * NFP + q];
+#pragma unroll
+      for (int d = 0; d < D; d++) sQo[d][k][q] = Qo[(((size_t)e * D + d) * NF + k) * NFP + q];
+    }
   }
   __syncthreads();
   double lU[NF];

@@ -19,18 +19,25 @@ hipError_t vjpViscousLaunchP(fr_ctx *c, const double *Us, const double *lam, dou
   const unsigned nbg = (unsigned)(((c->nE + G::EPB - 1) / G::EPB + 7) / 8 * 8);
   frdev::GradArgs<double> ga{Us, c->adjTr, c->JG64, c->detJ, c->fgeo64, c->fnbr, c->finfo, c->oned64, c->adjQf,
                              c->nE, c->ph, /*affine=*/0};
+  const unsigned nbq = (unsigned)(((long)c->nE * G::NFP + 255) / 256);
   if (bnd) {
     hipLaunchKernelGGL((frdev::k_grad<D, P, NF, double, true>), dim3(nbg), dim3(G::NTH), 0, c->stream, ga);
+    hipLaunchKernelGGL((frdev::k_vjp_face_vis<D, P, NF, true>), dim3(nbq), dim3(256), 0, c->stream, lam, c->adjTr,
+                       c->adjQf, c->adjA, c->fgeo64, c->fnbr, c->finfo, c->oned64, c->ph, c->nE, c->adjUo,
+                       c->adjQo, c->adjCj);
     hipLaunchKernelGGL((frdev::k_vjp_state_vis<D, P, NF, true>), dim3(c->nE), dim3(G::TPE), 0, c->stream, Us, lam,
-                       c->adjTr, c->adjQf, c->adjA, c->JG64, c->detJ, c->fgeo64, c->fnbr, c->finfo, c->oned64,
-                       c->ph, c->nE, out, c->adjB);
+                       c->adjUo, c->adjQo, c->adjCj, c->JG64, c->detJ, c->finfo, c->oned64, c->ph, c->nE, out,
+                       c->adjB);
     hipLaunchKernelGGL((frdev::k_vjp_ldg<D, P, NF, true>), dim3(c->nE), dim3(G::TPE), 0, c->stream, Us, c->adjB,
                        c->fgeo64, c->fnbr, c->finfo, c->oned64, c->ph, c->nE, out);
   } else {
     hipLaunchKernelGGL((frdev::k_grad<D, P, NF, double, false>), dim3(nbg), dim3(G::NTH), 0, c->stream, ga);
+    hipLaunchKernelGGL((frdev::k_vjp_face_vis<D, P, NF, false>), dim3(nbq), dim3(256), 0, c->stream, lam, c->adjTr,
+                       c->adjQf, c->adjA, c->fgeo64, c->fnbr, c->finfo, c->oned64, c->ph, c->nE, c->adjUo,
+                       c->adjQo, c->adjCj);
     hipLaunchKernelGGL((frdev::k_vjp_state_vis<D, P, NF, false>), dim3(c->nE), dim3(G::TPE), 0, c->stream, Us, lam,
-                       c->adjTr, c->adjQf, c->adjA, c->JG64, c->detJ, c->fgeo64, c->fnbr, c->finfo, c->oned64,
-                       c->ph, c->nE, out, c->adjB);
+                       c->adjUo, c->adjQo, c->adjCj, c->JG64, c->detJ, c->finfo, c->oned64, c->ph, c->nE, out,
+                       c->adjB);
     hipLaunchKernelGGL((frdev::k_vjp_ldg<D, P, NF, false>), dim3(c->nE), dim3(G::TPE), 0, c->stream, Us, c->adjB,
                        c->fgeo64, c->fnbr, c->finfo, c->oned64, c->ph, c->nE, out);
   }

@@ -24,7 +24,7 @@ void freeAll(fr_ctx *c) {
                   c->finfo,   c->dUf,     c->dtG,     c->dtE,     c->adjA,   c->adjTr, c->adjTr2,
                   c->adjLam,  c->adjOut,  c->adjNu,   c->adjW[0], c->adjW[1], c->adjS[0], c->adjS[1],
                   c->adjS[2], c->adjS[3], c->adjQf,   c->adjB,    c->JGc64,  c->detJc, c->fgeoc64,
-                  c->JGc32,   c->fgeoc32};
+                  c->JGc32,   c->fgeoc32, c->adjUo,   c->adjQo,   c->adjCj};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   for (auto &v : c->ev)
@@ -265,7 +265,10 @@ int adjCheck(fr_ctx *c) {
       return rc;
     for (int s = 1; s < c->nRK; s++)
       if ((rc = dmalloc(&c->adjS[s], nSt))) return rc;
-    if (c->cfg.viscous && ((rc = dmalloc(&c->adjQf, nTr * c->D)) || (rc = dmalloc(&c->adjB, nTr)))) return rc;
+    if (c->cfg.viscous && ((rc = dmalloc(&c->adjQf, nTr * c->D)) || (rc = dmalloc(&c->adjB, nTr)) ||
+                           (rc = dmalloc(&c->adjUo, nTr)) || (rc = dmalloc(&c->adjQo, nTr * c->D)) ||
+                           (rc = dmalloc(&c->adjCj, nTr))))
+      return rc;
   }
   return 0;
 }

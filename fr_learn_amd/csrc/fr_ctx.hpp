@@ -83,6 +83,7 @@ struct fr_ctx {
   double *adjA = nullptr, *adjTr = nullptr, *adjTr2 = nullptr;  // face cotangents, trace scratch
   double *adjLam = nullptr, *adjOut = nullptr, *adjNu = nullptr, *adjW[2] = {nullptr, nullptr};
   double *adjQf = nullptr, *adjB = nullptr;                     // viscous: gradient traces, jump cotangents
+  double *adjUo = nullptr, *adjQo = nullptr, *adjCj = nullptr;  // viscous: face pull-backs, forward jumps
   double *adjS[4] = {nullptr, nullptr, nullptr, nullptr};       // RK stage states
   int32_t *fnbr = nullptr;
   int8_t *finfo = nullptr;
