@@ -24,7 +24,7 @@ void freeAll(fr_ctx *c) {
                   c->finfo,   c->dUf,     c->dtG,     c->dtE,     c->adjA,   c->adjTr, c->adjTr2,
                   c->adjLam,  c->adjOut,  c->adjNu,   c->adjW[0], c->adjW[1], c->adjS[0], c->adjS[1],
                   c->adjS[2], c->adjS[3], c->adjQf,   c->adjB,    c->JGc64,  c->detJc, c->fgeoc64,
-                  c->JGc32,   c->fgeoc32, c->adjUo,   c->adjQo,   c->adjCj};
+                  c->JGc32,   c->fgeoc32, c->adjUo,   c->adjQo,   c->adjCj,   c->Fv};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   for (auto &v : c->ev)
@@ -107,6 +107,7 @@ frdev::StageArgs<T> makeStageArgs(fr_ctx *c, int in, int cur, int mode, int step
   a.dtE = dtSource(c).first;
   a.dtStride = dtSource(c).second;
   a.dUf = (const T *)c->dUf;
+  a.Fv = (const T *)c->Fv;
   a.mode = mode;
   a.nprev = step;
   a.nEles = c->nE;
@@ -404,7 +405,10 @@ int createImpl(const fr_config *cfg, const fr_mesh *m, int precision, fr_ctx *c)
   HIP_TRY(hipMalloc(&c->Us, nSt * ts));
   if ((rc = dmalloc(&c->dtG, 1))) return rc;
   HIP_TRY(hipMemcpy(c->dtG, &cfg->dt, sizeof(double), hipMemcpyHostToDevice));
-  if (cfg->viscous) HIP_TRY(hipMalloc(&c->dUf, (size_t)nE * D * c->NF * NFP * ts));
+  if (cfg->viscous) {
+    HIP_TRY(hipMalloc(&c->dUf, (size_t)nE * D * c->NF * NFP * ts));
+    HIP_TRY(hipMalloc(&c->Fv, (size_t)nE * c->NF * NFP * ts));
+  }
   HIP_TRY(hipMemset(c->Us, 0, nSt * ts));
   for (int s = 0; s < c->nRK; s++) {
     HIP_TRY(hipMalloc(&c->divF[s], nSt * ts));

@@ -76,6 +76,7 @@ struct fr_ctx {
   int traceOf = -1;                   // 0: U_spts, 1: stage state Us, -1: stale
   void *divF[4] = {nullptr, nullptr, nullptr, nullptr};
   void *dUf = nullptr;                // viscous: gradient traces [ele][dim][field][fpt]
+  void *Fv = nullptr;                 // viscous: common viscous normal flux [ele][field][fpt]
   double *dtG = nullptr;              // the global time step (one value)
   double *dtE = nullptr;              // per-element time steps (local time stepping)
   int dtType = 0;                     // 0 fixed dt, 1 global CFL dt, 2 local CFL dt

@@ -120,6 +120,7 @@ struct StageArgs {
   float *Uout32;
   const T *dprev0, *dprev1, *dprev2;
   const T *__restrict__ dUf;  // viscous: gradient traces [ele][dim][field][fpt]
+  const T *__restrict__ Fv;   // viscous: common viscous normal flux [ele][field][fpt] (k_vflux)
   const double *dtE;          // time step: dtE[ele * dtStride] (stride 0: one global dt)
   int dtStride;
   double c0, c1, c2, c3, rk;
@@ -219,6 +220,73 @@ __device__ __forceinline__ void correctedGradient(const T *s1, const T *sUe, con
       if constexpr (D == 3) acc = acc + gr[2][k] * J[2 * D + d1];
       g[d1][k] = invDet * acc;
     }
+}
+
+// ---------------------------------------------------------------------------
+// Viscous common normal flux at every flux point (face::calcViscousFlux,
+// face.cpp:177-287), one thread per (element, flux point), from the state
+// traces UfIn and the gradient traces dUf: boundaries take
+// boundFace::applyViscousBCs (boundFace.cpp:279-320), advection-diffusion
+// boundaries a zero outside gradient.  Right-side points evaluate it in the
+// left element's frame, as the inviscid flux.  Fv[ele][field][fpt] (before
+// the dA scaling), which the viscous stage kernel adds to its inviscid
+// common flux: one thread per point keeps the stage kernel's registers for
+// the rest of its work.
+// ---------------------------------------------------------------------------
+template <int D, int P, int NF, typename T, bool BND>
+__global__ __launch_bounds__(256) void k_vflux(const T *__restrict__ Uf, const T *__restrict__ dUf,
+                                               const T *__restrict__ fgeo, const int32_t *__restrict__ fnbr,
+                                               const int8_t *__restrict__ finfo, int affine, Phys ph, int nEles,
+                                               T *__restrict__ Fv) {
+  using G = ElemGeo<D, P>;
+  constexpr int NFP = G::NFP, NFPF = G::NFPF, NFACE = G::NFACE;
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)nEles * NFP) return;
+  const size_t e = (size_t)(idx / NFP);
+  const int q = (int)(idx - (long)e * NFP);
+  const GeoAddr<D, P> ga(affine, 0);
+  const int fi = finfo[e * NFACE + q / NFPF];
+  const bool bnd = BND && fi >= FINFO_BOUNDARY;
+  const bool right = !bnd && ((fi >> 5) & 1);
+  T n[D], uo[NF], un[NF], go[D][NF], gn[D][NF], F[NF];
+#pragma unroll
+  for (int c = 0; c < D; c++) n[c] = fgeo[ga.fg(e, c, q)];
+#pragma unroll
+  for (int k = 0; k < NF; k++) {
+    uo[k] = Uf[(e * NF + k) * NFP + q];
+    F[k] = (T)0;
+  }
+  const T *Go = dUf + e * D * NF * NFP + q;
+#pragma unroll
+  for (int d = 0; d < D; d++)
+#pragma unroll
+    for (int k = 0; k < NF; k++) go[d][k] = Go[(d * NF + k) * NFP];
+  int vb = 0;
+  if (bnd) {
+    const int bc = fi - FINFO_BOUNDARY;
+    vb = (bc == 10) ? 2 : 1;
+    boundaryState<D, NF, T>(ph, bc, uo, n, un);
+    if constexpr (NF > 1) {
+      boundaryGradient<D, NF, T>(ph, bc, uo, un, go, gn);
+    } else {
+#pragma unroll
+      for (int d = 0; d < D; d++) gn[d][0] = (T)0;
+    }
+  } else {
+    const size_t en = (size_t)fnbr[e * NFACE + q / NFPF];
+    const int qn = neighbourFpt<D, P>(q, fi);
+#pragma unroll
+    for (int k = 0; k < NF; k++) un[k] = Uf[(en * NF + k) * NFP + qn];
+    const T *Gn = dUf + en * D * NF * NFP + qn;
+#pragma unroll
+    for (int d = 0; d < D; d++)
+#pragma unroll
+      for (int k = 0; k < NF; k++) gn[d][k] = Gn[(d * NF + k) * NFP];
+  }
+  if (right) ldgViscousFlux<D, NF, T>(ph, 0, un, uo, gn, go, n, F);
+  else ldgViscousFlux<D, NF, T>(ph, vb, uo, un, go, gn, n, F);
+#pragma unroll
+  for (int k = 0; k < NF; k++) Fv[(e * NF + k) * NFP + q] = F[k];
 }
 
 // ---------------------------------------------------------------------------
@@ -626,34 +694,9 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
       if (right) commonFlux<D, NF, T>(a.ph, false, un, uo, n, Fn);
       else commonFlux<D, NF, T>(a.ph, bnd, uo, un, n, Fn);
       if constexpr (VIS) {
-        // face::calcViscousFlux (face.cpp:177-287) with the gradient traces;
-        // boundaries take boundFace::applyViscousBCs (boundFace.cpp:279-320),
-        // advection-diffusion boundaries a zero outside gradient
-        T go[D][NF], gn[D][NF];
-        const T *Go = a.dUf + (size_t)e * D * NF * NFP + q;
+        // the viscous common flux of this point (k_vflux, before this launch)
 #pragma unroll
-        for (int d = 0; d < D; d++)
-#pragma unroll
-          for (int k = 0; k < NF; k++) go[d][k] = Go[(d * NF + k) * NFP];
-        int vb = 0;
-        if (bnd) {
-          const int bc = fi - FINFO_BOUNDARY;
-          vb = (bc == 10) ? 2 : 1;
-          if constexpr (NF > 1) {
-            boundaryGradient<D, NF, T>(a.ph, bc, uo, un, go, gn);
-          } else {
-#pragma unroll
-            for (int d = 0; d < D; d++) gn[d][0] = (T)0;
-          }
-        } else {
-          const T *Gn = a.dUf + (size_t)a.fnbr[(size_t)e * NFACE + q / NFPF] * D * NF * NFP + neighbourFpt<D, P>(q, fi);
-#pragma unroll
-          for (int d = 0; d < D; d++)
-#pragma unroll
-            for (int k = 0; k < NF; k++) gn[d][k] = Gn[(d * NF + k) * NFP];
-        }
-        if (right) ldgViscousFlux<D, NF, T>(a.ph, 0, un, uo, gn, go, n, Fn);
-        else ldgViscousFlux<D, NF, T>(a.ph, vb, uo, un, go, gn, n, Fn);
+        for (int k = 0; k < NF; k++) Fn[k] += a.Fv[((size_t)e * NF + k) * NFP + q];
       }
       const T scale = right ? -dA : dA;
 #pragma unroll

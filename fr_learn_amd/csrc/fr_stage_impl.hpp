@@ -27,6 +27,13 @@ int launchStageArgs(fr_ctx *c, const frdev::StageArgs<T> &a, int mode) {
         TimeScope ts(c, 3);
         if (bnd) hipLaunchKernelGGL((frdev::k_grad<D, P, NF, T, true>), dim3(nb), dim3(G::NTH), 0, c->stream, g);
         else hipLaunchKernelGGL((frdev::k_grad<D, P, NF, T, false>), dim3(nb), dim3(G::NTH), 0, c->stream, g);
+        const unsigned nq = (unsigned)(((long)c->nE * G::NFP + 255) / 256);
+        if (bnd)
+          hipLaunchKernelGGL((frdev::k_vflux<D, P, NF, T, true>), dim3(nq), dim3(256), 0, c->stream, a.UfIn,
+                             (const T *)c->dUf, a.fgeo, a.fnbr, a.finfo, a.affine, a.ph, a.nEles, (T *)a.Fv);
+        else
+          hipLaunchKernelGGL((frdev::k_vflux<D, P, NF, T, false>), dim3(nq), dim3(256), 0, c->stream, a.UfIn,
+                             (const T *)c->dUf, a.fgeo, a.fnbr, a.finfo, a.affine, a.ph, a.nEles, (T *)a.Fv);
       }
       if ((err = hipGetLastError()) != hipSuccess) return;
       TimeScope ts(c, 0);
