@@ -275,21 +275,18 @@ __global__ __launch_bounds__(256) void k_vjp_face_vis(
     const int en = bnd ? 0 : fnbr[(size_t)e * NFACE + q / NFPF];
     const int qn = bnd ? 0 : neighbourFpt<D, P>(q, fi);
     double g[NF], unv[NF];
+    // state directions: dual passes through the inviscid and viscous fluxes
 #pragma unroll 1
-    for (int pass = 0; pass < (D + 1) * NF; pass++) {
-      const int grp = pass / NF, sj = pass - grp * NF;  // seeded group and field
+    for (int sj = 0; sj < NF; sj++) {
       DM uo[NF], un[NF], n[D], go[D][NF], gn[D][NF], Fn[NF];
 #pragma unroll
-      for (int k = 0; k < NF; k++) uo[k] = (grp == 0 && k == sj) ? DM::var(uov[k], 0) : DM(uov[k]);
+      for (int k = 0; k < NF; k++) uo[k] = (k == sj) ? DM::var(uov[k], 0) : DM(uov[k]);
 #pragma unroll
       for (int c = 0; c < D; c++) n[c] = DM(nv[c]);
 #pragma unroll
       for (int d = 0; d < D; d++)
 #pragma unroll
-        for (int k = 0; k < NF; k++) {
-          const double v = Qf[(((size_t)e * D + d) * NF + k) * NFP + q];
-          go[d][k] = (grp == d + 1 && k == sj) ? DM::var(v, 0) : DM(v);
-        }
+        for (int k = 0; k < NF; k++) go[d][k] = DM(Qf[(((size_t)e * D + d) * NF + k) * NFP + q]);
       if (bnd) {
         const int bc = fi - FINFO_BOUNDARY;
         boundaryState<D, NF, DM>(ph, bc, uo, n, un);
@@ -328,8 +325,64 @@ __global__ __launch_bounds__(256) void k_vjp_face_vis(
       double acc = 0;
 #pragma unroll
       for (int k = 0; k < NF; k++) acc += g[k] * Fn[k].d[0];
-      if (grp == 0) Uo[((size_t)e * NF + sj) * NFP + q] = acc;
-      else Qo[(((size_t)e * D + grp - 1) * NF + sj) * NFP + q] = acc;
+      Uo[((size_t)e * NF + sj) * NFP + q] = acc;
+    }
+    // gradient directions: the LDG viscous flux is affine in this point's
+    // gradient trace (viscous fluxes are linear in the gradient, and the
+    // boundary gradients are linear maps of it), so its derivative along
+    // e_dk is the flux at go = e_dk less the flux at go = 0, with the
+    // neighbour's gradient held at zero; the inviscid flux does not depend
+    // on it
+    {
+      double nd[D], uo[NF], un[NF];
+#pragma unroll
+      for (int c = 0; c < D; c++) nd[c] = nv[c];
+#pragma unroll
+      for (int k = 0; k < NF; k++) {
+        uo[k] = uov[k];
+        un[k] = unv[k];
+      }
+      const int bc = bnd ? fi - FINFO_BOUNDARY : 0;
+      auto viscF = [&](const double (&go)[D][NF], double *F) {
+        double gn[D][NF];
+#pragma unroll
+        for (int k = 0; k < NF; k++) F[k] = 0.0;
+        if (bnd) {
+          if constexpr (NF > 1) {
+            boundaryGradient<D, NF, double>(ph, bc, uo, un, go, gn);
+          } else {
+#pragma unroll
+            for (int d = 0; d < D; d++) gn[d][0] = 0.0;
+          }
+          ldgViscousFlux<D, NF, double>(ph, (bc == 10) ? 2 : 1, uo, un, go, gn, nd, F);
+        } else {
+#pragma unroll
+          for (int d = 0; d < D; d++)
+#pragma unroll
+            for (int k = 0; k < NF; k++) gn[d][k] = 0.0;
+          if (right) ldgViscousFlux<D, NF, double>(ph, 0, un, uo, gn, go, nd, F);
+          else ldgViscousFlux<D, NF, double>(ph, 0, uo, un, go, gn, nd, F);
+        }
+      };
+      double go[D][NF], F0[NF], F1[NF];
+#pragma unroll
+      for (int d = 0; d < D; d++)
+#pragma unroll
+        for (int k = 0; k < NF; k++) go[d][k] = 0.0;
+      viscF(go, F0);
+#pragma unroll 1
+      for (int dk = 0; dk < D * NF; dk++) {
+        const int d = dk / NF, sj = dk - d * NF;
+#pragma unroll
+        for (int d2 = 0; d2 < D; d2++)
+#pragma unroll
+          for (int k = 0; k < NF; k++) go[d2][k] = (d2 == d && k == sj) ? 1.0 : 0.0;
+        viscF(go, F1);
+        double acc = 0;
+#pragma unroll
+        for (int k = 0; k < NF; k++) acc += g[k] * (F1[k] - F0[k]);
+        Qo[(((size_t)e * D + d) * NF + sj) * NFP + q] = acc;
+      }
     }
     // forward LDG jump of this point (as k_grad forms it)
     double jmp[NF];
@@ -431,25 +484,47 @@ __global__ __launch_bounds__((ElemGeo<D, P>::TPE)) void k_vjp_state_vis(
         lF[d2][k] = acc;
       }
     double lQ[D][NF];
+    // state directions: dual passes
 #pragma unroll 1
-    for (int pass = 0; pass < (D + 1) * NF; pass++) {
-      const int grp = pass / NF, sj = pass - grp * NF;
+    for (int sj = 0; sj < NF; sj++) {
       DM u[NF], qd[D][NF], Fi[D][NF], Fv[D][NF];
 #pragma unroll
-      for (int k = 0; k < NF; k++) u[k] = (grp == 0 && k == sj) ? DM::var(sU[k][t], 0) : DM(sU[k][t]);
+      for (int k = 0; k < NF; k++) u[k] = (k == sj) ? DM::var(sU[k][t], 0) : DM(sU[k][t]);
 #pragma unroll
       for (int d = 0; d < D; d++)
 #pragma unroll
-        for (int k = 0; k < NF; k++) qd[d][k] = (grp == d + 1 && k == sj) ? DM::var(Q[d][k], 0) : DM(Q[d][k]);
+        for (int k = 0; k < NF; k++) qd[d][k] = DM(Q[d][k]);
       viscousFlux<D, NF, DM>(ph, u, qd, Fv);
-      if (grp == 0) inviscidFlux<D, NF, DM>(ph, u, Fi);
+      inviscidFlux<D, NF, DM>(ph, u, Fi);
       double acc = 0;
 #pragma unroll
       for (int d2 = 0; d2 < D; d2++)
 #pragma unroll
-        for (int k = 0; k < NF; k++) acc += lF[d2][k] * ((grp == 0 ? Fi[d2][k].d[0] : 0.0) + Fv[d2][k].d[0]);
-      if (grp == 0) lU[sj] = acc;
-      else lQ[grp - 1][sj] = acc;
+        for (int k = 0; k < NF; k++) acc += lF[d2][k] * (Fi[d2][k].d[0] + Fv[d2][k].d[0]);
+      lU[sj] = acc;
+    }
+    // gradient directions: the point viscous flux is linear in the gradient
+    // (zero at zero gradient), so its derivative along e_dk is its value there
+    {
+      double u[NF];
+#pragma unroll
+      for (int k = 0; k < NF; k++) u[k] = sU[k][t];
+#pragma unroll 1
+      for (int dk = 0; dk < D * NF; dk++) {
+        const int d = dk / NF, sj = dk - d * NF;
+        double qd[D][NF], Fv[D][NF];
+#pragma unroll
+        for (int d2 = 0; d2 < D; d2++)
+#pragma unroll
+          for (int k = 0; k < NF; k++) qd[d2][k] = (d2 == d && k == sj) ? 1.0 : 0.0;
+        viscousFlux<D, NF, double>(ph, u, qd, Fv);
+        double acc = 0;
+#pragma unroll
+        for (int d2 = 0; d2 < D; d2++)
+#pragma unroll
+          for (int k = 0; k < NF; k++) acc += lF[d2][k] * Fv[d2][k];
+        lQ[d][sj] = acc;
+      }
     }
     // gradient traces and state traces, transposed
 #pragma unroll
