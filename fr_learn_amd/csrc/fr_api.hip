@@ -108,6 +108,16 @@ frdev::StageArgs<T> makeStageArgs(fr_ctx *c, int in, int cur, int mode, int step
   a.dtStride = dtSource(c).second;
   a.dUf = (const T *)c->dUf;
   a.Fv = (const T *)c->Fv;
+  // Inviscid quads and hexes store no intermediate stage state: a stage that
+  // would read Us forms it from the base and the previous stage's residual
+  // (the viscous gradient kernel and the triangle kernels read Us itself).
+  if (!c->cfg.viscous && !c->tri) {
+    if (in == 1 && step > 0) {
+      a.rprev = (const T *)c->divF[step - 1];
+      a.rkPrev = c->RKa[step];
+    }
+    if (mode == 1) a.Unext = nullptr;
+  }
   a.mode = mode;
   a.nprev = step;
   a.nEles = c->nE;

@@ -72,7 +72,12 @@ __global__ __launch_bounds__(256) void k_trace(const T *__restrict__ U, T *__res
 //   updated state for the next stage (extrapolateU, solver.cpp:401-412).
 // mode 0: residual only (divF out).
 // mode 1: stage update Unext = UbaseT - rk * divF / detJ * dt (the base in
-//         the residual precision: the FP32 path's stage states are FP32).
+//         the residual precision: the FP32 path's stage states are FP32);
+//         Unext may be null when the next stage forms its state itself.
+// With rprev set, the stage state is not read: it is formed from the base
+// and the previous stage's residual, Us = base - rkPrev * rprev / detJ * dt
+// (the mode-1 update of the previous stage), so that an RK step stores no
+// intermediate stage state.
 // mode 2: final update U = Ubase - sum_r RKb[r] divF_r / detJ * dt, the
 //         subtractions in stage order as the timeStepB passes do.
 // mode 3: final Jameson-type update U = Ubase - rk * divF / detJ * dt.
@@ -119,6 +124,8 @@ struct StageArgs {
   double *Uout;
   float *Uout32;
   const T *dprev0, *dprev1, *dprev2;
+  const T *rprev;  // previous stage's residual, or null (stage state in Uin)
+  double rkPrev;   // its coefficient in the stage state
   const T *__restrict__ dUf;  // viscous: gradient traces [ele][dim][field][fpt]
   const T *__restrict__ Fv;   // viscous: common viscous normal flux [ele][field][fpt] (k_vflux)
   const double *dtE;          // time step: dtE[ele * dtStride] (stride 0: one global dt)
@@ -456,15 +463,15 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
   const int blockBase = b * EPB;
   auto inBlock = [&](int en) { return !VIS && (unsigned)(en - blockBase) < (unsigned)EPB; };
   // Loads.  Vector-memory counters retire in issue order, so the face records
-  // (which the neighbour-trace addresses depend on) go first, the state
-  // (needed in LDS before the barrier) next, then every other read of the
-  // element; only the records and the state are waited for before the
-  // barrier, the rest stays in flight through the LDS phases.  The loads are
+  // (which the neighbour-trace addresses depend on) go first, the state (or
+  // the residual it is formed from) and the base state, detJ and dt (needed
+  // in LDS before the barrier) next, then every other read of the element;
+  // only those are waited for before the barrier, the rest stays in flight
+  // through the LDS phases.  The loads are
   // branch-free (idle lanes read clamped, in-bounds indices; lanes with no
   // neighbour trace to fetch read the 1D table, which is cached), so that the
   // compiler's wait counts are exact.
   constexpr int NQ = (NFP + TPE - 1) / TPE;  // flux points per thread
-  constexpr int NU = (NF * NS + TPE - 1) / TPE;
   static_assert(O::SIZE <= G::NTH, "one thread per 1D table entry");
   const size_t ec = (size_t)(act ? e : a.nEles - 1);  // clamped element
   const int tc = t < NS ? t : NS - 1;                 // clamped solution point
@@ -476,22 +483,14 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
     pnb[r] = a.fnbr[ec * NFACE + qc / NFPF];
   }
   const T o1 = a.oned[min(tid, O::SIZE - 1)];
-  T ul[NU];
-  {
-    const T *Ue = a.Uin + ec * NF * NS;
-#pragma unroll
-    for (int j = 0; j < NU; j++) ul[j] = Ue[min(t + j * TPE, NF * NS - 1)];
-  }
   const GeoAddr<D, P> ga(a.affine, tc);
-  T pJ[D * D];  // adjugate metrics of this solution point
+  // Stage state at this thread's solution point: read from Uin, or (rprev
+  // set) the previous stage's residual, from which the state is formed below.
+  T ul[NF];
+  {
+    const T *src = (a.rprev ? a.rprev : a.Uin) + ec * NF * NS + tc;
 #pragma unroll
-  for (int c = 0; c < D * D; c++) pJ[c] = __builtin_nontemporal_load(a.JG + ga.jg(ec, c));
-  T pn[NQ][D + 1], pu[NQ][NF];  // face normal + dA, neighbour trace
-#pragma unroll
-  for (int r = 0; r < NQ; r++) {
-    const int qc = min(t + r * TPE, NFP - 1);
-#pragma unroll
-    for (int c = 0; c <= D; c++) pn[r][c] = __builtin_nontemporal_load(a.fgeo + ga.fg(ec, c, qc));
+    for (int k = 0; k < NF; k++) ul[k] = src[k * NS];
   }
   // stage base state, detJ and time step for the update (read in every mode:
   // in mode 0 the base is the input state, whose lines are already in cache)
@@ -502,6 +501,16 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
   for (int k = 0; k < NF; k++)
     pb[k] = (FIN || (VIS && a.mode >= 2)) ? a.Ubase[(ec * NF + k) * NS + tc]
                                           : (double)a.UbaseT[(ec * NF + k) * NS + tc];
+  T pJ[D * D];  // adjugate metrics of this solution point
+#pragma unroll
+  for (int c = 0; c < D * D; c++) pJ[c] = __builtin_nontemporal_load(a.JG + ga.jg(ec, c));
+  T pn[NQ][D + 1], pu[NQ][NF];  // face normal + dA, neighbour trace
+#pragma unroll
+  for (int r = 0; r < NQ; r++) {
+    const int qc = min(t + r * TPE, NFP - 1);
+#pragma unroll
+    for (int c = 0; c <= D; c++) pn[r][c] = __builtin_nontemporal_load(a.fgeo + ga.fg(ec, c, qc));
+  }
 #pragma unroll
   for (int r = 0; r < NQ; r++) {
     const int qc = min(t + r * TPE, NFP - 1);
@@ -514,25 +523,42 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
     if (bndFace) pnb[r] = -1;
   }
   // earlier stages' residuals for the final update (issued last: they are
-  // needed last)
+  // needed last); the latest one is already in ul when rprev is set
   T pd[FIN ? 3 : 1][NF];
   if constexpr (FIN) {
     const T *dp[3] = {a.dprev0, a.dprev1, a.dprev2};
 #pragma unroll
     for (int r = 0; r < 3; r++)
-      if (a.mode == 2 && r < a.nprev) {
+      if (a.mode == 2 && r < a.nprev && !(a.rprev && r == a.nprev - 1)) {
 #pragma unroll
         for (int k = 0; k < NF; k++) pd[r][k] = __builtin_nontemporal_load(dp[r] + (ec * NF + k) * NS + tc);
       }
   }
   if (tid < O::SIZE) s1[tid] = o1;
-  if (act) {
+  if (hasS) {
+    // Us = base - rkPrev * rprev * dt / detJ: the previous stage's update
+    // (the expression of step (4)), evaluated here instead of stored
+    const double dtJ = pdt / pdet;
 #pragma unroll
-    for (int j = 0; j < NU; j++)
-      if (t + j * TPE < NF * NS) {
-        const int f = t + j * TPE, k = f / NS;
-        sU[slot][k][padIdx<P>(f - k * NS)] = ul[j];
+    for (int k = 0; k < NF; k++) {
+      T v = ul[k];
+      if (a.rprev) {
+        double u = pb[k];
+        u = u - a.rkPrev * (double)ul[k] * dtJ;
+        v = (T)u;
       }
+      sU[slot][k][tP] = v;
+    }
+  }
+  if constexpr (FIN) {
+    if (a.rprev) {
+#pragma unroll
+      for (int r = 0; r < 3; r++)
+        if (r == a.nprev - 1) {
+#pragma unroll
+          for (int k = 0; k < NF; k++) pd[r][k] = ul[k];
+        }
+    }
   }
   __syncthreads();
 
@@ -647,6 +673,23 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
     elementSync<TPE>();
   }
 
+  // Final update (mode 2): the earlier stages' terms are subtracted from the
+  // base here, in stage order as at the update, so that their registers are
+  // free for the face phase
+  if constexpr (FIN) {
+    if (a.mode == 2) {
+      const double dtJ = pdt / pdet;
+#pragma unroll
+      for (int k = 0; k < NF; k++) {
+        double u = pb[k];
+        if (a.nprev > 0) u -= a.c0 * (double)pd[0][k] * dtJ;
+        if (a.nprev > 1) u -= a.c1 * (double)pd[1][k] * dtJ;
+        if (a.nprev > 2) u -= a.c2 * (double)pd[2][k] * dtJ;
+        pb[k] = u;
+      }
+    }
+  }
+
   // (3) common flux at every flux point (face::calcInviscidFlux,
   // face.cpp:132-175; intFace.cpp:85-120; boundFace.cpp:33-277)
 #pragma unroll
@@ -725,27 +768,23 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
       const double dtJ = pdt / pdet;
       double u = pb[k];
       if (a.mode == 2) {
-        // earlier residuals: prefetched (FIN) or read here (viscous variant)
-        T r0, r1, r2;
-        if constexpr (FIN) {
-          r0 = pd[0][k];
-          r1 = pd[1][k];
-          r2 = pd[2][k];
-        } else {
-          r0 = (a.nprev > 0) ? a.dprev0[gi] : (T)0;
-          r1 = (a.nprev > 1) ? a.dprev1[gi] : (T)0;
-          r2 = (a.nprev > 2) ? a.dprev2[gi] : (T)0;
+        // earlier residuals: already subtracted from pb (FIN, before (3)) or
+        // read here (viscous variant)
+        if constexpr (!FIN) {
+          const T r0 = (a.nprev > 0) ? a.dprev0[gi] : (T)0;
+          const T r1 = (a.nprev > 1) ? a.dprev1[gi] : (T)0;
+          const T r2 = (a.nprev > 2) ? a.dprev2[gi] : (T)0;
+          if (a.nprev > 0) u -= a.c0 * (double)r0 * dtJ;
+          if (a.nprev > 1) u -= a.c1 * (double)r1 * dtJ;
+          if (a.nprev > 2) u -= a.c2 * (double)r2 * dtJ;
         }
-        if (a.nprev > 0) u -= a.c0 * (double)r0 * dtJ;
-        if (a.nprev > 1) u -= a.c1 * (double)r1 * dtJ;
-        if (a.nprev > 2) u -= a.c2 * (double)r2 * dtJ;
         const double cl = (a.nprev == 0) ? a.c0 : (a.nprev == 1) ? a.c1 : (a.nprev == 2) ? a.c2 : a.c3;
         u -= cl * (double)dv * dtJ;
       } else {
         u = u - a.rk * (double)dv * dtJ;
       }
       if (a.mode == 1) {
-        __builtin_nontemporal_store((T)u, a.Unext + gi);
+        if (a.Unext) __builtin_nontemporal_store((T)u, a.Unext + gi);
       } else {
         __builtin_nontemporal_store(u, a.Uout + gi);
         if (a.Uout32) __builtin_nontemporal_store((float)u, a.Uout32 + gi);
@@ -780,8 +819,10 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
   }
 }
 
+// The inviscid final-stage variant asks for 3 waves per SIMD (at most 168
+// VGPRs; the FP64 hex P3 kernel needs 176 unbounded, which would allow 2).
 template <int D, int P, int NF, typename T, bool BND, bool VIS, bool FIN>
-__global__ __launch_bounds__((ElemGeo<D, P>::NTH)) void k_stage(StageArgs<T> a) {
+__global__ __launch_bounds__((ElemGeo<D, P>::NTH), (FIN && !VIS ? 3 : 1)) void k_stage(StageArgs<T> a) {
   __shared__ StageSmem<D, P, NF, T> sm;
   // Workgroups are dealt to the 8 XCDs in turn; remap so that each XCD walks a
   // contiguous range of elements and face neighbours share its L2.  The host
