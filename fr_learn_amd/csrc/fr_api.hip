@@ -434,7 +434,7 @@ int createImpl(const fr_config *cfg, const fr_mesh *m, int precision, fr_ctx *c)
   dispatch(D, c->P, c->NF, [&](auto Dc, auto Pc, auto) {
     c->epb = frdev::ElemGeo<decltype(Dc)::value, decltype(Pc)::value>::EPB;
   });
-  if (c->tri) c->epb = 8;  // frdev::TriGeo<P>::EPB
+  if (c->tri) c->epb = 256 / frdev::triTPE(c->P);  // frdev::TriGeo<P>::EPB
   {
     std::vector<double> cen((size_t)nE * 3, 0.0);
     for (int e = 0; e < nE; e++)

@@ -22,6 +22,14 @@ constexpr int nextPow2(int n) {
   return p;
 }
 
+// Threads per linear-triangle element of order P: one solution point
+// ((P+1)(P+2)/2) and one flux point (3(P+1)) per thread, rounded up to a power
+// of two (8, 16, 16, 16, 32 for P = 1..5); 256 / triTPE(P) elements per block.
+constexpr int triTPE(int P) {
+  const int ns = (P + 1) * (P + 2) / 2, nf = 3 * (P + 1);
+  return nextPow2(ns > nf ? ns : nf);
+}
+
 template <int D, int P>
 struct ElemGeo {
   static constexpr int P1 = P + 1;

@@ -1,9 +1,10 @@
 // fr_tri.hpp -- the stage kernel of the linear triangle (2D), dense operators.
 //
 // The triangle's FR operators (fr_tri_ops.cpp) are dense: (P+1)(P+2)/2 solution
-// points with no tensor lines.  A workgroup of 256 threads holds 8 elements,
-// 32 threads each (one solution point and one flux point per thread for
-// P <= 5: at most 21 and 18), two elements per wavefront.  The table
+// points with no tensor lines.  A workgroup of 256 threads holds 256 / TPE
+// elements of TPE = triTPE(P) threads each (one solution point and one flux
+// point per thread: 8 threads at P = 1, 16 at P = 2..4, 32 at P = 5), several
+// elements per wavefront.  The table
 // Dr | Ds | E | L (at most 1638 values) is staged in LDS once per workgroup.
 // The phases follow the quad kernel's (k_stage): physical and reference flux
 // at the solution points and the common flux at the flux points; the jump
@@ -20,7 +21,7 @@ namespace frdev {
 template <int P>
 struct TriGeo {
   static constexpr int P1 = P + 1, NS = P1 * (P + 2) / 2, NFPF = P1, NFACE = 3, NFP = 3 * P1;
-  static constexpr int TPE = 32, NTH = 256, EPB = NTH / TPE;
+  static constexpr int TPE = triTPE(P), NTH = 256, EPB = NTH / TPE;
   static constexpr int DR = 0, DS = NS * NS, E = 2 * NS * NS, L = 2 * NS * NS + NFP * NS;
   static constexpr int TAB = 2 * NS * NS + 2 * NFP * NS;
   static_assert(NS <= TPE && NFP <= TPE, "one point per thread");
