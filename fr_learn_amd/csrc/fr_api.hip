@@ -865,7 +865,6 @@ int fr_set_dt(fr_ctx *c, double dt) {
 int fr_calc_dt(fr_ctx *c, int dtType, double CFL, double *dtMin) {
   if (!c) return fail(FR_EINVAL, "null argument");
   if (dtType != 1 && dtType != 2) return fail(FR_EINVAL, "dtType must be 1 (global) or 2 (local)");
-  if (c->tri) return fail(FR_EINVAL, "the CFL time step is defined for quads and hexes");
   // getCFLLimit (global.cpp:45-68)
   static const double lim[6] = {1.393, 0.464, 0.235, 0.1453, 0.100, 0.0736};
   if (c->P < 0 || c->P > 5) return fail(FR_EINVAL, "CFL limit not available for this order!");
@@ -875,12 +874,21 @@ int fr_calc_dt(fr_ctx *c, int dtType, double CFL, double *dtMin) {
   if ((rc = dmalloc(&wsurf, c->ops.wtsFace.size()))) return rc;
   HIP_TRY(hipMemcpyAsync(wsurf, c->ops.wtsFace.data(), c->ops.wtsFace.size() * 8, hipMemcpyHostToDevice, c->stream));
   hipError_t err = hipSuccess;
-  dispatch(c->D, c->P, c->NF, [&](auto Dc, auto Pc, auto NFc) {
-    constexpr int D = decltype(Dc)::value, P = decltype(Pc)::value, NF = decltype(NFc)::value;
-    hipLaunchKernelGGL((frdev::k_dt<D, P, NF>), dim3(nblk(c->nE)), dim3(256), 0, c->stream, c->U, c->ownN, c->fgeo64,
-                       c->detJ, c->wts, wsurf, c->oned64, CFL * lim[c->P] * 2, c->ph, c->nE, c->dtE);
-    err = hipGetLastError();
-  });
+  if (c->tri) {
+    // triangles: the same formula with the element's dense traces; the limit
+    // table is the quads' (no reference value exists for triangles)
+    if ((rc = launchTriDt(c, wsurf, CFL * lim[c->P] * 2))) {
+      (void)hipFree(wsurf);
+      return rc;
+    }
+  } else {
+    dispatch(c->D, c->P, c->NF, [&](auto Dc, auto Pc, auto NFc) {
+      constexpr int D = decltype(Dc)::value, P = decltype(Pc)::value, NF = decltype(NFc)::value;
+      hipLaunchKernelGGL((frdev::k_dt<D, P, NF>), dim3(nblk(c->nE)), dim3(256), 0, c->stream, c->U, c->ownN,
+                         c->fgeo64, c->detJ, c->wts, wsurf, c->oned64, CFL * lim[c->P] * 2, c->ph, c->nE, c->dtE);
+      err = hipGetLastError();
+    });
+  }
   if (err != hipSuccess) {
     (void)hipFree(wsurf);
     return fail(FR_EHIP, std::string("k_dt launch: ") + hipGetErrorString(err));

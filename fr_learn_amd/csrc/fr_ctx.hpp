@@ -183,6 +183,8 @@ template <typename T>
 int launchTriStage(fr_ctx *c, const frdev::StageArgs<T> &a, int mode);
 template <typename T>
 int launchTriTrace(fr_ctx *c, const T *U, T *Uf, const T *tab);
+// CFL time steps of the triangles into c->dtE (k_tri_dt)
+int launchTriDt(fr_ctx *c, const double *wsurf, double factor);
 // Stage s of one batched step (fr_batch.hip); parity < 0 is the tuning launch,
 // Ponly != 0 restricts it to the meshes of that order.
 int batchLaunchStage(fr_batch *b, int parity, int s, int grid, hipStream_t st, int gOnly = 0);

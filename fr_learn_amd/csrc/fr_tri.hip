@@ -66,6 +66,19 @@ int launchTriTrace(fr_ctx *c, const T *U, T *Uf, const T *tab) {
   return 0;
 }
 
+int launchTriDt(fr_ctx *c, const double *wsurf, double factor) {
+  hipError_t err = hipSuccess;
+  bool ok = triDispatch(c->P, c->NF, [&](auto Pc, auto NFc) {
+    constexpr int P = decltype(Pc)::value, NF = decltype(NFc)::value;
+    hipLaunchKernelGGL((frdev::k_tri_dt<P, NF>), dim3(nblk(c->nE)), dim3(256), 0, c->stream, c->U, c->ownN,
+                       c->fgeo64, c->detJ, c->wts, wsurf, c->oned64, factor, c->ph, c->nE, c->dtE);
+    err = hipGetLastError();
+  });
+  if (!ok) return fail(FR_EINVAL, "no triangle kernel for this (order, nFields)");
+  if (err != hipSuccess) return fail(FR_EHIP, std::string("triangle dt launch: ") + hipGetErrorString(err));
+  return 0;
+}
+
 template int launchTriStage<double>(fr_ctx *, const frdev::StageArgs<double> &, int);
 template int launchTriStage<float>(fr_ctx *, const frdev::StageArgs<float> &, int);
 template int launchTriTrace<double>(fr_ctx *, const double *, double *, const double *);

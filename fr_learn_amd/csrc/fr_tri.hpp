@@ -233,6 +233,53 @@ __global__ __launch_bounds__(256) void k_batch_tri_stage(const StageArgs<T> *__r
   }
 }
 
+// CFL time step of each triangle, as k_dt for quads (ele::calcWaveSpFpts,
+// ele.cpp:1031-1084; ele::calcDt, ele.cpp:1086-1105): the traces E U at the
+// flux points, the wave speed |v.n| + c (Euler) or |a.n| (advection) with the
+// element's own normals, and dt = factor * vol / sum(waveSp * wSurf * dA * h),
+// where vol = sum(detJ * w) with the nodal basis' quadrature weights (the
+// element's area: detJ is constant on a linear triangle).  dA is relative to
+// the reference edge's unit normal; h (sqrt 2 on the hypotenuse v1v2, else 1)
+// makes sum(wSurf * dA * h) over an edge its physical length, as it is on
+// quads.  One thread per element.
+template <int P, int NF>
+__global__ __launch_bounds__(256) void k_tri_dt(const double *__restrict__ U, const double *__restrict__ ownN,
+                                                const double *__restrict__ fgeo, const double *__restrict__ detJ,
+                                                const double *__restrict__ wts, const double *__restrict__ wsurf,
+                                                const double *__restrict__ tab, double factor, Phys ph, int nEles,
+                                                double *__restrict__ dtE) {
+  using G = TriGeo<P>;
+  constexpr int NS = G::NS, NFP = G::NFP, NFPF = G::NFPF;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nEles) return;
+  double vol = 0;
+  for (int s = 0; s < NS; s++) vol += detJ[(size_t)e * NS + s] * wts[s];
+  const double *Ue = U + (size_t)e * NF * NS;
+  double intWave = 0;
+  for (int q = 0; q < NFP; q++) {
+    const double nx = ownN[((size_t)e * 2 + 0) * NFP + q], ny = ownN[((size_t)e * 2 + 1) * NFP + q];
+    double ws;
+    if constexpr (NF == 1) {
+      ws = fabs(ph.advect[0] * nx + ph.advect[1] * ny);
+    } else {
+      const double *Eq = tab + G::E + q * NS;
+      double u[NF];
+      for (int k = 0; k < NF; k++) {
+        double acc = 0;
+        for (int s = 0; s < NS; s++) acc += Eq[s] * Ue[k * NS + s];
+        u[k] = acc;
+      }
+      const double rho = u[0], vx = u[1] / rho, vy = u[2] / rho;
+      const double pr = (ph.gamma - 1) * (u[3] - 0.5 * rho * (vx * vx + vy * vy));
+      const double csq = fmax(ph.gamma * pr / rho, 0.0);
+      ws = fabs(vx * nx + vy * ny) + sqrt(csq);
+    }
+    const double h = (q / NFPF == 1) ? 1.41421356237309504880 : 1.0;
+    intWave += ws * wsurf[q % NFPF] * fgeo[((size_t)e * 3 + 2) * NFP + q] * h;
+  }
+  dtE[e] = factor * vol / intWave;
+}
+
 // Face traces U_fpts = E U_spts, one thread per (element, flux point).
 template <int P, int NF, typename T>
 __global__ void k_tri_trace(const T *__restrict__ U, T *__restrict__ Uf, const T *__restrict__ tab, int nEles) {

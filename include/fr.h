@@ -153,8 +153,10 @@ int fr_reduce(fr_ctx *ctx, int kind, double *out);
  * with its own dt (timeStepA/B with eles[e]->dt, solver.cpp:307-385) and
  * params->dt = the minimum.  The reference evaluates the wave speeds only
  * from the initial state (solver.cpp:294-297, 1497-1503), so one call after
- * fr_set_state reproduces its dt for the whole run.  fr_get_dt: the dt each
- * element advances with, in mesh element order. */
+ * fr_set_state reproduces its dt for the whole run.  Triangles (no reference
+ * counterpart) use the same formula with their dense traces, the element area
+ * as vol and the quads' CFL limit.  fr_get_dt: the dt each element advances
+ * with, in mesh element order. */
 int fr_set_dt(fr_ctx *ctx, double dt);
 int fr_calc_dt(fr_ctx *ctx, int dtType, double CFL, double *dtMin);
 int fr_get_dt(fr_ctx *ctx, double *dtEle /* nEles */);

@@ -147,11 +147,82 @@ def test_fp32_and_free_stream():
     assert np.abs(g64.residual(0)).max() < 1e-12
 
 
+CFL_LIMIT = [1.393, 0.464, 0.235, 0.1453, 0.100, 0.0736]
+
+
+def _edges(m):
+    """Per element and edge (v0v1, v1v2, v2v0): physical length and outward
+    unit normal of the counter-clockwise triangle, and the element's area."""
+    X = m.xv[m.c2v][:, :, :2]
+    a, b = X, np.roll(X, -1, axis=1)
+    d = b - a
+    L = np.linalg.norm(d, axis=-1)
+    n = np.stack([d[..., 1], -d[..., 0]], axis=-1) / L[..., None]
+    e1, e2 = X[:, 1] - X[:, 0], X[:, 2] - X[:, 0]
+    area = 0.5 * (e1[:, 0] * e2[:, 1] - e1[:, 1] * e2[:, 0])
+    return L, n, area
+
+
+@pytest.mark.parametrize("P", [1, 3, 5])
+@pytest.mark.parametrize("equation", [0, 1])
+def test_cfl_dt_uniform_state(P, equation):
+    """fr_calc_dt on triangles (dtType 2).  A uniform state has one wave speed
+    per straight edge, so dt_e = CFL lim(P) 2 area_e / sum_edges ws L has a
+    closed form in the vertices (ele::calcDt's formula with the physical edge
+    measure)."""
+    cfg = euler_cfg(2, P) if equation == 1 else advection_cfg(2, P)
+    m = _mesh(5, perturb=0.2, seed=4)
+    g = Solver(cfg, m)
+    L, n, area = _edges(m)
+    if equation == 1:
+        rho, u, v, p = 1.1, 0.3, -0.2, 1.7
+        gam = cfg["gamma"]
+        U = np.zeros(g.state_shape())
+        U[..., 0], U[..., 1], U[..., 2] = rho, rho * u, rho * v
+        U[..., 3] = p / (gam - 1) + 0.5 * rho * (u * u + v * v)
+        ws = np.abs(n[..., 0] * u + n[..., 1] * v) + np.sqrt(gam * p / rho)
+    else:
+        U = np.ones(g.state_shape())
+        ws = np.abs(n[..., 0] * cfg["advectVx"] + n[..., 1] * cfg["advectVy"])
+    g.set_state(U)
+    CFL = 0.7
+    dtmin = g.calc_dt(2, CFL)
+    want = CFL * CFL_LIMIT[P] * 2 * area / (ws * L).sum(axis=1)
+    got = g.dt()
+    assert np.abs(got - want).max() <= 1e-12 * want.max()
+    assert abs(dtmin - want.min()) <= 1e-12 * want.min()
+    assert abs(g.calc_dt(1, CFL) - want.min()) <= 1e-12 * want.min()
+    assert np.all(g.dt() == g.dt()[0])
+
+
+@pytest.mark.parametrize("P", [2, 4])
+def test_cfl_dt_varying_state(P):
+    """A varying state: the wave speeds at the flux points from the checker's
+    traces E U, own normals and dA, weighted by the edge Gauss weights and the
+    reference edge's half length (sqrt 2 on the hypotenuse)."""
+    cfg = euler_cfg(2, P)
+    m = _mesh(4, perturb=0.15, seed=6)
+    g = Solver(cfg, m)
+    U = _state(g, cfg, amp=5e-2)
+    ops, JG, detJ, nrm, dA, faces = _checker(g, cfg, m)
+    x1d, w1d = leggauss(P + 1)
+    Uf = np.einsum("qs,sek->qek", ops["E"], U)
+    gam = cfg["gamma"]
+    rho = Uf[..., 0]
+    vx, vy = Uf[..., 1] / rho, Uf[..., 2] / rho
+    pr = (gam - 1) * (Uf[..., 3] - 0.5 * rho * (vx * vx + vy * vy))
+    ws = np.abs(vx * nrm[..., 0] + vy * nrm[..., 1]) + np.sqrt(np.maximum(gam * pr / rho, 0))
+    q = np.arange(3 * (P + 1))
+    wq = w1d[q % (P + 1)] * np.where(q // (P + 1) == 1, np.sqrt(2.0), 1.0)
+    _, _, area = _edges(m)
+    want = 0.5 * CFL_LIMIT[P] * 2 * area / np.einsum("qe,q,qe->e", ws, wq, dA)
+    g.calc_dt(2, 0.5)
+    assert np.abs(g.dt() - want).max() <= 1e-12 * want.max()
+
+
 def test_unsupported_verbs_raise():
     cfg = euler_cfg(2, 2)
     g = Solver(cfg, _mesh(3))
     _state(g, cfg)
-    with pytest.raises(FrError):
-        g.calc_dt(1, 0.5)
     with pytest.raises(FrError):
         g.vjp_residual(np.zeros(g.state_shape()))
