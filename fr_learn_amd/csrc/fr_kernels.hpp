@@ -662,12 +662,17 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
       const int lb = padIdx<P>(base), ls = padStride<P>(stride);
       const T *l1 = s1 + (side ? O::LR : O::LL);
       const T sg = side ? (T)1 : (T)-1;
+      // the weights with the face sign and the negation of the stored value
+      // applied once, ahead of the fields (exact: sign changes only)
+      T lw[P1];
+#pragma unroll
+      for (int m = 0; m < P1; m++) lw[m] = -(l1[m] * sg);
 #pragma unroll
       for (int k = 0; k < NF; k++) {
         T acc = (T)0;
 #pragma unroll
-        for (int m = 0; m < P1; m++) acc += (l1[m] * sg) * sFd[slot][k][lb + m * ls];
-        sJ[slot][k][q] = -acc;
+        for (int m = 0; m < P1; m++) acc += lw[m] * sFd[slot][k][lb + m * ls];
+        sJ[slot][k][q] = acc;
       }
     }
     elementSync<TPE>();
@@ -754,12 +759,15 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
   if (hasS) {
     int cq[NFACE], cside[NFACE], cax[NFACE];
     corrEntries<D, P>(t, cq, cside, cax);
+    T cw[NFACE];  // correction weights, read once for all fields
+#pragma unroll
+    for (int f = 0; f < NFACE; f++) cw[f] = s1[(cside[f] ? O::GR : O::GL) + cax[f]];
     const size_t eo = (size_t)e * NF * NS;
 #pragma unroll
     for (int k = 0; k < NF; k++) {
       T corr = (T)0;
 #pragma unroll
-      for (int f = 0; f < NFACE; f++) corr += s1[(cside[f] ? O::GR : O::GL) + cax[f]] * sJ[slot][k][cq[f]];
+      for (int f = 0; f < NFACE; f++) corr += cw[f] * sJ[slot][k][cq[f]];
       const T dv = corr + div[k];
       const size_t gi = eo + (size_t)k * NS + t;
       if (a.divF) __builtin_nontemporal_store(dv, a.divF + gi);
