@@ -137,8 +137,15 @@ __global__ __launch_bounds__((ElemGeo<D, P>::NTH)) void k_vjp_state(
           un[k] = DN(Uf[((size_t)en * NF + k) * NFP + qn]);
           g[k] = lj[k] * (right ? -dA : dA) + A[((size_t)en * NF + k) * NFP + qn];
         }
-        if (right) commonFlux<D, NF, DN>(ph, false, un, uo, n, Fn);
-        else commonFlux<D, NF, DN>(ph, false, uo, un, n, Fn);
+        // the left element's state first, chosen per lane (one common flux
+        // per wavefront instead of one per side)
+        DN sL[NF], sR[NF];
+#pragma unroll
+        for (int k = 0; k < NF; k++) {
+          sL[k] = right ? un[k] : uo[k];
+          sR[k] = right ? uo[k] : un[k];
+        }
+        commonFlux<D, NF, DN>(ph, false, sL, sR, n, Fn);
       }
 #pragma unroll
       for (int j = 0; j < NF; j++) {

@@ -739,8 +739,16 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
 #pragma unroll
         for (int k = 0; k < NF; k++) un[k] = pu[r][k];
       }
-      if (right) commonFlux<D, NF, T>(a.ph, false, un, uo, n, Fn);
-      else commonFlux<D, NF, T>(a.ph, bnd, uo, un, n, Fn);
+      // the left element's state first, chosen per lane, so that a wavefront
+      // whose points lie on both sides of their faces runs the common flux
+      // once, not once per side (right implies !bnd)
+      T sL[NF], sR[NF];
+#pragma unroll
+      for (int k = 0; k < NF; k++) {
+        sL[k] = right ? un[k] : uo[k];
+        sR[k] = right ? uo[k] : un[k];
+      }
+      commonFlux<D, NF, T>(a.ph, bnd, sL, sR, n, Fn);
       if constexpr (VIS) {
         // the viscous common flux of this point (k_vflux, before this launch)
 #pragma unroll

@@ -120,8 +120,15 @@ __device__ __forceinline__ void triStageBody(const StageArgs<T> &a, const int b,
 #pragma unroll
       for (int k = 0; k < NF; k++) ub[k] = un[k];
     }
-    if (right) commonFlux<2, NF, T>(a.ph, false, ub, uo, n, Fc);
-    else commonFlux<2, NF, T>(a.ph, bnd, uo, ub, n, Fc);
+    // the left element's state first, chosen per lane (one common flux per
+    // wavefront instead of one per side; right implies !bnd)
+    T sL[NF], sR[NF];
+#pragma unroll
+    for (int k = 0; k < NF; k++) {
+      sL[k] = right ? ub[k] : uo[k];
+      sR[k] = right ? uo[k] : ub[k];
+    }
+    commonFlux<2, NF, T>(a.ph, bnd, sL, sR, n, Fc);
     const T scale = right ? -dA : dA;
 #pragma unroll
     for (int k = 0; k < NF; k++) Fn[k] = Fc[k] * scale;
