@@ -501,6 +501,20 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
   for (int k = 0; k < NF; k++)
     pb[k] = (FIN || (VIS && a.mode >= 2)) ? a.Ubase[(ec * NF + k) * NS + tc]
                                           : (double)a.UbaseT[(ec * NF + k) * NS + tc];
+  // earlier stages' residuals for the final update, issued with the state:
+  // they are folded into the base before the barrier, which frees their
+  // registers for the LDS phases (the latest one is already in ul when rprev
+  // is set)
+  T pd[FIN ? 3 : 1][NF];
+  if constexpr (FIN) {
+    const T *dp[3] = {a.dprev0, a.dprev1, a.dprev2};
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+      if (a.mode == 2 && r < a.nprev && !(a.rprev && r == a.nprev - 1)) {
+#pragma unroll
+        for (int k = 0; k < NF; k++) pd[r][k] = __builtin_nontemporal_load(dp[r] + (ec * NF + k) * NS + tc);
+      }
+  }
   T pJ[D * D];  // adjugate metrics of this solution point
 #pragma unroll
   for (int c = 0; c < D * D; c++) pJ[c] = __builtin_nontemporal_load(a.JG + ga.jg(ec, c));
@@ -521,18 +535,6 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
 #pragma unroll
     for (int k = 0; k < NF; k++) pu[r][k] = __builtin_nontemporal_load(Ur + k * st);
     if (bndFace) pnb[r] = -1;
-  }
-  // earlier stages' residuals for the final update (issued last: they are
-  // needed last); the latest one is already in ul when rprev is set
-  T pd[FIN ? 3 : 1][NF];
-  if constexpr (FIN) {
-    const T *dp[3] = {a.dprev0, a.dprev1, a.dprev2};
-#pragma unroll
-    for (int r = 0; r < 3; r++)
-      if (a.mode == 2 && r < a.nprev && !(a.rprev && r == a.nprev - 1)) {
-#pragma unroll
-        for (int k = 0; k < NF; k++) pd[r][k] = __builtin_nontemporal_load(dp[r] + (ec * NF + k) * NS + tc);
-      }
   }
   if (tid < O::SIZE) s1[tid] = o1;
   if (hasS) {
@@ -558,6 +560,20 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
 #pragma unroll
           for (int k = 0; k < NF; k++) pd[r][k] = ul[k];
         }
+    }
+    // Final update (mode 2): the earlier stages' terms are subtracted from
+    // the base now (after the stage state above used it), in stage order as
+    // at the update
+    if (a.mode == 2) {
+      const double dtJ = pdt / pdet;
+#pragma unroll
+      for (int k = 0; k < NF; k++) {
+        double u = pb[k];
+        if (a.nprev > 0) u -= a.c0 * (double)pd[0][k] * dtJ;
+        if (a.nprev > 1) u -= a.c1 * (double)pd[1][k] * dtJ;
+        if (a.nprev > 2) u -= a.c2 * (double)pd[2][k] * dtJ;
+        pb[k] = u;
+      }
     }
   }
   __syncthreads();
@@ -678,23 +694,6 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
     elementSync<TPE>();
   }
 
-  // Final update (mode 2): the earlier stages' terms are subtracted from the
-  // base here, in stage order as at the update, so that their registers are
-  // free for the face phase
-  if constexpr (FIN) {
-    if (a.mode == 2) {
-      const double dtJ = pdt / pdet;
-#pragma unroll
-      for (int k = 0; k < NF; k++) {
-        double u = pb[k];
-        if (a.nprev > 0) u -= a.c0 * (double)pd[0][k] * dtJ;
-        if (a.nprev > 1) u -= a.c1 * (double)pd[1][k] * dtJ;
-        if (a.nprev > 2) u -= a.c2 * (double)pd[2][k] * dtJ;
-        pb[k] = u;
-      }
-    }
-  }
-
   // (3) common flux at every flux point (face::calcInviscidFlux,
   // face.cpp:132-175; intFace.cpp:85-120; boundFace.cpp:33-277)
 #pragma unroll
@@ -784,7 +783,7 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
       const double dtJ = pdt / pdet;
       double u = pb[k];
       if (a.mode == 2) {
-        // earlier residuals: already subtracted from pb (FIN, before (3)) or
+        // earlier residuals: already subtracted from pb (FIN, before the barrier) or
         // read here (viscous variant)
         if constexpr (!FIN) {
           const T r0 = (a.nprev > 0) ? a.dprev0[gi] : (T)0;
@@ -836,7 +835,8 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
 }
 
 // The inviscid final-stage variant asks for 3 waves per SIMD (at most 168
-// VGPRs; the FP64 hex P3 kernel needs 176 unbounded, which would allow 2).
+// VGPRs): it holds the most prefetched data, and at 2 waves per SIMD the
+// final stage ran 12 % longer.
 template <int D, int P, int NF, typename T, bool BND, bool VIS, bool FIN>
 __global__ __launch_bounds__((ElemGeo<D, P>::NTH), (FIN && !VIS ? 3 : 1)) void k_stage(StageArgs<T> a) {
   __shared__ StageSmem<D, P, NF, T> sm;

@@ -64,25 +64,32 @@ __host__ __device__ __forceinline__ int padStride(int stride) {
 // with 1D weights l_R (side 1) or l_L (side 0); dn = reference normal axis.
 template <int D, int P>
 __host__ __device__ __forceinline__ void fptLine(int q, int &base, int &stride, int &side, int &dn) {
+  // Written with selects, not a switch over the face: the lanes of a
+  // wavefront cover several faces, and a switch would run every case.
+  //   hex   f: base                  stride  side dn
+  //         0: a + P1 b              P1^2    0    2
+  //         1: (P-a) + P1 b          P1^2    1    2
+  //         2: P1 (a + P1 b)         1       0    0
+  //         3: P1 ((P-a) + P1 b)     1       1    0
+  //         4: (P-a) + P1^2 b        P1      0    1
+  //         5: a + P1^2 b            P1      1    1
+  //   quad  f: 0: a, P1, 0, 1;  1: P1 a, 1, 1, 0;  2: P-a, P1, 1, 1;  3: P1 (P-a), 1, 0, 0
   constexpr int P1 = P + 1;
   if (D == 3) {
-    const int f = q / (P1 * P1), r = q - f * P1 * P1, a = r % P1, b = r / P1;
-    switch (f) {
-      case 0: base = a + P1 * b; stride = P1 * P1; side = 0; dn = 2; break;
-      case 1: base = (P - a) + P1 * b; stride = P1 * P1; side = 1; dn = 2; break;
-      case 2: base = P1 * (a + P1 * b); stride = 1; side = 0; dn = 0; break;
-      case 3: base = P1 * ((P - a) + P1 * b); stride = 1; side = 1; dn = 0; break;
-      case 4: base = (P - a) + P1 * P1 * b; stride = P1; side = 0; dn = 1; break;
-      default: base = a + P1 * P1 * b; stride = P1; side = 1; dn = 1; break;
-    }
+    constexpr int F2 = P1 * P1;
+    const int f = q / F2, r = q - f * F2, a = r % P1, b = r / P1;
+    const int aa = (f == 1 || f == 3 || f == 4) ? P - a : a;
+    base = ((f == 2 || f == 3) ? P1 : 1) * aa + ((f < 2) ? P1 : F2) * b;
+    stride = (f < 2) ? F2 : (f < 4) ? 1 : P1;
+    side = f & 1;
+    dn = (f < 2) ? 2 : (f < 4) ? 0 : 1;
   } else {
     const int f = q / P1, a = q - f * P1;
-    switch (f) {
-      case 0: base = a; stride = P1; side = 0; dn = 1; break;
-      case 1: base = P1 * a; stride = 1; side = 1; dn = 0; break;
-      case 2: base = P - a; stride = P1; side = 1; dn = 1; break;
-      default: base = P1 * (P - a); stride = 1; side = 0; dn = 0; break;
-    }
+    const int aa = (f >= 2) ? P - a : a;
+    base = (f & 1) ? P1 * aa : aa;
+    stride = (f & 1) ? 1 : P1;
+    side = (f == 1 || f == 2) ? 1 : 0;
+    dn = (f & 1) ? 0 : 1;
   }
 }
 
@@ -132,13 +139,9 @@ __host__ __device__ __forceinline__ int neighbourFpt(int q, int finfo) {
   const int nf = finfo & 7, rot = (finfo >> 3) & 3;
   int j;
   if (D == 3) {
-    const int a = i % P1, b = i / P1;
-    switch (rot) {
-      case 0: j = a * P1 + b; break;
-      case 1: j = (P - a) + b * P1; break;
-      case 2: j = NFPF - 1 - (a * P1 + b); break;
-      default: j = a + P1 * (P - b); break;
-    }
+    // rot 0: (b, a); 1: (P-a, b); 2: (P-b, P-a); 3: (a, P-b), as selects (see fptLine)
+    const int a = i % P1, b = i / P1, t = a * P1 + b;
+    j = (rot == 0) ? t : (rot == 1) ? (P - a) + b * P1 : (rot == 2) ? NFPF - 1 - t : a + P1 * (P - b);
   } else {
     j = P - i;
   }
