@@ -516,8 +516,13 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
       }
   }
   T pJ[D * D];  // adjugate metrics of this solution point
+  {
+    // components c at a uniform stride from c = 0 (one 64-bit add per load)
+    const T *jg0 = a.JG + ga.jg(ec, 0);
+    const size_t js = (size_t)ga.gS;
 #pragma unroll
-  for (int c = 0; c < D * D; c++) pJ[c] = __builtin_nontemporal_load(a.JG + ga.jg(ec, c));
+    for (int c = 0; c < D * D; c++) pJ[c] = __builtin_nontemporal_load(jg0 + c * js);
+  }
   T pn[NQ][D + 1], pu[NQ][NF];  // face normal + dA, neighbour trace
 #pragma unroll
   for (int r = 0; r < NQ; r++) {
