@@ -621,9 +621,11 @@ std::vector<int> blockOrder(const Connectivity &c, const std::vector<double> &ce
     members.assign(1, seed);
     cluster[seed] = id;
     // grow by the unassigned face neighbour with the most faces into the
-    // cluster (ties: lowest Z-order rank)
+    // cluster; ties: the one that leaves more unassigned elements touching the
+    // cluster twice (on a grid: turn a corner, so that the fourth element
+    // closes a 2x2 square instead of extending a line), then lowest Z-order rank
     while ((int)members.size() < epb) {
-      int best = -1, bestCnt = 0;
+      int best = -1, bestCnt = 0, bestLook = -1;
       for (int m : members)
         for (int f = 0; f < nFaceEle; f++) {
           const int n = c.fnbr[(size_t)m * nFaceEle + f];
@@ -633,9 +635,24 @@ std::vector<int> blockOrder(const Connectivity &c, const std::vector<double> &ce
             const int o = c.fnbr[(size_t)n * nFaceEle + g];
             if (o >= 0 && cluster[o] == id) cnt++;
           }
-          if (cnt > bestCnt || (cnt == bestCnt && best >= 0 && rank[n] < rank[best])) {
+          int look = 0;
+          if ((int)members.size() + 1 < epb)
+            for (int g = 0; g < nFaceEle; g++) {
+              const int o = c.fnbr[(size_t)n * nFaceEle + g];
+              if (o < 0 || o == n || cluster[o] >= 0) continue;
+              for (int h = 0; h < nFaceEle; h++) {
+                const int w = c.fnbr[(size_t)o * nFaceEle + h];
+                if (w >= 0 && cluster[w] == id) {
+                  look++;
+                  break;
+                }
+              }
+            }
+          if (cnt > bestCnt ||
+              (cnt == bestCnt && best >= 0 && (look > bestLook || (look == bestLook && rank[n] < rank[best])))) {
             best = n;
             bestCnt = cnt;
+            bestLook = look;
           }
         }
       if (best < 0) break;
