@@ -451,6 +451,9 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
   auto &sFd = sm.sFd;
   auto &sJ = sm.sJ;
   auto &s1 = sm.s1;
+  // the modes a variant serves: the inviscid FIN variants 2 and 3, the other
+  // inviscid variants 0 and 1 (the launches choose so), the viscous one all
+  constexpr bool M01 = VIS || !FIN, M23 = VIS || FIN;
   const int slot = tid / TPE, t = tid - slot * TPE;
   const int tP = padIdx<P>(t);  // padded LDS index of this thread's solution point
   const int e = b * EPB + slot;
@@ -783,11 +786,11 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
       const T dv = corr + div[k];
       const size_t gi = eo + (size_t)k * NS + t;
       if (a.divF) __builtin_nontemporal_store(dv, a.divF + gi);
-      if (a.mode == 0) continue;
+      if (M01 && a.mode == 0) continue;
       // stage terms RK * divF / detJ * dt, with dt / detJ formed once per point
       const double dtJ = pdt / pdet;
       double u = pb[k];
-      if (a.mode == 2) {
+      if (M23 && a.mode == 2) {
         // earlier residuals: already subtracted from pb (FIN, before the barrier) or
         // read here (viscous variant)
         if constexpr (!FIN) {
@@ -803,7 +806,7 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
       } else {
         u = u - a.rk * (double)dv * dtJ;
       }
-      if (a.mode == 1) {
+      if (!M23 || (M01 && a.mode == 1)) {
         if (a.Unext) __builtin_nontemporal_store((T)u, a.Unext + gi);
       } else {
         __builtin_nontemporal_store(u, a.Uout + gi);
@@ -812,7 +815,7 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
       sFd[slot][k][tP] = (T)u;
     }
   }
-  if (a.mode == 0) return;
+  if (M01 && a.mode == 0) return;
   elementSync<TPE>();
 
   // (5) face traces of the updated state for the next stage, for the faces
