@@ -3,8 +3,8 @@ import numpy as np
 
 from fr_learn_amd import cases, mesh
 
-# FP64 parity bar (north-star: GPU FP64 residual within 1e-12 relative of the
-# reference-algorithm oracle).  Relative error is measured per field as
+# FP64 parity bar: the GPU FP64 residual within 1e-12 relative of the
+# reference-algorithm oracle.  Relative error is measured per field as
 # max|a - b| / max|b| over the whole residual array.
 TOL_FP64 = 1e-12
 
