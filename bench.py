@@ -294,8 +294,8 @@ def main():
                     help="only the Navier-Stokes TGV adjoint workload (24^3 P3, 200 steps), its own JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU path is "
-              "not part of this build", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the library has no "
+              "multi-GPU path", file=sys.stderr)
         sys.exit(2)
 
     if args.copy_calibration:
