@@ -110,20 +110,23 @@ __device__ __forceinline__ void rusanovN(const Phys &ph, const T *UL, const T *U
 }
 
 // Roe flux with Harten-type entropy fix, 2D (face::roeFlux,
-// src/face.cpp:358-484).
+// src/face.cpp:358-484).  The reference's divisions by the densities, by eps
+// and by the Roe sound speed are multiplications by reciprocals (recip), an
+// ulp-level difference.
 template <typename T>
 __device__ __forceinline__ void roe2d(const Phys &ph, const T *UL, const T *UR, const T *n, T *Fn) {
   const T g = (T)ph.gamma;
+  const T riL = recip(UL[0]), riR = recip(UR[0]);
   T vL[2], vR[2], um[2], du[4];
   for (int i = 0; i < 2; i++) {
-    vL[i] = UL[i + 1] / UL[0];
-    vR[i] = UR[i + 1] / UR[0];
+    vL[i] = UL[i + 1] * riL;
+    vR[i] = UR[i + 1] * riR;
   }
   T pL = (g - (T)1.0) * (UL[3] - ((T)0.5 * UL[0] * (vL[0] * vL[0] + vL[1] * vL[1])));
   T pR = (g - (T)1.0) * (UR[3] - ((T)0.5 * UR[0] * (vR[0] * vR[0] + vR[1] * vR[1])));
-  T hL = (UL[3] + pL) / UL[0], hR = (UR[3] + pR) / UR[0];
-  T sq_rho = sqrt(UR[0] / UL[0]);
-  T rrho = (T)1. / (sq_rho + (T)1.);
+  T hL = (UL[3] + pL) * riL, hR = (UR[3] + pR) * riR;
+  T sq_rho = sqrt(UR[0] * riL);
+  T rrho = recip(sq_rho + (T)1.);
   T usq = 0, unm = 0;
   for (int i = 0; i < 2; i++) {
     um[i] = rrho * (vL[i] + sq_rho * vR[i]);
@@ -144,13 +147,14 @@ __device__ __forceinline__ void roe2d(const Phys &ph, const T *UL, const T *UR, 
   Fn[3] = rhoUnL * hL + rhoUnR * hR;
   for (int i = 0; i < 4; i++) du[i] = UR[i] - UL[i];
   T lambda0 = fabs(unm), lambdaP = fabs(unm + am), lambdaM = fabs(unm - am);
-  T eps = (T)0.5 * (fabs(rhoUnL / UL[0] - rhoUnR / UR[0]) + fabs(sqrt(g * pL / UL[0]) - sqrt(g * pR / UR[0])));
-  if (lambda0 < (T)2. * eps) lambda0 = (T)0.25 * lambda0 * lambda0 / eps + eps;
-  if (lambdaP < (T)2. * eps) lambdaP = (T)0.25 * lambdaP * lambdaP / eps + eps;
-  if (lambdaM < (T)2. * eps) lambdaM = (T)0.25 * lambdaM * lambdaM / eps + eps;
+  T eps = (T)0.5 * (fabs(rhoUnL * riL - rhoUnR * riR) + fabs(sqrt(g * pL * riL) - sqrt(g * pR * riR)));
+  const T reps = recip(eps);
+  if (lambda0 < (T)2. * eps) lambda0 = (T)0.25 * lambda0 * lambda0 * reps + eps;
+  if (lambdaP < (T)2. * eps) lambdaP = (T)0.25 * lambdaP * lambdaP * reps + eps;
+  if (lambdaM < (T)2. * eps) lambdaM = (T)0.25 * lambdaM * lambdaM * reps + eps;
   T a2 = (T)0.5 * (lambdaP + lambdaM) - lambda0;
-  T a3 = (T)0.5 * (lambdaP - lambdaM) / am;
-  T a1 = a2 * (g - (T)1.) / am_sq;
+  T a3 = (T)0.5 * (lambdaP - lambdaM) * recip(am);
+  T a1 = a2 * (g - (T)1.) * recip(am_sq);
   T a4 = a3 * (g - (T)1.);
   T a5 = usq * du[0] - um[0] * du[1] - um[1] * du[2] + du[3];
   T a6 = unm * du[0] - n[0] * du[1] - n[1] * du[2];
