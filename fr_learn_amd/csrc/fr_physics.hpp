@@ -294,11 +294,20 @@ __device__ __forceinline__ void viscousFlux(const Phys &ph, const T *U, const T 
 #pragma unroll
     for (int d = 0; d < D; d++) Fv[d][0] = -(T)ph.diffD * g[d][0];
   } else {
-    const T rho = U[0], u = U[1] / rho, v = U[2] / rho;
-    T e = U[D + 1] / rho - (T)0.5 * (u * u + v * v);
+    // FP64: one reciprocal of rho (recip) for the reference's divisions by
+    // rho; other types (FP32, the adjoint's dual numbers) divide, which is
+    // cheaper for them
+    const T rho = U[0];
+    const T ri = (sizeof(T) == 8) ? recip(rho) : (T)0;
+    auto over = [&](const T &x) -> T {
+      if constexpr (sizeof(T) == 8) return x * ri;
+      else return x / rho;
+    };
+    const T u = over(U[1]), v = over(U[2]);
+    T e = over(U[D + 1]) - (T)0.5 * (u * u + v * v);
     T w = (T)0;
     if constexpr (D == 3) {
-      w = U[3] / rho;
+      w = over(U[3]);
       e -= (T)0.5 * (w * w);
     }
     const T dRho_dx = g[0][0], dRhoU_dx = g[0][1], dRhoV_dx = g[0][2], dE_dx = g[0][D + 1];
@@ -318,15 +327,15 @@ __device__ __forceinline__ void viscousFlux(const Phys &ph, const T *U, const T 
       const T rt = ((T)ph.gamma - (T)1.0) * e / (T)ph.rt_inf;
       mu *= pow(rt, (T)1.5) * ((T)1. + (T)ph.c_sth) / (rt + (T)ph.c_sth);
     }
-    const T du_dx = (dRhoU_dx - dRho_dx * u) / rho, du_dy = (dRhoU_dy - dRho_dy * u) / rho;
-    const T dv_dx = (dRhoV_dx - dRho_dx * v) / rho, dv_dy = (dRhoV_dy - dRho_dy * v) / rho;
+    const T du_dx = over(dRhoU_dx - dRho_dx * u), du_dy = over(dRhoU_dy - dRho_dy * u);
+    const T dv_dx = over(dRhoV_dx - dRho_dx * v), dv_dy = over(dRhoV_dy - dRho_dy * v);
     T du_dz = 0, dv_dz = 0, dw_dx = 0, dw_dy = 0, dw_dz = 0;
     if constexpr (D == 3) {
-      du_dz = (dRhoU_dz - dRho_dz * u) / rho;
-      dv_dz = (dRhoV_dz - dRho_dz * v) / rho;
-      dw_dx = (dRhoW_dx - dRho_dx * w) / rho;
-      dw_dy = (dRhoW_dy - dRho_dy * w) / rho;
-      dw_dz = (dRhoW_dz - dRho_dz * w) / rho;
+      du_dz = over(dRhoU_dz - dRho_dz * u);
+      dv_dz = over(dRhoV_dz - dRho_dz * v);
+      dw_dx = over(dRhoW_dx - dRho_dx * w);
+      dw_dy = over(dRhoW_dy - dRho_dy * w);
+      dw_dz = over(dRhoW_dz - dRho_dz * w);
     }
     T dK_dx, dK_dy, dK_dz = 0;
     if constexpr (D == 2) {
@@ -337,10 +346,10 @@ __device__ __forceinline__ void viscousFlux(const Phys &ph, const T *U, const T 
       dK_dy = (T)0.5 * (u * u + v * v + w * w) * dRho_dy + rho * (u * du_dy + v * dv_dy + w * dw_dy);
       dK_dz = (T)0.5 * (u * u + v * v + w * w) * dRho_dz + rho * (u * du_dz + v * dv_dz + w * dw_dz);
     }
-    const T de_dx = (dE_dx - dK_dx - dRho_dx * e) / rho;
-    const T de_dy = (dE_dy - dK_dy - dRho_dy * e) / rho;
+    const T de_dx = over(dE_dx - dK_dx - dRho_dx * e);
+    const T de_dy = over(dE_dy - dK_dy - dRho_dy * e);
     T de_dz = 0;
-    if constexpr (D == 3) de_dz = (dE_dz - dK_dz - dRho_dz * e) / rho;
+    if constexpr (D == 3) de_dz = over(dE_dz - dK_dz - dRho_dz * e);
     const T diag = (du_dx + dv_dy + dw_dz) / (T)3.0;
     const T tauxx = (T)2.0 * mu * (du_dx - diag), tauyy = (T)2.0 * mu * (dv_dy - diag);
     const T tauxy = mu * (du_dy + dv_dx);
