@@ -164,17 +164,15 @@ def copy_calibration():
     print(json.dumps({"copy_calibration_bytes": n * 8, "copies": 3}), flush=True)
 
 
-def run_batch(n_inst=256, steps=20, warmup=3, seed=0):
-    """The batched 2D Euler workload: n_inst seeded periodic meshes of 8^2 to
-    64^2 cells, quadrilaterals or triangles (two per cell) with probability
-    1/2, P uniform in 1..5, isentropic vortices of random centre, strength
-    1..5 and Mach 0.2..0.6, Roe, RK4.  One persistent launch per stage,
-    element type and order for all meshes (replayed from a hipGraph) against
-    one update call per mesh."""
-    from fr_learn_amd import Batch, Solver, cases, mesh
+def batch_members(n_inst=256, seed=0):
+    """The instances of the batched 2D Euler workload: n_inst seeded periodic
+    meshes of 8^2 to 64^2 cells, quadrilaterals or triangles (two per cell)
+    with probability 1/2, P uniform in 1..5, isentropic vortices of random
+    centre, strength 1..5 and Mach 0.2..0.6, Roe, RK4 (dt 1e-3).  A list of
+    (config, mesh, initial-condition function, element type "quad" | "tri")."""
+    from fr_learn_amd import cases, mesh
     rng = np.random.default_rng(seed)
-    solvers = []
-    dof = 0
+    out = []
     for _ in range(n_inst):
         cells = int(rng.integers(8, 65))
         P = int(rng.integers(1, 6))
@@ -182,12 +180,26 @@ def run_batch(n_inst=256, steps=20, warmup=3, seed=0):
         beta, mach = rng.uniform(1.0, 5.0), rng.uniform(0.2, 0.6)
         cfg = cases.config(nDims=2, order=P, equation=1, riemannType=1, timeType=4, dt=1e-3, gamma=1.4)
         if rng.uniform() < 0.5:
-            m = mesh.tri_box((cells, cells), (-5.0, -5.0), (5.0, 5.0))
+            m, kind = mesh.tri_box((cells, cells), (-5.0, -5.0), (5.0, 5.0)), "tri"
         else:
-            m = mesh.box(2, (cells, cells), (-5.0, -5.0), (5.0, 5.0))
+            m, kind = mesh.box(2, (cells, cells), (-5.0, -5.0), (5.0, 5.0)), "quad"
+        ic = (lambda b, mc, x0, y0: lambda x, c: cases.ic_vortex(x, c, beta=b, mach=mc, center=(x0, y0)))(
+            beta, mach, cx, cy)
+        out.append((cfg, m, ic, kind))
+    return out
+
+
+def run_batch(n_inst=256, steps=20, warmup=3, seed=0):
+    """The batched 2D Euler workload (batch_members): one persistent launch per
+    stage, element type and order for all meshes (replayed from a hipGraph)
+    against one update call per mesh."""
+    from fr_learn_amd import Batch, Solver
+    solvers = []
+    dof = 0
+    for cfg, m, ic, _ in batch_members(n_inst, seed):
         s = Solver(cfg, m)
         s.geometry_mode(0)  # per-point metrics, as on the main workload
-        s.initialize(lambda x, c: cases.ic_vortex(x, c, beta=beta, mach=mach, center=(cx, cy)))
+        s.initialize(ic)
         solvers.append(s)
         dof += s.nEles * s.nSpts
     b = Batch(solvers)

@@ -24,7 +24,8 @@ void freeAll(fr_ctx *c) {
                   c->finfo,   c->dUf,     c->dtG,     c->dtE,     c->adjA,   c->adjTr, c->adjTr2,
                   c->adjLam,  c->adjOut,  c->adjNu,   c->adjW[0], c->adjW[1], c->adjS[0], c->adjS[1],
                   c->adjS[2], c->adjS[3], c->adjQf,   c->adjB,    c->JGc64,  c->detJc, c->fgeoc64,
-                  c->JGc32,   c->fgeoc32, c->adjUo,   c->adjQo,   c->adjCj,   c->Fv};
+                  c->JGc32,   c->fgeoc32, c->adjUo,   c->adjQo,   c->adjCj,   c->Fv,   c->waveSp,
+                  c->dtNum,   c->wsurf,   c->dtPart};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   for (auto &v : c->ev)
@@ -106,6 +107,9 @@ frdev::StageArgs<T> makeStageArgs(fr_ctx *c, int in, int cur, int mode, int step
   a.rk = rk;
   a.dtE = dtSource(c).first;
   a.dtStride = dtSource(c).second;
+  // CFL time stepping: the face fluxes store their wave speeds for the next
+  // update's time step (not while the adjoint replays given time steps)
+  a.waveSp = (c->dtType != 0 && !c->dtFrozen && !c->tri) ? c->waveSp : nullptr;
   a.dUf = (const T *)c->dUf;
   a.Fv = (const T *)c->Fv;
   // Inviscid quads and hexes store no intermediate stage state: a stage that
@@ -142,6 +146,9 @@ int launchStage(fr_ctx *c, int in, int mode, int step, double rk) {
 
 int residual(fr_ctx *c, int step) {
   if (step < 0 || step >= c->nRK) return fail(FR_EINVAL, "step out of range");
+  // the residual's face fluxes store new wave speeds (face.cpp:354), which
+  // the next update's time step then uses
+  if (c->dtType != 0 && !c->dtFirst && !c->dtFrozen && !c->tri) c->dtReady = false;
   return c->prec == FR_PREC_FP32 ? launchStage<float>(c, 0, 0, step, 0.0) : launchStage<double>(c, 0, 0, step, 0.0);
 }
 
@@ -167,13 +174,49 @@ int rkA(fr_ctx *c, int step, double rk) {
   TimeScope ts(c, 2);
   hipLaunchKernelGGL((frdev::k_rk_a<TD>), dim3(nblk(n)), dim3(256), 0, c->stream, c->U, c->U0,
                      (const TD *)c->divF[step], c->detJ, c->U32, rk, dtSource(c).first, dtSource(c).second, 0,
-                     c->NF, c->NS, n);
+                     c->NF, c->NS, n, c->ph.eOff);
   HIP_TRY(hipGetLastError());
   c->traceOf = -1;
   return 0;
 }
 
-int updateOnce(fr_ctx *c) { return c->prec == FR_PREC_FP32 ? updateT<float>(c) : updateT<double>(c); }
+// Forward Euler with a CFL time step after the first update: the reference
+// computes dt after the step's residual, from the wave speeds that residual
+// stored (solver.cpp:236-240), so the residual, the time step and the update
+// (timeStepB) are three launches here instead of one fused stage.
+int updateEulerCfl(fr_ctx *c) {
+  int rc = residual(c, 0);
+  if (rc) return rc;
+  if ((rc = dtPrepare(c, c->stream))) return rc;
+  long n = (long)c->nState();
+  if (c->prec == FR_PREC_FP32)
+    hipLaunchKernelGGL(frdev::k_rk_b1<float>, dim3(nblk(n)), dim3(256), 0, c->stream, c->U,
+                       (const float *)c->divF[0], c->detJ, c->U32, c->RKb[0], dtSource(c).first,
+                       dtSource(c).second, c->NF, c->NS, n, c->ph.eOff);
+  else
+    hipLaunchKernelGGL(frdev::k_rk_b1<double>, dim3(nblk(n)), dim3(256), 0, c->stream, c->U,
+                       (const double *)c->divF[0], c->detJ, c->U32, c->RKb[0], dtSource(c).first,
+                       dtSource(c).second, c->NF, c->NS, n, c->ph.eOff);
+  HIP_TRY(hipGetLastError());
+  c->traceOf = -1;
+  c->dtReady = false;
+  return 0;
+}
+
+// solver::update: the time step first (solver.cpp:218-219), then the stages.
+int updateOnce(fr_ctx *c) {
+  int rc;
+  if (c->dtType != 0 && !c->dtFrozen) {
+    if (c->nRK == 1 && !c->dtFirst && !c->tri) return updateEulerCfl(c);
+    if ((rc = dtPrepare(c, c->stream))) return rc;
+  }
+  rc = c->prec == FR_PREC_FP32 ? updateT<float>(c) : updateT<double>(c);
+  if (c->dtType != 0 && !c->dtFrozen) {
+    c->dtFirst = false;
+    c->dtReady = false;
+  }
+  return rc;
+}
 
 // reference layout [pt][ele][field] <-> device [device ele][field][pt]
 void refToDev(const fr_ctx *c, const double *src, double *dst, int NF, int NP) {
@@ -196,14 +239,57 @@ void devToRef(const fr_ctx *c, const double *src, double *dst, int NF, int NP) {
 int syncU32(fr_ctx *c) {
   if (c->prec != FR_PREC_FP32) return 0;
   long n = (long)c->nState();
-  hipLaunchKernelGGL(frdev::k_to_float, dim3(nblk(n)), dim3(256), 0, c->stream, c->U, c->U32, n);
+  hipLaunchKernelGGL(frdev::k_state_to_float, dim3(nblk(n)), dim3(256), 0, c->stream, c->U, c->U32, c->NF,
+                     c->NS, c->ph.eOff, n);
   HIP_TRY(hipGetLastError());
   return 0;
+}
+
+// The energy offset of the FP32 states (Phys::eOff) for state U (device
+// layout [ele][field][spt] or the reference layout): the mean pressure over the
+// solution points divided by gamma - 1.  Inviscid Euler quads and hexes in
+// FP32 only; zero otherwise.  A change bumps the settings generation.
+void setEnergyOffset(fr_ctx *c, const double *U, int layout) {
+  double eOff = 0.0;
+  if (c->prec == FR_PREC_FP32 && c->cfg.equation == 1 && !c->cfg.viscous && !c->tri) {
+    const int NF = c->NF, NS = c->NS, D = c->D;
+    const size_t nPts = (size_t)c->nE * NS;
+    double sum = 0.0;
+    for (size_t pt = 0; pt < nPts; pt++) {
+      double u[5];
+      if (layout == FR_LAYOUT_REF) {
+        for (int k = 0; k < NF; k++) u[k] = U[pt * NF + k];
+      } else {
+        const size_t e = pt / NS, s = pt % NS;
+        for (int k = 0; k < NF; k++) u[k] = U[(e * NF + k) * NS + s];
+      }
+      double m2 = 0.0;
+      for (int d = 0; d < D; d++) m2 += u[d + 1] * u[d + 1];
+      sum += (c->cfg.gamma - 1.0) * (u[NF - 1] - 0.5 * m2 / u[0]);
+    }
+    const double pMean = sum / (double)nPts;
+    if (std::isfinite(pMean)) eOff = pMean / (c->cfg.gamma - 1.0);
+  }
+  if (eOff != c->ph.eOff) {
+    c->ph.eOff = eOff;
+    c->ph.pOff = eOff * (c->cfg.gamma - 1.0);
+    c->ph.hOff = eOff * c->cfg.gamma;
+    c->gen++;
+  }
+}
+
+// A verb changed the state: a first update after fr_calc_dt takes its wave
+// speeds from the state, so its time step must be formed again.
+void stateChanged(fr_ctx *c) {
+  if (c->dtFirst) c->dtReady = false;
 }
 
 // ---------------------------------------------------------------------------
 // batches
 // ---------------------------------------------------------------------------
+// The members' stage arguments in device memory, which the captured graphs
+// read: built at fr_batch_create, and rebuilt in place (same buffers, so the
+// graphs stay valid) when a member's settings changed since.
 template <typename T>
 int batchArgs(fr_batch *b) {
   const size_t n = b->ctx.size();
@@ -217,14 +303,28 @@ int batchArgs(fr_batch *b) {
         h[(size_t)s * n + i] = makeStageArgs<T>(b->ctx[i], s == 0 ? 0 : 1, p ^ (s & 1), mode, s, rk);
       }
     int rc;
-    if ((rc = dmalloc((frdev::StageArgs<T> **)&b->args[p], h.size()))) return rc;
+    if (!b->args[p] && (rc = dmalloc((frdev::StageArgs<T> **)&b->args[p], h.size()))) return rc;
     HIP_TRY(hipMemcpy(b->args[p], h.data(), h.size() * sizeof(frdev::StageArgs<T>), hipMemcpyHostToDevice));
   }
   std::vector<frdev::StageArgs<T>> h(n);
-  for (size_t i = 0; i < n; i++) h[i] = makeStageArgs<T>(b->ctx[i], 0, 0, 0, 0, 0.0);
+  for (size_t i = 0; i < n; i++) {
+    h[i] = makeStageArgs<T>(b->ctx[i], 0, 0, 0, 0, 0.0);
+    h[i].waveSp = nullptr;  // the tuning launches leave the CFL wave speeds alone
+  }
   int rc;
-  if ((rc = dmalloc((frdev::StageArgs<T> **)&b->argsTune, n))) return rc;
+  if (!b->argsTune && (rc = dmalloc((frdev::StageArgs<T> **)&b->argsTune, n))) return rc;
   HIP_TRY(hipMemcpy(b->argsTune, h.data(), n * sizeof(frdev::StageArgs<T>), hipMemcpyHostToDevice));
+  b->gen.resize(n);
+  for (size_t i = 0; i < n; i++) b->gen[i] = b->ctx[i]->gen;
+  return 0;
+}
+
+// A member's settings must suit the batched step: a fixed or CFL time step
+// (forward Euler with a CFL time step takes its dt after the step's residual,
+// which the fused batched stage cannot).
+int batchMemberCheck(const fr_ctx *c) {
+  if (c->dtType != 0 && c->nRK == 1)
+    return fail(FR_EINVAL, "batched forward Euler needs a fixed time step (fr_set_dt)");
   return 0;
 }
 
@@ -246,6 +346,16 @@ int batchPrepare(fr_batch *b, int parity) {
 }
 
 void batchFree(fr_batch *b) {
+  // the live members forget this batch
+  for (size_t k = 0; k < b->ctx.size(); k++)
+    if (k < b->alive.size() && b->alive[k]) {
+      auto &v = b->ctx[k]->batches;
+      for (size_t i = 0; i < v.size(); i++)
+        if (v[i] == b) {
+          v.erase(v.begin() + i);
+          break;
+        }
+    }
   for (int p = 0; p < 2; p++) {
     if (b->exec[p]) (void)hipGraphExecDestroy(b->exec[p]);
     if (b->graph[p]) (void)hipGraphDestroy(b->graph[p]);
@@ -305,6 +415,7 @@ int stageStates(fr_ctx *c, const double **S) {
     a.Unext = c->adjS[s + 1];
     a.Uout = nullptr;
     a.Uout32 = nullptr;
+    a.waveSp = nullptr;  // the recomputation leaves the CFL wave speeds alone
     int rc = launchStageArgs<double>(c, a, 1);
     if (rc) return rc;
     S[s + 1] = c->adjS[s + 1];
@@ -557,6 +668,21 @@ int createImpl(const fr_config *cfg, const fr_mesh *m, int precision, fr_ctx *c)
   return 0;
 }
 
+// Buffers of CFL time stepping (allocated by the first fr_calc_dt).
+int cflBuffers(fr_ctx *c) {
+  int rc;
+  const size_t nb = nblk(c->nE);
+  if ((!c->dtE && (rc = dmalloc(&c->dtE, (size_t)c->nE))) ||
+      (!c->waveSp && (rc = dmalloc(&c->waveSp, (size_t)c->nE * c->NFP))) ||
+      (!c->dtNum && (rc = dmalloc(&c->dtNum, (size_t)c->nE))) || (!c->dtPart && (rc = dmalloc(&c->dtPart, nb))))
+    return rc;
+  if (!c->wsurf) {
+    if ((rc = dmalloc(&c->wsurf, c->ops.wtsFace.size()))) return rc;
+    HIP_TRY(hipMemcpy(c->wsurf, c->ops.wtsFace.data(), c->ops.wtsFace.size() * 8, hipMemcpyHostToDevice));
+  }
+  return 0;
+}
+
 int needPart(fr_ctx *c, size_t n) {
   if (c->partCap >= n) return 0;
   if (c->part) (void)hipFree(c->part);
@@ -568,6 +694,46 @@ int needPart(fr_ctx *c, size_t n) {
 }
 
 }  // namespace
+
+namespace frimpl {
+
+int dtPrepare(fr_ctx *c, hipStream_t st) {
+  if (c->dtType == 0 || c->dtReady || c->dtFrozen) return 0;
+  // getCFLLimit (global.cpp:45-68)
+  static const double lim[6] = {1.393, 0.464, 0.235, 0.1453, 0.100, 0.0736};
+  const double factor = c->cfl * lim[c->P] * 2;
+  const unsigned nb = nblk(c->nE);
+  hipError_t err = hipSuccess;
+  if (c->tri) {
+    // triangles (no reference counterpart): the wave speeds of the current
+    // state at every update
+    int rc = launchTriDt(c, c->wsurf, factor, st);
+    if (rc) return rc;
+    hipLaunchKernelGGL(frdev::k_block_min, dim3(nb), dim3(256), 0, st, c->dtE, c->nE, c->dtPart);
+  } else if (c->dtFirst) {
+    // the first update: ele::calcWaveSpFpts from the state (solver.cpp:294-297)
+    dispatch(c->D, c->P, c->NF, [&](auto Dc, auto Pc, auto NFc) {
+      constexpr int D = decltype(Dc)::value, P = decltype(Pc)::value, NF = decltype(NFc)::value;
+      hipLaunchKernelGGL((frdev::k_dt<D, P, NF>), dim3(nb), dim3(256), 0, st, c->U, c->ownN, c->fgeo64, c->detJ,
+                         c->wts, c->wsurf, c->oned64, factor, c->ph, c->nE, c->dtE, c->waveSp, c->dtNum);
+    });
+    if ((err = hipGetLastError()) != hipSuccess) return fail(FR_EHIP, std::string("k_dt: ") + hipGetErrorString(err));
+    hipLaunchKernelGGL(frdev::k_block_min, dim3(nb), dim3(256), 0, st, c->dtE, c->nE, c->dtPart);
+  } else {
+    // later updates: the wave speeds the last residual's face fluxes stored
+    dispatch(c->D, c->P, c->NF, [&](auto Dc, auto Pc, auto) {
+      using G = frdev::ElemGeo<decltype(Dc)::value, decltype(Pc)::value>;
+      hipLaunchKernelGGL((frdev::k_dt_ws<G::NFP, G::NFPF>), dim3(nb), dim3(256), 0, st, c->waveSp, c->fgeo64,
+                         c->wsurf, c->dtNum, c->D, c->nE, c->dtE, c->dtPart);
+    });
+  }
+  hipLaunchKernelGGL(frdev::k_dt_min, dim3(1), dim3(256), 0, st, c->dtPart, (int)nb, c->dtG);
+  if ((err = hipGetLastError()) != hipSuccess) return fail(FR_EHIP, std::string("time step: ") + hipGetErrorString(err));
+  c->dtReady = true;
+  return 0;
+}
+
+}  // namespace frimpl
 
 extern "C" {
 
@@ -592,6 +758,12 @@ int fr_create(const fr_config *cfg, const fr_mesh *mesh, int precision, fr_ctx *
 int fr_destroy(fr_ctx *c) {
   if (!c) return 0;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  // a batch holding this context can no longer step
+  for (fr_batch *b : c->batches) {
+    b->broken = true;
+    for (size_t k = 0; k < b->ctx.size(); k++)
+      if (b->ctx[k] == c) b->alive[k] = false;
+  }
   freeAll(c);
   delete c;
   return 0;
@@ -682,6 +854,9 @@ int fr_get_faces(fr_ctx *c, int *o6) {
 int fr_set_state(fr_ctx *c, const double *U, int layout) {
   if (!c || !U) return fail(FR_EINVAL, "null argument");
   const size_t n = c->nState();
+  if (layout != FR_LAYOUT_REF && layout != FR_LAYOUT_DEVICE) return fail(FR_EINVAL, "unknown layout");
+  setEnergyOffset(c, U, layout);
+  stateChanged(c);
   if (layout == FR_LAYOUT_REF) {
     std::vector<double> h(n);
     refToDev(c, U, h.data(), c->NF, c->NS);
@@ -749,7 +924,9 @@ int fr_get_traces(fr_ctx *c, double *out) {
   if (c->prec == FR_PREC_FP32) {
     std::vector<float> f(n);
     HIP_TRY(hipMemcpy(f.data(), c->Uf[c->cur], n * 4, hipMemcpyDeviceToHost));
-    for (size_t i = 0; i < n; i++) h[i] = f[i];
+    // the FP32 energy traces are offset (Phys::eOff)
+    for (size_t i = 0; i < n; i++)
+      h[i] = (double)f[i] + (((int)((i / c->NFP) % c->NF) == c->NF - 1) ? c->ph.eOff : 0.0);
   } else {
     HIP_TRY(hipMemcpy(h.data(), c->Uf[c->cur], n * 8, hipMemcpyDeviceToHost));
   }
@@ -769,6 +946,7 @@ int fr_update(fr_ctx *c, int nSteps) {
 int fr_time_step_a(fr_ctx *c, int step, double rkval) {
   if (!c) return fail(FR_EINVAL, "null argument");
   if (step < 0 || step >= c->nRK) return fail(FR_EINVAL, "step out of range");
+  stateChanged(c);
   return c->prec == FR_PREC_FP32 ? rkA<float>(c, step, rkval) : rkA<double>(c, step, rkval);
 }
 
@@ -776,14 +954,15 @@ int fr_time_step_b(fr_ctx *c, int step) {
   if (!c) return fail(FR_EINVAL, "null argument");
   if (step < 0 || step >= c->nRK) return fail(FR_EINVAL, "step out of range");
   long n = (long)c->nState();
+  stateChanged(c);
   if (c->prec == FR_PREC_FP32)
     hipLaunchKernelGGL(frdev::k_rk_b1<float>, dim3(nblk(n)), dim3(256), 0, c->stream, c->U,
                        (const float *)c->divF[step], c->detJ, c->U32, c->RKb[step], dtSource(c).first,
-                       dtSource(c).second, c->NF, c->NS, n);
+                       dtSource(c).second, c->NF, c->NS, n, c->ph.eOff);
   else
     hipLaunchKernelGGL(frdev::k_rk_b1<double>, dim3(nblk(n)), dim3(256), 0, c->stream, c->U,
                        (const double *)c->divF[step], c->detJ, c->U32, c->RKb[step], dtSource(c).first,
-                       dtSource(c).second, c->NF, c->NS, n);
+                       dtSource(c).second, c->NF, c->NS, n, c->ph.eOff);
   HIP_TRY(hipGetLastError());
   c->traceOf = -1;
   return 0;
@@ -797,6 +976,7 @@ int fr_copy_u_u0(fr_ctx *c) {
 
 int fr_copy_u0_u(fr_ctx *c) {
   if (!c) return fail(FR_EINVAL, "null argument");
+  stateChanged(c);
   HIP_TRY(hipMemcpyAsync(c->U, c->U0, c->nState() * 8, hipMemcpyDeviceToDevice, c->stream));
   c->traceOf = -1;
   return syncU32(c);
@@ -820,7 +1000,7 @@ int fr_reduce(fr_ctx *c, int kind, double *out) {
     HIP_TRY(hipStreamSynchronize(c->stream));
     for (int k = 0; k < c->NF; k++) {
       double r = 0;
-      for (int b = 0; b < nb; b++) r = (kind == 3) ? std::fmax(r, h[(size_t)b * 5 + k]) : r + h[(size_t)b * 5 + k];
+      for (int b = 0; b < nb; b++) r = (kind == 3) ? frdev::maxKeepNaN(r, h[(size_t)b * 5 + k]) : r + h[(size_t)b * 5 + k];
       if (kind == 2) r = std::sqrt(r);
       if (std::isnan(r)) return fail(FR_ENAN, "NaN Encountered in Solution Residual!");
       out[k] = r;
@@ -858,6 +1038,8 @@ int fr_set_dt(fr_ctx *c, double dt) {
   HIP_TRY(hipStreamSynchronize(c->stream));
   c->cfg.dt = dt;
   c->dtType = 0;
+  c->dtFirst = c->dtReady = false;
+  c->gen++;
   HIP_TRY(hipMemcpy(c->dtG, &dt, sizeof(double), hipMemcpyHostToDevice));
   return 0;
 }
@@ -865,44 +1047,21 @@ int fr_set_dt(fr_ctx *c, double dt) {
 int fr_calc_dt(fr_ctx *c, int dtType, double CFL, double *dtMin) {
   if (!c) return fail(FR_EINVAL, "null argument");
   if (dtType != 1 && dtType != 2) return fail(FR_EINVAL, "dtType must be 1 (global) or 2 (local)");
-  // getCFLLimit (global.cpp:45-68)
-  static const double lim[6] = {1.393, 0.464, 0.235, 0.1453, 0.100, 0.0736};
   if (c->P < 0 || c->P > 5) return fail(FR_EINVAL, "CFL limit not available for this order!");
+  if (!(CFL > 0)) return fail(FR_EINVAL, "CFL must be positive");
   int rc;
-  if (!c->dtE && (rc = dmalloc(&c->dtE, (size_t)c->nE))) return rc;
-  double *wsurf;
-  if ((rc = dmalloc(&wsurf, c->ops.wtsFace.size()))) return rc;
-  HIP_TRY(hipMemcpyAsync(wsurf, c->ops.wtsFace.data(), c->ops.wtsFace.size() * 8, hipMemcpyHostToDevice, c->stream));
-  hipError_t err = hipSuccess;
-  if (c->tri) {
-    // triangles: the same formula with the element's dense traces; the limit
-    // table is the quads' (no reference value exists for triangles)
-    if ((rc = launchTriDt(c, wsurf, CFL * lim[c->P] * 2))) {
-      (void)hipFree(wsurf);
-      return rc;
-    }
-  } else {
-    dispatch(c->D, c->P, c->NF, [&](auto Dc, auto Pc, auto NFc) {
-      constexpr int D = decltype(Dc)::value, P = decltype(Pc)::value, NF = decltype(NFc)::value;
-      hipLaunchKernelGGL((frdev::k_dt<D, P, NF>), dim3(nblk(c->nE)), dim3(256), 0, c->stream, c->U, c->ownN,
-                         c->fgeo64, c->detJ, c->wts, wsurf, c->oned64, CFL * lim[c->P] * 2, c->ph, c->nE, c->dtE);
-      err = hipGetLastError();
-    });
-  }
-  if (err != hipSuccess) {
-    (void)hipFree(wsurf);
-    return fail(FR_EHIP, std::string("k_dt launch: ") + hipGetErrorString(err));
-  }
-  std::vector<double> h(c->nE);
-  HIP_TRY(hipMemcpyAsync(h.data(), c->dtE, h.size() * 8, hipMemcpyDeviceToHost, c->stream));
+  if ((rc = cflBuffers(c))) return rc;
+  c->cfl = CFL;
+  c->dtType = dtType;
+  c->dtFirst = true;
+  c->dtReady = false;
+  c->gen++;
+  if ((rc = dtPrepare(c, c->stream))) return rc;
+  double m = 0;
+  HIP_TRY(hipMemcpyAsync(&m, c->dtG, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
-  (void)hipFree(wsurf);
-  double m = INFINITY;
-  for (double v : h) m = std::fmin(m, v);
   if (!(m > 0) || std::isinf(m)) return fail(FR_ENAN, "CFL time step is not positive and finite");
   c->cfg.dt = m;
-  c->dtType = dtType;
-  HIP_TRY(hipMemcpy(c->dtG, &m, sizeof(double), hipMemcpyHostToDevice));
   if (dtMin) *dtMin = m;
   return 0;
 }
@@ -910,6 +1069,8 @@ int fr_calc_dt(fr_ctx *c, int dtType, double CFL, double *dtMin) {
 int fr_get_dt(fr_ctx *c, double *dtEle) {
   if (!c || !dtEle) return fail(FR_EINVAL, "null argument");
   HIP_TRY(hipStreamSynchronize(c->stream));
+  // params->dt of the last update (or of fr_calc_dt / fr_set_dt)
+  HIP_TRY(hipMemcpy(&c->cfg.dt, c->dtG, sizeof(double), hipMemcpyDeviceToHost));
   if (c->dtType != 2) {
     for (int e = 0; e < c->nE; e++) dtEle[e] = c->cfg.dt;
     return 0;
@@ -939,6 +1100,9 @@ int fr_vjp_step(fr_ctx *c, const double *lam1, double *lam0) {
   if (!c || !lam1 || !lam0) return fail(FR_EINVAL, "null argument");
   int rc = adjCheck(c);
   if (rc) return rc;
+  // the step's time step is the one the next update takes (a constant of
+  // the step); forming it here leaves it for that update
+  if ((rc = dtPrepare(c, c->stream))) return rc;
   const size_t n = c->nState();
   std::vector<double> h(n);
   refToDev(c, lam1, h.data(), c->NF, c->NS);
@@ -957,28 +1121,51 @@ int fr_vjp_steps(fr_ctx *c, int nSteps, int every, const double *lamN, double *l
   if (rc) return rc;
   const size_t n = c->nState();
   const int nCk = (nSteps + every - 1) / every;
+  // CFL time stepping: each step's time step is recorded in the forward sweep
+  // and given back to its recomputation and its adjoint
+  const bool cfl = c->dtType != 0;
+  const size_t nDt = (c->dtType == 2) ? (size_t)c->nE : 1;
+  double *dtArr = (c->dtType == 2) ? c->dtE : c->dtG;
   std::vector<double *> ck(nCk, nullptr), seg(every, nullptr);
-  double *uN = nullptr;
+  double *uN = nullptr, *dtHist = nullptr, *wsKeep = nullptr;
   auto cleanup = [&]() {
     for (double *p : ck) if (p) (void)hipFree(p);
     for (double *p : seg) if (p) (void)hipFree(p);
     if (uN) (void)hipFree(uN);
+    if (dtHist) (void)hipFree(dtHist);
+    if (wsKeep) (void)hipFree(wsKeep);
+    c->dtFrozen = false;
   };
   auto copy = [&](double *dst, const double *src) {
     return hipMemcpyAsync(dst, src, n * 8, hipMemcpyDeviceToDevice, c->stream);
+  };
+  auto copyDt = [&](double *dst, const double *src) {
+    return hipMemcpyAsync(dst, src, nDt * 8, hipMemcpyDeviceToDevice, c->stream);
   };
   for (int i = 0; i < nCk; i++)
     if ((rc = dmalloc(&ck[i], n))) { cleanup(); return rc; }
   for (int i = 0; i < every; i++)
     if ((rc = dmalloc(&seg[i], n))) { cleanup(); return rc; }
   if ((rc = dmalloc(&uN, n))) { cleanup(); return rc; }
+  if (cfl && ((rc = dmalloc(&dtHist, nDt * nSteps)) || (!c->tri && (rc = dmalloc(&wsKeep, (size_t)c->nE * c->NFP))))) {
+    cleanup();
+    return rc;
+  }
   // forward sweep, keeping the state at every `every`-th step
   for (int i = 0; i < nSteps; i++) {
     if (i % every == 0 && copy(ck[i / every], c->U) != hipSuccess) { cleanup(); return fail(FR_EHIP, "copy"); }
     if ((rc = updateOnce(c))) { cleanup(); return rc; }
+    if (cfl && copyDt(dtHist + (size_t)i * nDt, dtArr) != hipSuccess) { cleanup(); return fail(FR_EHIP, "copy"); }
   }
   if (copy(uN, c->U) != hipSuccess) { cleanup(); return fail(FR_EHIP, "copy"); }
+  if (wsKeep && hipMemcpyAsync(wsKeep, c->waveSp, (size_t)c->nE * c->NFP * 8, hipMemcpyDeviceToDevice,
+                               c->stream) != hipSuccess) {
+    cleanup();
+    return fail(FR_EHIP, "copy");
+  }
   // reverse sweep, recomputing each segment's step states from its checkpoint
+  // with the recorded time steps
+  c->dtFrozen = cfl;
   std::vector<double> h(n);
   refToDev(c, lamN, h.data(), c->NF, c->NS);
   HIP_TRY(hipMemcpyAsync(c->adjLam, h.data(), n * 8, hipMemcpyHostToDevice, c->stream));
@@ -989,16 +1176,31 @@ int fr_vjp_steps(fr_ctx *c, int nSteps, int every, const double *lamN, double *l
     c->traceOf = -1;
     for (int i = start; i < end; i++) {
       if (copy(seg[i - start], c->U) != hipSuccess) { cleanup(); return fail(FR_EHIP, "copy"); }
-      if (i + 1 < end && (rc = updateOnce(c))) { cleanup(); return rc; }
+      if (i + 1 < end) {
+        if (cfl && copyDt(dtArr, dtHist + (size_t)i * nDt) != hipSuccess) { cleanup(); return fail(FR_EHIP, "copy"); }
+        if ((rc = updateOnce(c))) { cleanup(); return rc; }
+      }
     }
     for (int i = end - 1; i >= start; i--) {
       if (copy(c->U, seg[i - start]) != hipSuccess) { cleanup(); return fail(FR_EHIP, "copy"); }
+      if (cfl && copyDt(dtArr, dtHist + (size_t)i * nDt) != hipSuccess) { cleanup(); return fail(FR_EHIP, "copy"); }
       if ((rc = vjpStepDev(c, lamA, lamB))) { cleanup(); return rc; }
       std::swap(lamA, lamB);
     }
   }
   HIP_TRY(hipMemcpyAsync(h.data(), lamA, n * 8, hipMemcpyDeviceToHost, c->stream));
   if (copy(c->U, uN) != hipSuccess) { cleanup(); return fail(FR_EHIP, "copy"); }
+  // the context as the forward sweep left it: the last step's time step and
+  // the wave speeds of its last residual
+  if (cfl) {
+    if (copyDt(dtArr, dtHist + (size_t)(nSteps - 1) * nDt) != hipSuccess ||
+        (wsKeep && hipMemcpyAsync(c->waveSp, wsKeep, (size_t)c->nE * c->NFP * 8, hipMemcpyDeviceToDevice,
+                                  c->stream) != hipSuccess)) {
+      cleanup();
+      return fail(FR_EHIP, "copy");
+    }
+    c->dtReady = false;
+  }
   HIP_TRY(hipStreamSynchronize(c->stream));
   c->traceOf = -1;
   cleanup();
@@ -1054,6 +1256,9 @@ int fr_batch_create(fr_ctx *const *ctxs, int n, fr_batch **out) {
       return bad(FR_EINVAL, "batched meshes must share nDims, equation, precision and timeType");
     if (c->cfg.viscous) return bad(FR_EINVAL, "the batched kernel is inviscid");
     if (c->P < 1 || c->P > 5) return bad(FR_EINVAL, "batched meshes need P in 1..5");
+    for (fr_ctx *o : b->ctx)
+      if (o == c) return bad(FR_EINVAL, "a context appears twice in the batch");
+    if (int rc0 = batchMemberCheck(c)) return bad(rc0, g_err);
     b->ctx.push_back(c);
   }
   // work items grouped by kernel (one launch per element type and order),
@@ -1136,19 +1341,40 @@ int fr_batch_create(fr_ctx *const *ctxs, int n, fr_batch **out) {
       return bad(FR_EHIP, "graph instantiate");
   }
   // the tuning launches wrote only divF[0]; the states and traces are as before
+  // (the wave speeds of CFL members are rewritten by the next step's stages)
+  for (fr_ctx *c : b->ctx) c->batches.push_back(b);
+  b->alive.assign(b->ctx.size(), true);
   *out = b;
   return 0;
 }
 
 int fr_batch_update(fr_batch *b, int nSteps) {
   if (!b) return fail(FR_EINVAL, "null argument");
+  if (b->broken) return fail(FR_EINVAL, "a member context of this batch was destroyed");
+  int rc;
+  bool stale = false;
+  for (size_t i = 0; i < b->ctx.size(); i++) {
+    if ((rc = batchMemberCheck(b->ctx[i]))) return rc;
+    stale = stale || b->ctx[i]->gen != b->gen[i];
+  }
+  // a member's time step, geometry layout or FP32 energy offset changed since
+  // the arguments were built: rebuild them in place
+  if (stale) {
+    for (fr_ctx *c : b->ctx) HIP_TRY(hipStreamSynchronize(c->stream));
+    rc = (b->prec == FR_PREC_FP32) ? batchArgs<float>(b) : batchArgs<double>(b);
+    if (rc) return rc;
+  }
   // an even number of stages returns the traces to the buffer they started
   // in, so one graph (parity 0) serves every step; odd counts alternate
   int parity = (b->nRK & 1) ? (b->ctx[0]->cur & 1) : 0;
-  int rc = batchPrepare(b, parity);
-  if (rc) return rc;
+  if ((rc = batchPrepare(b, parity))) return rc;
   for (int i = 0; i < nSteps; i++) {
+    // members with a CFL time step form it first, on the batch stream
+    for (fr_ctx *c : b->ctx)
+      if (c->dtType != 0 && (rc = dtPrepare(c, b->stream))) return rc;
     HIP_TRY(hipGraphLaunch(b->exec[parity], b->stream));
+    for (fr_ctx *c : b->ctx)
+      if (c->dtType != 0) c->dtFirst = c->dtReady = false;
     parity ^= (b->nRK & 1);
   }
   HIP_TRY(hipStreamSynchronize(b->stream));
@@ -1190,7 +1416,10 @@ int fr_synchronize(fr_ctx *c) {
 
 int fr_geometry_mode(fr_ctx *c, int request, int *active) {
   if (!c || request < -1 || request > 1) return fail(FR_EINVAL, "bad argument");
-  if (request >= 0) c->affineUse = request == 1;
+  if (request >= 0 && c->affineUse != (request == 1)) {
+    c->affineUse = request == 1;
+    c->gen++;
+  }
   if (active) *active = (c->affineMesh && c->affineUse) ? 1 : 0;
   return 0;
 }

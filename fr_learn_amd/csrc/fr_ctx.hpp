@@ -52,6 +52,8 @@ inline unsigned nblk(long n, int b = 256) { return (unsigned)((n + b - 1) / b); 
 
 }  // namespace frimpl
 
+struct fr_batch;
+
 struct fr_ctx {
   fr_config cfg{};
   int prec = FR_PREC_FP64;
@@ -80,6 +82,16 @@ struct fr_ctx {
   double *dtG = nullptr;              // the global time step (one value)
   double *dtE = nullptr;              // per-element time steps (local time stepping)
   int dtType = 0;                     // 0 fixed dt, 1 global CFL dt, 2 local CFL dt
+  // CFL time stepping (dtType != 0; solver::calcDt, solver.cpp:293-305): the
+  // wave speeds waveSp[ele][fpt] (the reference's waveSp_fpts), the per-element
+  // numerators CFL * limit(P) * 2 * vol, the surface weights, block minima
+  double *waveSp = nullptr, *dtNum = nullptr, *wsurf = nullptr, *dtPart = nullptr;
+  double cfl = 0;         // the CFL number of fr_calc_dt
+  bool dtFirst = false;   // the next update is the first since fr_calc_dt (wave speeds from the state)
+  bool dtReady = false;   // dtG / dtE already hold the next update's time step
+  bool dtFrozen = false;  // the adjoint's replay: dt given, wave speeds untouched
+  unsigned gen = 0;       // bumped by every setting the batched launch arguments capture
+  std::vector<fr_batch *> batches;  // batches this context is a member of
   // adjoint work buffers (FP64, allocated on first use)
   double *adjA = nullptr, *adjTr = nullptr, *adjTr2 = nullptr;  // face cotangents, trace scratch
   double *adjLam = nullptr, *adjOut = nullptr, *adjNu = nullptr, *adjW[2] = {nullptr, nullptr};
@@ -108,6 +120,9 @@ struct fr_ctx {
 // all their element blocks, the launches of one step captured in a hipGraph.
 struct fr_batch {
   std::vector<fr_ctx *> ctx;
+  std::vector<unsigned> gen;  // each member's settings generation the arguments were built at
+  std::vector<bool> alive;    // members not yet destroyed
+  bool broken = false;        // a member was destroyed
   int D = 2, NF = 4, nRK = 1, prec = FR_PREC_FP64;
   hipStream_t stream = nullptr;
   void *args[2] = {nullptr, nullptr};  // StageArgs per [trace parity][stage][instance]
@@ -183,8 +198,12 @@ template <typename T>
 int launchTriStage(fr_ctx *c, const frdev::StageArgs<T> &a, int mode);
 template <typename T>
 int launchTriTrace(fr_ctx *c, const T *U, T *Uf, const T *tab);
-// CFL time steps of the triangles into c->dtE (k_tri_dt)
-int launchTriDt(fr_ctx *c, const double *wsurf, double factor);
+// CFL time steps of the triangles into c->dtE (k_tri_dt), on stream st
+int launchTriDt(fr_ctx *c, const double *wsurf, double factor, hipStream_t st);
+// The time step of the next update into dtG / dtE on stream st (fr_api.hip):
+// from the state on the first update after fr_calc_dt (and always for
+// triangles), else from the wave speeds the last residual stored.
+int dtPrepare(fr_ctx *c, hipStream_t st);
 // Stage s of one batched step (fr_batch.hip); parity < 0 is the tuning launch,
 // Ponly != 0 restricts it to the meshes of that order.
 int batchLaunchStage(fr_batch *b, int parity, int s, int grid, hipStream_t st, int gOnly = 0);

@@ -23,16 +23,32 @@
 namespace frdev {
 
 // Common normal flux at one face point, (UL, UR) in the left element's frame.
+// ws (may be null) receives the wave speed the reference stores at the face's
+// left flux point for the CFL time step: max(eigL, eigR) of the Rusanov flux
+// (face.cpp:354; boundary faces always use Rusanov, face.cpp:140-150) or
+// |a.n| of Lax-Friedrichs (face.cpp:498-508).  Returns false where the
+// reference stores none (the Roe flux, face.cpp:358-484).
 template <int D, int NF, typename T>
-__device__ __forceinline__ void commonFlux(const Phys &ph, bool boundary, const T *UL, const T *UR, const T *n,
-                                           T *Fn) {
+__device__ __forceinline__ bool commonFlux(const Phys &ph, bool boundary, const T *UL, const T *UR, const T *n,
+                                           T *Fn, T *ws = nullptr) {
   if constexpr (NF == 1) {
     laxFriedrichs<D, T>(ph, UL, UR, n, Fn);
+    if (ws) {
+      T vN = (T)ph.advect[0] * n[0] + (T)ph.advect[1] * n[1];
+      if constexpr (D == 3) vN += (T)ph.advect[2] * n[2];
+      *ws = fabs(vN);
+    }
+    return true;
   } else if constexpr (D == 2) {
-    if (!boundary && ph.riemannType == 1) roe2d<T>(ph, UL, UR, n, Fn);
-    else rusanovN<D, T>(ph, UL, UR, n, Fn);
+    if (!boundary && ph.riemannType == 1) {
+      roe2d<T>(ph, UL, UR, n, Fn);
+      return false;
+    }
+    rusanovN<D, T>(ph, UL, UR, n, Fn, ws);
+    return true;
   } else {
-    rusanovN<D, T>(ph, UL, UR, n, Fn);
+    rusanovN<D, T>(ph, UL, UR, n, Fn, ws);
+    return true;
   }
 }
 
@@ -130,6 +146,7 @@ struct StageArgs {
   const T *__restrict__ Fv;   // viscous: common viscous normal flux [ele][field][fpt] (k_vflux)
   const double *dtE;          // time step: dtE[ele * dtStride] (stride 0: one global dt)
   int dtStride;
+  double *waveSp;             // CFL wave speeds [ele][fpt] written at left / boundary points, or null
   double c0, c1, c2, c3, rk;
   int mode, nprev, nEles;
   Phys ph;
@@ -545,6 +562,9 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
     if (bndFace) pnb[r] = -1;
   }
   if (tid < O::SIZE) s1[tid] = o1;
+  // the FP64 base of the final stages is absolute; FP32 stage states hold the
+  // energy less Phys::eOff
+  const bool base64 = FIN || (VIS && a.mode >= 2);
   if (hasS) {
     // Us = base - rkPrev * rprev * dt / detJ: the previous stage's update
     // (the expression of step (4)), evaluated here instead of stored
@@ -555,6 +575,8 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
       if (a.rprev) {
         double u = pb[k];
         u = u - a.rkPrev * (double)ul[k] * dtJ;
+        if constexpr (hasEnergyOffset<T>())
+          if (k == NF - 1 && base64) u = u - a.ph.eOff;
         v = (T)u;
       }
       sU[slot][k][tP] = v;
@@ -755,7 +777,11 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
         sL[k] = right ? un[k] : uo[k];
         sR[k] = right ? uo[k] : un[k];
       }
-      commonFlux<D, NF, T>(a.ph, bnd, sL, sR, n, Fn);
+      T ws;
+      const bool hasWs = commonFlux<D, NF, T>(a.ph, bnd, sL, sR, n, Fn, &ws);
+      // the CFL wave speed at the face's left points (and boundary points);
+      // the right element's points keep theirs (face.cpp:74-80)
+      if (a.waveSp && hasWs && !right) a.waveSp[(size_t)e * NFP + q] = (double)ws;
       if constexpr (VIS) {
         // the viscous common flux of this point (k_vflux, before this launch)
 #pragma unroll
@@ -810,7 +836,12 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
         if (a.Unext) __builtin_nontemporal_store((T)u, a.Unext + gi);
       } else {
         __builtin_nontemporal_store(u, a.Uout + gi);
-        if (a.Uout32) __builtin_nontemporal_store((float)u, a.Uout32 + gi);
+        if (a.Uout32) {
+          double u32 = u;
+          if constexpr (hasEnergyOffset<T>())
+            if (k == NF - 1) u32 = u - a.ph.eOff;
+          __builtin_nontemporal_store((float)u32, a.Uout32 + gi);
+        }
       }
       sFd[slot][k][tP] = (T)u;
     }
@@ -943,12 +974,19 @@ __global__ void k_fgeo(const double *__restrict__ ownN, const int32_t *__restric
 // Runge-Kutta stage updates (solver::timeStepA / timeStepB,
 // solver.cpp:307-385), elementwise over [ele][field][spt].
 // ---------------------------------------------------------------------------
+// FP32 copy of one FP64 state entry i of [ele][field][spt]: the energy field
+// less the offset eOff (Phys::eOff), the other fields as they are.
+__device__ __forceinline__ float stateToFloat(double u, long i, int NF, int NS, double eOff) {
+  return (float)(((int)((i / NS) % NF) == NF - 1) ? u - eOff : u);
+}
+
 // U = U0 - rk * divF / detJ * dt; with copy != 0 first sets U0 = U
 // (copyUspts_U0, solver.cpp:387-392).  U32 (optional) receives a float copy.
 template <typename TD>
 __global__ void k_rk_a(double *__restrict__ U, double *__restrict__ U0, const TD *__restrict__ divF,
                        const double *__restrict__ detJ, float *__restrict__ U32, double rk,
-                       const double *__restrict__ dtE, int dtStride, int copy, int NF, int NS, long n) {
+                       const double *__restrict__ dtE, int dtStride, int copy, int NF, int NS, long n,
+                       double eOff) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const long e = i / ((long)NF * NS);
@@ -962,21 +1000,28 @@ __global__ void k_rk_a(double *__restrict__ U, double *__restrict__ U0, const TD
   }
   const double v = u0 - rk * (double)divF[i] / detJ[e * NS + s] * dtE[e * dtStride];
   U[i] = v;
-  if (U32) U32[i] = (float)v;
+  if (U32) U32[i] = stateToFloat(v, i, NF, NS, eOff);
 }
 
 // Single timeStepB pass (solver.cpp:347-385) for the ABI entry point.
 template <typename TD>
 __global__ void k_rk_b1(double *__restrict__ U, const TD *__restrict__ dF, const double *__restrict__ detJ,
                         float *__restrict__ U32, double b, const double *__restrict__ dtE, int dtStride, int NF,
-                        int NS, long n) {
+                        int NS, long n, double eOff) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const long e = i / ((long)NF * NS);
   const int s = (int)(i % NS);
   const double u = U[i] - b * (double)dF[i] / detJ[e * NS + s] * dtE[e * dtStride];
   U[i] = u;
-  if (U32) U32[i] = (float)u;
+  if (U32) U32[i] = stateToFloat(u, i, NF, NS, eOff);
+}
+
+// FP32 copy of the FP64 state (stateToFloat).
+static __global__ void k_state_to_float(const double *__restrict__ U, float *__restrict__ U32, int NF, int NS,
+                                        double eOff, long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) U32[i] = stateToFloat(U[i], i, NF, NS, eOff);
 }
 
 // CFL time step of each element from the current state, FP64: the traces
@@ -984,12 +1029,15 @@ __global__ void k_rk_b1(double *__restrict__ U, const TD *__restrict__ dF, const
 // every flux point (ele::calcWaveSpFpts, ele.cpp:1031-1084, static grids) and
 // dt = CFL * limit(P) * 2 * vol / sum(waveSp * wSurf * dA) (ele::calcDt,
 // ele.cpp:1086-1105).  One thread per element; runs only when asked for.
+// waveSp[ele][fpt] keeps the wave speeds (the reference's waveSp_fpts) and
+// dtNum[ele] the numerator CFL * limit(P) * 2 * vol for k_dt_ws.
 template <int D, int P, int NF>
 __global__ __launch_bounds__(256) void k_dt(const double *__restrict__ U, const double *__restrict__ ownN,
                                             const double *__restrict__ fgeo, const double *__restrict__ detJ,
                                             const double *__restrict__ wts, const double *__restrict__ wsurf,
                                             const double *__restrict__ oned, double factor, Phys ph, int nEles,
-                                            double *__restrict__ dtE) {
+                                            double *__restrict__ dtE, double *__restrict__ waveSp,
+                                            double *__restrict__ dtNum) {
   using G = ElemGeo<D, P>;
   using O = OneD<P>;
   constexpr int NS = G::NS, NFP = G::NFP, NFPF = G::NFPF, P1 = G::P1;
@@ -1025,9 +1073,69 @@ __global__ __launch_bounds__(256) void k_dt(const double *__restrict__ U, const 
       const double csq = fmax(ph.gamma * pr / rho, 0.0);
       ws = fabs(vN) + sqrt(csq);
     }
+    waveSp[(size_t)e * NFP + q] = ws;
     intWave += ws * wsurf[q % NFPF] * fgeo[((size_t)e * (D + 1) + D) * NFP + q];
   }
   dtE[e] = factor * vol / intWave;
+  dtNum[e] = factor * vol;
+}
+
+// The CFL time step of every element from the stored wave speeds
+// (ele::calcDt, ele.cpp:1086-1105, with the waveSp_fpts the face fluxes of
+// the last residual left: solver::calcDt after the first iteration,
+// solver.cpp:293-305), and the minimum over each block of elements into
+// part[block].  One thread per element.
+template <int NFP, int NFPF>
+__global__ __launch_bounds__(256) void k_dt_ws(const double *__restrict__ waveSp, const double *__restrict__ fgeo,
+                                               const double *__restrict__ wsurf, const double *__restrict__ dtNum,
+                                               int D, int nEles, double *__restrict__ dtE, double *__restrict__ part) {
+  __shared__ double red[256];
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  double dt = INFINITY;
+  if (e < nEles) {
+    const double *ws = waveSp + (size_t)e * NFP;
+    const double *dA = fgeo + ((size_t)e * (D + 1) + D) * NFP;
+    double intWave = 0;
+    for (int q = 0; q < NFP; q++) intWave += ws[q] * wsurf[q % NFPF] * dA[q];
+    dt = dtNum[e] / intWave;
+    dtE[e] = dt;
+  }
+  red[threadIdx.x] = dt;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// part[block] = the minimum of x over each block of 256 entries.
+static __global__ __launch_bounds__(256) void k_block_min(const double *__restrict__ x, int n,
+                                                          double *__restrict__ part) {
+  __shared__ double red[256];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  red[threadIdx.x] = i < n ? x[i] : INFINITY;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// dtG[0] = the minimum of part[0..n) (params->dt of solver::calcDt), one block.
+static __global__ __launch_bounds__(256) void k_dt_min(const double *__restrict__ part, int n,
+                                                       double *__restrict__ dtG) {
+  __shared__ double red[256];
+  double m = INFINITY;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) m = fmin(m, part[i]);
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) dtG[0] = red[0];
 }
 
 // Affine-element test and compact geometry (GeoAddr): element e passes when
@@ -1073,6 +1181,10 @@ static __global__ void k_to_float(const double *__restrict__ a, float *__restric
 // loop order, then a block tree reduction; partial results per block go to
 // `part` and are summed on the host in block order (deterministic).
 // ---------------------------------------------------------------------------
+// Maximum that keeps a NaN operand (fmax drops it): an infinity norm over a
+// residual holding a NaN is NaN, so the NaN check of the reduction fires.
+__host__ __device__ __forceinline__ double maxKeepNaN(double a, double b) { return (a != a || a > b) ? a : b; }
+
 // residual norm of divF_spts[0] (ele::getNormResidual, ele.cpp:1882-1903)
 template <typename TD>
 __global__ __launch_bounds__(256) void k_norm(const TD *__restrict__ divF, const double *__restrict__ detJ,
@@ -1088,7 +1200,7 @@ __global__ __launch_bounds__(256) void k_norm(const TD *__restrict__ divF, const
         const double v = (double)divF[((size_t)e * NF + k) * NS + s];
         if (normType == 1) r[k] += fabs(v) * wts[s];
         else if (normType == 2) r[k] += v * v / J * wts[s];
-        else r[k] = fmax(fabs(v) / J, r[k]);
+        else r[k] = maxKeepNaN(fabs(v) / J, r[k]);
       }
     }
   }
@@ -1098,7 +1210,7 @@ __global__ __launch_bounds__(256) void k_norm(const TD *__restrict__ divF, const
     if ((int)threadIdx.x < w)
       for (int k = 0; k < 5; k++) {
         const double o = red[threadIdx.x + w][k];
-        red[threadIdx.x][k] = (normType == 3) ? fmax(red[threadIdx.x][k], o) : red[threadIdx.x][k] + o;
+        red[threadIdx.x][k] = (normType == 3) ? maxKeepNaN(red[threadIdx.x][k], o) : red[threadIdx.x][k] + o;
       }
     __syncthreads();
   }

@@ -15,7 +15,23 @@ struct Phys {
   double TWall, RGas, uWall, vWall, wWall;
   double penFact, tau, mu_inf, prandtl, rt_inf, c_sth;
   int equation, riemannType, fixVis, viscFix;
+  // Energy offset of the FP32 states (inviscid Euler): they hold E - eOff,
+  // where eOff = pOff / (gamma - 1) for a reference pressure pOff near the
+  // flow's mean.  The FP32 flux routines then work with p - pOff, whose
+  // rounding is relative to the pressure variation instead of the (low-Mach)
+  // background; hOff = gamma * eOff restores the total enthalpy of the energy
+  // flux.  The constant pOff dropped from the momentum flux is a
+  // divergence-free term of the discretisation (free-stream preservation).
+  // Zero in FP64 contexts, whose routines ignore them.
+  double eOff, pOff, hOff;
 };
+
+// FP32 states carry the energy offset (Phys::eOff); FP64 and dual-number
+// states do not.
+template <typename T>
+constexpr bool hasEnergyOffset() {
+  return sizeof(T) == 4;
+}
 
 // 1 / x for the flux routines.  FP64: the hardware reciprocal estimate
 // refined by two Newton-Raphson steps, e = 1 - x r, r <- r + r e (each step
@@ -49,20 +65,25 @@ __device__ __forceinline__ void inviscidFlux(const Phys &ph, const T *U, T F[D][
     const T rho = U[0], ri = recip(rho);
     T u = U[1] * ri, v = U[2] * ri, E = U[3];
     T p = (g - (T)1.0) * (E - (T)0.5 * rho * (u * u + v * v));
+    // total enthalpy rho*H = E + p (FP32: offset-free E and p plus hOff)
+    T Ep = E + p;
+    if constexpr (hasEnergyOffset<T>()) Ep = Ep + (T)ph.hOff;
     F[0][0] = rho * u;         F[1][0] = rho * v;
     F[0][1] = rho * u * u + p; F[1][1] = rho * u * v;
     F[0][2] = rho * v * u;     F[1][2] = rho * v * v + p;
-    F[0][3] = (E + p) * u;     F[1][3] = (E + p) * v;
+    F[0][3] = Ep * u;          F[1][3] = Ep * v;
   } else {
     const T g = (T)ph.gamma;
     const T rho = U[0], ri = recip(rho);
     T u = U[1] * ri, v = U[2] * ri, w = U[3] * ri, E = U[4];
     T p = (g - (T)1.0) * (E - (T)0.5 * rho * (u * u + v * v + w * w));
+    T Ep = E + p;
+    if constexpr (hasEnergyOffset<T>()) Ep = Ep + (T)ph.hOff;
     F[0][0] = rho * u;         F[1][0] = rho * v;         F[2][0] = rho * w;
     F[0][1] = rho * u * u + p; F[1][1] = rho * u * v;     F[2][1] = rho * u * w;
     F[0][2] = rho * v * u;     F[1][2] = rho * v * v + p; F[2][2] = rho * v * w;
     F[0][3] = rho * w * u;     F[1][3] = rho * w * v;     F[2][3] = rho * w * w + p;
-    F[0][4] = (E + p) * u;     F[1][4] = (E + p) * v;     F[2][4] = (E + p) * w;
+    F[0][4] = Ep * u;          F[1][4] = Ep * v;          F[2][4] = Ep * w;
   }
 }
 
@@ -72,7 +93,7 @@ __device__ __forceinline__ void inviscidFlux(const Phys &ph, const T *U, T F[D][
 // holding the D x NF flux table.  One reciprocal of rho (recip) replaces the
 // reference's divisions (an ulp-level difference).
 template <int D, typename T>
-__device__ __forceinline__ void normalFlux(const T g, const T *U, const T *n, T *Fn, T &vn, T &csq) {
+__device__ __forceinline__ void normalFlux(const Phys &ph, const T g, const T *U, const T *n, T *Fn, T &vn, T &csq) {
   constexpr int NF = D + 2;
   const T rho = U[0], E = U[D + 1], ri = recip(rho);
   T v[D];
@@ -81,6 +102,11 @@ __device__ __forceinline__ void normalFlux(const T g, const T *U, const T *n, T 
   T q = v[0] * v[0] + v[1] * v[1];
   if (D == 3) q = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
   const T p = (g - (T)1.0) * (E - (T)0.5 * rho * q);
+  T Ep = E + p, pAbs = p;  // FP32: offset-free E and p (Phys::eOff)
+  if constexpr (hasEnergyOffset<T>()) {
+    Ep = Ep + (T)ph.hOff;
+    pAbs = p + (T)ph.pOff;
+  }
 #pragma unroll
   for (int i = 0; i < NF; i++) Fn[i] = (T)0;
   vn = (T)0;
@@ -90,21 +116,25 @@ __device__ __forceinline__ void normalFlux(const T g, const T *U, const T *n, T 
     Fn[0] += n[d] * (rho * v[d]);
 #pragma unroll
     for (int c = 0; c < D; c++) Fn[1 + c] += n[d] * ((c == d) ? rho * v[c] * v[d] + p : rho * v[c] * v[d]);
-    Fn[D + 1] += n[d] * ((E + p) * v[d]);
+    Fn[D + 1] += n[d] * (Ep * v[d]);
   }
-  csq = fmax(g * p * ri, (T)0.0);
+  csq = fmax(g * pAbs * ri, (T)0.0);
 }
 
 // Rusanov / local Lax-Friedrichs flux for Euler (face::rusanovFlux,
-// src/face.cpp:289-356, static mesh).
+// src/face.cpp:289-356, static mesh).  eigOut (may be null) receives
+// max(eigL, eigR), the wave speed the reference stores for the CFL time step
+// (face.cpp:354).
 template <int D, typename T>
-__device__ __forceinline__ void rusanovN(const Phys &ph, const T *UL, const T *UR, const T *n, T *Fn) {
+__device__ __forceinline__ void rusanovN(const Phys &ph, const T *UL, const T *UR, const T *n, T *Fn,
+                                         T *eigOut = nullptr) {
   constexpr int NF = D + 2;
   const T g = (T)ph.gamma;
   T FnL[NF], FnR[NF], vnL, vnR, csqL, csqR;
-  normalFlux<D, T>(g, UL, n, FnL, vnL, csqL);
-  normalFlux<D, T>(g, UR, n, FnR, vnR, csqR);
+  normalFlux<D, T>(ph, g, UL, n, FnL, vnL, csqL);
+  normalFlux<D, T>(ph, g, UR, n, FnR, vnR, csqR);
   const T eig = fmax(fabs(vnL) + sqrt(csqL), fabs(vnR) + sqrt(csqR));
+  if (eigOut) *eigOut = eig;
 #pragma unroll
   for (int i = 0; i < NF; i++) Fn[i] = (T)0.5 * (FnL[i] + FnR[i] - eig * (UR[i] - UL[i]));
 }
@@ -124,7 +154,16 @@ __device__ __forceinline__ void roe2d(const Phys &ph, const T *UL, const T *UR, 
   }
   T pL = (g - (T)1.0) * (UL[3] - ((T)0.5 * UL[0] * (vL[0] * vL[0] + vL[1] * vL[1])));
   T pR = (g - (T)1.0) * (UR[3] - ((T)0.5 * UR[0] * (vR[0] * vR[0] + vR[1] * vR[1])));
-  T hL = (UL[3] + pL) * riL, hR = (UR[3] + pR) * riR;
+  // FP32 (Phys::eOff): pL, pR are offset-free; the enthalpies and sound
+  // speeds take the absolute values
+  T EpL = UL[3] + pL, EpR = UR[3] + pR, pAL = pL, pAR = pR;
+  if constexpr (hasEnergyOffset<T>()) {
+    EpL = EpL + (T)ph.hOff;
+    EpR = EpR + (T)ph.hOff;
+    pAL = pL + (T)ph.pOff;
+    pAR = pR + (T)ph.pOff;
+  }
+  T hL = EpL * riL, hR = EpR * riR;
   T sq_rho = sqrt(UR[0] * riL);
   T rrho = recip(sq_rho + (T)1.);
   T usq = 0, unm = 0;
@@ -147,7 +186,7 @@ __device__ __forceinline__ void roe2d(const Phys &ph, const T *UL, const T *UR, 
   Fn[3] = rhoUnL * hL + rhoUnR * hR;
   for (int i = 0; i < 4; i++) du[i] = UR[i] - UL[i];
   T lambda0 = fabs(unm), lambdaP = fabs(unm + am), lambdaM = fabs(unm - am);
-  T eps = (T)0.5 * (fabs(rhoUnL * riL - rhoUnR * riR) + fabs(sqrt(g * pL * riL) - sqrt(g * pR * riR)));
+  T eps = (T)0.5 * (fabs(rhoUnL * riL - rhoUnR * riR) + fabs(sqrt(g * pAL * riL) - sqrt(g * pAR * riR)));
   const T reps = recip(eps);
   if (lambda0 < (T)2. * eps) lambda0 = (T)0.25 * lambda0 * lambda0 * reps + eps;
   if (lambdaP < (T)2. * eps) lambdaP = (T)0.25 * lambdaP * lambdaP * reps + eps;
@@ -185,6 +224,10 @@ __device__ __forceinline__ void boundaryState(const Phys &ph, int bc, const T *U
     UR[0] = (T)0;  // the reference leaves the advection ghost state at zero
   } else {
     const T g = (T)ph.gamma, gmo = g - (T)1.;
+    // FP32 states hold E - eOff (Phys::eOff): pL and the ghost energies below
+    // are then offset-free (p - pOff, E - eOff), the absolute pressure pLa is
+    // used where the formulas need it.  FP64: pLa = pL, pOff = 0.
+    constexpr bool OFF = hasEnergyOffset<T>();
     T rhoR = 0, pR = 0, ER = 0;
     T vL[3] = {0, 0, 0}, vR[3] = {0, 0, 0};
     const T vB[3] = {(T)ph.uBound, (T)ph.vBound, (T)ph.wBound};
@@ -193,6 +236,9 @@ __device__ __forceinline__ void boundaryState(const Phys &ph, int bc, const T *U
     T vSq = 0;
     for (int i = 0; i < D; i++) vSq += vL[i] * vL[i];
     T pL = (g - (T)1.0) * (eL - (T)0.5 * rhoL * vSq);
+    const T pLa = OFF ? pL + (T)ph.pOff : pL;
+    // the ghost pressure pR is offset-free like pL
+    const T pBnd = OFF ? (T)(ph.pBound - ph.pOff) : (T)ph.pBound;
     auto energy = [&](void) {
       T s = 0;
       for (int i = 0; i < D; i++) s += vR[i] * vR[i];
@@ -208,13 +254,13 @@ __device__ __forceinline__ void boundaryState(const Phys &ph, int bc, const T *U
       case 5:  // SUB_OUT
         rhoR = rhoL;
         for (int i = 0; i < D; i++) vR[i] = vL[i];
-        pR = (T)ph.pBound;
+        pR = pBnd;
         ER = energy();
         break;
       case 2:  // SUP_IN
         rhoR = (T)ph.rhoBound;
         vR[0] = vB[0]; vR[1] = vB[1]; vR[2] = vB[2];
-        pR = (T)ph.pBound;
+        pR = pBnd;
         ER = energy();
         break;
       case 3:  // SUP_OUT
@@ -233,7 +279,7 @@ __device__ __forceinline__ void boundaryState(const Phys &ph, int bc, const T *U
       }
       case 13:  // ISOTHERMAL_NOSLIP_MOVING
         pR = pL;
-        rhoR = pR / ((T)ph.RGas * (T)ph.TWall);
+        rhoR = pLa / ((T)ph.RGas * (T)ph.TWall);
         vR[0] = (T)ph.uWall;
         vR[1] = (T)ph.vWall;
         if (D == 3) vR[2] = (T)ph.wWall;
@@ -241,7 +287,7 @@ __device__ __forceinline__ void boundaryState(const Phys &ph, int bc, const T *U
         break;
       case 9:  // ISOTHERMAL_NOSLIP
         pR = pL;
-        rhoR = pR / ((T)ph.RGas * (T)ph.TWall);
+        rhoR = pLa / ((T)ph.RGas * (T)ph.TWall);
         ER = energy();
         break;
       case 10:  // ADIABATIC_NOSLIP
@@ -253,15 +299,16 @@ __device__ __forceinline__ void boundaryState(const Phys &ph, int bc, const T *U
         T vnL = 0, vnB = 0;
         for (int i = 0; i < D; i++) vnL += vL[i] * n[i];
         for (int i = 0; i < D; i++) vnB += vB[i] * n[i];
-        T cL = sqrt(g * pL / rhoL), cB = sqrt(g * (T)ph.pBound / (T)ph.rhoBound);
+        T cL = sqrt(g * pLa / rhoL), cB = sqrt(g * (T)ph.pBound / (T)ph.rhoBound);
         T R_L = (fabs(vnB) >= cB && vnL >= 0) ? vnB + (T)2. / gmo * cB : vnL + (T)2. / gmo * cL;
         T R_B = (fabs(vnB) >= cB && vnL < 0) ? vnL - (T)2. / gmo * cL : vnB - (T)2. / gmo * cB;
         T rhoFac = (T).25 * (g - (T)1.) * (R_L - R_B);
         T rhoBG = rhoFac * rhoFac / g;
         if (vnL < 0) rhoBG *= pow((T)ph.rhoBound, g) / (T)ph.pBound;
-        else rhoBG *= pow(rhoL, g) / pL;
+        else rhoBG *= pow(rhoL, g) / pLa;
         rhoR = pow(rhoBG, (T)1. / gmo);
         pR = rhoFac * rhoFac * rhoR / g;
+        if constexpr (OFF) pR = pR - (T)ph.pOff;
         T VB = (T)0.5 * (R_L + R_B);
         for (int i = 0; i < D; i++) vR[i] = (vnL < 0) ? vB[i] + (VB - vnB) * n[i] : vL[i] + (VB - vnL) * n[i];
         T vM2 = 0;

@@ -66,11 +66,11 @@ int launchTriTrace(fr_ctx *c, const T *U, T *Uf, const T *tab) {
   return 0;
 }
 
-int launchTriDt(fr_ctx *c, const double *wsurf, double factor) {
+int launchTriDt(fr_ctx *c, const double *wsurf, double factor, hipStream_t st) {
   hipError_t err = hipSuccess;
   bool ok = triDispatch(c->P, c->NF, [&](auto Pc, auto NFc) {
     constexpr int P = decltype(Pc)::value, NF = decltype(NFc)::value;
-    hipLaunchKernelGGL((frdev::k_tri_dt<P, NF>), dim3(nblk(c->nE)), dim3(256), 0, c->stream, c->U, c->ownN,
+    hipLaunchKernelGGL((frdev::k_tri_dt<P, NF>), dim3(nblk(c->nE)), dim3(256), 0, st, c->U, c->ownN,
                        c->fgeo64, c->detJ, c->wts, wsurf, c->oned64, factor, c->ph, c->nE, c->dtE);
     err = hipGetLastError();
   });

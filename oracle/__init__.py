@@ -222,7 +222,7 @@ class Oracle:
     def array(self, which):
         D, nS, nF, nE, K = self.nDims, self.nSpts, self.nFpts, self.nEles, self.nFields
         shapes = {0: (nS, nE, K), 1: (nF, nE, K), 2: (D, nS, nE, K), 3: (nF, nE, K), 4: (nF, nE, K),
-                  5: (D, nS, nE, K), 6: (D, nF, nE, K), 7: (nF, nE, K), 8: (nS, nE, K)}
+                  5: (D, nS, nE, K), 6: (D, nF, nE, K), 7: (nF, nE, K), 8: (nS, nE, K), 9: (nF, nE)}
         shape = shapes.get(which, (nS, nE, K))
         out = np.zeros(shape)
         if self._lib.or_get_array(self._h, which, _d(out)) != 0:

@@ -201,6 +201,13 @@ struct or_solver {
   double *faceUC; /* [nFaces][nFptsFace][nFields] LDG common solution */
   int dtType;      /* 0 fixed, 1 global CFL dt, 2 local (per-element) CFL dt */
   double *dtEle;   /* [nEles] ele::dt for dtType 2 */
+  /* CFL time stepping: ele::waveSp_fpts as [fpt][ele] (written by
+   * calcWaveSpFpts and by the faces' Rusanov / Lax-Friedrichs fluxes at the
+   * left element's points), the CFL number, and whether the next update is
+   * the first (params->iter == initIter + 1, solver.cpp:294) */
+  double *waveSp;
+  double CFL;
+  int dtFirst;
 };
 
 #define IDX3(a, b, c, B, C) ((c) + (C) * ((b) + (B) * (a)))
@@ -727,6 +734,7 @@ void or_destroy(or_solver *s) {
   for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; i++) free(ptrs[i]);
   for (int r = 0; r < 4; r++) free(s->divF[r]);
   free(s->dtEle);
+  free(s->waveSp);
   int *iptrs[] = {s->fL, s->flocL, s->fR, s->flocR, s->frot, s->fbc, s->fptR};
   for (size_t i = 0; i < sizeof iptrs / sizeof iptrs[0]; i++) free(iptrs[i]);
   free(s);
@@ -792,6 +800,7 @@ int or_get_array(const or_solver *s, int which, double *out) {
   case 6: src = s->dU_fpts; n = D * nF * nE * nK; break;
   case 7: src = s->dUc_fpts; n = nF * nE * nK; break;
   case 8: src = s->U0; n = nS * nE * nK; break;
+  case 9: src = s->waveSp; n = nF * nE; break;
   default:
     if (which >= 10 && which < 10 + s->nRK) { src = s->divF[which - 10]; n = nS * nE * nK; }
   }
@@ -910,7 +919,8 @@ static void viscous_flux_ad(const or_params *p, const double gradU[3][5], double
 /* common fluxes, src/face.cpp                                               */
 /* ------------------------------------------------------------------------ */
 /* rusanovFlux, face.cpp:289-356 (static mesh) */
-static void rusanov(const or_params *p, const double *UL, const double *UR, const double *n, double *Fn) {
+static void rusanov(const or_params *p, const double *UL, const double *UR, const double *n, double *Fn,
+                    double *waveSp) {
   double FL[3][5], FR[3][5];
   int D = p->nDims, K = D + 2;
   inviscid_flux(p, UL, FL);
@@ -939,6 +949,8 @@ static void rusanov(const or_params *p, const double *UL, const double *UR, cons
   double eigL = fabs(vnL) + sqrt(csqL), eigR = fabs(vnR) + sqrt(csqR);
   double eig = fmax(eigL, eigR);
   for (int i = 0; i < K; i++) Fn[i] = 0.5 * (FnL[i] + FnR[i] - eig * (UR[i] - UL[i]));
+  /* wave speed for calculation of allowable dt (face.cpp:354) */
+  if (waveSp) *waveSp = fmax(eigL, eigR);
 }
 
 /* roeFlux, face.cpp:358-484 (2D; entropy fix 436-446) */
@@ -992,11 +1004,13 @@ static void roe(const or_params *p, const double *UL, const double *UR, const do
 }
 
 /* laxFriedrichsFlux (advection), face.cpp:486-514 */
-static void lax_friedrichs(const or_params *p, const double *UL, const double *UR, const double *n, double *Fn) {
+static void lax_friedrichs(const or_params *p, const double *UL, const double *UR, const double *n, double *Fn,
+                           double *waveSp) {
   double uAvg = 0.5 * (UL[0] + UR[0]), uDiff = UL[0] - UR[0];
   double vNorm = p->advectVx * n[0] + p->advectVy * n[1];
   if (p->nDims == 3) vNorm += p->advectVz * n[2];
   Fn[0] = vNorm * uAvg + 0.5 * p->lambda * fabs(vNorm) * uDiff;
+  if (waveSp) *waveSp = fabs(vNorm); /* face.cpp:504-508 (static mesh) */
 }
 
 /* boundFace::applyBCs, boundFace.cpp:33-277 (static mesh, vG = 0) */
@@ -1150,8 +1164,11 @@ static void inviscid_faces(or_solver *s) {
       double dAL = s->dA_fpts[(size_t)fptL * nE + eL];
       if (isBnd) apply_bc(p, bc, UL, n, UR);
       else for (int k = 0; k < K; k++) UR[k] = UF(s, fptR, eR, k);
-      if (p->equation == OR_ADVECTION_DIFFUSION) lax_friedrichs(p, UL, UR, n, Fn);
-      else if (isBnd || p->riemannType == 0) rusanov(p, UL, UR, n, Fn);
+      /* the left element's wave speed at this point (face::getPointers,
+       * face.cpp:74-80); roeFlux stores none */
+      double *ws = s->waveSp ? s->waveSp + (size_t)fptL * nE + eL : NULL;
+      if (p->equation == OR_ADVECTION_DIFFUSION) lax_friedrichs(p, UL, UR, n, Fn, ws);
+      else if (isBnd || p->riemannType == 0) rusanov(p, UL, UR, n, Fn, ws);
       else roe(p, UL, UR, n, Fn);
       double *keep = s->faceFn + ((size_t)f * nfp + i) * K;
       for (int k = 0; k < K; k++) {
@@ -1406,26 +1423,13 @@ static double cfl_limit(int order) {
   return (order >= 0 && order <= 5) ? lim[order] : -1.0;
 }
 
-/* CFL time step from the current state: extrapolateU, ele::calcWaveSpFpts
- * (ele.cpp:1031-1084, static grids), ele::calcDt (1086-1105) and
- * solver::calcDt (solver.cpp:293-305).  dtType 1 sets params->dt to the
- * minimum over elements; dtType 2 also keeps each element's dt for the
- * stage updates; 0 returns to the fixed dt given.  Returns the minimum. */
-double or_calc_dt(or_solver *s, int dtType, double CFL) {
+/* ele::calcWaveSpFpts (ele.cpp:1031-1084, static grids): the wave speed at
+ * every flux point from the traces U_fpts, |v.n| + c (Euler) or |a.n|
+ * (advection) with the element's own normal. */
+static void calc_wave_sp_fpts(or_solver *s) {
   const or_params *p = &s->p;
-  int D = s->nDims, K = s->nFields, nS = s->nSpts, nF = s->nFpts, nE = s->nEles, nfp = s->nFptsFace, P1 = s->P1;
-  oracle_dgemm(nF, nE * K, nS, 1.0, s->opp_spts_to_fpts, s->U_spts, 0.0, s->U_fpts);
-  double wS[64];
-  for (int m = 0; m < nfp; m++) {
-    const double *w1 = GL_WTS[s->order];
-    wS[m] = (D == 3) ? w1[m / P1] * w1[m % P1] : w1[m];
-  }
-  if (!s->dtEle) s->dtEle = xcalloc(nE, 8);
-  double dtMin = INFINITY;
-  for (int e = 0; e < nE; e++) {
-    double vol = 0;
-    for (int spt = 0; spt < nS; spt++) vol += s->detJac_spts[(size_t)spt * nE + e] * s->wts[spt];
-    double intWave = 0;
+  int D = s->nDims, K = s->nFields, nF = s->nFpts, nE = s->nEles;
+  for (int e = 0; e < nE; e++)
     for (int fpt = 0; fpt < nF; fpt++) {
       const double *n = s->norm_fpts + ((size_t)fpt * nE + e) * D;
       double ws;
@@ -1443,14 +1447,63 @@ double or_calc_dt(or_solver *s, int dtType, double CFL) {
         double csq = fmax(p->gamma * pr / rho, 0.0);
         ws = fabs(vN) + sqrt(csq);
       }
-      intWave += ws * wS[fpt % nfp] * s->dA_fpts[(size_t)fpt * nE + e];
+      s->waveSp[(size_t)fpt * nE + e] = ws;
     }
-    s->dtEle[e] = CFL * cfl_limit(s->order) * 2 * vol / intWave;
+}
+
+/* ele::calcDt (ele.cpp:1086-1105) from waveSp, then solver::calcDt's minimum
+ * (solver.cpp:298-305): dtType 1 sets params->dt to it, dtType 2 also keeps
+ * each element's dt for the stage updates.  Returns the minimum. */
+static double calc_dt_ws(or_solver *s) {
+  int D = s->nDims, nS = s->nSpts, nF = s->nFpts, nE = s->nEles, nfp = s->nFptsFace, P1 = s->P1;
+  double wS[64];
+  for (int m = 0; m < nfp; m++) {
+    const double *w1 = GL_WTS[s->order];
+    wS[m] = (D == 3) ? w1[m / P1] * w1[m % P1] : w1[m];
+  }
+  if (!s->dtEle) s->dtEle = xcalloc(nE, 8);
+  double dtMin = INFINITY;
+  for (int e = 0; e < nE; e++) {
+    double vol = 0;
+    for (int spt = 0; spt < nS; spt++) vol += s->detJac_spts[(size_t)spt * nE + e] * s->wts[spt];
+    double intWave = 0;
+    for (int fpt = 0; fpt < nF; fpt++)
+      intWave += s->waveSp[(size_t)fpt * nE + e] * wS[fpt % nfp] * s->dA_fpts[(size_t)fpt * nE + e];
+    s->dtEle[e] = s->CFL * cfl_limit(s->order) * 2 * vol / intWave;
     dtMin = fmin(dtMin, s->dtEle[e]);
   }
-  s->dtType = dtType;
-  if (dtType != 0) s->p.dt = dtMin;
+  if (s->dtType != 0) s->p.dt = dtMin;
   return dtMin;
+}
+
+/* solver::calcDt (solver.cpp:293-305): on the first iteration the wave
+ * speeds come from the traces of the current state (calcWaveSpFpts; there
+ * U_fpts still holds initializeSolution's extrapolation of the initial state,
+ * solver.cpp:1497-1503), afterwards from what the face fluxes of the last
+ * residual stored. */
+static double calc_dt(or_solver *s) {
+  if (s->dtFirst) {
+    oracle_dgemm(s->nFpts, s->nEles * s->nFields, s->nSpts, 1.0, s->opp_spts_to_fpts, s->U_spts, 0.0, s->U_fpts);
+    calc_wave_sp_fpts(s);
+    s->dtFirst = 0;
+  }
+  return calc_dt_ws(s);
+}
+
+/* CFL time stepping from the current state (solver::initializeSolution's
+ * extrapolateU + calcWaveSpFpts, solver.cpp:1497-1503, and the dt the first
+ * update computes): dtType 1 global, 2 local; returns the minimum.  The next
+ * update is the first: it forms the wave speeds from the state again. */
+double or_calc_dt(or_solver *s, int dtType, double CFL) {
+  int K = s->nFields, nS = s->nSpts, nF = s->nFpts, nE = s->nEles;
+  oracle_dgemm(nF, nE * K, nS, 1.0, s->opp_spts_to_fpts, s->U_spts, 0.0, s->U_fpts);
+  if (!s->waveSp) s->waveSp = xcalloc((size_t)nF * nE, 8);
+  s->dtType = dtType;
+  s->CFL = CFL;
+  calc_wave_sp_fpts(s);
+  double m = calc_dt_ws(s);
+  s->dtFirst = 1;
+  return m;
 }
 void or_copy_u_u0(or_solver *s) { memcpy(s->U0, s->U_spts, (size_t)s->nSpts * s->nEles * s->nFields * 8); }
 void or_copy_u0_u(or_solver *s) { memcpy(s->U_spts, s->U0, (size_t)s->nSpts * s->nEles * s->nFields * 8); }
@@ -1458,6 +1511,7 @@ void or_copy_u0_u(or_solver *s) { memcpy(s->U_spts, s->U0, (size_t)s->nSpts * s-
 void or_update(or_solver *s) {
   int nRK = s->nRK;
   for (int step = 0; step < nRK - 1; step++) {
+    if (step == 0 && s->dtType != 0) calc_dt(s);
     if (step == 0) or_copy_u_u0(s);
     or_calc_residual(s, step);
     or_time_step_a(s, step, s->RKa[step + 1]);
@@ -1465,6 +1519,7 @@ void or_update(or_solver *s) {
   or_calc_residual(s, nRK - 1);
   if (s->p.timeType < 5) {
     if (nRK > 1) or_copy_u0_u(s);
+    else if (s->dtType != 0) calc_dt(s);
     for (int step = 0; step < nRK; step++) or_time_step_b(s, step);
   } else {
     or_time_step_a(s, nRK - 1, s->RKa[nRK]);
@@ -1504,9 +1559,9 @@ void or_point_inviscid_flux(const or_params *p, const double *U, double *F /* [3
 /* kind: 0 Rusanov, 1 Roe, 2 Lax-Friedrichs (advection) */
 void or_point_common_flux(const or_params *p, int kind, const double *UL, const double *UR, const double *n,
                           double *Fn) {
-  if (kind == 0) rusanov(p, UL, UR, n, Fn);
+  if (kind == 0) rusanov(p, UL, UR, n, Fn, NULL);
   else if (kind == 1) roe(p, UL, UR, n, Fn);
-  else lax_friedrichs(p, UL, UR, n, Fn);
+  else lax_friedrichs(p, UL, UR, n, Fn, NULL);
 }
 void or_point_boundary_state(const or_params *p, int bc, const double *UL, const double *n, double *UR) {
   apply_bc(p, bc, UL, n, UR);

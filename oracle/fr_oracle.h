@@ -88,7 +88,8 @@ int or_get_faces(const or_solver *s, int *out6, int *fptR);
 void or_set_state(or_solver *s, const double *U);
 void or_get_state(const or_solver *s, double *U);
 /* arrays: 0 U_spts, 1 U_fpts, 2 F_spts, 3 Fn_fpts, 4 disFn_fpts,
- * 5 dU_spts, 6 dU_fpts, 7 dUc_fpts, 8 U0, 10+s divF_spts[s] */
+ * 5 dU_spts, 6 dU_fpts, 7 dUc_fpts, 8 U0, 9 waveSp_fpts [fpt][ele] (after
+ * or_calc_dt), 10+s divF_spts[s] */
 int or_get_array(const or_solver *s, int which, double *out);
 
 void or_calc_residual(or_solver *s, int step);

@@ -123,3 +123,52 @@ def test_batch_fp32():
     solo, batched, _, _ = _run(members, 3, precision="fp32")
     for a, b in zip(solo, batched):
         assert rel_err(b.get_state(), a.get_state()).max() <= 1e-7
+
+
+def test_batch_follows_member_settings():
+    """Settings changed on a member between batch calls -- a fixed dt, a CFL
+    time step (global or local, recomputed every step), the geometry layout
+    -- are what the next batched steps use: the batch equals the same
+    members advanced alone with the same changes."""
+    rng = np.random.default_rng(5)
+    members = _quad_members(rng, 6)
+    solo, batched, _, B = _run(members, 2)
+    for i, (a, b) in enumerate(zip(solo, batched)):
+        assert rel_err(b.get_state(), a.get_state()).max() <= 1e-14
+    for s in (solo, batched):
+        s[0].set_dt(5e-4)
+        s[1].geometry_mode(0)
+        s[2].calc_dt(2, 0.3)
+        s[3].calc_dt(1, 0.3)
+    for a in solo:
+        a.update(3)
+    B.update(3)
+    for i, (a, b) in enumerate(zip(solo, batched)):
+        err = rel_err(b.get_state(), a.get_state()).max()
+        print(i, "err", err)
+        assert err <= 1e-14
+        assert np.abs(b.dt() - a.dt()).max() <= 1e-14 * a.dt().max()
+    # back to fixed time steps
+    for s in (solo, batched):
+        s[2].set_dt(1e-3)
+        s[3].set_dt(1e-3)
+        s[1].geometry_mode(1)
+    for a in solo:
+        a.update(2)
+    B.update(2)
+    for a, b in zip(solo, batched):
+        assert rel_err(b.get_state(), a.get_state()).max() <= 1e-14
+
+
+def test_batch_with_destroyed_member_refuses_to_step():
+    from fr_learn_amd._lib import FrError
+    rng = np.random.default_rng(6)
+    members = _quad_members(rng, 3)
+    _, batched, _, B = _run(members, 1)
+    batched[1].close()
+    with pytest.raises(FrError) as ei:
+        B.update(1)
+    assert ei.value.code == -1
+    B.close()
+    batched[0].close()
+    batched[2].close()

@@ -49,13 +49,11 @@ def test_validation_errors():
 
 
 def test_device_error_without_gpu():
-    """Valid input reaches the device stage; without a GPU that is FR_EHIP."""
-    try:
-        import torch
-        if torch.cuda.device_count() > 0:
-            pytest.skip("a GPU is present")
-    except ImportError:
-        pass
+    """Valid input reaches the device stage; without a GPU that is FR_EHIP.
+    The GPU is detected by the kernel driver's device node, not by a HIP
+    runtime of another library in this process."""
+    if os.path.exists("/dev/kfd"):
+        pytest.skip("a GPU driver is present")
     m = mesh.box(3, (3, 3, 3), (0, 0, 0), (2 * np.pi,) * 3)
     _expect(-4, cases.config(nDims=3), m)
 

@@ -712,3 +712,108 @@ def test_residual_norms_quadrature():
     assert np.allclose(o.norm_residual(2), ref2, rtol=1e-13)
     assert np.allclose(o.norm_residual(1), ref1, rtol=1e-13)
     assert np.allclose(o.norm_residual(3), refi, rtol=1e-13)
+
+
+def _wave_speeds_after_residual(o, cfg, ws_before):
+    """The reference's waveSp_fpts after one residual evaluation, restated in
+    numpy from its traces: at the left element's point of every face the
+    Rusanov flux stores max(|vL.n| + cL, |vR.n| + cR) with the left normal
+    (face.cpp:74-80, 354; boundary faces always use Rusanov, face.cpp:140-150),
+    Lax-Friedrichs |a.n| (face.cpp:504-508), an interior Roe face nothing;
+    the right element's points keep their earlier values."""
+    D, g = o.nDims, cfg["gamma"]
+    Uf, nrm = o.array(1), o.geometry(2)
+    faces, fptR = o.faces()
+    nfp = o.nFpts // (2 * D)
+    pt = Point(cfg)
+    ws = ws_before.copy()
+
+    def speed(U, n):
+        rho, v = U[0], U[1:D + 1] / U[0]
+        p = (g - 1) * (U[D + 1] - 0.5 * rho * (v @ v))
+        return abs(v @ n) + np.sqrt(max(g * p / rho, 0.0))
+
+    for f, (eL, fL, eR, fR, rot, bc) in enumerate(faces):
+        for i in range(nfp):
+            qL = fL * nfp + i
+            n = nrm[qL, eL]
+            if cfg["equation"] == 0:
+                ws[qL, eL] = abs(np.dot([cfg["advectVx"], cfg["advectVy"], cfg["advectVz"]][:D], n))
+                continue
+            UL = Uf[qL, eL]
+            if bc >= 0:
+                UR = pt.boundary_state(bc, UL, n)
+            elif cfg["riemannType"] == 1:
+                continue
+            else:
+                UR = Uf[fptR[f, i], eR]
+            ws[qL, eL] = max(speed(UL, n), speed(UR, n))
+    return ws
+
+
+def _dt_from_wave_speeds(o, cfg, ws, CFL):
+    """ele::calcDt (ele.cpp:1086-1105) per element from given wave speeds."""
+    D, P = o.nDims, cfg["order"]
+    x, w = ind.gauss_legendre(P)
+    wsurf = np.outer(w, w).T.ravel() if D == 3 else w  # point a + (P+1) b of a face
+    vol = (o.operator(8)[:, None] * o.geometry(1)).sum(axis=0)
+    nfp = o.nFpts // (2 * D)
+    intw = (ws * np.tile(wsurf, 2 * D)[:, None] * o.geometry(3)).sum(axis=0)
+    return CFL * [1.393, 0.464, 0.235, 0.1453, 0.100, 0.0736][P] * 2 * vol / intw
+
+
+@pytest.mark.parametrize("case", ["hex_rusanov", "quad_roe_bc", "advection", "euler_fe"])
+def test_cfl_time_step_per_update(case):
+    """solver::update recomputes the CFL time step at every step
+    (solver.cpp:218-219, 236-240): the first update from the state's wave
+    speeds (calcWaveSpFpts), later ones from the wave speeds the face fluxes
+    of the last residual stored (solver::calcDt, solver.cpp:293-305).  The
+    oracle's dt for the second and third updates against that rule restated
+    in numpy from the oracle's traces."""
+    CFL = 0.3
+    if case == "hex_rusanov":
+        cfg, m = euler_cfg(3, 2, timeType=4), hex_box(3, perturb=0.2, seed=3)
+    elif case == "quad_roe_bc":
+        cfg, m = euler_cfg(2, 3, riemannType=1, timeType=4), quad_box(4, periodic=False, bc=QUAD_BCS, seed=2)
+    elif case == "advection":
+        cfg, m = advection_cfg(3, 2, timeType=4), hex_box(3, perturb=0.2, seed=3)
+    else:
+        cfg, m = euler_cfg(3, 2, timeType=0), hex_box(3, perturb=0.2, seed=3)
+    o = Oracle(cfg, m)
+    if cfg["equation"] == 0:
+        U = perturbed_state(o.initialize(cases.ic_gaussian), seed=1, amp=2e-2)
+    elif o.nDims == 3:
+        U = perturbed_state(o.initialize(cases.ic_tgv), seed=1, amp=5e-2)
+    else:
+        U = perturbed_state(o.initialize(lambda x, c: cases.ic_vortex(x, c, beta=2.0, mach=0.4)), seed=1, amp=2e-2)
+    o.set_state(U)
+    for dtType in (1, 2):
+        o.set_state(U)
+        dt0 = o.calc_dt(dtType, CFL)
+        ws = o.array(9)
+        for step in range(3):
+            if case == "euler_fe" and step > 0:
+                # forward Euler takes its dt after the step's residual
+                o.calc_residual(0)
+                ws = _wave_speeds_after_residual(o, cfg, ws)
+                want = _dt_from_wave_speeds(o, cfg, ws, CFL)
+            elif step > 0:
+                want = _dt_from_wave_speeds(o, cfg, ws, CFL)
+            else:
+                want = np.array([o.dt(e) for e in range(o.nEles)]) if dtType == 2 else None
+            o.update(1)
+            assert np.isfinite(o.get_state()).all()
+            got = np.array([o.dt(e if dtType == 2 else -1) for e in range(o.nEles)])
+            if step == 0:
+                assert abs(o.dt() - dt0) <= 1e-15 * dt0
+            else:
+                ref = want if dtType == 2 else np.full(o.nEles, want.min())
+                assert np.abs(got - ref).max() <= 1e-13 * ref.max(), (case, dtType, step)
+            if case != "euler_fe":
+                ws = _wave_speeds_after_residual(o, cfg, ws)
+                assert np.abs(o.array(9) - ws).max() <= 1e-13 * ws.max()
+            else:
+                ws = o.array(9)
+        if case != "advection":
+            # the time step follows the flow: it differs from the first one
+            assert abs(o.dt() - dt0) > 1e-9 * dt0
