@@ -44,13 +44,17 @@ def tgv_setup(n, order=3, dt=2e-4):
     return cfg, m
 
 
-def run_gpu(cfg, m, steps, warmup, precision, compact=False):
-    """The TGV workload's RK4 steps.  The stage kernels read per-point metrics
-    and face normals (general hex elements) unless compact=True, which lets
-    them use the per-element / per-face geometry of the affine mesh."""
+GEOMETRY = {"point": 0, "compact": 1, "nodal": 2, "nodal-spt": 3}
+
+
+def run_gpu(cfg, m, steps, warmup, precision, geometry="point"):
+    """The TGV workload's RK4 steps with the stage kernels' geometry layout
+    `geometry` (GEOMETRY): per-point metrics and face normals read from HBM,
+    the affine mesh's per-element / per-face values, or the metrics and
+    normals formed in the kernel from each element's corner nodes."""
     from fr_learn_amd import Solver, cases
     s = Solver(cfg, m, precision=precision)
-    assert s.geometry_mode(1 if compact else 0) == compact
+    assert s.geometry_mode(GEOMETRY[geometry]) == GEOMETRY[geometry]
     s.initialize(cases.ic_tgv)
     s.update(warmup)
     s.synchronize()
@@ -380,7 +384,7 @@ def main():
             "kernels_ms": {"stage": r32["res_ms"]},
         }
     if not args.no_compact and main_prec == "fp64":
-        rc = run_gpu(cfg, m, args.steps, args.warmup, "fp64", compact=True)
+        rc = run_gpu(cfg, m, args.steps, args.warmup, "fp64", geometry="compact")
         vc = dof * 4 * args.steps / rc["wall"] / 1e9
         line["compact_geometry"] = {
             "note": "not the measured path: the same FP64 workload with one metric set per element and one "

@@ -49,6 +49,17 @@ class FrMesh(ctypes.Structure):
     ]
 
 
+class FrErrorParams(ctypes.Structure):
+    """Mirror of fr_error_params (include/fr.h)."""
+    _fields_ = [
+        ("testCase", ctypes.c_int), ("errorNorm", ctypes.c_int), ("quadOrder", ctypes.c_int),
+        ("icType", ctypes.c_int),
+        ("time", ctypes.c_double), ("rhoIC", ctypes.c_double), ("pIC", ctypes.c_double),
+        ("vortexAngle", ctypes.c_double), ("xmin", ctypes.c_double), ("xmax", ctypes.c_double),
+        ("ymin", ctypes.c_double), ("ymax", ctypes.c_double),
+    ]
+
+
 # every symbol include/fr.h declares, with (restype, argtypes)
 _P = ctypes.c_void_p
 _DP = ctypes.POINTER(ctypes.c_double)
@@ -88,6 +99,7 @@ SIGNATURES = {
     "fr_set_stream": (ctypes.c_int, [_P, _P]),
     "fr_synchronize": (ctypes.c_int, [_P]),
     "fr_geometry_mode": (ctypes.c_int, [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
+    "fr_integrate_error": (ctypes.c_int, [_P, ctypes.POINTER(FrErrorParams), _DP]),
     "fr_write_data": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_double]),
     "fr_read_restart": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.c_int, _DP]),
     "fr_write_checkpoint": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.c_int, ctypes.c_double]),

@@ -14,9 +14,10 @@ LIB = os.path.join(HERE, "libfrhip.so")
 
 # translation units, compiled in parallel (the stage kernels per precision,
 # the adjoint and the batch kernels each in their own)
-SOURCES = ["fr_adjoint_v3.hip", "fr_adjoint_v2.hip", "fr_stage64.hip", "fr_stage32.hip", "fr_adjoint.hip",
+SOURCES = ["fr_stage64_3d.hip", "fr_stage32_3d.hip", "fr_adjoint_v3.hip", "fr_adjoint_v2.hip", "fr_stage64.hip",
+           "fr_stage32.hip", "fr_adjoint.hip",
            "fr_batch.hip", "fr_adjoint_v3ad.hip", "fr_api.hip", "fr_host.cpp", "fr_output.cpp", "fr_tri_ops.cpp",
-           "fr_tri.hip"]
+           "fr_tri.hip", "fr_error.hip"]
 HEADERS = ["fr_ctx.hpp", "fr_host.hpp", "fr_kernels.hpp", "fr_layout.hpp", "fr_physics.hpp", "fr_adjoint.hpp",
            "fr_adjoint_vis.hpp", "fr_dual.hpp", "fr_batch.hpp", "fr_stage_impl.hpp", "fr_tri.hpp"]
 

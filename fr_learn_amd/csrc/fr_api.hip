@@ -25,7 +25,7 @@ void freeAll(fr_ctx *c) {
                   c->adjLam,  c->adjOut,  c->adjNu,   c->adjW[0], c->adjW[1], c->adjS[0], c->adjS[1],
                   c->adjS[2], c->adjS[3], c->adjQf,   c->adjB,    c->JGc64,  c->detJc, c->fgeoc64,
                   c->JGc32,   c->fgeoc32, c->adjUo,   c->adjQo,   c->adjCj,   c->Fv,   c->waveSp,
-                  c->dtNum,   c->wsurf,   c->dtPart};
+                  c->dtNum,   c->wsurf,   c->dtPart,  c->geoN64,  c->geoN32,  c->errTab, c->Xd};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   for (auto &v : c->ev)
@@ -78,8 +78,11 @@ frdev::StageArgs<T> makeStageArgs(fr_ctx *c, int in, int cur, int mode, int step
   a.Uin = stateIn<T>(c, in);
   a.UfIn = (const T *)c->Uf[cur];
   a.UfOut = (T *)c->Uf[cur ^ 1];
-  const bool aff = c->affineMesh && c->affineUse;
+  const int gm = c->geoActive();
+  const bool aff = gm == 1;
   a.affine = aff ? 1 : 0;
+  a.nodal = gm == 2 ? 1 : gm == 3 ? 2 : 0;
+  a.geoN = f32 ? (const T *)c->geoN32 : (const T *)c->geoN64;
   if (aff) {
     a.JG = f32 ? (const T *)c->JGc32 : (const T *)c->JGc64;
     a.fgeo = f32 ? (const T *)c->fgeoc32 : (const T *)c->fgeoc64;
@@ -654,6 +657,39 @@ int createImpl(const fr_config *cfg, const fr_mesh *m, int precision, fr_ctx *c)
       HIP_TRY(hipGetLastError());
     }
   }
+  // nodal geometry of linear quads and hexes: the coefficients of each
+  // element's map (frdev::LinearMap) from its corner nodes, as sums of node
+  // differences (node 0 subtracted first) so that they keep the precision of
+  // the element's size
+  if (!c->tri && ((D == 3 && nN == 8) || (D == 2 && nN == 4))) {
+    const int NC = D == 3 ? frdev::LinearMap<3>::NC : frdev::LinearMap<2>::NC;
+    static const int SI[8] = {-1, 1, 1, -1, -1, 1, 1, -1}, SJ[8] = {-1, -1, 1, 1, -1, -1, 1, 1},
+                     SK[8] = {-1, -1, -1, -1, 1, 1, 1, 1};
+    std::vector<double> cf((size_t)nE * NC, 0.0);
+    for (int de = 0; de < nE; de++) {
+      const double *Xe = &c->X[(size_t)de * nN * 3];
+      double *o = &cf[(size_t)de * NC];
+      for (int n = 1; n < nN; n++)
+        for (int d = 0; d < D; d++) {
+          const double y = Xe[n * 3 + d] - Xe[d];
+          if (D == 3) {
+            const int s[7] = {SI[n], SJ[n], SK[n], SI[n] * SJ[n], SI[n] * SK[n], SJ[n] * SK[n], SI[n] * SJ[n] * SK[n]};
+            for (int k = 0; k < 7; k++) o[3 * k + d] += s[k] * y;
+          } else {
+            const int s[3] = {SI[n], SJ[n], SI[n] * SJ[n]};
+            for (int k = 0; k < 3; k++) o[2 * k + d] += s[k] * y;
+          }
+        }
+      for (int k = 0; k < NC; k++) o[k] *= (D == 3) ? 0.125 : 0.25;
+    }
+    if ((rc = dmalloc(&c->geoN64, cf.size()))) return rc;
+    HIP_TRY(hipMemcpy(c->geoN64, cf.data(), cf.size() * 8, hipMemcpyHostToDevice));
+    if (c->prec == FR_PREC_FP32) {
+      std::vector<float> cf32(cf.begin(), cf.end());
+      if ((rc = dmalloc(&c->geoN32, cf32.size()))) return rc;
+      HIP_TRY(hipMemcpy(c->geoN32, cf32.data(), cf32.size() * 4, hipMemcpyHostToDevice));
+    }
+  }
   HIP_TRY(hipMemcpy(c->wts, c->ops.wts.data(), NS * 8, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(c->oned64, c->ops.oned.data(), c->ops.oned.size() * 8, hipMemcpyHostToDevice));
   if (c->prec == FR_PREC_FP32) {
@@ -683,6 +719,10 @@ int cflBuffers(fr_ctx *c) {
   return 0;
 }
 
+}  // namespace
+
+namespace frimpl {
+
 int needPart(fr_ctx *c, size_t n) {
   if (c->partCap >= n) return 0;
   if (c->part) (void)hipFree(c->part);
@@ -692,10 +732,6 @@ int needPart(fr_ctx *c, size_t n) {
   if (!rc) c->partCap = n;
   return rc;
 }
-
-}  // namespace
-
-namespace frimpl {
 
 int dtPrepare(fr_ctx *c, hipStream_t st) {
   if (c->dtType == 0 || c->dtReady || c->dtFrozen) return 0;
@@ -1415,12 +1451,12 @@ int fr_synchronize(fr_ctx *c) {
 }
 
 int fr_geometry_mode(fr_ctx *c, int request, int *active) {
-  if (!c || request < -1 || request > 1) return fail(FR_EINVAL, "bad argument");
-  if (request >= 0 && c->affineUse != (request == 1)) {
-    c->affineUse = request == 1;
+  if (!c || request < -1 || request > 3) return fail(FR_EINVAL, "bad argument");
+  if (request >= 0 && c->geoReq != request) {
+    c->geoReq = request;
     c->gen++;
   }
-  if (active) *active = (c->affineMesh && c->affineUse) ? 1 : 0;
+  if (active) *active = c->geoActive();
   return 0;
 }
 

@@ -71,7 +71,16 @@ struct fr_ctx {
   double *JGc64 = nullptr, *detJc = nullptr, *fgeoc64 = nullptr;
   float *JGc32 = nullptr, *fgeoc32 = nullptr;
   bool affineMesh = false;  // every element affine (compact arrays built)
-  bool affineUse = true;    // the stage kernels read the compact arrays when affineMesh
+  // nodal geometry of linear quads / hexes (frdev::LinearMap): the map
+  // coefficients per element, from which the stage kernel forms its metrics
+  double *geoN64 = nullptr;
+  float *geoN32 = nullptr;
+  // requested stage-kernel geometry (fr_geometry_mode): 0 per point,
+  // 1 compact when affine, 2 nodal, 3 nodal metrics with stored face geometry
+  int geoReq = 1;
+  // error norms (fr_integrate_error): point tables and the element nodes
+  double *errTab = nullptr, *Xd = nullptr;
+  size_t errTabCap = 0;
   void *Us = nullptr;                 // RK stage state (residual precision)
   void *Uf[2] = {nullptr, nullptr};   // face traces, double-buffered
   int cur = 0;                        // Uf[cur] holds the traces of `traceOf`
@@ -113,6 +122,13 @@ struct fr_ctx {
   long cnt[4] = {0, 0, 0, 0};
 
   size_t nState() const { return (size_t)nE * NF * NS; }
+  // the geometry layout the stage kernels use (the values of geoReq): the
+  // request where the context supports it, else the per-point arrays
+  int geoActive() const {
+    if (geoReq == 1 && affineMesh) return 1;
+    if ((geoReq == 2 || geoReq == 3) && geoN64 && !cfg.viscous && !tri) return geoReq;
+    return 0;
+  }
   size_t tsize() const { return prec == FR_PREC_FP32 ? 4 : 8; }
 };
 
@@ -200,6 +216,9 @@ template <typename T>
 int launchTriTrace(fr_ctx *c, const T *U, T *Uf, const T *tab);
 // CFL time steps of the triangles into c->dtE (k_tri_dt), on stream st
 int launchTriDt(fr_ctx *c, const double *wsurf, double factor, hipStream_t st);
+// c->part holds at least n doubles (the block partial sums of the
+// reductions), allocated on first use (fr_api.hip).
+int needPart(fr_ctx *c, size_t n);
 // The time step of the next update into dtG / dtE on stream st (fr_api.hip):
 // from the state on the first update after fr_calc_dt (and always for
 // triangles), else from the wave speeds the last residual stored.

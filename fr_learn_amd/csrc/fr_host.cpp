@@ -32,7 +32,7 @@ namespace {
 // 1D abscissae / weights.  The values are the reference's 15-digit tables
 // (src/points.cpp:169-253 Gauss-Legendre, 254-330 Gauss-Lobatto, 376-469
 // weights), kept digit for digit because operator parity depends on them.
-const double kGL[7][7] = {
+const double kGL[11][11] = {
     {0.0},
     {-0.577350269189626, 0.577350269189626},
     {-0.774596669241483, 0.000000000000000, 0.774596669241483},
@@ -41,7 +41,16 @@ const double kGL[7][7] = {
     {-0.932469514203152, -0.661209386466264, -0.238619186083197, 0.238619186083197, 0.661209386466265,
      0.932469514203152},
     {-0.949107912342758, -0.741531185599395, -0.405845151377397, 0.000000000000000, 0.405845151377397,
-     0.741531185599394, 0.949107912342758}};
+     0.741531185599394, 0.949107912342758},
+    {-0.960289856497536, -0.796666477413627, -0.525532409916329, -0.183434642495650, 0.183434642495650,
+     0.525532409916329, 0.796666477413627, 0.960289856497536},
+    {-0.968160239507626, -0.836031107326636, -0.613371432700591, -0.324253423403809, 0.000000000000000,
+     0.324253423403809, 0.613371432700591, 0.836031107326636, 0.968160239507626},
+    {-0.973906528517172, -0.865063366688985, -0.679409568299024, -0.433395394129247, -0.148874338981631,
+     0.148874338981632, 0.433395394129247, 0.679409568299024, 0.865063366688984, 0.973906528517171},
+    {-0.978228658146057, -0.887062599768095, -0.730152005574049, -0.519096129206812, -0.269543155952345,
+     0.000000000000000, 0.269543155952345, 0.519096129206812, 0.730152005574049, 0.887062599768095,
+     0.978228658146057}};
 const double kGLL[7][7] = {
     {0.0},
     {-1.000000000000000, 1.000000000000000},
@@ -52,7 +61,7 @@ const double kGLL[7][7] = {
      1.000000000000000},
     {-1.000000000000000, -0.830223896278567, -0.468848793470714, 0.000000000000000, 0.468848793470714,
      0.830223896278567, 1.000000000000000}};
-const double kGW[7][7] = {
+const double kGW[11][11] = {
     {2.0},
     {1.0, 1.0},
     {0.5555555555555556, 0.8888888888888888, 0.5555555555555556},
@@ -61,7 +70,16 @@ const double kGW[7][7] = {
     {0.1713244923791704, 0.3607615730481386, 0.4679139345726910, 0.4679139345726910, 0.3607615730481386,
      0.1713244923791704},
     {0.1294849661688697, 0.2797053914892766, 0.3818300505051189, 0.4179591836734694, 0.3818300505051189,
-     0.2797053914892766, 0.1294849661688697}};
+     0.2797053914892766, 0.1294849661688697},
+    {0.1012285362903763, 0.2223810344533745, 0.3137066458778873, 0.3626837833783620, 0.3626837833783620,
+     0.3137066458778873, 0.2223810344533745, 0.1012285362903763},
+    {0.0812743883615744, 0.1806481606948574, 0.2606106964029354, 0.3123470770400029, 0.3302393550012598,
+     0.3123470770400029, 0.2606106964029354, 0.1806481606948574, 0.0812743883615744},
+    {0.0666713443086881, 0.1494513491505806, 0.2190863625159820, 0.2692667193099963, 0.2955242247147529,
+     0.2955242247147529, 0.2692667193099963, 0.2190863625159820, 0.1494513491505806, 0.0666713443086881},
+    {0.0556685671161737, 0.1255803694649046, 0.1862902109277343, 0.2331937645919905, 0.2628045445102467,
+     0.2729250867779006, 0.2628045445102467, 0.2331937645919905, 0.1862902109277343, 0.1255803694649046,
+     0.0556685671161737}};
 
 // Lagrange basis l_mode(y) on nodes x (product form of polynomials.cpp:9-19).
 double lag(const std::vector<double> &x, double y, int mode) {
@@ -386,9 +404,10 @@ std::string buildOperators(const fr_config &cfg, Operators &o) {
   // 1D factors for the kernels and a check that the kernels' index maps
   // (fr_layout.hpp) enumerate exactly the nonzeros of the dense operators
   {
-    const int L = 4 * P1 + P1 * P1;
+    const int L = 5 * P1 + P1 * P1;
     o.oned.assign(L, 0.0);
     for (int m = 0; m < P1; m++) {
+      o.oned[4 * P1 + P1 * P1 + m] = x[m];
       o.oned[m] = lag(x, -1.0, m);
       o.oned[P1 + m] = lag(x, 1.0, m);
       o.oned[2 * P1 + m] = -dvcjh(x[m], 0, P, eta);
@@ -432,10 +451,16 @@ std::string buildOperators(const fr_config &cfg, Operators &o) {
 }
 
 std::vector<double> pts1D(int sptsType, int order) {
-  if (order < 0 || order > 6 || (sptsType != 0 && sptsType != 1)) return {};
+  if (order < 0 || (sptsType != 0 && sptsType != 1)) return {};
+  if ((sptsType == 0 && order > 10) || (sptsType == 1 && order > 6)) return {};
   std::vector<double> x(order + 1);
   for (int i = 0; i <= order; i++) x[i] = (sptsType == 0) ? kGL[order][i] : kGLL[order][i];
   return x;
+}
+
+std::vector<double> gaussWeights1D(int order) {
+  if (order < 0 || order > 10) return {};
+  return std::vector<double>(kGW[order], kGW[order] + order + 1);
 }
 
 double lagrange(const std::vector<double> &x, double y, int mode) { return lag(x, y, mode); }

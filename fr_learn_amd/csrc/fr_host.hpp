@@ -63,8 +63,13 @@ std::vector<int> blockOrder(const Connectivity &conn, const std::vector<double> 
                             int epb);
 
 // 1D solution-point abscissae (getPts1D, points.cpp:162-330): sptsType 0
-// Gauss-Legendre, 1 Gauss-Lobatto; order 0..6.  Empty for other orders.
+// Gauss-Legendre (order 0..10), 1 Gauss-Lobatto (order 0..6).  Empty for
+// other orders.
 std::vector<double> pts1D(int sptsType, int order);
+
+// 1D Gauss-Legendre weights (getQptWeights1D, points.cpp:376-469), order
+// 0..10; empty for other orders.
+std::vector<double> gaussWeights1D(int order);
 
 // Lagrange basis l_mode(y) on the nodes x (polynomials.cpp:9-19).
 double lagrange(const std::vector<double> &x, double y, int mode);

@@ -122,6 +122,64 @@ struct GeoAddr {
   int aff;
 };
 
+// Nodal geometry of linear elements (8-node hexes, 4-node quads): the map
+// from reference to physical coordinates written in monomials of the
+// reference coordinates r,
+//   hex:  x(r) = a + b r0 + c r1 + d r2 + e r0 r1 + f r0 r2 + g r1 r2 + h r0 r1 r2
+//   quad: x(r) = a + b r0 + c r1 + e r0 r1
+// (the trilinear / bilinear shape functions of funcs.cpp:178-260 expanded).
+// geoN[ele][NC] holds b c d e f g h (hex) or b c e (quad), D components each.
+// They are combinations of node differences, so the only absolute position,
+// a, is not needed and the coefficients keep the precision of the element's
+// size.  The stage kernel forms the metrics at its solution points and the
+// normals at its flux points from them instead of reading per-point arrays.
+template <int D>
+struct LinearMap {
+  static constexpr int NC = (D == 3) ? 21 : 6;
+};
+
+// Jacobian J[dimRef][dimPhys] = dx_dimPhys / dr_dimRef of a linear element at
+// reference point r from its map coefficients cf (LinearMap).
+template <int D, typename T>
+__host__ __device__ __forceinline__ void linearJacobian(const T *cf, const T *r, T J[D][D]) {
+  if constexpr (D == 3) {
+    const T r01 = r[0] * r[1], r02 = r[0] * r[2], r12 = r[1] * r[2];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      J[0][c] = cf[c] + cf[9 + c] * r[1] + cf[12 + c] * r[2] + cf[18 + c] * r12;
+      J[1][c] = cf[3 + c] + cf[9 + c] * r[0] + cf[15 + c] * r[2] + cf[18 + c] * r02;
+      J[2][c] = cf[6 + c] + cf[12 + c] * r[0] + cf[15 + c] * r[1] + cf[18 + c] * r01;
+    }
+  } else {
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+      J[0][c] = cf[c] + cf[4 + c] * r[1];
+      J[1][c] = cf[2 + c] + cf[4 + c] * r[0];
+    }
+  }
+}
+
+// Adjugate G = det(J) J^-1, G[dimRef][dimPhys] (the reference's JGinv,
+// solver.cpp:1273-1281), and det(J) as the return value.
+template <int D, typename T>
+__host__ __device__ __forceinline__ T adjugate(const T J[D][D], T G[D][D]) {
+  if constexpr (D == 2) {
+    G[0][0] = J[1][1];
+    G[0][1] = -J[1][0];
+    G[1][0] = -J[0][1];
+    G[1][1] = J[0][0];
+    return J[0][0] * J[1][1] - J[0][1] * J[1][0];
+  } else {
+    const T xr = J[0][0], xs = J[1][0], xt = J[2][0];
+    const T yr = J[0][1], ys = J[1][1], yt = J[2][1];
+    const T zr = J[0][2], zs = J[1][2], zt = J[2][2];
+    G[0][0] = ys * zt - yt * zs; G[0][1] = xt * zs - xs * zt; G[0][2] = xs * yt - xt * ys;
+    G[1][0] = yt * zr - yr * zt; G[1][1] = xr * zt - xt * zr; G[1][2] = xt * yr - xr * yt;
+    G[2][0] = yr * zs - ys * zr; G[2][1] = xs * zr - xr * zs; G[2][2] = xr * ys - xs * yr;
+    return xr * (ys * zt - yt * zs) - xs * (yr * zt - yt * zr) + xt * (yr * zs - ys * zr);
+  }
+}
+
 template <typename T>
 struct StageArgs {
   const T *Uin;
@@ -151,6 +209,8 @@ struct StageArgs {
   int mode, nprev, nEles;
   Phys ph;
   int affine;  // geometry layout (GeoAddr)
+  int nodal;   // 1: the stage kernel forms its metrics and normals from geoN (GM = 1)
+  const T *__restrict__ geoN;  // linear map coefficients [ele][LinearMap<D>::NC]
 };
 
 // Face f of the two faces normal to reference axis d (hex: x 2,3; y 4,5;
@@ -456,9 +516,17 @@ struct StageSmem {
 // FIN = false serves modes 0 and 1 with the base in the residual precision.
 // The stage computation of element block b (the elements b*EPB ..), shared
 // by the per-mesh kernel and the batched persistent kernel.
-template <int D, int P, int NF, typename T, bool BND, bool VIS, bool FIN>
+// GM = 0 reads the metrics, detJ and face geometry from the per-point (or
+// compact) arrays; GM = 1 (inviscid linear elements, StageArgs::nodal) forms
+// them from the element's map coefficients geoN (LinearMap): the adjugate
+// metrics and detJ at each solution point, and at each flux point the outward
+// area-weighted normal, whose length is dA; a right-side point uses the
+// negated own normal for the left element's (equal but for rounding).
+// GM = 2 forms the solution-point metrics and reads the face geometry.
+template <int D, int P, int NF, typename T, bool BND, bool VIS, bool FIN, int GM = 0>
 __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, StageSmem<D, P, NF, T> &sm,
                                           const int tid) {
+  static_assert(GM == 0 || !VIS, "the viscous stage reads the per-point geometry");
   using G = ElemGeo<D, P>;
   using O = OneD<P>;
   constexpr int NS = G::NS, NFP = G::NFP, NFPF = G::NFPF, P1 = G::P1, TPE = G::TPE, EPB = G::EPB;
@@ -493,8 +561,21 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
   // compiler's wait counts are exact.
   constexpr int NQ = (NFP + TPE - 1) / TPE;  // flux points per thread
   static_assert(O::SIZE <= G::NTH, "one thread per 1D table entry");
-  const size_t ec = (size_t)(act ? e : a.nEles - 1);  // clamped element
-  const int tc = t < NS ? t : NS - 1;                 // clamped solution point
+  // clamped element (wave-uniform when an element fills whole wavefronts, so
+  // that its map coefficients come through scalar loads)
+  size_t ec = (size_t)(act ? e : a.nEles - 1);
+  if constexpr (TPE >= 64) ec = (size_t)__builtin_amdgcn_readfirstlane((int)ec);
+  const int tc = t < NS ? t : NS - 1;  // clamped solution point
+  int si, sj, sk;  // tensor index of this thread's solution point
+  if (D == 3) {
+    sk = tc / (P1 * P1);
+    sj = (tc - P1 * P1 * sk) / P1;
+    si = tc - P1 * sj - P1 * P1 * sk;
+  } else {
+    si = tc % P1;
+    sj = tc / P1;
+    sk = 0;
+  }
   int pfi[NQ], pnb[NQ];  // face record, neighbour element (-1: boundary)
 #pragma unroll
   for (int r = 0; r < NQ; r++) {
@@ -515,7 +596,22 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
   // stage base state, detJ and time step for the update (read in every mode:
   // in mode 0 the base is the input state, whose lines are already in cache)
   double pb[NF];
-  const double pdet = a.detJ[ga.det(ec)];
+  T pJ[D * D];  // adjugate metrics of this solution point
+  double pdet;
+  if constexpr (GM == 0) {
+    pdet = a.detJ[ga.det(ec)];
+  } else {
+    // metrics from the element's linear map at this solution point
+    const T *cf = a.geoN + ec * LinearMap<D>::NC;
+    const T r[3] = {a.oned[O::X1 + si], a.oned[O::X1 + sj], a.oned[O::X1 + sk]};
+    T Jm[D][D], Gm[D][D];
+    linearJacobian<D, T>(cf, r, Jm);
+    pdet = (double)adjugate<D, T>(Jm, Gm);
+#pragma unroll
+    for (int d = 0; d < D; d++)
+#pragma unroll
+      for (int d2 = 0; d2 < D; d2++) pJ[d * D + d2] = Gm[d][d2];
+  }
   const double pdt = a.dtE[ec * a.dtStride];
 #pragma unroll
   for (int k = 0; k < NF; k++)
@@ -535,8 +631,7 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
         for (int k = 0; k < NF; k++) pd[r][k] = __builtin_nontemporal_load(dp[r] + (ec * NF + k) * NS + tc);
       }
   }
-  T pJ[D * D];  // adjugate metrics of this solution point
-  {
+  if constexpr (GM == 0) {
     // components c at a uniform stride from c = 0 (one 64-bit add per load)
     const T *jg0 = a.JG + ga.jg(ec, 0);
     const size_t js = (size_t)ga.gS;
@@ -544,11 +639,13 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
     for (int c = 0; c < D * D; c++) pJ[c] = __builtin_nontemporal_load(jg0 + c * js);
   }
   T pn[NQ][D + 1], pu[NQ][NF];  // face normal + dA, neighbour trace
+  if constexpr (GM != 1) {
 #pragma unroll
-  for (int r = 0; r < NQ; r++) {
-    const int qc = min(t + r * TPE, NFP - 1);
+    for (int r = 0; r < NQ; r++) {
+      const int qc = min(t + r * TPE, NFP - 1);
 #pragma unroll
-    for (int c = 0; c <= D; c++) pn[r][c] = __builtin_nontemporal_load(a.fgeo + ga.fg(ec, c, qc));
+      for (int c = 0; c <= D; c++) pn[r][c] = __builtin_nontemporal_load(a.fgeo + ga.fg(ec, c, qc));
+    }
   }
 #pragma unroll
   for (int r = 0; r < NQ; r++) {
@@ -608,16 +705,6 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
   }
   __syncthreads();
 
-  int si, sj, sk;  // tensor index of this thread's solution point
-  if (D == 3) {
-    sk = t / (P1 * P1);
-    sj = (t - P1 * P1 * sk) / P1;
-    si = t - P1 * sj - P1 * P1 * sk;
-  } else {
-    si = t % P1;
-    sj = t / P1;
-    sk = 0;
-  }
   // (0) viscous: LDG jumps at the flux points (held in sJ until the first
   // barrier of (2), which every read of them precedes)
   if constexpr (VIS) {
@@ -745,8 +832,38 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
       const int fi = pfi[r];
       const bool bnd = BND && fi >= FINFO_BOUNDARY;
       const bool right = !bnd && ((fi >> 5) & 1);
-      const T *n = pn[r];
-      const T dA = pn[r][D];
+      T n[D], dA;
+      if constexpr (GM != 1) {
+#pragma unroll
+        for (int c = 0; c < D; c++) n[c] = pn[r][c];
+        dA = pn[r][D];
+      } else {
+        // reference coordinates of the point: -1 / +1 along the face's axis,
+        // the line's abscissae across it
+        const int bi = base % P1, bj = (base / P1) % P1, bk = base / (P1 * P1);
+        const T one = side ? (T)1 : (T)-1;
+        const T rq[3] = {dn == 0 ? one : s1[O::X1 + bi], dn == 1 ? one : s1[O::X1 + bj],
+                         dn == 2 ? one : s1[O::X1 + bk]};
+        T Jm[D][D], Gm[D][D];
+        linearJacobian<D, T>(a.geoN + ec * LinearMap<D>::NC, rq, Jm);
+        adjugate<D, T>(Jm, Gm);
+        // outward normal times dA: row dn of the adjugate, signed by the side;
+        // the left element's normal at a right-side point is its negation
+        const T sg = (side != (int)right) ? (T)1 : (T)-1;
+        T ss = (T)0;
+#pragma unroll
+        for (int c = 0; c < D; c++) {
+          T g = Gm[0][c];
+          if constexpr (D >= 2) g = (dn == 1) ? Gm[1][c] : g;
+          if constexpr (D == 3) g = (dn == 2) ? Gm[2][c] : g;
+          n[c] = sg * g;
+          ss += g * g;
+        }
+        dA = sqrt(ss);
+        const T rdA = recip(dA);
+#pragma unroll
+        for (int c = 0; c < D; c++) n[c] = n[c] * rdA;
+      }
       T un[NF], Fn[NF];
       if (bnd) {
         boundaryState<D, NF, T>(a.ph, fi - FINFO_BOUNDARY, uo, n, un);
@@ -832,18 +949,19 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
       } else {
         u = u - a.rk * (double)dv * dtJ;
       }
+      // the updated state in the residual precision, for the traces: FP32
+      // states hold the energy less Phys::eOff, and the final stages' FP64
+      // base (hence u) is absolute
+      double ut = u;
+      if constexpr (hasEnergyOffset<T>())
+        if (k == NF - 1 && base64) ut = u - a.ph.eOff;
       if (!M23 || (M01 && a.mode == 1)) {
         if (a.Unext) __builtin_nontemporal_store((T)u, a.Unext + gi);
       } else {
         __builtin_nontemporal_store(u, a.Uout + gi);
-        if (a.Uout32) {
-          double u32 = u;
-          if constexpr (hasEnergyOffset<T>())
-            if (k == NF - 1) u32 = u - a.ph.eOff;
-          __builtin_nontemporal_store((float)u32, a.Uout32 + gi);
-        }
+        if (a.Uout32) __builtin_nontemporal_store((float)ut, a.Uout32 + gi);
       }
-      sFd[slot][k][tP] = (T)u;
+      sFd[slot][k][tP] = (T)ut;
     }
   }
   if (M01 && a.mode == 0) return;
@@ -876,14 +994,14 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
 // The inviscid final-stage variant asks for 3 waves per SIMD (at most 168
 // VGPRs): it holds the most prefetched data, and at 2 waves per SIMD the
 // final stage ran 12 % longer.
-template <int D, int P, int NF, typename T, bool BND, bool VIS, bool FIN>
+template <int D, int P, int NF, typename T, bool BND, bool VIS, bool FIN, int GM = 0>
 __global__ __launch_bounds__((ElemGeo<D, P>::NTH), (FIN && !VIS ? 3 : 1)) void k_stage(StageArgs<T> a) {
   __shared__ StageSmem<D, P, NF, T> sm;
   // Workgroups are dealt to the 8 XCDs in turn; remap so that each XCD walks a
   // contiguous range of elements and face neighbours share its L2.  The host
   // pads the grid to a multiple of 8.
   const int b = (blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);
-  stageBody<D, P, NF, T, BND, VIS, FIN>(a, b, sm, (int)threadIdx.x);
+  stageBody<D, P, NF, T, BND, VIS, FIN, GM>(a, b, sm, (int)threadIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -911,20 +1029,13 @@ __global__ void k_metrics(const double *__restrict__ X, const double *__restrict
       J[d][d2] = acc;
     }
   double G[3][3], det;
-  if (D == 2) {
-    det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
-    G[0][0] = J[1][1];
-    G[0][1] = -J[1][0];
-    G[1][0] = -J[0][1];
-    G[1][1] = J[0][0];
+  if constexpr (D == 2) {
+    double J2[2][2] = {{J[0][0], J[0][1]}, {J[1][0], J[1][1]}}, G2[2][2];
+    det = adjugate<2, double>(J2, G2);
+    for (int i = 0; i < 2; i++)
+      for (int j = 0; j < 2; j++) G[i][j] = G2[i][j];
   } else {
-    const double xr = J[0][0], xs = J[1][0], xt = J[2][0];
-    const double yr = J[0][1], ys = J[1][1], yt = J[2][1];
-    const double zr = J[0][2], zs = J[1][2], zt = J[2][2];
-    det = xr * (ys * zt - yt * zs) - xs * (yr * zt - yt * zr) + xt * (yr * zs - ys * zr);
-    G[0][0] = ys * zt - yt * zs; G[0][1] = xt * zs - xs * zt; G[0][2] = xs * yt - xt * ys;
-    G[1][0] = yt * zr - yr * zt; G[1][1] = xr * zt - xt * zr; G[1][2] = xt * yr - xr * yt;
-    G[2][0] = yr * zs - ys * zr; G[2][1] = xs * zr - xr * zs; G[2][2] = xr * ys - xs * yr;
+    det = adjugate<3, double>(J, G);
   }
   if (isS) {
     for (int d1 = 0; d1 < D; d1++)

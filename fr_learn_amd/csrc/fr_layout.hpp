@@ -149,12 +149,13 @@ __host__ __device__ __forceinline__ int neighbourFpt(int q, int finfo) {
 }
 
 // Layout of the per-order 1D factor table: l_L[P1], l_R[P1], g_L[P1], g_R[P1],
-// Dm[P1][P1] (Dm[i][m] = derivative of basis m at node i).
+// Dm[P1][P1] (Dm[i][m] = derivative of basis m at node i), then the 1D
+// solution-point abscissae x[P1] (for the metrics of the nodal geometry).
 template <int P>
 struct OneD {
   static constexpr int P1 = P + 1;
-  static constexpr int LL = 0, LR = P1, GL = 2 * P1, GR = 3 * P1, DM = 4 * P1;
-  static constexpr int SIZE = 4 * P1 + P1 * P1;
+  static constexpr int LL = 0, LR = P1, GL = 2 * P1, GR = 3 * P1, DM = 4 * P1, X1 = 4 * P1 + P1 * P1;
+  static constexpr int SIZE = 5 * P1 + P1 * P1;
 };
 
 }  // namespace frdev

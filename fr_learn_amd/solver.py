@@ -126,6 +126,17 @@ class Solver:
         _lib.check(self._lib.fr_reduce(self._h, kind, _lib.dptr(out)))
         return out
 
+    def integrate_error(self, testCase=1, errorNorm=1, quadOrder=10, icType=0, time=0.0, rhoIC=1.0, pIC=1.0,
+                        vortexAngle=0.4636476090008061, box=(-5.0, 5.0, -5.0, 5.0)):
+        """Error norms of the current state (fr_integrate_error; the reference's
+        solver::integrateError).  Defaults are the reference's (errorNorm 1,
+        quadOrder 10, vortexAngle atan(1/2), the standard box [-5, 5]^2)."""
+        p = _lib.FrErrorParams(int(testCase), int(errorNorm), int(quadOrder), int(icType), float(time),
+                               float(rhoIC), float(pIC), float(vortexAngle), *[float(b) for b in box])
+        out = np.zeros(self.nFields)
+        _lib.check(self._lib.fr_integrate_error(self._h, ctypes.byref(p), _lib.dptr(out)))
+        return out
+
     def kinetic_energy(self):
         out = np.zeros(2)
         _lib.check(self._lib.fr_reduce(self._h, 10, _lib.dptr(out)))
@@ -231,12 +242,14 @@ class Solver:
         return it.value, t.value
 
     def geometry_mode(self, request=-1):
-        """Stage-kernel geometry layout: request 1 compact per element/face
-        when the mesh is affine (default), 0 per point, -1 query.  Returns
-        True when the compact arrays are in use."""
+        """Stage-kernel geometry layout (fr_geometry_mode): request 0 per-point
+        arrays, 1 compact per element / face when the mesh is affine (the
+        default), 2 nodal (metrics and normals formed from each linear
+        element's corner nodes), 3 nodal metrics with the stored face
+        geometry, -1 query.  Returns the layout in use (0..3)."""
         act = ctypes.c_int()
         _lib.check(self._lib.fr_geometry_mode(self._h, int(request), ctypes.byref(act)))
-        return bool(act.value)
+        return int(act.value)
 
     def residual_bytes(self):
         b = ctypes.c_double()

@@ -146,6 +146,25 @@ int fr_copy_u0_u(fr_ctx *ctx);
  * (out[1]) by the spt quadrature of solver.cpp:1510-1526 */
 int fr_reduce(fr_ctx *ctx, int kind, double *out);
 
+/* Error norms of U_spts: solver::integrateError (solver.cpp:1507-1574) with
+ * calcError (funcs.cpp:1875-2056), the numbers writeError prints
+ * (output.cpp:1055-1123, testCase 1).  The fields mirror the `input` members
+ * they read (input.cpp:224-248, 276, 379-380). */
+typedef struct fr_error_params {
+  int testCase;   /* 0: the solution itself is integrated; 1: its error against the analytic solution */
+  int errorNorm;  /* 0: |integral of the error - integral of U| at the solution points; 1 L1; 2 L2 */
+  int quadOrder;  /* errorNorm 1, 2: (quadOrder+1)^nDims Gauss points per element, 0..10 (reference default 10) */
+  int icType;     /* Euler / NS: 0 free-stream entropy, 1 isentropic vortex, 2 Liang's vortex;
+                     advection: 0 Gaussian, 1 sine, 2 cosine product */
+  double time;    /* the flow time of the analytic solution */
+  double rhoIC, pIC;                /* icType 0 (Euler / NS): the free stream */
+  double vortexAngle;               /* icType 2 (Euler / NS) */
+  double xmin, xmax, ymin, ymax;    /* the periodic box of the analytic solutions (calcError's
+                                       CREATE_MESH box, or its 'standard' box) */
+} fr_error_params;
+/* out[nFields]; quads and hexes (FR_EINVAL for triangles) */
+int fr_integrate_error(fr_ctx *ctx, const fr_error_params *p, double *out);
+
 /* Time step.  fr_set_dt: fixed params->dt (dtType 0).  fr_calc_dt: CFL time
  * step from the current U_spts -- ele::calcWaveSpFpts (ele.cpp:1031-1084,
  * static grids), ele::calcDt (1086-1105), solver::calcDt (solver.cpp:293-305);
@@ -224,14 +243,22 @@ int fr_read_restart(fr_ctx *ctx, const char *dataFileName, int restartIter, doub
 int fr_write_checkpoint(fr_ctx *ctx, const char *path, int iter, double time);
 int fr_read_checkpoint(fr_ctx *ctx, const char *path, int *iter, double *time);
 
-/* Geometry layout of the stage kernels.  On a mesh whose elements are all
- * affine (metrics and detJ constant over each element, normal and dA
- * constant over each face, to 1e-13 relative) the stage kernels read one
- * metric set per element and one normal per face instead of per-point
- * copies; otherwise, or with request 0, they read the per-point arrays.
- * request: -1 query, 0 per-point, 1 compact when the mesh is affine (the
- * default).  *active (may be NULL) = 1 when the compact arrays are in use.
- * No reference counterpart: FlurryPP always stores metrics per point
+/* Geometry layout of the stage kernels.  request:
+ *   -1  query;
+ *    0  per-point arrays (metrics and detJ at every solution point, normal
+ *       and dA at every flux point);
+ *    1  compact (the default): on a mesh whose elements are all affine
+ *       (metrics and detJ constant over each element, normal and dA constant
+ *       over each face, to 1e-13 relative) one metric set per element and one
+ *       normal per face, else per point;
+ *    2  nodal: on inviscid meshes of linear quads / hexes (4 / 8 nodes, any
+ *       shape) the stage kernels form the metrics and detJ at each solution
+ *       point and the normal and dA at each flux point from the element's
+ *       corner nodes (21 coefficients per hex) instead of reading them,
+ *       else per point;
+ *    3  as 2 for the solution-point metrics, with the stored face geometry.
+ * *active (may be NULL) receives the layout in use (0..3).  No reference
+ * counterpart: FlurryPP always stores metrics per point
  * (ele::calcTransforms, ele.cpp:202-342); the results agree to rounding. */
 int fr_geometry_mode(fr_ctx *ctx, int request, int *active);
 

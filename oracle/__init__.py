@@ -32,6 +32,27 @@ class OrParams(ctypes.Structure):
     ]
 
 
+class OrErrorParams(ctypes.Structure):
+    """or_error_params: the `input` fields solver::integrateError and
+    calcError read (solver.cpp:1507-1574, funcs.cpp:1875-2056)."""
+    _fields_ = [
+        ("testCase", ctypes.c_int), ("errorNorm", ctypes.c_int), ("quadOrder", ctypes.c_int),
+        ("icType", ctypes.c_int),
+        ("time", ctypes.c_double), ("rhoIC", ctypes.c_double), ("pIC", ctypes.c_double),
+        ("vortexAngle", ctypes.c_double), ("xmin", ctypes.c_double), ("xmax", ctypes.c_double),
+        ("ymin", ctypes.c_double), ("ymax", ctypes.c_double),
+    ]
+
+
+def error_params(testCase=1, errorNorm=1, quadOrder=10, icType=0, time=0.0, rhoIC=1.0, pIC=1.0,
+                 vortexAngle=0.4636476090008061, box=(-5.0, 5.0, -5.0, 5.0)):
+    """Error-norm parameters with the reference's defaults (input.cpp:224-248,
+    276, 379-380): errorNorm 1, quadOrder 10, vortexAngle atan(1/2), and the
+    'standard' box [-5, 5]^2 calcError assumes for a mesh it did not create."""
+    return OrErrorParams(int(testCase), int(errorNorm), int(quadOrder), int(icType), float(time), float(rhoIC),
+                         float(pIC), float(vortexAngle), *[float(b) for b in box])
+
+
 _DP = ctypes.POINTER(ctypes.c_double)
 _IP = ctypes.POINTER(ctypes.c_int)
 _lib = None
@@ -66,6 +87,7 @@ def load():
         lib.or_time_step_b.argtypes = [ctypes.c_void_p, ctypes.c_int]
         lib.or_norm_residual.argtypes = [ctypes.c_void_p, ctypes.c_int, _DP]
         lib.or_set_threads.argtypes = [ctypes.c_int]
+        lib.or_integrate_error.argtypes = [ctypes.c_void_p, ctypes.POINTER(OrErrorParams), _DP]
         P = ctypes.POINTER(OrParams)
         lib.or_point_inviscid_flux.argtypes = [P, _DP, _DP]
         lib.or_point_common_flux.argtypes = [P, ctypes.c_int, _DP, _DP, _DP, _DP]
@@ -253,6 +275,15 @@ class Oracle:
 
     def copy_u0_u(self):
         self._lib.or_copy_u0_u(self._h)
+
+    def integrate_error(self, **kw):
+        """solver::integrateError (solver.cpp:1507-1574) of the current state;
+        keyword arguments as error_params."""
+        out = np.zeros(self.nFields)
+        ep = error_params(**kw)
+        if self._lib.or_integrate_error(self._h, ctypes.byref(ep), _d(out)) != 0:
+            raise RuntimeError("oracle: " + self._lib.or_last_error().decode())
+        return out
 
     def norm_residual(self, kind=2):
         out = np.zeros(self.nFields)

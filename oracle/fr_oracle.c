@@ -1548,6 +1548,173 @@ void or_norm_residual(const or_solver *s, int normType, double *out) {
 }
 
 /* ------------------------------------------------------------------------ */
+/* error norms: solver::integrateError (solver.cpp:1507-1574) with           */
+/* ele::calcEleError (ele.cpp:792-810), calcError (funcs.cpp:1875-2056) and  */
+/* ele::calcPos (ele.cpp:389-404)                                            */
+/* ------------------------------------------------------------------------ */
+static const double OR_PI = 3.141592653589793; /* 4 atan(1), global.hpp:37 */
+
+/* calcError (funcs.cpp:1875-2056): the error of state U at physical point pos
+ * against the analytic solution of the test case (the solution itself when
+ * testCase == 0), then |.| (errorNorm 1) or squared (errorNorm 2) */
+static void calc_error(const or_solver *s, const or_error_params *ep, const double *U, const double *pos,
+                       double *err) {
+  const int D = s->nDims, K = s->nFields;
+  if (ep->testCase == 0) {
+    for (int k = 0; k < K; k++) err[k] = U[k];
+    return;
+  }
+  for (int k = 0; k < K; k++) err[k] = 0.0;
+  const double px = pos[0], py = pos[1], pz = D == 3 ? pos[2] : 0.0;
+  if (s->p.equation == OR_NAVIER_STOKES) {
+    const double gamma = s->p.gamma;
+    if (ep->icType == 0) {
+      /* uniform free stream: entropy error (funcs.cpp:1891-1903) */
+      double pfs = ep->pIC, rhofsg = pow(ep->rhoIC, gamma);
+      double rho = U[0], u = U[1] / rho, v = U[2] / rho, w = (D == 2) ? 0 : U[3] / rho;
+      double p = (gamma - 1.) * (U[D + 1] - .5 * rho * (u * u + v * v + w * w));
+      err[0] = p * rhofsg / (pow(rho, gamma) * pfs) - 1.;
+    } else if (ep->icType == 1) {
+      /* isentropic vortex of strength 5 convected along (1, 1) (1904-1946) */
+      double eps = 5.0;
+      double xmin = ep->xmin, xmax = ep->xmax, ymin = ep->ymin, ymax = ep->ymax;
+      double x = fmod((px - ep->time), (xmax - xmin));
+      double y = fmod((py - ep->time), (ymax - ymin));
+      if (x > xmax) x -= (xmax - xmin);
+      if (y > ymax) y -= (ymax - ymin);
+      if (x < xmin) x += (xmax - xmin);
+      if (y < ymin) y += (ymax - ymin);
+      double f = 1.0 - (x * x + y * y);
+      double rho = fmax(pow(1. - eps * eps * (gamma - 1.) / (8. * gamma * OR_PI * OR_PI) * exp(f), 1.0 / (gamma - 1.0)),
+                        1e-10);
+      double vx = 1. - eps * y / (2. * OR_PI) * exp(f / 2.);
+      double vy = 1. + eps * x / (2. * OR_PI) * exp(f / 2.);
+      double p = pow(rho, gamma);
+      err[0] = rho;
+      err[1] = rho * vx;
+      err[2] = rho * vy;
+      if (D == 3) err[3] = 0.;
+      err[D + 1] = p / (gamma - 1) + (0.5 * rho * (vx * vx + vy * vy));
+      for (int i = 0; i < K; i++) err[i] = U[i] - err[i];
+    } else if (ep->icType == 2) {
+      /* Liang & Miyaji's vortex (1947-2000) */
+      double eps = 1.0, rc = 1.0, Minf = .3, Uinf = 1, rhoInf = 1, theta = ep->vortexAngle;
+      double Pinf = pow(Minf, -2) / gamma;
+      double eM = (eps * Minf) * (eps * Minf);
+      double xmin = ep->xmin, xmax = ep->xmax, ymin = ep->ymin, ymax = ep->ymax;
+      double advX = Uinf * cos(theta) * ep->time, advY = Uinf * sin(theta) * ep->time;
+      double x = (fmod((px - advX), (xmax - xmin)));
+      double y = (fmod((py - advY), (ymax - ymin)));
+      if (x > xmax) x -= (xmax - xmin);
+      if (x < xmin) x += (xmax - xmin);
+      if (y > ymax) y -= (ymax - ymin);
+      if (y < ymin) y += (ymax - ymin);
+      double f = -(x * x + y * y) / (rc * rc);
+      double vx = Uinf * (cos(theta) - y * eps / rc * exp(f / 2.));
+      double vy = Uinf * (sin(theta) + x * eps / rc * exp(f / 2.));
+      double rho = rhoInf * pow(1. - (gamma - 1.) / 2. * eM * exp(f), gamma / (gamma - 1.0));
+      double p = Pinf * pow(1. - (gamma - 1.) / 2. * eM * exp(f), gamma / (gamma - 1.0));
+      err[0] = rho;
+      err[1] = rho * vx;
+      err[2] = rho * vy;
+      if (D == 3) err[3] = 0.;
+      err[D + 1] = p / (gamma - 1) + (0.5 * rho * (vx * vx + vy * vy));
+      for (int i = 0; i < K; i++) err[i] = U[i] - err[i];
+    }
+  } else {
+    /* advection-diffusion (2001-2044) */
+    double xmin = ep->xmin, xmax = ep->xmax, ymin = ep->ymin, ymax = ep->ymax;
+    double x = fabs(fmod((px - ep->time), (xmax - xmin)));
+    double y = fabs(fmod((py - ep->time), (ymax - ymin)));
+    if (x > (xmax - xmin) / 2) x -= (xmax - xmin);
+    if (y > (ymax - ymin) / 2) y -= (ymax - ymin);
+    if (ep->icType == 0) {
+      double r2 = x * x + y * y;
+      err[0] = exp(-r2);
+    } else if (ep->icType == 1) {
+      err[0] = 1 + sin(2. * OR_PI * (px + 5. - ep->time) / 10.);
+    } else if (ep->icType == 2) {
+      err[0] = cos(2 * OR_PI * px / 6.) * cos(2 * OR_PI * py / 6.) * cos(2 * OR_PI * pz / 6.);
+    }
+    for (int i = 0; i < K; i++) err[i] = U[i] - err[i];
+  }
+  if (ep->errorNorm == 1)
+    for (int k = 0; k < K; k++) err[k] = fabs(err[k]);
+  else if (ep->errorNorm == 2)
+    for (int k = 0; k < K; k++) err[k] *= err[k];
+}
+
+int or_integrate_error(const or_solver *s, const or_error_params *ep, double *out) {
+  const int D = s->nDims, nS = s->nSpts, nE = s->nEles, K = s->nFields, nN = s->nNodes, P1 = s->P1;
+  double LpErr[5] = {0, 0, 0, 0, 0}, err[5], pos[3];
+  if (ep->errorNorm == 0) {
+    /* the error at the solution points against the integral of the solution
+     * (solver.cpp:1510-1526; quadrature weights of the solution order) */
+    double intU[5] = {0, 0, 0, 0, 0};
+    for (int ic = 0; ic < nE; ic++)
+      for (int spt = 0; spt < nS; spt++) {
+        const double *U = s->U_spts + (size_t)K * ((size_t)spt * nE + ic);
+        for (int d = 0; d < D; d++) pos[d] = s->pos_spts[IDX3(spt, ic, d, nE, D)];
+        if (ep->testCase) calc_error(s, ep, U, pos, err);
+        else for (int k = 0; k < K; k++) err[k] = U[k];
+        const double wJ = s->detJac_spts[(size_t)spt * nE + ic];
+        for (int k = 0; k < K; k++) {
+          LpErr[k] += err[k] * s->wts[spt] * wJ;
+          intU[k] += U[k] * s->wts[spt] * wJ;
+        }
+      }
+    for (int k = 0; k < K; k++) LpErr[k] = fabs(LpErr[k] - intU[k]);
+  } else {
+    /* quadrature points of order quadOrder (setupInterpolateSptsQpts,
+     * operators.cpp:305-316; getQptWeights, points.cpp:347-374) */
+    const int qo = ep->quadOrder, nq1 = qo + 1;
+    if (qo < 0 || qo > 10) { set_err("quadrature order %d not tabulated", qo); return -1; }
+    const int nQ = D == 3 ? nq1 * nq1 * nq1 : nq1 * nq1;
+    const double *q1 = GL_PTS[qo], *w1 = GL_WTS[qo];
+    double *loc = xcalloc((size_t)nQ * 3, 8), *wq = xcalloc(nQ, 8);
+    for (int q = 0; q < nQ; q++) {
+      int i = q % nq1, j = (q / nq1) % nq1, k = q / (nq1 * nq1);
+      loc[3 * q] = q1[i];
+      loc[3 * q + 1] = q1[j];
+      loc[3 * q + 2] = D == 3 ? q1[k] : 0.0;
+      wq[q] = D == 3 ? w1[k] * w1[j] * w1[i] : w1[j] * w1[i];
+    }
+    /* opp_spts_to_qpts (setupInterpolateSptsIpts, operators.cpp:259-303) */
+    double *A = xcalloc((size_t)nQ * nS, 8);
+    for (int q = 0; q < nQ; q++)
+      for (int spt = 0; spt < nS; spt++) {
+        int ks = spt / (P1 * P1), js = (spt - P1 * P1 * ks) / P1, is = spt - P1 * js - P1 * P1 * ks;
+        if (D == 2) {
+          is = spt % P1;
+          js = spt / P1;
+        }
+        double v = lagrange(s->spts1D, P1, loc[3 * q], is) * lagrange(s->spts1D, P1, loc[3 * q + 1], js);
+        if (D == 3) v = v * lagrange(s->spts1D, P1, loc[3 * q + 2], ks);
+        A[(size_t)q * nS + spt] = v;
+      }
+    double *Uq = xcalloc((size_t)nQ * nE * K, 8), *Jq = xcalloc((size_t)nQ * nE, 8);
+    oracle_dgemm(nQ, nE * K, nS, 1.0, A, s->U_spts, 0.0, Uq);
+    oracle_dgemm(nQ, nE, nS, 1.0, A, s->detJac_spts, 0.0, Jq);
+    double sh[8];
+    for (int ic = 0; ic < nE; ic++)
+      for (int q = 0; q < nQ; q++) {
+        /* ele::calcPos: the shape functions at the point times the nodes */
+        shape_dshape(D, loc + 3 * q, sh, NULL);
+        pos[0] = pos[1] = pos[2] = 0;
+        for (int n = 0; n < nN; n++)
+          for (int d = 0; d < D; d++) pos[d] += sh[n] * s->nodes[IDX3(n, ic, d, nE, D)];
+        calc_error(s, ep, Uq + (size_t)K * ((size_t)q * nE + ic), pos, err);
+        for (int k = 0; k < K; k++) LpErr[k] += err[k] * wq[q] * Jq[(size_t)q * nE + ic];
+      }
+    free(loc); free(wq); free(A); free(Uq); free(Jq);
+  }
+  if (ep->errorNorm == 2)
+    for (int k = 0; k < K; k++) LpErr[k] = sqrt(fabs(LpErr[k]));
+  for (int k = 0; k < K; k++) out[k] = LpErr[k];
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
 /* test hooks: pointwise routines exposed so that tests can check them        */
 /* against independent textbook implementations                              */
 /* ------------------------------------------------------------------------ */

@@ -101,6 +101,13 @@ void or_copy_u0_u(or_solver *s);
 /* residual norm of divF_spts[0] (ele.cpp:1882-1903 summed as output.cpp:846-892) */
 void or_norm_residual(const or_solver *s, int normType, double *out);
 void or_set_threads(int n);
+/* Error norms (solver::integrateError, solver.cpp:1507-1574; calcError,
+ * funcs.cpp:1875-2056).  out[nFields]. */
+typedef struct or_error_params {
+  int testCase, errorNorm, quadOrder, icType;
+  double time, rhoIC, pIC, vortexAngle, xmin, xmax, ymin, ymax;
+} or_error_params;
+int or_integrate_error(const or_solver *s, const or_error_params *ep, double *out);
 
 /* pointwise test hooks (flux.cpp / face.cpp / boundFace.cpp / polynomials.cpp
  * restatements), so tests can pin them against textbook formulas */
