@@ -463,8 +463,8 @@ int createImpl(const fr_config *cfg, const fr_mesh *m, int precision, fr_ctx *c)
   if (cfg->riemannType != 0 && cfg->riemannType != 1) return fail(FR_EINVAL, "riemannType must be 0 or 1");
   if (cfg->viscous != 0 && cfg->viscous != 1) return fail(FR_EINVAL, "viscous must be 0 or 1");
   if (!m || m->nEles <= 0) return fail(FR_EMESH, "Cannot setup elements array - nEles = 0");
-  if (m->nNodes != (cfg->nDims == 3 ? 8 : 4) && !(cfg->nDims == 2 && m->nNodes == 3))
-    return fail(FR_EINVAL, "only linear hexes (8 nodes), quads (4) or triangles (3)");
+  if (!(cfg->nDims == 3 && frhost::hexNodeCount(m->nNodes)) && !(cfg->nDims == 2 && (m->nNodes == 4 || m->nNodes == 3)))
+    return fail(FR_EINVAL, "hexes of 8, 20, 27, 64, 125 or 216 nodes, linear quads (4) or triangles (3)");
   c->tri = (m->nNodes == 3);
   for (int i = 0; i < m->nEles * m->nNodes; i++)
     if (m->c2v[i] < 0 || m->c2v[i] >= m->nVerts) return fail(FR_EMESH, "c2v vertex id out of range");
@@ -485,7 +485,8 @@ int createImpl(const fr_config *cfg, const fr_mesh *m, int precision, fr_ctx *c)
       break;
     default: return fail(FR_EINVAL, "Time-Stepping type not supported.");
   }
-  std::string err = c->tri ? frhost::buildTriOperators(*cfg, c->ops) : frhost::buildOperators(*cfg, c->ops);
+  std::string err =
+      c->tri ? frhost::buildTriOperators(*cfg, c->ops) : frhost::buildOperators(*cfg, c->ops, m->nNodes);
   if (!err.empty()) return fail(FR_EINVAL, err);
   err = frhost::buildConnectivity(*cfg, *m, c->ops, c->conn);
   if (!err.empty()) return fail(FR_EMESH, err);

@@ -202,7 +202,7 @@ extern "C" int fr_integrate_error(fr_ctx *c, const fr_error_params *p, double *o
   }
   const int nPts = (int)w.size();
   std::vector<double> sh((size_t)nPts * nN);
-  for (int q = 0; q < nPts; q++) frhost::shapeLinear(D, &loc[3 * q], &sh[(size_t)q * nN], nN);
+  for (int q = 0; q < nPts; q++) frhost::shapeFunctions(D, &loc[3 * q], &sh[(size_t)q * nN], nN);
   // tables and the element nodes to the device (kept between calls)
   const size_t nTab = Lq.size() + sh.size() + w.size();
   int rc;

@@ -258,7 +258,7 @@ std::string verifyIndexMaps(const Operators &o) {
 
 }  // namespace
 
-std::string buildOperators(const fr_config &cfg, Operators &o) {
+std::string buildOperators(const fr_config &cfg, Operators &o, int nNodes) {
   const int D = cfg.nDims, P = cfg.order;
   if (D != 2 && D != 3) return "nDims must be 2 or 3";
   if (P < 1 || P > 5) return "order must be in 1..5";
@@ -272,7 +272,8 @@ std::string buildOperators(const fr_config &cfg, Operators &o) {
   o.nFptsFace = (D == 3) ? P1 * P1 : P1;
   o.nFacesEle = 2 * D;
   o.nFpts = o.nFacesEle * o.nFptsFace;
-  o.nNodes = (D == 3) ? 8 : 4;
+  o.nNodes = (D == 3) ? (nNodes ? nNodes : 8) : 4;
+  if (D == 3 && !hexNodeCount(o.nNodes)) return "hexes have 8, 20, 27, 64, 125 or 216 nodes";
   const int nS = o.nSpts, nF = o.nFpts;
   std::vector<double> x(P1);
   for (int i = 0; i < P1; i++) x[i] = (cfg.sptsType == 0) ? kGL[P][i] : kGLL[P][i];
@@ -418,34 +419,21 @@ std::string buildOperators(const fr_config &cfg, Operators &o) {
     if (!err.empty()) return err;
   }
 
-  // linear-element shape-function derivatives (Lagrange on {-1, 1}, Gmsh
-  // corner order: funcs.cpp:178-260, 663-800)
-  static const int CI[8] = {0, 1, 1, 0, 0, 1, 1, 0}, CJ[8] = {0, 0, 1, 1, 0, 0, 1, 1},
-                   CK[8] = {0, 0, 0, 0, 1, 1, 1, 1};
-  const std::vector<double> xl = {-1., 1.};
+  // shape-function derivatives of the element's nodes at the solution and
+  // flux points (dshape_quad / dshape_hex, funcs.cpp:570-1607)
   const int nN = o.nNodes;
   auto fillDshape = [&](const std::vector<double> &loc, int nPts, std::vector<double> &out) {
     out.assign((size_t)D * nPts * nN, 0.0);
+    std::vector<double> sh(nN), dsh((size_t)3 * nN);
     for (int p = 0; p < nPts; p++) {
-      const double *r = &loc[3 * p];
-      for (int n = 0; n < nN; n++) {
-        double la = lag(xl, r[0], CI[n]), lb = lag(xl, r[1], CJ[n]);
-        double da = dlag(xl, r[0], CI[n]), db = dlag(xl, r[1], CJ[n]);
-        if (D == 3) {
-          double lc = lag(xl, r[2], CK[n]), dc = dlag(xl, r[2], CK[n]);
-          out[((size_t)0 * nPts + p) * nN + n] = da * lb * lc;
-          out[((size_t)1 * nPts + p) * nN + n] = la * db * lc;
-          out[((size_t)2 * nPts + p) * nN + n] = la * lb * dc;
-        } else {
-          out[((size_t)0 * nPts + p) * nN + n] = da * lb;
-          out[((size_t)1 * nPts + p) * nN + n] = la * db;
-        }
-      }
+      shapeFunctions(D, &loc[3 * p], sh.data(), nN, dsh.data());
+      for (int n = 0; n < nN; n++)
+        for (int d = 0; d < D; d++) out[((size_t)d * nPts + p) * nN + n] = dsh[3 * n + d];
     }
   };
   fillDshape(o.locSpts, nS, o.dshapeSpts);
   o.shapeSpts.assign((size_t)nS * nN, 0.0);
-  for (int p = 0; p < nS; p++) shapeLinear(D, &o.locSpts[3 * p], &o.shapeSpts[(size_t)p * nN]);
+  for (int p = 0; p < nS; p++) shapeFunctions(D, &o.locSpts[3 * p], &o.shapeSpts[(size_t)p * nN], nN);
   fillDshape(o.locFpts, nF, o.dshapeFpts);
   return "";
 }
@@ -465,22 +453,152 @@ std::vector<double> gaussWeights1D(int order) {
 
 double lagrange(const std::vector<double> &x, double y, int mode) { return lag(x, y, mode); }
 
-void shapeLinear(int nDims, const double *r, double *out, int nNodes) {
+std::vector<std::array<int, 3>> hexLattice(int nSide) {
+  std::vector<std::array<int, 3>> nd;
+  const int nLevels = nSide / 2, c = nSide / 2;
+  for (int i = 0; i < nLevels; i++) {
+    const int i2 = (nSide - 1) - i;
+    // the shell's corners
+    const int cs[8][3] = {{i, i, i}, {i2, i, i}, {i2, i2, i}, {i, i2, i}, {i, i, i2}, {i2, i, i2}, {i2, i2, i2}, {i, i2, i2}};
+    for (auto &q : cs) nd.push_back({q[0], q[1], q[2]});
+    // its edges: slot s of 12 holds the nSide2 points of one edge
+    const int nSide2 = nSide - 2 * (i + 1);
+    std::vector<std::array<int, 3>> ed((size_t)12 * nSide2);
+    for (int j = 0; j < nSide2; j++) {
+      const int a = i + 1 + j, b = i2 - 1 - j;
+      ed[0 * nSide2 + j] = {a, i, i};
+      ed[1 * nSide2 + j] = {i, a, i};
+      ed[2 * nSide2 + j] = {i, i, a};
+      ed[3 * nSide2 + j] = {i2, a, i};
+      ed[4 * nSide2 + j] = {i2, i, a};
+      ed[5 * nSide2 + j] = {b, i2, i};
+      ed[6 * nSide2 + j] = {i2, i2, a};
+      ed[7 * nSide2 + j] = {i, i2, a};
+      ed[8 * nSide2 + j] = {a, i, i2};
+      ed[9 * nSide2 + j] = {i, a, i2};
+      ed[10 * nSide2 + j] = {i2, a, i2};
+      ed[11 * nSide2 + j] = {b, i2, i2};
+    }
+    nd.insert(nd.end(), ed.begin(), ed.end());
+    // its faces (bottom, front, left, right, back, top), each by rings of 4
+    // corners and 4 edges, then the face centre when nSide2 is odd.  f(u, v)
+    // places the face's in-plane indices (u, v) at the face's fixed index.
+    const int nLevels2 = nSide2 / 2;
+    auto face = [&](auto f, const int (*crn)[2], const int (*edge)[4]) {
+      for (int j0 = 0; j0 < nLevels2; j0++) {
+        const int j = j0 + i + 1, j2 = i + 1 + (nSide2 - 1) - j0;
+        const int val[2] = {j, j2};
+        for (int q = 0; q < 4; q++) nd.push_back(f(val[crn[q][0]], val[crn[q][1]]));
+        const int nSide3 = nSide2 - 2 * (j0 + 1);
+        std::vector<std::array<int, 3>> fe((size_t)4 * nSide3);
+        for (int k = 0; k < nSide3; k++) {
+          // per edge: (u, v) = (A + sa*k, B + sb*k) with A, B from {j, j2, j+1, j2-1}
+          const int opts[4] = {j, j2, j + 1 + k, j2 - 1 - k};
+          for (int q = 0; q < 4; q++) fe[(size_t)q * nSide3 + k] = f(opts[edge[q][0]], opts[edge[q][1]]);
+        }
+        nd.insert(nd.end(), fe.begin(), fe.end());
+      }
+      if (nSide2 % 2) nd.push_back(f(c, c));
+    };
+    // corner orders as indices into (j, j2); edge orders as indices into
+    // (j, j2, j+1+k, j2-1-k), both as (u, v) -- funcs.cpp:292-556
+    static const int cBot[4][2] = {{0, 0}, {0, 1}, {1, 1}, {1, 0}}, eBot[4][4] = {{0, 2}, {2, 1}, {1, 3}, {3, 0}};
+    static const int cFro[4][2] = {{0, 0}, {1, 0}, {1, 1}, {0, 1}}, eFro[4][4] = {{2, 0}, {1, 2}, {3, 1}, {0, 3}};
+    static const int cLef[4][2] = {{0, 0}, {0, 1}, {1, 1}, {1, 0}}, eLef[4][4] = {{0, 2}, {2, 1}, {1, 3}, {3, 0}};
+    static const int cRig[4][2] = {{0, 0}, {1, 0}, {1, 1}, {0, 1}}, eRig[4][4] = {{2, 0}, {1, 2}, {3, 1}, {0, 3}};
+    static const int cBac[4][2] = {{1, 0}, {0, 0}, {0, 1}, {1, 1}}, eBac[4][4] = {{3, 0}, {0, 2}, {2, 1}, {1, 3}};
+    static const int cTop[4][2] = {{0, 0}, {1, 0}, {1, 1}, {0, 1}}, eTop[4][4] = {{2, 0}, {1, 2}, {3, 1}, {0, 3}};
+    face([&](int u, int v) { return std::array<int, 3>{u, v, i}; }, cBot, eBot);    // (xi, eta) at mu = i
+    face([&](int u, int v) { return std::array<int, 3>{u, i, v}; }, cFro, eFro);    // (xi, mu) at eta = i
+    face([&](int u, int v) { return std::array<int, 3>{i, u, v}; }, cLef, eLef);    // (eta, mu) at xi = i
+    face([&](int u, int v) { return std::array<int, 3>{i2, u, v}; }, cRig, eRig);   // (eta, mu) at xi = i2
+    face([&](int u, int v) { return std::array<int, 3>{u, i2, v}; }, cBac, eBac);   // (xi, mu) at eta = i2
+    face([&](int u, int v) { return std::array<int, 3>{u, v, i2}; }, cTop, eTop);   // (xi, eta) at mu = i2
+  }
+  if (nSide % 2) nd.push_back({c, c, c});
+  return nd;
+}
+
+void shapeFunctions(int nDims, const double *r, double *out, int nNodes, double *dout) {
   if (nNodes == 3) {
     out[0] = -(r[0] + r[1]) / 2;
     out[1] = (1 + r[0]) / 2;
     out[2] = (1 + r[1]) / 2;
+    if (dout) {
+      const double d[3][3] = {{-0.5, -0.5, 0}, {0.5, 0, 0}, {0, 0.5, 0}};
+      for (int n = 0; n < 3; n++)
+        for (int k = 0; k < 3; k++) dout[3 * n + k] = d[n][k];
+    }
     return;
   }
-  static const int CI[8] = {0, 1, 1, 0, 0, 1, 1, 0}, CJ[8] = {0, 0, 1, 1, 0, 0, 1, 1},
-                   CK[8] = {0, 0, 0, 0, 1, 1, 1, 1};
-  const std::vector<double> xl = {-1., 1.};
-  const int nN = (nDims == 3) ? 8 : 4;
-  for (int n = 0; n < nN; n++) {
-    double v = lag(xl, r[0], CI[n]) * lag(xl, r[1], CJ[n]);
-    if (nDims == 3) v = v * lag(xl, r[2], CK[n]);
-    out[n] = v;
+  if (nDims == 2) nNodes = 4;
+  if (nDims == 3 && nNodes == 0) nNodes = 8;
+  if (nDims == 3 && nNodes == 20) {
+    // quadratic serendipity hex (shape_hex / dshape_hex, funcs.cpp:188-212, 685-750)
+    static const double XI[8] = {-1, 1, 1, -1, -1, 1, 1, -1}, ETA[8] = {-1, -1, 1, 1, -1, -1, 1, 1},
+                        MU[8] = {-1, -1, -1, -1, 1, 1, 1, 1};
+    const double xi = r[0], eta = r[1], mu = r[2];
+    for (int i = 0; i < 8; i++) {
+      const double a = 1 + xi * XI[i], b = 1 + eta * ETA[i], cc = 1 + mu * MU[i];
+      out[i] = .125 * a * b * cc * (xi * XI[i] + eta * ETA[i] + mu * MU[i] - 2);
+      if (dout) {
+        dout[3 * i + 0] = .125 * XI[i] * b * cc * (2 * xi * XI[i] + eta * ETA[i] + mu * MU[i] - 1);
+        dout[3 * i + 1] = .125 * ETA[i] * a * cc * (xi * XI[i] + 2 * eta * ETA[i] + mu * MU[i] - 1);
+        dout[3 * i + 2] = .125 * MU[i] * a * b * (xi * XI[i] + eta * ETA[i] + 2 * mu * MU[i] - 1);
+      }
+    }
+    // mid-edge nodes: (node, axis along the edge, signs of the two others)
+    static const int EDG[12][4] = {{8, 0, -1, -1}, {10, 0, 1, -1}, {16, 0, -1, 1}, {18, 0, 1, 1},
+                                   {9, 1, 1, -1},  {11, 1, -1, -1}, {17, 1, 1, 1}, {19, 1, -1, 1},
+                                   {12, 2, -1, -1}, {13, 2, 1, -1}, {14, 2, 1, 1}, {15, 2, -1, 1}};
+    for (auto &e : EDG) {
+      const int n = e[0], ax = e[1];
+      const int o1 = ax == 0 ? 1 : 0, o2 = ax == 2 ? 1 : 2;  // the other two axes, ascending
+      const double s = r[ax], p = 1 + e[2] * r[o1], q = 1 + e[3] * r[o2];
+      out[n] = .25 * (1 - s * s) * p * q;
+      if (dout) {
+        dout[3 * n + ax] = -.5 * s * p * q;
+        dout[3 * n + o1] = .25 * (1 - s * s) * e[2] * q;
+        dout[3 * n + o2] = .25 * (1 - s * s) * p * e[3];
+      }
+    }
+    return;
   }
+  // Lagrange elements on equispaced nodes: the node order of the quad
+  // (Gmsh corners) or of the hex recursion (hexLattice)
+  const int nSide = (nDims == 3) ? (int)std::lround(std::cbrt((double)nNodes)) : 2;
+  std::vector<double> xl(nSide);
+  for (int i = 0; i < nSide; i++) xl[i] = -1. + i * (2. / (nSide - 1));
+  if (nDims == 2) {
+    static const int QI[4] = {0, 1, 1, 0}, QJ[4] = {0, 0, 1, 1};
+    for (int n = 0; n < 4; n++) {
+      out[n] = lag(xl, r[0], QI[n]) * lag(xl, r[1], QJ[n]);
+      if (dout) {
+        dout[3 * n + 0] = dlag(xl, r[0], QI[n]) * lag(xl, r[1], QJ[n]);
+        dout[3 * n + 1] = lag(xl, r[0], QI[n]) * dlag(xl, r[1], QJ[n]);
+        dout[3 * n + 2] = 0;
+      }
+    }
+    return;
+  }
+  const std::vector<std::array<int, 3>> nd = hexLattice(nSide);
+  for (int n = 0; n < nNodes; n++) {
+    const int a = nd[n][0], b = nd[n][1], cc = nd[n][2];
+    const double la = lag(xl, r[0], a), lb = lag(xl, r[1], b), lc = lag(xl, r[2], cc);
+    out[n] = la * lb * lc;
+    if (dout) {
+      dout[3 * n + 0] = dlag(xl, r[0], a) * lb * lc;
+      dout[3 * n + 1] = la * dlag(xl, r[1], b) * lc;
+      dout[3 * n + 2] = la * lb * dlag(xl, r[2], cc);
+    }
+  }
+}
+
+bool hexNodeCount(int nNodes) {
+  if (nNodes == 8 || nNodes == 20) return true;
+  for (int n = 3; n <= 6; n++)
+    if (nNodes == n * n * n) return true;
+  return false;
 }
 
 std::string buildConnectivity(const fr_config &cfg, const fr_mesh &m, const Operators &o, Connectivity &c) {

@@ -4,6 +4,7 @@
 #pragma once
 
 #include <cstdint>
+#include <array>
 #include <string>
 #include <vector>
 
@@ -36,7 +37,7 @@ struct Operators {
 };
 
 // Builds everything above; returns "" or an error message.
-std::string buildOperators(const fr_config &cfg, Operators &ops);
+std::string buildOperators(const fr_config &cfg, Operators &ops, int nNodes = 0);
 
 struct FaceRec {
   int eL, fL, eR, fR, rot, bc;  // bc = -1 for interior faces
@@ -74,10 +75,23 @@ std::vector<double> gaussWeights1D(int order);
 // Lagrange basis l_mode(y) on the nodes x (polynomials.cpp:9-19).
 double lagrange(const std::vector<double> &x, double y, int mode);
 
-// Shape functions of the linear element (Gmsh corner order; shape_quad,
-// funcs.cpp:121-176, shape_hex 178-260) at the reference point r[3]: out[4 or 8];
-// nNodes 3: the linear triangle (-1,-1), (1,-1), (-1,1).
-void shapeLinear(int nDims, const double *r, double *out, int nNodes = 0);
+// Shape functions of the element's nodes at the reference point r[3] (and,
+// with dout, their derivatives dout[node][3]): the 4-node quad, the hex of
+// 8 (default), 20 (quadratic serendipity), 27, 64, 125 or 216 (Lagrange)
+// nodes in the reference's node order (shape_quad / shape_hex and
+// dshape_quad / dshape_hex, funcs.cpp:121-1607), or with nNodes 3 the linear
+// triangle (-1,-1), (1,-1), (-1,1).
+void shapeFunctions(int nDims, const double *r, double *out, int nNodes = 0, double *dout = nullptr);
+
+// Lattice indices (i, j, k) on the nSide^3 equispaced nodes of each node of
+// the Lagrange hex, in the reference's node order: nested shells from the
+// outside in, each with its 8 corners, its 12 edges and its 6 faces (bottom,
+// front, left, right, back, top; each face by rings of corners and edges,
+// then its centre), and the volume centre last (shape_hex, funcs.cpp:213-566).
+std::vector<std::array<int, 3>> hexLattice(int nSide);
+
+// Node counts of the supported hexes: 8, 20, 27, 64, 125, 216.
+bool hexNodeCount(int nNodes);
 
 // FR operators of the linear triangle (fr_tri_ops.cpp); "" or an error message.
 std::string buildTriOperators(const fr_config &cfg, Operators &ops);

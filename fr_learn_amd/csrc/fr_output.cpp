@@ -161,7 +161,7 @@ int writeParaview(fr_ctx *c, const char *name, int iter, double time) {
   std::vector<double> shp((size_t)nPpts * nN);  // shape_ppts(ppt, node)
   for (int q = 0; q < nPpts; q++) {
     const double r[3] = {p1d[q % NP], p1d[(q / NP) % NP], p1d[q / (NP * NP)]};
-    frhost::shapeLinear(3, r, &shp[(size_t)q * nN]);
+    frhost::shapeFunctions(3, r, &shp[(size_t)q * nN], nN);
   }
   // sub-cell connectivity, offsets and types: the same for every element
   std::string cells;
@@ -256,8 +256,8 @@ int writeCSV(fr_ctx *c, const char *name, int iter) {
   if (int rc = hostState(c, Ue)) return rc;
   const int D = c->D, NF = c->NF, NS = c->NS, NFP = c->NFP, nN = c->ops.nNodes;
   std::vector<double> shS((size_t)NS * nN), shF((size_t)NFP * nN);
-  for (int p = 0; p < NS; p++) frhost::shapeLinear(D, &c->ops.locSpts[3 * p], &shS[(size_t)p * nN], nN);
-  for (int p = 0; p < NFP; p++) frhost::shapeLinear(D, &c->ops.locFpts[3 * p], &shF[(size_t)p * nN], nN);
+  for (int p = 0; p < NS; p++) frhost::shapeFunctions(D, &c->ops.locSpts[3 * p], &shS[(size_t)p * nN], nN);
+  for (int p = 0; p < NFP; p++) frhost::shapeFunctions(D, &c->ops.locFpts[3 * p], &shF[(size_t)p * nN], nN);
   const std::vector<double> &E = c->ops.spts_to_fpts;  // nFpts x nSpts
 
   std::ofstream out(iterName(name, "%s.csv.%.09d", iter), std::ios::binary);

@@ -295,7 +295,7 @@ std::string buildTriOperators(const fr_config &cfg, Operators &o) {
   fill(o.locSpts, nS, o.dshapeSpts);
   fill(o.locFpts, nF, o.dshapeFpts);
   o.shapeSpts.assign((size_t)nS * 3, 0.0);
-  for (int p = 0; p < nS; p++) shapeLinear(2, &o.locSpts[3 * p], &o.shapeSpts[(size_t)p * 3], 3);
+  for (int p = 0; p < nS; p++) shapeFunctions(2, &o.locSpts[3 * p], &o.shapeSpts[(size_t)p * 3], 3);
   // kernel table: Dr | Ds | E | L
   o.oned.clear();
   o.oned.insert(o.oned.end(), Dr.begin(), Dr.end());

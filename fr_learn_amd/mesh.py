@@ -1,4 +1,4 @@
-"""Linear hex / quad meshes for the FR engine.
+"""Hex / quad / triangle meshes for the FR engine.
 
 A mesh is what fr_create consumes (include/fr.h `fr_mesh`): vertex
 coordinates, element -> vertex lists in Gmsh corner order, an optional
@@ -169,7 +169,8 @@ def write_gmsh(mesh, path, names=None):
     """Write a Gmsh 2.2 ASCII mesh with one physical volume "fluid" and one
     physical surface per boundary (the format read by geo::readGmsh,
     src/geo.cpp:448-770).  Boundary faces are emitted as linear quads/lines;
-    cells as hexes, quads or triangles (Gmsh types 5, 3, 2)."""
+    cells as hexes (Gmsh types 5, 17, 12, 92, 93, 94 by node count), quads or
+    triangles (3, 2)."""
     D = mesh.nDims
     names = names or [f"bnd{i}" for i in range(len(mesh.bounds))]
     lines = ["$MeshFormat", "2.2 0 8", "$EndMeshFormat", "$PhysicalNames", str(len(mesh.bounds) + 1)]
@@ -196,7 +197,10 @@ def write_gmsh(mesh, path, names=None):
                     elems.append((3 if D == 3 else 1, b + 1, vs))
                     break
     for e in range(mesh.nEles):
-        elems.append((5 if D == 3 else 2 if tri else 3, len(mesh.bounds) + 1, mesh.c2v[e]))
+        cell = mesh.c2v[e]
+        if D == 3 and mesh.nNodes == 20:
+            cell = np.concatenate([cell[:8], cell[EDGE20]])  # the file order of the edge nodes
+        elems.append((GMSH_HEX_TYPE[mesh.nNodes] if D == 3 else 2 if tri else 3, len(mesh.bounds) + 1, cell))
     lines.append("$Elements")
     lines.append(str(len(elems)))
     for i, (t, tag, vs) in enumerate(elems):
@@ -209,6 +213,15 @@ def write_gmsh(mesh, path, names=None):
 # Boundary-condition names of the mesh file / input (bcStr2Num, src/global.cpp:11-25);
 # "periodic" and "overset" have no counterpart here.
 BC_NAMES = dict(BC, none=-1, fluid=-1)
+
+# cell node counts by Gmsh element type: hexahedra of 8 nodes (type 5), 20
+# (quadratic serendipity, 17), 27, 64, 125, 216 (Lagrange, 12 / 92 / 93 / 94)
+# (geo.cpp:586-647); in 2D linear quads (3) and triangles (2)
+GMSH_HEX_NODES = {5: 8, 17: 20, 12: 27, 92: 64, 93: 125, 94: 216}
+GMSH_HEX_TYPE = {v: k for k, v in GMSH_HEX_NODES.items()}
+# the 20-node hex's edge nodes: the k-th edge node of the file goes to node
+# EDGE20[k] (geo.cpp:603-609)
+EDGE20 = [8, 11, 12, 9, 13, 10, 14, 15, 16, 19, 17, 18]
 
 # vertices per boundary element, by Gmsh element type (geo.cpp:670-730)
 _GMSH_FACE_PTS = {1: 2, 2: 3, 3: 4, 10: 4, 16: 4, 36: 4, 37: 4, 38: 4, 8: 3, 26: 4, 27: 5, 28: 6, 62: 7,
@@ -224,7 +237,10 @@ def read_gmsh(path, mesh_bounds=None):
     linear quads (type 3) or linear triangles (type 2; one type per mesh;
     triangles are turned counter-clockwise, the library's corner order --
     FlurryPP itself reads no triangles); every other element is a boundary
-    face whose vertices join its group's vertex set.  Vertices are read in
+    face whose vertices join its group's vertex set.  Hexes may be curved:
+    20-node serendipity (type 17, its edge nodes permuted as the reference
+    stores them) and 27 / 64 / 125 / 216-node Lagrange (types 12, 92, 93, 94,
+    in file order); faces are matched by their corners.  Vertices are read in
     file order.  Errors raise ValueError with the reference's message."""
     mesh_bounds = {k.lower(): v for k, v in (mesh_bounds or {}).items()}
     with open(path) as fh:
@@ -266,7 +282,7 @@ def read_gmsh(path, mesh_bounds=None):
     i = section("$Elements")
     nEl = int(lines[i].split()[0])
     cells, bnd = [], [set() for _ in bcs]
-    cell_types = {5: 8} if nDims == 3 else {3: 4, 2: 3}
+    cell_types = GMSH_HEX_NODES if nDims == 3 else {3: 4, 2: 3}
     cell_type = None
     for k in range(nEl):
         tok = [int(t) for t in lines[i + 1 + k].split()]
@@ -278,12 +294,18 @@ def read_gmsh(path, mesh_bounds=None):
         if g == -1:
             if etype not in cell_types:
                 raise ValueError(f"Gmsh element ID {k}, Gmsh Element Type = {etype}: element type not recognized "
-                                 f"(linear {'hexahedra' if nDims == 3 else 'quadrilaterals or triangles'} only)")
+                                 f"({'hexahedra' if nDims == 3 else 'linear quadrilaterals or triangles'} only)")
             if cell_type is None:
                 cell_type = etype
             elif etype != cell_type:
                 raise ValueError(f"Gmsh element ID {k}: mixed cell types in one mesh")
-            cells.append([v - 1 for v in nodes[:cell_types[etype]]])
+            cell = [v - 1 for v in nodes[:cell_types[etype]]]
+            if etype == 17:
+                reordered = cell[:8] + [0] * 12
+                for k2, n in enumerate(EDGE20):
+                    reordered[n] = cell[8 + k2]
+                cell = reordered
+            cells.append(cell)
         else:
             if etype not in _GMSH_FACE_PTS:
                 raise ValueError(f"Gmsh element ID {k}, Gmsh Element Type = {etype}: "

@@ -73,14 +73,18 @@ typedef struct fr_config {
   double TWall, RGas, uWall, vWall, wWall;
 } fr_config;
 
-/* Unstructured linear mesh (hex: 8 Gmsh-ordered nodes, quad: 4).
+/* Unstructured mesh.  Hexes of 8 nodes (linear), 20 (quadratic
+ * serendipity) or 27, 64, 125, 216 (Lagrange, equispaced, of order 2..5)
+ * in the node order geo::readGmsh stores (geo.cpp:597-647: Gmsh's order,
+ * the 20-node edges permuted) and shape_hex uses (funcs.cpp:178-566); quads
+ * of 4 nodes; triangles of 3.  Faces are matched by their corner nodes.
  * vmap (optional) gives a canonical vertex id used only for face matching:
  * identifying the vertices of opposite box faces makes them periodic. */
 typedef struct fr_mesh {
   int nVerts;
   const double *xv;   /* [nVerts][3] */
   int nEles;
-  int nNodes;         /* nodes per element: 8 (hex) or 4 (quad) */
+  int nNodes;         /* nodes per element: 8, 20, 27, 64, 125, 216 (hex), 4 (quad), 3 (triangle) */
   const int *c2v;     /* [nEles][nNodes] */
   const int *vmap;    /* [nVerts] or NULL */
   int nBounds;

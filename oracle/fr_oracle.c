@@ -395,35 +395,193 @@ static void setup_operators(or_solver *s) {
 }
 
 /* ------------------------------------------------------------------------ */
-/* shape functions, src/funcs.cpp: linear Lagrange hex (8 nodes, 178-568 /   */
-/* 663-1607) and quad (4 nodes); node order is Gmsh's                          */
+/* shape functions, src/funcs.cpp: quad (4 nodes, shape_quad 121-176) and    */
+/* hex (shape_hex 178-566, dshape_hex 680-1607): 8 and (N+1)^3 nodes by the  */
+/* Lagrange recursion over nested shells, 20 nodes serendipity; Gmsh order    */
 /* ------------------------------------------------------------------------ */
-static const double XL[2] = {-1., 1.};
-static void shape_dshape(int nDims, const double *r, double *sh, double *dsh /* [node][3] */) {
-  static const int HI[8] = {0, 1, 1, 0, 0, 1, 1, 0}, HJ[8] = {0, 0, 1, 1, 0, 0, 1, 1}, HK[8] = {0, 0, 0, 0, 1, 1, 1, 1};
-  if (nDims == 3) {
-    for (int n = 0; n < 8; n++) {
-      int a = HI[n], b = HJ[n], c = HK[n];
-      double la = lagrange(XL, 2, r[0], a), lb = lagrange(XL, 2, r[1], b), lc = lagrange(XL, 2, r[2], c);
-      if (sh) sh[n] = la * lb * lc;
-      if (dsh) {
-        dsh[3 * n + 0] = dlagrange(XL, 2, r[0], a) * lb * lc;
-        dsh[3 * n + 1] = la * dlagrange(XL, 2, r[1], b) * lc;
-        dsh[3 * n + 2] = la * lb * dlagrange(XL, 2, r[2], c);
-      }
+/* one node of the Lagrange hex: lattice indices along xi, eta, mu */
+typedef struct { int a, b, c; } or_node;
+
+/* the shell recursion of shape_hex (funcs.cpp:213-566), as the list of each
+ * node's lattice indices in node order; returns the node count */
+static int hex_nodes(int nSide, or_node *nd) {
+  int n = 0;
+  const int nLevels = nSide / 2, isOdd = nSide % 2, h = nSide / 2;
+#define PUT(x, y, z) (nd[n].a = (x), nd[n].b = (y), nd[n].c = (z), n++)
+  for (int i = 0; i < nLevels; i++) {
+    int i2 = (nSide - 1) - i;
+    PUT(i, i, i); PUT(i2, i, i); PUT(i2, i2, i); PUT(i, i2, i);
+    PUT(i, i, i2); PUT(i2, i, i2); PUT(i2, i2, i2); PUT(i, i2, i2);
+    int nSide2 = nSide - 2 * (i + 1);
+    /* edges: 12 blocks of nSide2, written slot by slot */
+    or_node *e = nd + n;
+    for (int j = 0; j < nSide2; j++) {
+#define E(slot, x, y, z) (e[(slot) * nSide2 + j].a = (x), e[(slot) * nSide2 + j].b = (y), e[(slot) * nSide2 + j].c = (z))
+      E(0, i + 1 + j, i, i);      E(3, i2, i + 1 + j, i);      E(5, i2 - 1 - j, i2, i);  E(1, i, i + 1 + j, i);
+      E(2, i, i, i + 1 + j);      E(4, i2, i, i + 1 + j);      E(6, i2, i2, i + 1 + j);  E(7, i, i2, i + 1 + j);
+      E(8, i + 1 + j, i, i2);     E(10, i2, i + 1 + j, i2);    E(11, i2 - 1 - j, i2, i2); E(9, i, i + 1 + j, i2);
+#undef E
     }
-  } else {
-    for (int n = 0; n < 4; n++) {
-      int a = HI[n], b = HJ[n];
-      double la = lagrange(XL, 2, r[0], a), lb = lagrange(XL, 2, r[1], b);
-      if (sh) sh[n] = la * lb;
-      if (dsh) {
-        dsh[3 * n + 0] = dlagrange(XL, 2, r[0], a) * lb;
-        dsh[3 * n + 1] = la * dlagrange(XL, 2, r[1], b);
-        dsh[3 * n + 2] = 0;
+    n += 12 * nSide2;
+    int nLevels2 = nSide2 / 2, isOdd2 = nSide2 % 2;
+    /* faces: bottom (mu = i), front (eta = i), left (xi = i), right (xi = i2),
+     * back (eta = i2), top (mu = i2); each ring: 4 corners, then 4 edges of nSide3 */
+    for (int f = 0; f < 6; f++) {
+      for (int j0 = 0; j0 < nLevels2; j0++) {
+        int j = j0 + i + 1, j2 = i + 1 + (nSide2 - 1) - j0, nSide3 = nSide2 - 2 * (j0 + 1);
+        switch (f) {
+        case 0: PUT(j, j, i); PUT(j, j2, i); PUT(j2, j2, i); PUT(j2, j, i); break;
+        case 1: PUT(j, i, j); PUT(j2, i, j); PUT(j2, i, j2); PUT(j, i, j2); break;
+        case 2: PUT(i, j, j); PUT(i, j, j2); PUT(i, j2, j2); PUT(i, j2, j); break;
+        case 3: PUT(i2, j, j); PUT(i2, j2, j); PUT(i2, j2, j2); PUT(i2, j, j2); break;
+        case 4: PUT(j2, i2, j); PUT(j, i2, j); PUT(j, i2, j2); PUT(j2, i2, j2); break;
+        default: PUT(j, j, i2); PUT(j2, j, i2); PUT(j2, j2, i2); PUT(j, j2, i2); break;
+        }
+        or_node *q = nd + n;
+        for (int k = 0; k < nSide3; k++) {
+          int up = j + 1 + k, dn = j2 - 1 - k;
+#define Q(slot, x, y, z) (q[(slot) * nSide3 + k].a = (x), q[(slot) * nSide3 + k].b = (y), q[(slot) * nSide3 + k].c = (z))
+          switch (f) {
+          case 0: Q(0, j, up, i); Q(1, up, j2, i); Q(2, j2, dn, i); Q(3, dn, j, i); break;
+          case 1: Q(0, up, i, j); Q(1, j2, i, up); Q(2, dn, i, j2); Q(3, j, i, dn); break;
+          case 2: Q(0, i, j, up); Q(1, i, up, j2); Q(2, i, j2, dn); Q(3, i, dn, j); break;
+          case 3: Q(0, i2, up, j); Q(1, i2, j2, up); Q(2, i2, dn, j2); Q(3, i2, j, dn); break;
+          case 4: Q(0, dn, i2, j); Q(1, j, i2, up); Q(2, up, i2, j2); Q(3, j2, i2, dn); break;
+          default: Q(0, up, j, i2); Q(1, j2, up, i2); Q(2, dn, j2, i2); Q(3, j, dn, i2); break;
+          }
+#undef Q
+        }
+        n += 4 * nSide3;
+      }
+      if (isOdd2) {
+        switch (f) {
+        case 0: PUT(h, h, i); break;
+        case 1: PUT(h, i, h); break;
+        case 2: PUT(i, h, h); break;
+        case 3: PUT(i2, h, h); break;
+        case 4: PUT(h, i2, h); break;
+        default: PUT(h, h, i2); break;
+        }
       }
     }
   }
+  if (isOdd) PUT(h, h, h);
+#undef PUT
+  return n;
+}
+
+static const double XL[2] = {-1., 1.};
+/* shape values sh[node] and derivatives dsh[node][3] (either may be NULL) */
+static void shape_dshape_n(int nDims, int nNodes, const double *r, double *sh, double *dsh) {
+  if (nDims == 2) {
+    static const int QI[4] = {0, 1, 1, 0}, QJ[4] = {0, 0, 1, 1};
+    for (int n = 0; n < 4; n++) {
+      double la = lagrange(XL, 2, r[0], QI[n]), lb = lagrange(XL, 2, r[1], QJ[n]);
+      if (sh) sh[n] = la * lb;
+      if (dsh) {
+        dsh[3 * n + 0] = dlagrange(XL, 2, r[0], QI[n]) * lb;
+        dsh[3 * n + 1] = la * dlagrange(XL, 2, r[1], QJ[n]);
+        dsh[3 * n + 2] = 0;
+      }
+    }
+    return;
+  }
+  if (nNodes == 20) {
+    static const double XI[8] = {-1, 1, 1, -1, -1, 1, 1, -1}, ETA[8] = {-1, -1, 1, 1, -1, -1, 1, 1},
+                        MU[8] = {-1, -1, -1, -1, 1, 1, 1, 1};
+    double xi = r[0], eta = r[1], mu = r[2];
+    for (int i = 0; i < 8; i++) {
+      if (sh) sh[i] = .125 * (1 + xi * XI[i]) * (1 + eta * ETA[i]) * (1 + mu * MU[i]) *
+                      (xi * XI[i] + eta * ETA[i] + mu * MU[i] - 2);
+      if (dsh) {
+        dsh[3 * i + 0] = .125 * XI[i] * (1 + eta * ETA[i]) * (1 + mu * MU[i]) *
+                         (2 * xi * XI[i] + eta * ETA[i] + mu * MU[i] - 1);
+        dsh[3 * i + 1] = .125 * ETA[i] * (1 + xi * XI[i]) * (1 + mu * MU[i]) *
+                         (xi * XI[i] + 2 * eta * ETA[i] + mu * MU[i] - 1);
+        dsh[3 * i + 2] = .125 * MU[i] * (1 + xi * XI[i]) * (1 + eta * ETA[i]) *
+                         (xi * XI[i] + eta * ETA[i] + 2 * mu * MU[i] - 1);
+      }
+    }
+    /* edge nodes (funcs.cpp:197-211, 705-749) */
+    if (sh) {
+      sh[8] = .25 * (1 - xi * xi) * (1 - eta) * (1 - mu);
+      sh[10] = .25 * (1 - xi * xi) * (1 + eta) * (1 - mu);
+      sh[16] = .25 * (1 - xi * xi) * (1 - eta) * (1 + mu);
+      sh[18] = .25 * (1 - xi * xi) * (1 + eta) * (1 + mu);
+      sh[9] = .25 * (1 - eta * eta) * (1 + xi) * (1 - mu);
+      sh[11] = .25 * (1 - eta * eta) * (1 - xi) * (1 - mu);
+      sh[17] = .25 * (1 - eta * eta) * (1 + xi) * (1 + mu);
+      sh[19] = .25 * (1 - eta * eta) * (1 - xi) * (1 + mu);
+      sh[12] = .25 * (1 - mu * mu) * (1 - xi) * (1 - eta);
+      sh[13] = .25 * (1 - mu * mu) * (1 + xi) * (1 - eta);
+      sh[14] = .25 * (1 - mu * mu) * (1 + xi) * (1 + eta);
+      sh[15] = .25 * (1 - mu * mu) * (1 - xi) * (1 + eta);
+    }
+    if (dsh) {
+      double *o = dsh;
+      o[3 * 8 + 0] = -.5 * xi * (1 - eta) * (1 - mu);  o[3 * 8 + 1] = -.25 * (1 - xi * xi) * (1 - mu);
+      o[3 * 8 + 2] = -.25 * (1 - xi * xi) * (1 - eta);
+      o[3 * 10 + 0] = -.5 * xi * (1 + eta) * (1 - mu); o[3 * 10 + 1] = .25 * (1 - xi * xi) * (1 - mu);
+      o[3 * 10 + 2] = -.25 * (1 - xi * xi) * (1 + eta);
+      o[3 * 16 + 0] = -.5 * xi * (1 - eta) * (1 + mu); o[3 * 16 + 1] = -.25 * (1 - xi * xi) * (1 + mu);
+      o[3 * 16 + 2] = .25 * (1 - xi * xi) * (1 - eta);
+      o[3 * 18 + 0] = -.5 * xi * (1 + eta) * (1 + mu); o[3 * 18 + 1] = .25 * (1 - xi * xi) * (1 + mu);
+      o[3 * 18 + 2] = .25 * (1 - xi * xi) * (1 + eta);
+      o[3 * 9 + 1] = -.5 * eta * (1 + xi) * (1 - mu);  o[3 * 9 + 0] = .25 * (1 - eta * eta) * (1 - mu);
+      o[3 * 9 + 2] = -.25 * (1 - eta * eta) * (1 + xi);
+      o[3 * 11 + 1] = -.5 * eta * (1 - xi) * (1 - mu); o[3 * 11 + 0] = -.25 * (1 - eta * eta) * (1 - mu);
+      o[3 * 11 + 2] = -.25 * (1 - eta * eta) * (1 - xi);
+      o[3 * 17 + 1] = -.5 * eta * (1 + xi) * (1 + mu); o[3 * 17 + 0] = .25 * (1 - eta * eta) * (1 + mu);
+      o[3 * 17 + 2] = .25 * (1 - eta * eta) * (1 + xi);
+      o[3 * 19 + 1] = -.5 * eta * (1 - xi) * (1 + mu); o[3 * 19 + 0] = -.25 * (1 - eta * eta) * (1 + mu);
+      o[3 * 19 + 2] = .25 * (1 - eta * eta) * (1 - xi);
+      o[3 * 12 + 2] = -.5 * mu * (1 - xi) * (1 - eta); o[3 * 12 + 0] = -.25 * (1 - mu * mu) * (1 - eta);
+      o[3 * 12 + 1] = -.25 * (1 - mu * mu) * (1 - xi);
+      o[3 * 13 + 2] = -.5 * mu * (1 + xi) * (1 - eta); o[3 * 13 + 0] = .25 * (1 - mu * mu) * (1 - eta);
+      o[3 * 13 + 1] = -.25 * (1 - mu * mu) * (1 + xi);
+      o[3 * 14 + 2] = -.5 * mu * (1 + xi) * (1 + eta); o[3 * 14 + 0] = .25 * (1 - mu * mu) * (1 + eta);
+      o[3 * 14 + 1] = .25 * (1 - mu * mu) * (1 + xi);
+      o[3 * 15 + 2] = -.5 * mu * (1 - xi) * (1 + eta); o[3 * 15 + 0] = -.25 * (1 - mu * mu) * (1 + eta);
+      o[3 * 15 + 1] = .25 * (1 - mu * mu) * (1 - xi);
+    }
+    return;
+  }
+  int nSide = (int)lround(cbrt((double)nNodes));
+  double xl[6];
+  for (int i = 0; i < nSide; i++) xl[i] = -1. + i * (2. / (nSide - 1));
+  or_node nd[216];
+  hex_nodes(nSide, nd);
+  for (int n = 0; n < nNodes; n++) {
+    double la = lagrange(xl, nSide, r[0], nd[n].a), lb = lagrange(xl, nSide, r[1], nd[n].b),
+           lc = lagrange(xl, nSide, r[2], nd[n].c);
+    if (sh) sh[n] = la * lb * lc;
+    if (dsh) {
+      dsh[3 * n + 0] = dlagrange(xl, nSide, r[0], nd[n].a) * lb * lc;
+      dsh[3 * n + 1] = la * dlagrange(xl, nSide, r[1], nd[n].b) * lc;
+      dsh[3 * n + 2] = la * lb * dlagrange(xl, nSide, r[2], nd[n].c);
+    }
+  }
+}
+/* linear elements (8-node hex, 4-node quad) */
+static void shape_dshape(int nDims, const double *r, double *sh, double *dsh) {
+  shape_dshape_n(nDims, nDims == 3 ? 8 : 4, r, sh, dsh);
+}
+
+/* test hook: shape values / derivatives of an nNodes element at r[3] */
+void or_shape(int nDims, int nNodes, const double *r, double *sh, double *dsh) {
+  shape_dshape_n(nDims, nNodes, r, sh, dsh);
+}
+/* test hook: lattice indices of the nodes of the nSide^3 Lagrange hex */
+int or_hex_lattice(int nSide, int *abc) {
+  or_node nd[216];
+  if (nSide < 2 || nSide > 6) return -1;
+  int n = hex_nodes(nSide, nd);
+  for (int i = 0; i < n; i++) {
+    abc[3 * i] = nd[i].a;
+    abc[3 * i + 1] = nd[i].b;
+    abc[3 * i + 2] = nd[i].c;
+  }
+  return n;
 }
 
 /* setPosSptsFpts (solver.cpp:904-951) + calcTransforms (solver.cpp:1227-1356) */
@@ -434,16 +592,16 @@ static int setup_geometry(or_solver *s, const double *xv, const int *c2v) {
       for (int d = 0; d < D; d++) s->nodes[IDX3(n, e, d, nE, D)] = xv[3 * c2v[e * nN + n] + d];
   double *shs = xcalloc((size_t)nS * nN, 8), *dshs = xcalloc((size_t)D * nS * nN, 8);
   double *dshf = xcalloc((size_t)D * nF * nN, 8);
-  double tmp[8], dtmp[24];
+  double tmp[216], dtmp[3 * 216];
   for (int spt = 0; spt < nS; spt++) {
-    shape_dshape(D, s->loc_spts + 3 * spt, tmp, dtmp);
+    shape_dshape_n(D, nN, s->loc_spts + 3 * spt, tmp, dtmp);
     for (int n = 0; n < nN; n++) {
       shs[spt * nN + n] = tmp[n];
       for (int d = 0; d < D; d++) dshs[((size_t)d * nS + spt) * nN + n] = dtmp[3 * n + d];
     }
   }
   for (int fpt = 0; fpt < nF; fpt++) {
-    shape_dshape(D, s->loc_fpts + 3 * fpt, tmp, dtmp);
+    shape_dshape_n(D, nN, s->loc_fpts + 3 * fpt, tmp, dtmp);
     for (int n = 0; n < nN; n++)
       for (int d = 0; d < D; d++) dshf[((size_t)d * nF + fpt) * nN + n] = dtmp[3 * n + d];
   }
@@ -654,7 +812,11 @@ or_solver *or_create(const or_params *p, int nVerts, const double *xv, int nEles
                      const int *vmap, int nBounds, const int *bcType, const int *bndOff, const int *bndPts) {
   if (p->nDims != 2 && p->nDims != 3) { set_err("nDims must be 2 or 3"); return NULL; }
   if (p->order < 0 || p->order > 10) { set_err("order out of range"); return NULL; }
-  if (nNodes != (p->nDims == 3 ? 8 : 4)) { set_err("only linear hex/quad elements"); return NULL; }
+  {
+    int ok = p->nDims == 2 ? nNodes == 4 : (nNodes == 8 || nNodes == 20 || nNodes == 27 || nNodes == 64 ||
+                                            nNodes == 125 || nNodes == 216);
+    if (!ok) { set_err("hexes of 8, 20, 27, 64, 125 or 216 nodes, or 4-node quads"); return NULL; }
+  }
   if (p->nDims == 3 && p->equation == OR_NAVIER_STOKES && p->riemannType == 1) {
     set_err("Roe not implemented in 3D"); /* face.cpp:362-363 */
     return NULL;
@@ -1695,11 +1857,11 @@ int or_integrate_error(const or_solver *s, const or_error_params *ep, double *ou
     double *Uq = xcalloc((size_t)nQ * nE * K, 8), *Jq = xcalloc((size_t)nQ * nE, 8);
     oracle_dgemm(nQ, nE * K, nS, 1.0, A, s->U_spts, 0.0, Uq);
     oracle_dgemm(nQ, nE, nS, 1.0, A, s->detJac_spts, 0.0, Jq);
-    double sh[8];
+    double sh[216];
     for (int ic = 0; ic < nE; ic++)
       for (int q = 0; q < nQ; q++) {
         /* ele::calcPos: the shape functions at the point times the nodes */
-        shape_dshape(D, loc + 3 * q, sh, NULL);
+        shape_dshape_n(D, nN, loc + 3 * q, sh, NULL);
         pos[0] = pos[1] = pos[2] = 0;
         for (int n = 0; n < nN; n++)
           for (int d = 0; d < D; d++) pos[d] += sh[n] * s->nodes[IDX3(n, ic, d, nE, D)];

@@ -108,6 +108,11 @@ typedef struct or_error_params {
   double time, rhoIC, pIC, vortexAngle, xmin, xmax, ymin, ymax;
 } or_error_params;
 int or_integrate_error(const or_solver *s, const or_error_params *ep, double *out);
+/* test hooks: shape functions of an nNodes element at r[3] (sh[nNodes],
+ * dsh[nNodes][3], either may be NULL), and the lattice indices of the nodes
+ * of the nSide^3 Lagrange hex in node order (abc[n][3]; returns the count) */
+void or_shape(int nDims, int nNodes, const double *r, double *sh, double *dsh);
+int or_hex_lattice(int nSide, int *abc);
 
 /* pointwise test hooks (flux.cpp / face.cpp / boundFace.cpp / polynomials.cpp
  * restatements), so tests can pin them against textbook formulas */

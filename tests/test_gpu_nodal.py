@@ -114,7 +114,11 @@ def test_tgv_48_nodal_matches_per_point():
     g.set_state(U0)
     assert g.geometry_mode(0) == 0
     g.update(5)
-    err = rel_err(a, g.get_state()).max()
-    print("nodal vs per point after 5 steps", err, flush=True)
+    b = g.get_state()
+    # against the state's largest entry: rho*w starts at zero, so its own
+    # maximum after five steps (~1e-4) would magnify the ulp-level metric
+    # differences of the other fields' fluxes (the pressure is ~71)
+    err = np.abs(a - b).max() / np.abs(b).max()
+    print("nodal vs per point after 5 steps", err, "per field", rel_err(a, b), flush=True)
     assert err <= 1e-13
     g.close()
