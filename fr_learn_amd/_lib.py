@@ -100,6 +100,7 @@ SIGNATURES = {
     "fr_synchronize": (ctypes.c_int, [_P]),
     "fr_geometry_mode": (ctypes.c_int, [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
     "fr_integrate_error": (ctypes.c_int, [_P, ctypes.POINTER(FrErrorParams), _DP]),
+    "fr_reduce_noalloc": (ctypes.c_int, [_P, ctypes.c_int, _DP]),
     "fr_write_data": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_double]),
     "fr_read_restart": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.c_int, _DP]),
     "fr_write_checkpoint": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.c_int, ctypes.c_double]),

@@ -4,6 +4,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <utility>
@@ -511,6 +512,7 @@ int createImpl(const fr_config *cfg, const fr_mesh *m, int precision, fr_ctx *c)
 
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(FR_EHIP, "no HIP device available");
+  HIP_TRY(hipGetDevice(&c->dev));
   HIP_TRY(hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking));
   c->stream = c->own;
 
@@ -1493,6 +1495,133 @@ int fr_residual_bytes(fr_ctx *c, double *bytes) {
   const double s = (double)c->tsize(), F = (double)c->conn.faces.size();
   *bytes = (double)c->nE * c->NS * (2.0 * c->NF * s + (double)c->D * c->D * s) +
            F * c->NFPF * (c->D + 1) * s + 20.0 * F;
+  return 0;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// fr_reduce_noalloc: fr_reduce with its block partial sums in `scratch`, a
+// fixed device array, instead of the context's buffer (which fr_reduce
+// allocates on first use).  `scratch` is a host-side stand-in in host code:
+// the device address of each GPU's copy comes from hipGetSymbolAddress,
+// taken once per device with that device current, and only that address is
+// passed to kernels.  Every call on a GPU shares that GPU's copy; calls that
+// overlap on one GPU (on different streams or threads) race.  Each call
+// writes every partial it reads (one entry per block and field), so nothing
+// needs clearing.
+// ---------------------------------------------------------------------------
+namespace {
+
+__device__ __align__(256) double scratch[FR_NOALLOC_SCRATCH_DOUBLES];
+
+constexpr int kMaxDevices = 64;
+std::mutex gScratchMutex;
+double *gScratchPtr[kMaxDevices] = {};
+
+// Device address of device dev's copy of `scratch` (dev is current).
+int scratchAddress(int dev, double **out) {
+  if (dev < 0 || dev >= kMaxDevices) return fail(FR_EINVAL, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(gScratchMutex);
+  if (!gScratchPtr[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(FR_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    gScratchPtr[dev] = (double *)p;
+  }
+  *out = gScratchPtr[dev];
+  return 0;
+}
+
+// Makes device `dev` current for the lifetime of the guard.
+struct DeviceGuard {
+  int prev = -1;
+  hipError_t err = hipSuccess;
+  explicit DeviceGuard(int dev) {
+    int cur = -1;
+    err = hipGetDevice(&cur);
+    if (err == hipSuccess && cur != dev) {
+      err = hipSetDevice(dev);
+      if (err == hipSuccess) prev = cur;
+    }
+  }
+  ~DeviceGuard() {
+    if (prev >= 0) (void)hipSetDevice(prev);
+  }
+};
+
+// out[k] = the block partial sums part[b * stride + k], b = 0 .. nb-1, added
+// (or, max != 0, combined by the NaN-keeping maximum) in block order, as the
+// host finishes them in fr_reduce: one thread per k.
+__global__ void k_finish_partials(const double *__restrict__ part, int nb, int stride, int nk, int max,
+                                  double *__restrict__ out) {
+  const int k = threadIdx.x;
+  if (k >= nk) return;
+  double r = 0.0;
+  for (int b = 0; b < nb; b++) {
+    const double v = part[(size_t)b * stride + k];
+    r = max ? frdev::maxKeepNaN(r, v) : r + v;
+  }
+  out[k] = r;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fr_reduce_noalloc(fr_ctx *c, int kind, double *out) {
+  if (!c || !out) return fail(FR_EINVAL, "null argument");
+  if (!(kind >= 1 && kind <= 3) && kind != 10) return fail(FR_EINVAL, "unknown reduction kind");
+  if (kind == 10 && c->cfg.equation != 1)
+    return fail(FR_EINVAL, "kinetic energy needs the Euler / Navier-Stokes equations");
+  const int nb = (int)nblk(c->nE);
+  const int stride = kind == 10 ? 2 : 5, nk = kind == 10 ? 2 : c->NF;
+  // the block partials, then the finished values
+  const size_t need = (size_t)nb * stride + 8;
+  if (need > (size_t)FR_NOALLOC_SCRATCH_DOUBLES)
+    return fail(FR_EINVAL, "fr_reduce_noalloc: the mesh needs " + std::to_string(need) +
+                               " doubles of scratch, more than FR_NOALLOC_SCRATCH_DOUBLES (" +
+                               std::to_string(FR_NOALLOC_SCRATCH_DOUBLES) + "): at most " +
+                               std::to_string((FR_NOALLOC_SCRATCH_DOUBLES - 8) / 5 * 256) + " elements");
+  DeviceGuard dg(c->dev);
+  if (dg.err != hipSuccess) return fail(FR_EHIP, std::string("hipSetDevice: ") + hipGetErrorString(dg.err));
+  double *sp = nullptr;
+  int rc = scratchAddress(c->dev, &sp);
+  if (rc) return rc;
+  double *part = sp, *fin = sp + (size_t)nb * stride;
+  if (kind == 10) {
+    if (c->D == 3)
+      hipLaunchKernelGGL(frdev::k_ke<3>, dim3(nb), dim3(256), 0, c->stream, c->U, c->detJ, c->wts, c->NS, c->nE,
+                         part);
+    else
+      hipLaunchKernelGGL(frdev::k_ke<2>, dim3(nb), dim3(256), 0, c->stream, c->U, c->detJ, c->wts, c->NS, c->nE,
+                         part);
+  } else if (c->prec == FR_PREC_FP32) {
+    hipLaunchKernelGGL(frdev::k_norm<float>, dim3(nb), dim3(256), 0, c->stream, (const float *)c->divF[0], c->detJ,
+                       c->wts, kind, c->NF, c->NS, c->nE, part);
+  } else {
+    hipLaunchKernelGGL(frdev::k_norm<double>, dim3(nb), dim3(256), 0, c->stream, (const double *)c->divF[0],
+                       c->detJ, c->wts, kind, c->NF, c->NS, c->nE, part);
+  }
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(k_finish_partials, dim3(1), dim3(64), 0, c->stream, part, nb, stride, nk, kind == 3 ? 1 : 0,
+                     fin);
+  HIP_TRY(hipGetLastError());
+  double h[8];
+  HIP_TRY(hipMemcpyAsync(h, fin, (size_t)nk * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (kind == 10) {
+    out[0] = h[0];
+    out[1] = h[1];
+    return 0;
+  }
+  for (int k = 0; k < c->NF; k++) {
+    double r = h[k];
+    if (kind == 2) r = std::sqrt(r);
+    if (std::isnan(r)) return fail(FR_ENAN, "NaN Encountered in Solution Residual!");
+    out[k] = r;
+  }
   return 0;
 }
 

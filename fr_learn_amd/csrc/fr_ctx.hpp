@@ -57,6 +57,7 @@ struct fr_batch;
 struct fr_ctx {
   fr_config cfg{};
   int prec = FR_PREC_FP64;
+  int dev = 0;  // the HIP device the context was created on
   frhost::Operators ops;
   frhost::Connectivity conn;
   int D = 3, P = 3, NS = 0, NFP = 0, NFPF = 0, NFACE = 0, NF = 0, nE = 0, nRK = 0;

@@ -137,6 +137,13 @@ class Solver:
         _lib.check(self._lib.fr_integrate_error(self._h, ctypes.byref(p), _lib.dptr(out)))
         return out
 
+    def reduce_noalloc(self, kind):
+        """fr_reduce_noalloc: fr_reduce's values (kind 1-3 residual norms of
+        divF_spts[0], 10 kinetic energy and volume) with no device allocation."""
+        out = np.zeros(max(self.nFields, 2))
+        _lib.check(self._lib.fr_reduce_noalloc(self._h, int(kind), _lib.dptr(out)))
+        return out[:2] if kind == 10 else out[:self.nFields]
+
     def kinetic_energy(self):
         out = np.zeros(2)
         _lib.check(self._lib.fr_reduce(self._h, 10, _lib.dptr(out)))

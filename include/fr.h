@@ -150,6 +150,18 @@ int fr_copy_u0_u(fr_ctx *ctx);
  * (out[1]) by the spt quadrature of solver.cpp:1510-1526 */
 int fr_reduce(fr_ctx *ctx, int kind, double *out);
 
+/* fr_reduce without device allocation.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc).  Same kinds and results as
+ * fr_reduce, bit for bit; the block partial sums go to a fixed device array
+ * of FR_NOALLOC_SCRATCH_DOUBLES doubles (512 KiB, one per GPU, held until the
+ * process ends) instead of the context's buffer, so meshes of at most
+ * (FR_NOALLOC_SCRATCH_DOUBLES - 8) / 5 * 256 elements (3,354,624) are
+ * accepted and larger ones get FR_EINVAL.  Calls that overlap on one GPU
+ * share that array and race. */
+#define FR_NOALLOC_SCRATCH_DOUBLES 65536
+int fr_reduce_noalloc(fr_ctx *ctx, int kind, double *out);
+
 /* Error norms of U_spts: solver::integrateError (solver.cpp:1507-1574) with
  * calcError (funcs.cpp:1875-2056), the numbers writeError prints
  * (output.cpp:1055-1123, testCase 1).  The fields mirror the `input` members
