@@ -37,10 +37,13 @@ METRIC = "residual GDOF/s, 3D Euler hex P3, FP64 vs FP32-MFMA; % HBM roofline"
 HBM_PEAK = 8.0e12
 
 
-def tgv_setup(n, order=3, dt=2e-4):
+def tgv_setup(n, order=3, dt=2e-4, perturb=0.0):
+    """The TGV workload's config and mesh; perturb > 0 displaces the interior
+    vertices randomly (a fraction of the cell size) so that no element is
+    affine."""
     from fr_learn_amd import cases, mesh
     cfg = cases.config(nDims=3, order=order, equation=1, riemannType=0, timeType=4, dt=dt, gamma=1.4)
-    m = mesh.box(3, (n, n, n), (0, 0, 0), (2 * np.pi,) * 3)
+    m = mesh.box(3, (n, n, n), (0, 0, 0), (2 * np.pi,) * 3, perturb=perturb, seed=11)
     return cfg, m
 
 
@@ -117,7 +120,8 @@ def run_cpu_baseline(cfg, m, budget_s=10.0):
         pass
     return dict(value=gdof, unit="GDOF/s", cores=threads, kind="port",
                 sample=f"{nsteps} RK4 steps ({4 * nsteps} residuals) of the same 48^3 P3 TGV workload, "
-                       f"{dt:.1f} s, dense-operator oracle, {threads} OpenMP threads",
+                       f"{dt:.1f} s, dense-operator oracle with its own blocked dgemm (no OpenBLAS: the "
+                       f"image has no CBLAS), {threads} OpenMP threads",
                 single_core={"value": o.nEles * o.nSpts * 4 / dt1 / 1e9, "unit": "GDOF/s",
                              "sample": f"1 RK4 step of the same workload, {dt1:.1f} s, 1 thread"},
                 host={"cpu": model, "logical_cpus": os.cpu_count()})
@@ -282,10 +286,10 @@ def tgv_ns_adjoint(n=24, steps=200, every=20, eps=1e-6, seed=0):
             "eps": eps}
 
 
-def traffic_from_profile(precision):
+def traffic_from_profile(precision, geometry):
     """HBM bytes per launch of the residual kernel from the committed rocprofv3
     PMC summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor), if any."""
-    path = os.path.join(ROOT, "profiles", f"k_stage_tgv48_p3_{precision}_traffic.json")
+    path = os.path.join(ROOT, "profiles", f"k_stage_tgv48_p3_{precision}_{geometry}_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as fh:
@@ -301,9 +305,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 path")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-mesh workload")
-    ap.add_argument("--no-compact", action="store_true",
-                    help="skip the extra FP64 run with the affine mesh's compact geometry")
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
+    ap.add_argument("--geometry", choices=list(GEOMETRY), default="nodal",
+                    help="stage-kernel geometry of the headline run (default: metrics and normals formed in the "
+                         "kernel from each hex's corner nodes, valid for any linear hex)")
+    ap.add_argument("--no-variants", action="store_true",
+                    help="skip the extra FP64 runs with the other geometry layouts and the perturbed mesh")
     ap.add_argument("--copy-calibration", action="store_true", help="only the 256 MiB copy for PMC scaling")
     ap.add_argument("--copy-bandwidth", action="store_true", help="only measure the 1 GiB copy bandwidth")
     ap.add_argument("--adjoint", action="store_true",
@@ -327,7 +334,7 @@ def main():
     copy_gbps = copy_bandwidth_child()  # before this process touches the GPU
     cfg, m = tgv_setup(args.n)
     main_prec = args.precision
-    r = run_gpu(cfg, m, args.steps, args.warmup, main_prec)
+    r = run_gpu(cfg, m, args.steps, args.warmup, main_prec, geometry=args.geometry)
     dof = r["nEles"] * r["nSpts"]
     value = dof * 4 * args.steps / r["wall"] / 1e9
     achieved = r["bytes"] / (r["res_ms"] * 1e-3)
@@ -348,11 +355,16 @@ def main():
             "workload": "3D Euler Taylor-Green vortex, periodic [0,2pi]^3, 48^3 linear hexes, P=3, Ma=0.1, "
                         "Rusanov, RK4 (timeType 4), dt=2e-4",
             "cells": [args.n] * 3, "order": 3, "nEles": r["nEles"], "dof_per_field": dof, "fields": 5,
-            "precision": main_prec, "geometry": "per-point metrics and face normals (general hex layout)",
+            "precision": main_prec, "geometry": {
+                "nodal": "metrics at the solution points and normals at the flux points formed in the kernel "
+                         "from each element's 8 corner nodes (any linear hex; no affine assumption)",
+                "point": "per-point metrics and face normals read from HBM",
+                "compact": "one metric set per element and one normal per face (affine meshes only)",
+                "nodal-spt": "solution-point metrics formed in the kernel, face normals read from HBM"}[args.geometry],
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK, "traffic": traffic_from_profile(main_prec),
+            "frac": achieved / HBM_PEAK, "traffic": traffic_from_profile(main_prec, args.geometry),
             "kernel": "k_stage<3,3,5,...> (residual + RK stage update + next-stage face traces; "
                       "average over the RK4 stages, the final stage's variant included)",
             "algorithmic_bytes_per_launch": r["bytes"],
@@ -369,7 +381,7 @@ def main():
         },
     }
     if not args.no_fp32 and main_prec == "fp64":
-        r32 = run_gpu(cfg, m, args.steps, args.warmup, "fp32")
+        r32 = run_gpu(cfg, m, args.steps, args.warmup, "fp32", geometry=args.geometry)
         v32 = dof * 4 * args.steps / r32["wall"] / 1e9
         a, b = r32["state"], r["state"]
         l2 = float(np.linalg.norm(a - b) / np.linalg.norm(b))
@@ -380,19 +392,27 @@ def main():
             "speedup_vs_fp64": v32 / value, "rel_l2_vs_fp64": l2, "rel_l2_per_field_vs_fp64": l2f,
             "roofline": {"bound": "hbm", "achieved": a32 / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                          "frac": a32 / HBM_PEAK, "avg_launch_ms": r32["res_ms"],
-                         "algorithmic_bytes_per_launch": r32["bytes"]},
+                         "algorithmic_bytes_per_launch": r32["bytes"],
+                         "traffic": traffic_from_profile("fp32", args.geometry)},
             "kernels_ms": {"stage": r32["res_ms"]},
         }
-    if not args.no_compact and main_prec == "fp64":
-        rc = run_gpu(cfg, m, args.steps, args.warmup, "fp64", geometry="compact")
-        vc = dof * 4 * args.steps / rc["wall"] / 1e9
-        line["compact_geometry"] = {
-            "note": "not the measured path: the same FP64 workload with one metric set per element and one "
-                    "normal per face, which the library uses by default on affine meshes",
-            "value": vc, "unit": "GDOF/s", "ms_per_step": rc["wall"] / args.steps * 1e3,
-            "avg_launch_ms": rc["res_ms"], "speedup_vs_per_point": vc / value,
-            "max_abs_state_diff_vs_per_point": float(np.abs(rc["state"] - r["state"]).max()),
-        }
+    if not args.no_variants and main_prec == "fp64":
+        # the same FP64 workload with the other geometry layouts, and the
+        # headline layout on a mesh whose interior vertices are displaced by
+        # up to 10 % of the cell size (no element affine)
+        var = {}
+        for geo in [g for g in ("point", "compact") if g != args.geometry]:
+            rv = run_gpu(cfg, m, args.steps, args.warmup, "fp64", geometry=geo)
+            var[geo] = {"value": dof * 4 * args.steps / rv["wall"] / 1e9, "unit": "GDOF/s",
+                        "avg_launch_ms": rv["res_ms"], "frac": rv["bytes"] / (rv["res_ms"] * 1e-3) / HBM_PEAK,
+                        "max_abs_state_diff_vs_headline": float(np.abs(rv["state"] - r["state"]).max())}
+        cfgp, mp = tgv_setup(args.n, perturb=0.1)
+        rp = run_gpu(cfgp, mp, args.steps, args.warmup, "fp64", geometry=args.geometry)
+        var[f"{args.geometry}_perturbed_mesh"] = {
+            "value": dof * 4 * args.steps / rp["wall"] / 1e9, "unit": "GDOF/s", "avg_launch_ms": rp["res_ms"],
+            "frac": rp["bytes"] / (rp["res_ms"] * 1e-3) / HBM_PEAK,
+            "mesh": "interior vertices displaced by up to 0.1 cell sizes (seed 11)"}
+        line["geometry_variants"] = var
     if not args.no_batch:
         line["batched"] = run_batch()
     if not args.no_cpu:
