@@ -39,17 +39,18 @@ def hipcc():
     raise RuntimeError("hipcc not found")
 
 
-def build(verbose=False, jobs=None):
+def build(verbose=False, jobs=None, lib=LIB, objdir=None, defines=()):
     """Compile libfrhip.so for gfx950: every translation unit to an object in
-    parallel, then one link.  Always rebuilds."""
+    parallel, then one link.  Always rebuilds.  lib / objdir / defines (-D
+    switches, e.g. the stage kernel's FR_LDS_PAD) build a variant elsewhere."""
     from concurrent.futures import ThreadPoolExecutor
-    objdir = os.path.join(HERE, "_obj")
+    objdir = objdir or os.path.join(HERE, "_obj")
     os.makedirs(objdir, exist_ok=True)
     inc = ["-I", os.path.join(ROOT, "include")]
 
     def compile_one(src):
         obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
-        cmd = [hipcc()] + HIPCC_FLAGS + inc + ["-c", os.path.join(CSRC, src), "-o", obj]
+        cmd = [hipcc()] + HIPCC_FLAGS + ["-D" + d for d in defines] + inc + ["-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         r = subprocess.run(cmd, capture_output=True, text=True)
@@ -63,7 +64,7 @@ def build(verbose=False, jobs=None):
         sys.stderr.write(f"--- {src}\n" + r.stdout + r.stderr)
     if failed:
         raise RuntimeError("hipcc failed building libfrhip.so")
-    tmp = LIB + ".tmp"
+    tmp = lib + ".tmp"
     cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC"] + [obj for _, obj, _ in results] + ["-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)
@@ -71,8 +72,8 @@ def build(verbose=False, jobs=None):
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("linking libfrhip.so failed")
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, lib)
+    return lib
 
 
 if __name__ == "__main__":

@@ -20,6 +20,16 @@
 #include "fr_layout.hpp"
 #include "fr_physics.hpp"
 
+#ifndef FR_DEFER_NBR
+#define FR_DEFER_NBR 0  // intermediate inviscid stages load the neighbour traces in the face phase
+#endif
+#ifndef FR_MFMA
+#define FR_MFMA 1  // hex P3 inviscid stages: the line contractions of F~ on the matrix cores
+#endif
+#ifndef FR_STAGE_MINW
+#define FR_STAGE_MINW 1  // waves per SIMD asked of the intermediate inviscid stage kernel
+#endif
+
 namespace frdev {
 
 // Common normal flux at one face point, (UL, UR) in the left element's frame.
@@ -506,6 +516,130 @@ struct StageSmem {
   T s1[OneD<P>::SIZE];        // 1D factor table
 };
 
+// Phase (2) of the stage kernel for hex P3 on the matrix cores.  One element
+// is one wavefront, lane = solution point s = i + 4j + 16k.  Along each
+// reference axis the element has 16 lines of 4 points; one 16x16x4 MFMA per
+// field multiplies the 16x4 operator A by the 4x16 matrix B of the 16 lines'
+// values (B[k][n] = point k of line n, at lane n + 16k) and yields, in lane
+// n + 16g, the derivative at point g of line n (register 0: rows of A holding
+// the rows of Dm), and in the lanes of g = 0 the line's extrapolations to its
+// two faces (registers 1, 2: rows holding l_L and l_R).  For the z axis
+// (lines n = i + 4j, point k) B is the lanes' own values and the derivative
+// lands on its own solution point; for x (n = j + 4k, point i) and y
+// (n = i + 4k, point j) the lanes read B from LDS at the permuted point and
+// the derivatives go back through LDS.  The same operator entries as the
+// VALU path's line sums (calcDivF_spts, extrapolateNormalFlux); the MFMA sums
+// its four products in its own order.
+template <typename T>
+struct MfmaVec;
+template <>
+struct MfmaVec<float> {
+  typedef float type __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ type mma(float a, float b) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, type{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+  }
+  // row of A holding Dm row g; rows of A holding l_L, l_R (f32 layout:
+  // C[m][n] in lane n + 16 (m / 4), register m % 4)
+  static __device__ __forceinline__ int rowD(int g) { return 4 * g; }
+  static constexpr int ROWL = 1, ROWR = 2;
+};
+template <>
+struct MfmaVec<double> {
+  typedef double type __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ type mma(double a, double b) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, type{0., 0., 0., 0.}, 0, 0, 0);
+  }
+  // f64 layout: C[m][n] in lane n + 16 (m % 4), register m / 4
+  static __device__ __forceinline__ int rowD(int g) { return g; }
+  static constexpr int ROWL = 4, ROWR = 8;
+};
+
+// The lane's part of the 16x16x4 products of a hex P3 element (lane t):
+// its A entry, its line n and point of the B / C layout, the LDS index of the
+// solution point it stands for on the x and y axes (on z it is the lane
+// itself), and the two face flux points of line n on each axis (fptLine's
+// numbering): x faces 2, 3 (n = j + 4k), y faces 4, 5 (n = i + 4k), z faces 0, 1
+// (n = i + 4j).
+template <typename T>
+struct MfmaLane {
+  using V = MfmaVec<T>;
+  T aval;
+  int ln, lp, px, py, qL[3], qR[3];
+  __device__ __forceinline__ MfmaLane(const T *s1, int t) {
+    using O = OneD<3>;
+    // A[m][k] at lane m + 16k: Dm rows, l_L, l_R, zeros elsewhere
+    const int am = t & 15, ak = t >> 4;
+    aval = (T)0;
+#pragma unroll
+    for (int g = 0; g < 4; g++)
+      if (am == V::rowD(g)) aval = s1[O::DM + g * 4 + ak];
+    if (am == V::ROWL) aval = s1[O::LL + ak];
+    if (am == V::ROWR) aval = s1[O::LR + ak];
+    ln = t & 15;
+    lp = t >> 4;
+    px = padIdx<3>(lp + 4 * (ln & 3) + 16 * (ln >> 2));
+    py = padIdx<3>((ln & 3) + 4 * lp + 16 * (ln >> 2));
+    const int rev = (3 - (ln & 3)) + 4 * (ln >> 2);
+    qL[0] = 32 + ln; qL[1] = 64 + rev; qL[2] = ln;
+    qR[0] = 48 + rev; qR[1] = 80 + ln; qR[2] = 16 + rev;
+  }
+};
+
+template <int NF, typename T, typename SM>
+__device__ __forceinline__ void mfmaAxes(SM &sm, const int slot, const int t, const int tP, const T *pJ,
+                                         const T (&F)[3][NF], T (&div)[NF]) {
+  using V = MfmaVec<T>;
+  auto &sFd = sm.sFd;
+  auto &sJ = sm.sJ;
+  const MfmaLane<T> L(sm.s1, t);
+  const T aval = L.aval;
+  const int lp = L.lp, px = L.px, py = L.py;
+  const int *qL = L.qL, *qR = L.qR;
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    T ft[NF];
+#pragma unroll
+    for (int k = 0; k < NF; k++) {
+      T acc = (T)0;
+#pragma unroll
+      for (int d2 = 0; d2 < 3; d2++) acc += pJ[d * 3 + d2] * F[d2][k];
+      ft[k] = acc;
+    }
+    const int pd = d == 0 ? px : py;
+    T b[NF];
+    if (d < 2) {
+#pragma unroll
+      for (int k = 0; k < NF; k++) sFd[slot][k][tP] = ft[k];
+      elementSync<64>();
+#pragma unroll
+      for (int k = 0; k < NF; k++) b[k] = sFd[slot][k][pd];
+    } else {
+#pragma unroll
+      for (int k = 0; k < NF; k++) b[k] = ft[k];
+    }
+    typename V::type c[NF];
+#pragma unroll
+    for (int k = 0; k < NF; k++) c[k] = V::mma(aval, b[k]);
+    if (d < 2) elementSync<64>();  // every lane has read its B before the derivatives overwrite sFd
+#pragma unroll
+    for (int k = 0; k < NF; k++) {
+      if (d < 2) sFd[slot][k][pd] = c[k][0];
+      else div[k] = c[k][0] + div[k];
+      if (lp == 0) {
+        sJ[slot][k][qL[d]] = c[k][1];
+        sJ[slot][k][qR[d]] = -c[k][2];
+      }
+    }
+    if (d < 2) {
+      elementSync<64>();
+#pragma unroll
+      for (int k = 0; k < NF; k++) div[k] = (d == 0) ? sFd[slot][k][tP] : sFd[slot][k][tP] + div[k];
+      elementSync<64>();  // before the next axis writes sFd
+    }
+  }
+  elementSync<64>();
+}
+
 // BND = false is the variant for meshes without boundary faces (no ghost
 // states, which keeps its register allocation small).  VIS adds the viscous
 // terms (LDG): the corrected gradient at the solution points is recomputed
@@ -539,6 +673,8 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
   // the modes a variant serves: the inviscid FIN variants 2 and 3, the other
   // inviscid variants 0 and 1 (the launches choose so), the viscous one all
   constexpr bool M01 = VIS || !FIN, M23 = VIS || FIN;
+  // hex P3, inviscid: the line contractions of (2) on the matrix cores
+  constexpr bool MF = FR_MFMA && D == 3 && P == 3 && !VIS;
   const int slot = tid / TPE, t = tid - slot * TPE;
   const int tP = padIdx<P>(t);  // padded LDS index of this thread's solution point
   const int e = b * EPB + slot;
@@ -647,17 +783,24 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
       for (int c = 0; c <= D; c++) pn[r][c] = __builtin_nontemporal_load(a.fgeo + ga.fg(ec, c, qc));
     }
   }
+  // neighbour traces: prefetched here, or (DEF: the intermediate inviscid
+  // variants when FR_DEFER_NBR is set) loaded at the start of the face phase,
+  // which shortens their registers' live range to that phase
+  auto loadNbr = [&]() {
 #pragma unroll
-  for (int r = 0; r < NQ; r++) {
-    const int qc = min(t + r * TPE, NFP - 1);
-    const bool bndFace = BND && pfi[r] >= FINFO_BOUNDARY;
-    const bool fetch = !bndFace && !inBlock(pnb[r]);
-    const T *Ur = fetch ? a.UfIn + (size_t)pnb[r] * NF * NFP + neighbourFpt<D, P>(qc, pfi[r]) : a.oned;
-    const int st = fetch ? NFP : 0;
+    for (int r = 0; r < NQ; r++) {
+      const int qc = min(t + r * TPE, NFP - 1);
+      const bool bndFace = BND && pfi[r] >= FINFO_BOUNDARY;
+      const bool fetch = !bndFace && !inBlock(pnb[r]);
+      const T *Ur = fetch ? a.UfIn + (size_t)pnb[r] * NF * NFP + neighbourFpt<D, P>(qc, pfi[r]) : a.oned;
+      const int st = fetch ? NFP : 0;
 #pragma unroll
-    for (int k = 0; k < NF; k++) pu[r][k] = __builtin_nontemporal_load(Ur + k * st);
-    if (bndFace) pnb[r] = -1;
-  }
+      for (int k = 0; k < NF; k++) pu[r][k] = __builtin_nontemporal_load(Ur + k * st);
+      if (bndFace) pnb[r] = -1;
+    }
+  };
+  constexpr bool DEF = FR_DEFER_NBR && !VIS && !FIN;
+  if constexpr (!DEF) loadNbr();
   if (tid < O::SIZE) s1[tid] = o1;
   // the FP64 base of the final stages is absolute; FP32 stage states hold the
   // energy less Phys::eOff
@@ -760,59 +903,64 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
   // (2) per reference axis d: F~_d = sum_d2 JGinv(d,d2) F_d2, its derivative
   // along d (calcDivF_spts) and its extrapolation to the two faces normal to d
   // (extrapolateNormalFlux), stored negated as the start of the jump
+  if constexpr (MF) {
+    if (act) mfmaAxes<NF, T>(sm, slot, t, tP, pJ, F, div);
+  } else {
 #pragma unroll
-  for (int d = 0; d < D; d++) {
-    if (hasS) {
+    for (int d = 0; d < D; d++) {
+      if (hasS) {
 #pragma unroll
-      for (int k = 0; k < NF; k++) {
-        T acc = (T)0;
+        for (int k = 0; k < NF; k++) {
+          T acc = (T)0;
 #pragma unroll
-        for (int d2 = 0; d2 < D; d2++) acc += pJ[d * D + d2] * F[d2][k];
-        sFd[slot][k][tP] = acc;
-      }
-    }
-    elementSync<TPE>();
-    if (hasS) {
-      const int id = (d == 0) ? si : (d == 1) ? sj : sk;
-#pragma unroll
-      for (int k = 0; k < NF; k++) {
-        T acc = (T)0;
-#pragma unroll
-        for (int m = 0; m < P1; m++) {
-          // padded index of the line point (padIdx of the unpadded one)
-          const int sp = (d == 0)   ? m + (P1 + 1) * (sj + P1 * sk)
-                         : (d == 1) ? si + (P1 + 1) * (m + P1 * sk)
-                                    : si + (P1 + 1) * (sj + P1 * m);
-          acc += s1[O::DM + id * P1 + m] * sFd[slot][k][sp];
+          for (int d2 = 0; d2 < D; d2++) acc += pJ[d * D + d2] * F[d2][k];
+          sFd[slot][k][tP] = acc;
         }
-        div[k] = (d == 0) ? acc : acc + div[k];
       }
-    }
-    if (act && t < 2 * NFPF) {
-      const int q = dirFace<D>(d, t / NFPF) * NFPF + t % NFPF;
-      int base, stride, side, dn;
-      fptLine<D, P>(q, base, stride, side, dn);
-      const int lb = padIdx<P>(base), ls = padStride<P>(stride);
-      const T *l1 = s1 + (side ? O::LR : O::LL);
-      const T sg = side ? (T)1 : (T)-1;
-      // the weights with the face sign and the negation of the stored value
-      // applied once, ahead of the fields (exact: sign changes only)
-      T lw[P1];
+      elementSync<TPE>();
+      if (hasS) {
+        const int id = (d == 0) ? si : (d == 1) ? sj : sk;
 #pragma unroll
-      for (int m = 0; m < P1; m++) lw[m] = -(l1[m] * sg);
+        for (int k = 0; k < NF; k++) {
+          T acc = (T)0;
 #pragma unroll
-      for (int k = 0; k < NF; k++) {
-        T acc = (T)0;
-#pragma unroll
-        for (int m = 0; m < P1; m++) acc += lw[m] * sFd[slot][k][lb + m * ls];
-        sJ[slot][k][q] = acc;
+          for (int m = 0; m < P1; m++) {
+            // LDS index of the line point
+            const int sp = padIdx<P>((d == 0) ? m + P1 * (sj + P1 * sk)
+                                     : (d == 1) ? si + P1 * (m + P1 * sk)
+                                                : si + P1 * (sj + P1 * m));
+            acc += s1[O::DM + id * P1 + m] * sFd[slot][k][sp];
+          }
+          div[k] = (d == 0) ? acc : acc + div[k];
+        }
       }
+      if (act && t < 2 * NFPF) {
+        const int q = dirFace<D>(d, t / NFPF) * NFPF + t % NFPF;
+        int base, stride, side, dn;
+        fptLine<D, P>(q, base, stride, side, dn);
+        const int lb = padIdx<P>(base), ls = padStride<P>(stride);
+        const T *l1 = s1 + (side ? O::LR : O::LL);
+        const T sg = side ? (T)1 : (T)-1;
+        // the weights with the face sign and the negation of the stored value
+        // applied once, ahead of the fields (exact: sign changes only)
+        T lw[P1];
+#pragma unroll
+        for (int m = 0; m < P1; m++) lw[m] = -(l1[m] * sg);
+#pragma unroll
+        for (int k = 0; k < NF; k++) {
+          T acc = (T)0;
+#pragma unroll
+          for (int m = 0; m < P1; m++) acc += lw[m] * sFd[slot][k][lb + m * ls];
+          sJ[slot][k][q] = acc;
+        }
+      }
+      elementSync<TPE>();
     }
-    elementSync<TPE>();
   }
 
   // (3) common flux at every flux point (face::calcInviscidFlux,
   // face.cpp:132-175; intFace.cpp:85-120; boundFace.cpp:33-277)
+  if constexpr (DEF) loadNbr();
 #pragma unroll
   for (int r = 0; r < NQ; r++) {
     const int q = t + r * TPE;
@@ -933,6 +1081,8 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
       // stage terms RK * divF / detJ * dt, with dt / detJ formed once per point
       const double dtJ = pdt / pdet;
       double u = pb[k];
+      // DEF: the base state read again here instead of kept in registers
+      if constexpr (DEF) u = (double)a.UbaseT[gi];
       if (M23 && a.mode == 2) {
         // earlier residuals: already subtracted from pb (FIN, before the barrier) or
         // read here (viscous variant)
@@ -993,9 +1143,14 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
 
 // The inviscid final-stage variant asks for 3 waves per SIMD (at most 168
 // VGPRs): it holds the most prefetched data, and at 2 waves per SIMD the
-// final stage ran 12 % longer.
+// final stage ran 12 % longer.  So do the hex P3 variants with the MFMA
+// contractions, whose accumulators would otherwise take them to 2 waves.
 template <int D, int P, int NF, typename T, bool BND, bool VIS, bool FIN, int GM = 0>
-__global__ __launch_bounds__((ElemGeo<D, P>::NTH), (FIN && !VIS ? 3 : 1)) void k_stage(StageArgs<T> a) {
+__global__ __launch_bounds__((ElemGeo<D, P>::NTH),
+                             (FIN && !VIS ? 3
+                              : VIS       ? 1
+                              : (FR_MFMA && D == 3 && P == 3) ? 3
+                                                              : FR_STAGE_MINW)) void k_stage(StageArgs<T> a) {
   __shared__ StageSmem<D, P, NF, T> sm;
   // Workgroups are dealt to the 8 XCDs in turn; remap so that each XCD walks a
   // contiguous range of elements and face neighbours share its L2.  The host

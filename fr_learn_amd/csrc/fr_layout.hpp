@@ -14,6 +14,11 @@
 
 #include <hip/hip_runtime.h>
 
+// Build-time switches of the stage kernel's layout (compile with -D to change)
+#ifndef FR_LDS_PAD
+#define FR_LDS_PAD 1  // pad the solution-point rows in LDS (padIdx)
+#endif
+
 namespace frdev {
 
 constexpr int nextPow2(int n) {
@@ -40,24 +45,24 @@ struct ElemGeo {
   static constexpr int TPE = nextPow2(NS) < 8 ? 8 : nextPow2(NS);  // threads per element
   static constexpr int EPB = TPE >= 256 ? 1 : 256 / TPE;           // elements per block
   static constexpr int NTH = TPE * EPB;
-  // LDS rows of solution-point data are padded by one entry per x-line, so
-  // that lanes walking y- or x-face lines (strides P+1, 1 with bases P+1
-  // apart) fall in different banks
-  static constexpr int NSP = NS + NS / P1;
+  // LDS rows of solution-point data can be padded by one entry per x-line
+  // (FR_LDS_PAD), so that lanes walking y- or x-face lines (strides P+1, 1
+  // with bases P+1 apart) fall in different banks
+  static constexpr int NSP = FR_LDS_PAD ? NS + NS / P1 : NS;
 };
 
-// Padded LDS index of solution point s (one pad entry per x-line).
+// LDS index of solution point s (padded: one pad entry per x-line).
 template <int P>
 __host__ __device__ __forceinline__ int padIdx(int s) {
-  return s + s / (P + 1);
+  return FR_LDS_PAD ? s + s / (P + 1) : s;
 }
 
-// Padded stride of a solution-point line of stride `stride` (1, P+1 or
+// LDS stride of a solution-point line of stride `stride` (1, P+1 or
 // (P+1)^2): every line starts on an x-line boundary or moves along x, so
 // padIdx(base + m*stride) = padIdx(base) + m*padStride(stride).
 template <int P>
 __host__ __device__ __forceinline__ int padStride(int stride) {
-  return stride + stride / (P + 1);
+  return FR_LDS_PAD ? stride + stride / (P + 1) : stride;
 }
 
 // Solution-point line behind flux point q: spts base + m*stride (m = 0..P)
