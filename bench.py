@@ -309,6 +309,9 @@ def main():
     ap.add_argument("--geometry", choices=list(GEOMETRY), default="nodal",
                     help="stage-kernel geometry of the headline run (default: metrics and normals formed in the "
                          "kernel from each hex's corner nodes, valid for any linear hex)")
+    ap.add_argument("--lib", default=None,
+                    help="load this build of libfrhip.so instead of the package's (e.g. one built by "
+                         "`python -m fr_learn_amd.build --lib PATH -D FR_MFMA=0`)")
     ap.add_argument("--no-variants", action="store_true",
                     help="skip the extra FP64 runs with the other geometry layouts and the perturbed mesh")
     ap.add_argument("--copy-calibration", action="store_true", help="only the 256 MiB copy for PMC scaling")
@@ -321,6 +324,9 @@ def main():
               "multi-GPU path", file=sys.stderr)
         sys.exit(2)
 
+    if args.lib:
+        import fr_learn_amd._lib as frlib
+        frlib.LIB_PATH = os.path.abspath(args.lib)
     if args.copy_calibration:
         copy_calibration()
         return

@@ -77,4 +77,11 @@ def build(verbose=False, jobs=None, lib=LIB, objdir=None, defines=()):
 
 
 if __name__ == "__main__":
-    print(build(verbose=True))
+    import argparse
+    ap = argparse.ArgumentParser(description="build libfrhip.so (or a variant of it elsewhere)")
+    ap.add_argument("--lib", default=LIB, help="output path of the shared library")
+    ap.add_argument("-D", dest="defines", action="append", default=[],
+                    help="a build switch NAME=VALUE, e.g. FR_MFMA=0 for the VALU line contractions")
+    args = ap.parse_args()
+    objdir = None if args.lib == LIB else args.lib + ".obj"
+    print(build(verbose=True, lib=args.lib, objdir=objdir, defines=args.defines))

@@ -26,6 +26,12 @@
 #ifndef FR_MFMA
 #define FR_MFMA 1  // hex P3 inviscid stages: the line contractions of F~ on the matrix cores
 #endif
+#ifndef FR_MFMA_BPERM
+#define FR_MFMA_BPERM 1  // the x / y operand permutation of the MFMA path by lane permutes, not through LDS
+#endif
+#ifndef FR_INBLOCK
+#define FR_INBLOCK 1  // neighbours in the same workgroup: traces extrapolated from their LDS state, not stored
+#endif
 #ifndef FR_STAGE_MINW
 #define FR_STAGE_MINW 1  // waves per SIMD asked of the intermediate inviscid stage kernel
 #endif
@@ -59,6 +65,26 @@ __device__ __forceinline__ bool commonFlux(const Phys &ph, bool boundary, const 
   } else {
     rusanovN<D, T>(ph, UL, UR, n, Fn, ws);
     return true;
+  }
+}
+
+// commonFlux of the left / right pair (right ? (UB, UA) : (UA, UB)): the
+// Rusanov flux takes the order as a sign (rusanovN), the others get the
+// states selected per field.
+template <int D, int NF, typename T>
+__device__ __forceinline__ bool commonFluxSided(const Phys &ph, bool boundary, bool right, const T *UA, const T *UB,
+                                                const T *n, T *Fn, T *ws = nullptr) {
+  if constexpr (NF > 1 && D == 3) {
+    rusanovN<D, T>(ph, UA, UB, n, Fn, ws, right);
+    return true;
+  } else {
+    T sL[NF], sR[NF];
+#pragma unroll
+    for (int k = 0; k < NF; k++) {
+      sL[k] = right ? UB[k] : UA[k];
+      sR[k] = right ? UA[k] : UB[k];
+    }
+    return commonFlux<D, NF, T>(ph, boundary, sL, sR, n, Fn, ws);
   }
 }
 
@@ -554,17 +580,30 @@ struct MfmaVec<double> {
   static constexpr int ROWL = 4, ROWR = 8;
 };
 
+// Value of v in lane addr / 4 of the wavefront (ds_bpermute: the LDS unit's
+// lane crossbar, no LDS memory).  Every lane of the wave is active here.
+__device__ __forceinline__ float lanePull(int addr, float v) {
+  return __int_as_float(__builtin_amdgcn_ds_bpermute(addr, __float_as_int(v)));
+}
+__device__ __forceinline__ double lanePull(int addr, double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_ds_bpermute(addr, (int)(b & 0xffffffffLL));
+  const int hi = __builtin_amdgcn_ds_bpermute(addr, (int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 // The lane's part of the 16x16x4 products of a hex P3 element (lane t):
-// its A entry, its line n and point of the B / C layout, the LDS index of the
-// solution point it stands for on the x and y axes (on z it is the lane
-// itself), and the two face flux points of line n on each axis (fptLine's
-// numbering): x faces 2, 3 (n = j + 4k), y faces 4, 5 (n = i + 4k), z faces 0, 1
-// (n = i + 4j).
+// its A entry, its line n and point of the B / C layout, the solution point
+// it stands for on the x and y axes (on z it is the lane itself) as an LDS
+// index (px, py) and as a lane byte address (bx, by), the lane holding the
+// x / y derivative of the lane's own point (ix, iy, byte addresses), and the
+// two face flux points of line n on each axis (fptLine's numbering): x faces
+// 2, 3 (n = j + 4k), y faces 4, 5 (n = i + 4k), z faces 0, 1 (n = i + 4j).
 template <typename T>
 struct MfmaLane {
   using V = MfmaVec<T>;
   T aval;
-  int ln, lp, px, py, qL[3], qR[3];
+  int ln, lp, px, py, bx, by, ix, iy, qL[3], qR[3];
   __device__ __forceinline__ MfmaLane(const T *s1, int t) {
     using O = OneD<3>;
     // A[m][k] at lane m + 16k: Dm rows, l_L, l_R, zeros elsewhere
@@ -577,8 +616,15 @@ struct MfmaLane {
     if (am == V::ROWR) aval = s1[O::LR + ak];
     ln = t & 15;
     lp = t >> 4;
-    px = padIdx<3>(lp + 4 * (ln & 3) + 16 * (ln >> 2));
-    py = padIdx<3>((ln & 3) + 4 * lp + 16 * (ln >> 2));
+    const int sx = lp + 4 * (ln & 3) + 16 * (ln >> 2), sy = (ln & 3) + 4 * lp + 16 * (ln >> 2);
+    px = padIdx<3>(sx);
+    py = padIdx<3>(sy);
+    bx = 4 * sx;
+    by = 4 * sy;
+    // point t = i + 4j + 16k is point i of x line j + 4k (lane j + 4k + 16i)
+    // and point j of y line i + 4k (lane i + 4k + 16j)
+    ix = 4 * ((t >> 2) + 16 * (t & 3));
+    iy = 4 * ((t & 3) + 4 * (t >> 4) + 16 * ((t >> 2) & 3));
     const int rev = (3 - (ln & 3)) + 4 * (ln >> 2);
     qL[0] = 32 + ln; qL[1] = 64 + rev; qL[2] = ln;
     qR[0] = 48 + rev; qR[1] = 80 + ln; qR[2] = 16 + rev;
@@ -607,6 +653,21 @@ __device__ __forceinline__ void mfmaAxes(SM &sm, const int slot, const int t, co
     }
     const int pd = d == 0 ? px : py;
     T b[NF];
+    if constexpr (FR_MFMA_BPERM) {
+      typename V::type c[NF];
+#pragma unroll
+      for (int k = 0; k < NF; k++) c[k] = V::mma(aval, d < 2 ? lanePull(d == 0 ? L.bx : L.by, ft[k]) : ft[k]);
+#pragma unroll
+      for (int k = 0; k < NF; k++) {
+        const T dv = d < 2 ? lanePull(d == 0 ? L.ix : L.iy, c[k][0]) : c[k][0];
+        div[k] = d == 0 ? dv : dv + div[k];
+        if (lp == 0) {
+          sJ[slot][k][qL[d]] = c[k][1];
+          sJ[slot][k][qR[d]] = -c[k][2];
+        }
+      }
+      continue;
+    }
     if (d < 2) {
 #pragma unroll
       for (int k = 0; k < NF; k++) sFd[slot][k][tP] = ft[k];
@@ -685,7 +746,7 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
   // order groups face neighbours into blocks).  The viscous variant keeps the
   // full trace exchange, which k_grad reads.
   const int blockBase = b * EPB;
-  auto inBlock = [&](int en) { return !VIS && (unsigned)(en - blockBase) < (unsigned)EPB; };
+  auto inBlock = [&](int en) { return FR_INBLOCK && !VIS && (unsigned)(en - blockBase) < (unsigned)EPB; };
   // Loads.  Vector-memory counters retire in issue order, so the face records
   // (which the neighbour-trace addresses depend on) go first, the state (or
   // the residual it is formed from) and the base state, detJ and dt (needed
@@ -1007,8 +1068,8 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
           n[c] = sg * g;
           ss += g * g;
         }
-        dA = sqrt(ss);
-        const T rdA = recip(dA);
+        T rdA;
+        rootAndInv(ss, dA, rdA);
 #pragma unroll
         for (int c = 0; c < D; c++) n[c] = n[c] * rdA;
       }
@@ -1033,17 +1094,11 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
 #pragma unroll
         for (int k = 0; k < NF; k++) un[k] = pu[r][k];
       }
-      // the left element's state first, chosen per lane, so that a wavefront
-      // whose points lie on both sides of their faces runs the common flux
-      // once, not once per side (right implies !bnd)
-      T sL[NF], sR[NF];
-#pragma unroll
-      for (int k = 0; k < NF; k++) {
-        sL[k] = right ? un[k] : uo[k];
-        sR[k] = right ? uo[k] : un[k];
-      }
+      // the left element's state first, per lane, so that a wavefront whose
+      // points lie on both sides of their faces runs the common flux once,
+      // not once per side (right implies !bnd)
       T ws;
-      const bool hasWs = commonFlux<D, NF, T>(a.ph, bnd, sL, sR, n, Fn, &ws);
+      const bool hasWs = commonFluxSided<D, NF, T>(a.ph, bnd, right, uo, un, n, Fn, &ws);
       // the CFL wave speed at the face's left points (and boundary points);
       // the right element's points keep theirs (face.cpp:74-80)
       if (a.waveSp && hasWs && !right) a.waveSp[(size_t)e * NFP + q] = (double)ws;

@@ -51,6 +51,50 @@ __device__ __forceinline__ T recip(T x) {
   }
 }
 
+// sqrt(x) of a non-negative x of ordinary magnitude (sound speeds, face
+// areas); rootAndInv also gives 1 / sqrt(x).  FP64: Goldschmidt's coupled
+// iteration for g -> sqrt(x), h -> 1 / (2 sqrt(x)) from the hardware
+// reciprocal square root estimate y (relative error below 2^-22): g = x y,
+// h = y / 2, then per step r = 1/2 - g h, g <- g + g r, h <- h + h r, which
+// squares the relative error; a Newton correction g <- g + (x - g^2) h
+// after the step leaves g within an ulp of the correctly rounded root.  An x
+// below the smallest normal double is taken as that value (root 1.5e-154
+// instead of 0: the estimate of 0 is infinite).  FP32: the library root.
+__device__ __forceinline__ void rootSteps(double x, double &g, double &h, int steps) {
+  x = fmax(x, 2.2250738585072014e-308);
+  const double y = __builtin_amdgcn_rsq(x);
+  g = x * y;
+  h = 0.5 * y;
+  for (int i = 0; i < steps; i++) {
+    const double r = fma(-g, h, 0.5);
+    g = fma(g, r, g);
+    h = fma(h, r, h);
+  }
+  g = fma(fma(-g, g, x), h, g);
+}
+template <typename T>
+__device__ __forceinline__ T rootPos(T x) {
+  if constexpr (sizeof(T) == 8) {
+    double g, h;
+    rootSteps(x, g, h, 1);
+    return g;
+  } else {
+    return sqrt(x);
+  }
+}
+template <typename T>
+__device__ __forceinline__ void rootAndInv(T x, T &root, T &inv) {
+  if constexpr (sizeof(T) == 8) {
+    double g, h;
+    rootSteps(x, g, h, 2);
+    root = g;
+    inv = h + h;
+  } else {
+    root = sqrt(x);
+    inv = (T)1 / root;
+  }
+}
+
 // Euler flux F[d][k] (inviscidFlux, reference src/flux.cpp:37-88) or linear
 // advection flux.
 template <int D, int NF, typename T>
@@ -88,10 +132,12 @@ __device__ __forceinline__ void inviscidFlux(const Phys &ph, const T *U, T F[D][
 }
 
 // Normal Euler flux F(U).n of one side plus its normal velocity and squared
-// sound speed: each inviscidFlux term F[d][i] (flux.cpp:37-88) multiplied by
-// n[d] and summed over d in order, as face::rusanovFlux does, without
-// holding the D x NF flux table.  One reciprocal of rho (recip) replaces the
-// reference's divisions (an ulp-level difference).
+// sound speed: the sum over d of n[d] F[d][i] that face::rusanovFlux forms
+// from inviscidFlux (flux.cpp:37-88), factored through the normal velocity
+// vn = v.n -- mass rho vn, momentum rho vn v_c + p n_c, energy (E + p) vn --
+// without holding the D x NF flux table (the same terms, summed in another
+// order: an ulp-level difference).  One reciprocal of rho (recip) replaces
+// the reference's divisions (also an ulp-level difference).
 template <int D, typename T>
 __device__ __forceinline__ void normalFlux(const Phys &ph, const T g, const T *U, const T *n, T *Fn, T &vn, T &csq) {
   constexpr int NF = D + 2;
@@ -107,36 +153,39 @@ __device__ __forceinline__ void normalFlux(const Phys &ph, const T g, const T *U
     Ep = Ep + (T)ph.hOff;
     pAbs = p + (T)ph.pOff;
   }
+  vn = n[0] * v[0];
 #pragma unroll
-  for (int i = 0; i < NF; i++) Fn[i] = (T)0;
-  vn = (T)0;
+  for (int d = 1; d < D; d++) vn += n[d] * v[d];
+  const T mn = rho * vn;
+  Fn[0] = mn;
 #pragma unroll
-  for (int d = 0; d < D; d++) {
-    vn += n[d] * v[d];
-    Fn[0] += n[d] * (rho * v[d]);
-#pragma unroll
-    for (int c = 0; c < D; c++) Fn[1 + c] += n[d] * ((c == d) ? rho * v[c] * v[d] + p : rho * v[c] * v[d]);
-    Fn[D + 1] += n[d] * (Ep * v[d]);
-  }
+  for (int c = 0; c < D; c++) Fn[1 + c] = mn * v[c] + p * n[c];
+  Fn[D + 1] = Ep * vn;
   csq = fmax(g * pAbs * ri, (T)0.0);
 }
 
 // Rusanov / local Lax-Friedrichs flux for Euler (face::rusanovFlux,
 // src/face.cpp:289-356, static mesh).  eigOut (may be null) receives
 // max(eigL, eigR), the wave speed the reference stores for the CFL time step
-// (face.cpp:354).
+// (face.cpp:354).  swap = true takes UA as the right state and UB as the
+// left one: the flux is symmetric in the two sides but for the sign of the
+// dissipation term eig (UR - UL), and sums, fmax and a negated product are
+// exact under the exchange, so the result is bit for bit the flux of the
+// exchanged arguments -- a lane picks its side by a sign, not by selecting
+// every field.
 template <int D, typename T>
-__device__ __forceinline__ void rusanovN(const Phys &ph, const T *UL, const T *UR, const T *n, T *Fn,
-                                         T *eigOut = nullptr) {
+__device__ __forceinline__ void rusanovN(const Phys &ph, const T *UA, const T *UB, const T *n, T *Fn,
+                                         T *eigOut = nullptr, bool swap = false) {
   constexpr int NF = D + 2;
   const T g = (T)ph.gamma;
-  T FnL[NF], FnR[NF], vnL, vnR, csqL, csqR;
-  normalFlux<D, T>(ph, g, UL, n, FnL, vnL, csqL);
-  normalFlux<D, T>(ph, g, UR, n, FnR, vnR, csqR);
-  const T eig = fmax(fabs(vnL) + sqrt(csqL), fabs(vnR) + sqrt(csqR));
+  T FnA[NF], FnB[NF], vnA, vnB, csqA, csqB;
+  normalFlux<D, T>(ph, g, UA, n, FnA, vnA, csqA);
+  normalFlux<D, T>(ph, g, UB, n, FnB, vnB, csqB);
+  const T eig = fmax(fabs(vnA) + rootPos(csqA), fabs(vnB) + rootPos(csqB));
   if (eigOut) *eigOut = eig;
+  const T eigS = swap ? -eig : eig;
 #pragma unroll
-  for (int i = 0; i < NF; i++) Fn[i] = (T)0.5 * (FnL[i] + FnR[i] - eig * (UR[i] - UL[i]));
+  for (int i = 0; i < NF; i++) Fn[i] = (T)0.5 * (FnA[i] + FnB[i] - eigS * (UB[i] - UA[i]));
 }
 
 // Roe flux with Harten-type entropy fix, 2D (face::roeFlux,
