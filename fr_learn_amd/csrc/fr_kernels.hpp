@@ -26,11 +26,11 @@
 #ifndef FR_MFMA
 #define FR_MFMA 1  // hex P3 inviscid stages: the line contractions of F~ on the matrix cores
 #endif
-#ifndef FR_MFMA_BPERM
-#define FR_MFMA_BPERM 1  // the x / y operand permutation of the MFMA path by lane permutes, not through LDS
-#endif
 #ifndef FR_INBLOCK
-#define FR_INBLOCK 1  // neighbours in the same workgroup: traces extrapolated from their LDS state, not stored
+#define FR_INBLOCK 1  // neighbours in the same workgroup: traces extrapolated from their LDS state (VALU variants)
+#endif
+#ifndef FR_MF_WAVES
+#define FR_MF_WAVES 3  // waves per SIMD asked of the hex P3 matrix-core stage kernels
 #endif
 #ifndef FR_STAGE_MINW
 #define FR_STAGE_MINW 1  // waves per SIMD asked of the intermediate inviscid stage kernel
@@ -542,42 +542,110 @@ struct StageSmem {
   T s1[OneD<P>::SIZE];        // 1D factor table
 };
 
-// Phase (2) of the stage kernel for hex P3 on the matrix cores.  One element
-// is one wavefront, lane = solution point s = i + 4j + 16k.  Along each
-// reference axis the element has 16 lines of 4 points; one 16x16x4 MFMA per
-// field multiplies the 16x4 operator A by the 4x16 matrix B of the 16 lines'
-// values (B[k][n] = point k of line n, at lane n + 16k) and yields, in lane
-// n + 16g, the derivative at point g of line n (register 0: rows of A holding
-// the rows of Dm), and in the lanes of g = 0 the line's extrapolations to its
-// two faces (registers 1, 2: rows holding l_L and l_R).  For the z axis
-// (lines n = i + 4j, point k) B is the lanes' own values and the derivative
-// lands on its own solution point; for x (n = j + 4k, point i) and y
-// (n = i + 4k, point j) the lanes read B from LDS at the permuted point and
-// the derivatives go back through LDS.  The same operator entries as the
-// VALU path's line sums (calcDivF_spts, extrapolateNormalFlux); the MFMA sums
-// its four products in its own order.
+// Hex P3 line operations on the matrix cores.  One element is one wavefront,
+// lane t = solution point i + 4j + 16k.  Along each reference axis the
+// element has 16 lines of 4 points, and every line operation of the stage is
+// a product A B of a small operator A (rows: the derivative matrix Dm, the
+// extrapolations l_L and l_R to the line's two faces) and the 4 x 16 matrix
+// B of the lines' values:
+//  - FP64: v_mfma_f64_4x4x4_4b, four independent 4x4x4 products; block b
+//    holds lines 4b .. 4b+3, with A_b[m][k] at lane m + 4b + 16k,
+//    B_b[k][n] at lane n + 4b + 16k and C_b[m][n] at lane n + 4b + 16m.
+//    The derivative (A = Dm) and the extrapolations (A rows l_L, l_R) are
+//    separate products.
+//  - FP32: v_mfma_f32_16x16x4f32, A[m][k] at lane m + 16k, B[k][n] at lane
+//    n + 16k, C[m][n] at lane n + 16 (m / 4), register m % 4.  Rows 4g of A
+//    hold Dm row g (derivative at point g in register 0) and rows 1 + 4s of
+//    A hold the extrapolations (register 1), so one product gives both.
+// In both, line n = n0 + 4 n1 of the axis takes its point k from lane
+// n + 16k and returns its derivative at point g in lane n + 16g: for the z
+// axis (n = i + 4j) that is the lane's own point, so B is the lanes' own
+// values and the derivative lands in place; for x (n = j + 4k, point i) and
+// y (n = i + 4k, point j) the operand comes from the permuted lane (a lane
+// permute, or LDS for state already there) and the derivative goes back
+// through the inverse permutation.  The extrapolations land in the lanes of
+// flux points: z lines put their left face value in lane n and right face
+// value in lane 16 + n, x lines in lanes 32 + n and 48 + n (rows 2, 3 of A
+// in FP64, rows 9, 13 in FP32) and y lines in lanes n and 16 + n, so that
+// one accumulator holds the z and x face values (the 64 flux points of faces
+// 0-3) and another the y values (faces 4, 5) -- lane t handles flux point
+// mfmaFpt(t, r) in round r of the face phases.  The operator entries are
+// those of the VALU line sums (calcDivF_spts, extrapolateNormalFlux,
+// extrapolateU); the matrix cores sum the four products in their own order.
 template <typename T>
 struct MfmaVec;
 template <>
-struct MfmaVec<float> {
-  typedef float type __attribute__((ext_vector_type(4)));
-  static __device__ __forceinline__ type mma(float a, float b) {
-    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, type{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+struct MfmaVec<double> {
+  static __device__ __forceinline__ double mma(double a, double b, double c) {
+    return __builtin_amdgcn_mfma_f64_4x4x4f64(a, b, c, 0, 0, 0);
   }
-  // row of A holding Dm row g; rows of A holding l_L, l_R (f32 layout:
-  // C[m][n] in lane n + 16 (m / 4), register m % 4)
-  static __device__ __forceinline__ int rowD(int g) { return 4 * g; }
-  static constexpr int ROWL = 1, ROWR = 2;
+  // derivative (aD) and face values added to acc (aE) of one axis
+  static __device__ __forceinline__ void axis(double aD, double aE, double b, double &acc, double &der) {
+    der = mma(aD, b, 0.0);
+    acc = mma(aE, b, acc);
+  }
+  static __device__ __forceinline__ double ext(double aE, double b, double acc) { return mma(aE, b, acc); }
 };
 template <>
-struct MfmaVec<double> {
-  typedef double type __attribute__((ext_vector_type(4)));
-  static __device__ __forceinline__ type mma(double a, double b) {
-    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, type{0., 0., 0., 0.}, 0, 0, 0);
+struct MfmaVec<float> {
+  typedef float type __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ type mma(float a, float b, float c1) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, type{0.f, c1, 0.f, 0.f}, 0, 0, 0);
   }
-  // f64 layout: C[m][n] in lane n + 16 (m % 4), register m / 4
-  static __device__ __forceinline__ int rowD(int g) { return g; }
-  static constexpr int ROWL = 4, ROWR = 8;
+  static __device__ __forceinline__ void axis(float aD, float aE, float b, float &acc, float &der) {
+    (void)aD;
+    const type c = mma(aE, b, acc);
+    der = c[0];
+    acc = c[1];
+  }
+  static __device__ __forceinline__ float ext(float aE, float b, float acc) { return mma(aE, b, acc)[1]; }
+};
+
+// Flux point of lane t in round r (0, 1) of the face phases of a hex P3
+// element: round 0 holds faces 0-3 (lanes 16f ..), round 1 faces 4 and 5;
+// within a face, the lane of line n has the point fptLine numbers n or, on
+// faces 1, 3 and 4 (whose point order runs against the line order), n ^ 3.
+__device__ __forceinline__ int mfmaFpt(int t, int r) {
+  const int flip = r == 0 ? (t >> 4) & 1 : (t >> 4) == 0;
+  return 64 * r + (t ^ (3 * flip));
+}
+
+// The lane's part of the products of a hex P3 element (lane t): its A
+// entries (the derivative aD; the face values of the z and y axes aEzy and
+// of the x axis aEx), the solution point it stands for in the x and y
+// operands as an LDS index (px, py) and a lane byte address (bx, by), and
+// the lane holding the x / y derivative of its own point (ix, iy).
+template <typename T>
+struct MfmaLane {
+  T aD, aEzy, aEx;
+  int px, py, bx, by, ix, iy;
+  __device__ __forceinline__ MfmaLane(const T *s1, int t) {
+    using O = OneD<3>;
+    const int k = t >> 4;
+    const T lL = s1[O::LL + k], lR = s1[O::LR + k];
+    if constexpr (sizeof(T) == 8) {
+      const int m = t & 3;
+      aD = s1[O::DM + m * 4 + k];
+      aEzy = m == 0 ? lL : m == 1 ? lR : (T)0;
+      aEx = m == 2 ? lL : m == 3 ? lR : (T)0;
+    } else {
+      const int m = t & 15;
+      const T d = (m & 3) == 0 ? s1[O::DM + (m >> 2) * 4 + k] : (T)0;
+      aD = d;
+      aEzy = m == 1 ? lL : m == 5 ? lR : d;
+      aEx = m == 9 ? lL : m == 13 ? lR : d;
+    }
+    const int ln = t & 15, lp = t >> 4;
+    const int sx = lp + 4 * (ln & 3) + 16 * (ln >> 2), sy = (ln & 3) + 4 * lp + 16 * (ln >> 2);
+    px = padIdx<3>(sx);
+    py = padIdx<3>(sy);
+    bx = 4 * sx;
+    by = 4 * sy;
+    // point t = i + 4j + 16k is point i of x line j + 4k (lane j + 4k + 16i)
+    // and point j of y line i + 4k (lane i + 4k + 16j)
+    ix = 4 * ((t >> 2) + 16 * (t & 3));
+    iy = 4 * ((t & 3) + 4 * (t >> 4) + 16 * ((t >> 2) & 3));
+  }
 };
 
 // Value of v in lane addr / 4 of the wavefront (ds_bpermute: the LDS unit's
@@ -592,113 +660,58 @@ __device__ __forceinline__ double lanePull(int addr, double v) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-// The lane's part of the 16x16x4 products of a hex P3 element (lane t):
-// its A entry, its line n and point of the B / C layout, the solution point
-// it stands for on the x and y axes (on z it is the lane itself) as an LDS
-// index (px, py) and as a lane byte address (bx, by), the lane holding the
-// x / y derivative of the lane's own point (ix, iy, byte addresses), and the
-// two face flux points of line n on each axis (fptLine's numbering): x faces
-// 2, 3 (n = j + 4k), y faces 4, 5 (n = i + 4k), z faces 0, 1 (n = i + 4j).
-template <typename T>
-struct MfmaLane {
-  using V = MfmaVec<T>;
-  T aval;
-  int ln, lp, px, py, bx, by, ix, iy, qL[3], qR[3];
-  __device__ __forceinline__ MfmaLane(const T *s1, int t) {
-    using O = OneD<3>;
-    // A[m][k] at lane m + 16k: Dm rows, l_L, l_R, zeros elsewhere
-    const int am = t & 15, ak = t >> 4;
-    aval = (T)0;
-#pragma unroll
-    for (int g = 0; g < 4; g++)
-      if (am == V::rowD(g)) aval = s1[O::DM + g * 4 + ak];
-    if (am == V::ROWL) aval = s1[O::LL + ak];
-    if (am == V::ROWR) aval = s1[O::LR + ak];
-    ln = t & 15;
-    lp = t >> 4;
-    const int sx = lp + 4 * (ln & 3) + 16 * (ln >> 2), sy = (ln & 3) + 4 * lp + 16 * (ln >> 2);
-    px = padIdx<3>(sx);
-    py = padIdx<3>(sy);
-    bx = 4 * sx;
-    by = 4 * sy;
-    // point t = i + 4j + 16k is point i of x line j + 4k (lane j + 4k + 16i)
-    // and point j of y line i + 4k (lane i + 4k + 16j)
-    ix = 4 * ((t >> 2) + 16 * (t & 3));
-    iy = 4 * ((t & 3) + 4 * (t >> 4) + 16 * ((t >> 2) & 3));
-    const int rev = (3 - (ln & 3)) + 4 * (ln >> 2);
-    qL[0] = 32 + ln; qL[1] = 64 + rev; qL[2] = ln;
-    qR[0] = 48 + rev; qR[1] = 80 + ln; qR[2] = 16 + rev;
-  }
-};
-
+// Phase (2) on the matrix cores: per reference axis d the contravariant
+// flux F~_d = sum_d2 JGinv(d,d2) F_d2, its derivative along d added to div,
+// and its values at the two faces normal to d, stored negated as the start
+// of the jump (the left face's outward flux is -F~_d): sJ at the round-0
+// and round-1 flux points of the lane.
 template <int NF, typename T, typename SM>
-__device__ __forceinline__ void mfmaAxes(SM &sm, const int slot, const int t, const int tP, const T *pJ,
-                                         const T (&F)[3][NF], T (&div)[NF]) {
+__device__ __forceinline__ void mfmaAxes(SM &sm, const int slot, const int t, const T *pJ, const T (&F)[3][NF],
+                                         T (&div)[NF]) {
   using V = MfmaVec<T>;
-  auto &sFd = sm.sFd;
   auto &sJ = sm.sJ;
   const MfmaLane<T> L(sm.s1, t);
-  const T aval = L.aval;
-  const int lp = L.lp, px = L.px, py = L.py;
-  const int *qL = L.qL, *qR = L.qR;
+  T e0[NF], e1[NF];
+#pragma unroll
+  for (int k = 0; k < NF; k++) e0[k] = e1[k] = (T)0;
 #pragma unroll
   for (int d = 0; d < 3; d++) {
-    T ft[NF];
 #pragma unroll
     for (int k = 0; k < NF; k++) {
-      T acc = (T)0;
+      T ft = (T)0;
 #pragma unroll
-      for (int d2 = 0; d2 < 3; d2++) acc += pJ[d * 3 + d2] * F[d2][k];
-      ft[k] = acc;
-    }
-    const int pd = d == 0 ? px : py;
-    T b[NF];
-    if constexpr (FR_MFMA_BPERM) {
-      typename V::type c[NF];
-#pragma unroll
-      for (int k = 0; k < NF; k++) c[k] = V::mma(aval, d < 2 ? lanePull(d == 0 ? L.bx : L.by, ft[k]) : ft[k]);
-#pragma unroll
-      for (int k = 0; k < NF; k++) {
-        const T dv = d < 2 ? lanePull(d == 0 ? L.ix : L.iy, c[k][0]) : c[k][0];
-        div[k] = d == 0 ? dv : dv + div[k];
-        if (lp == 0) {
-          sJ[slot][k][qL[d]] = c[k][1];
-          sJ[slot][k][qR[d]] = -c[k][2];
-        }
-      }
-      continue;
-    }
-    if (d < 2) {
-#pragma unroll
-      for (int k = 0; k < NF; k++) sFd[slot][k][tP] = ft[k];
-      elementSync<64>();
-#pragma unroll
-      for (int k = 0; k < NF; k++) b[k] = sFd[slot][k][pd];
-    } else {
-#pragma unroll
-      for (int k = 0; k < NF; k++) b[k] = ft[k];
-    }
-    typename V::type c[NF];
-#pragma unroll
-    for (int k = 0; k < NF; k++) c[k] = V::mma(aval, b[k]);
-    if (d < 2) elementSync<64>();  // every lane has read its B before the derivatives overwrite sFd
-#pragma unroll
-    for (int k = 0; k < NF; k++) {
-      if (d < 2) sFd[slot][k][pd] = c[k][0];
-      else div[k] = c[k][0] + div[k];
-      if (lp == 0) {
-        sJ[slot][k][qL[d]] = c[k][1];
-        sJ[slot][k][qR[d]] = -c[k][2];
-      }
-    }
-    if (d < 2) {
-      elementSync<64>();
-#pragma unroll
-      for (int k = 0; k < NF; k++) div[k] = (d == 0) ? sFd[slot][k][tP] : sFd[slot][k][tP] + div[k];
-      elementSync<64>();  // before the next axis writes sFd
+      for (int d2 = 0; d2 < 3; d2++) ft += pJ[d * 3 + d2] * F[d2][k];
+      const T b = d < 2 ? lanePull(d == 0 ? L.bx : L.by, ft) : ft;
+      T der;
+      V::axis(L.aD, d == 0 ? L.aEx : L.aEzy, b, d == 1 ? e1[k] : e0[k], der);
+      const T dv = d < 2 ? lanePull(d == 0 ? L.ix : L.iy, der) : der;
+      div[k] = d == 0 ? dv : dv + div[k];
     }
   }
+  const T sg = ((t >> 4) & 1) ? (T)-1 : (T)1;
+  const int q0 = mfmaFpt(t, 0), q1 = mfmaFpt(t, 1);
+#pragma unroll
+  for (int k = 0; k < NF; k++) {
+    sJ[slot][k][q0] = sg * e0[k];
+    if (t < 32) sJ[slot][k][q1] = sg * e1[k];
+  }
   elementSync<64>();
+}
+
+// Face values of a state S held in LDS (the padded layout of one element,
+// fields NSP apart) at the lane's flux point of round r (mfmaFpt): round 0
+// (faces 0-3) from the z lines -- the lane's own values own[] -- and the x
+// lines, round 1 (faces 4, 5) from the y lines.  All lanes take part.
+template <int NF, typename T>
+__device__ __forceinline__ void mfmaFaceRound(const T *s1, const T *S, const int t, const T (&own)[NF], const int r,
+                                              T (&v)[NF]) {
+  using V = MfmaVec<T>;
+  const MfmaLane<T> L(s1, t);
+  constexpr int NSP = ElemGeo<3, 3>::NSP;
+#pragma unroll
+  for (int k = 0; k < NF; k++)
+    v[k] = r == 0 ? V::ext(L.aEx, S[k * NSP + L.px], V::ext(L.aEzy, own[k], (T)0))
+                  : V::ext(L.aEzy, S[k * NSP + L.py], (T)0);
 }
 
 // BND = false is the variant for meshes without boundary faces (no ghost
@@ -736,17 +749,26 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
   constexpr bool M01 = VIS || !FIN, M23 = VIS || FIN;
   // hex P3, inviscid: the line contractions of (2) on the matrix cores
   constexpr bool MF = FR_MFMA && D == 3 && P == 3 && !VIS;
+  // the face values of the state in (3) and (5) too: FP64 only (three FP32
+  // 16x16x4 products per field cost more than the FP32 line sums they replace)
+  constexpr bool MFF = MF && sizeof(T) == 8;
   const int slot = tid / TPE, t = tid - slot * TPE;
   const int tP = padIdx<P>(t);  // padded LDS index of this thread's solution point
+  // flux point of this thread in round r of the face phases: the matrix-core
+  // variant's lane order (mfmaFpt), otherwise t + r * TPE
+  auto fq = [&](int r) { return MF ? mfmaFpt(t, r) : t + r * TPE; };
   const int e = b * EPB + slot;
   const bool act = e < a.nEles;
   const bool hasS = act && t < NS;
   // Faces between two elements of this block take the neighbour's trace from
   // its state in LDS, and their traces are not written to HBM (the element
   // order groups face neighbours into blocks).  The viscous variant keeps the
-  // full trace exchange, which k_grad reads.
+  // full trace exchange, which k_grad reads, and so does the matrix-core
+  // variant, whose face values come out of the products at the lanes' own
+  // flux points (the LDS line sums for the neighbour's points cost more there
+  // than the trace reads they save).
   const int blockBase = b * EPB;
-  auto inBlock = [&](int en) { return FR_INBLOCK && !VIS && (unsigned)(en - blockBase) < (unsigned)EPB; };
+  auto inBlock = [&](int en) { return FR_INBLOCK && !MF && !VIS && (unsigned)(en - blockBase) < (unsigned)EPB; };
   // Loads.  Vector-memory counters retire in issue order, so the face records
   // (which the neighbour-trace addresses depend on) go first, the state (or
   // the residual it is formed from) and the base state, detJ and dt (needed
@@ -776,7 +798,7 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
   int pfi[NQ], pnb[NQ];  // face record, neighbour element (-1: boundary)
 #pragma unroll
   for (int r = 0; r < NQ; r++) {
-    const int qc = min(t + r * TPE, NFP - 1);
+    const int qc = min(fq(r), NFP - 1);
     pfi[r] = a.finfo[ec * NFACE + qc / NFPF];
     pnb[r] = a.fnbr[ec * NFACE + qc / NFPF];
   }
@@ -839,7 +861,7 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
   if constexpr (GM != 1) {
 #pragma unroll
     for (int r = 0; r < NQ; r++) {
-      const int qc = min(t + r * TPE, NFP - 1);
+      const int qc = min(fq(r), NFP - 1);
 #pragma unroll
       for (int c = 0; c <= D; c++) pn[r][c] = __builtin_nontemporal_load(a.fgeo + ga.fg(ec, c, qc));
     }
@@ -850,7 +872,7 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
   auto loadNbr = [&]() {
 #pragma unroll
     for (int r = 0; r < NQ; r++) {
-      const int qc = min(t + r * TPE, NFP - 1);
+      const int qc = min(fq(r), NFP - 1);
       const bool bndFace = BND && pfi[r] >= FINFO_BOUNDARY;
       const bool fetch = !bndFace && !inBlock(pnb[r]);
       const T *Ur = fetch ? a.UfIn + (size_t)pnb[r] * NF * NFP + neighbourFpt<D, P>(qc, pfi[r]) : a.oned;
@@ -914,7 +936,7 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
   if constexpr (VIS) {
 #pragma unroll
     for (int r = 0; r < NQ; r++) {
-      const int q = t + r * TPE;
+      const int q = fq(r);
       if (act && q < NFP) {
         int base, stride, side, dn;
         fptLine<D, P>(q, base, stride, side, dn);
@@ -965,7 +987,7 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
   // along d (calcDivF_spts) and its extrapolation to the two faces normal to d
   // (extrapolateNormalFlux), stored negated as the start of the jump
   if constexpr (MF) {
-    if (act) mfmaAxes<NF, T>(sm, slot, t, tP, pJ, F, div);
+    if (act) mfmaAxes<NF, T>(sm, slot, t, pJ, F, div);
   } else {
 #pragma unroll
     for (int d = 0; d < D; d++) {
@@ -1024,19 +1046,30 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
   if constexpr (DEF) loadNbr();
 #pragma unroll
   for (int r = 0; r < NQ; r++) {
-    const int q = t + r * TPE;
+    const int q = fq(r);
+    // the state at the flux point
+    T uo[NF];
+    if constexpr (MFF) {
+      if (act) {
+        T own[NF];
+#pragma unroll
+        for (int k = 0; k < NF; k++) own[k] = r == 0 ? sU[slot][k][tP] : (T)0;
+        mfmaFaceRound<NF, T>(s1, &sU[slot][0][0], t, own, r, uo);
+      }
+    }
     if (act && q < NFP) {
       int base, stride, side, dn;
       fptLine<D, P>(q, base, stride, side, dn);
-      const int lb = padIdx<P>(base), ls = padStride<P>(stride);
-      const T *l1 = s1 + (side ? O::LR : O::LL);
-      T uo[NF];
+      if constexpr (!MFF) {
+        const int lb = padIdx<P>(base), ls = padStride<P>(stride);
+        const T *l1 = s1 + (side ? O::LR : O::LL);
 #pragma unroll
-      for (int k = 0; k < NF; k++) {
-        T acc = (T)0;
+        for (int k = 0; k < NF; k++) {
+          T acc = (T)0;
 #pragma unroll
-        for (int m = 0; m < P1; m++) acc += l1[m] * sU[slot][k][lb + m * ls];
-        uo[k] = acc;
+          for (int m = 0; m < P1; m++) acc += l1[m] * sU[slot][k][lb + m * ls];
+          uo[k] = acc;
+        }
       }
       const int fi = pfi[r];
       const bool bnd = BND && fi >= FINFO_BOUNDARY;
@@ -1117,6 +1150,7 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
   // (4) correction of the jump (correctDivFlux, solver.cpp:789-806) and the
   // stage update; the updated state goes to sFd (free after (2)) for its
   // traces, so that sU keeps the stage state other elements of the block read
+  T utv[NF];  // the updated state at this thread's point, for the traces of (5)
   if (hasS) {
     int cq[NFACE], cside[NFACE], cax[NFACE];
     corrEntries<D, P>(t, cq, cside, cax);
@@ -1167,6 +1201,7 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
         if (a.Uout32) __builtin_nontemporal_store((float)ut, a.Uout32 + gi);
       }
       sFd[slot][k][tP] = (T)ut;
+      utv[k] = (T)ut;
     }
   }
   if (M01 && a.mode == 0) return;
@@ -1178,9 +1213,16 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
     T *Fe = a.UfOut + (size_t)e * NF * NFP;
 #pragma unroll
     for (int r = 0; r < NQ; r++) {
-      const int q = t + r * TPE;
+      const int q = fq(r);
+      T tv[NF];
+      if constexpr (MFF) mfmaFaceRound<NF, T>(s1, &sFd[slot][0][0], t, utv, r, tv);
       if (q >= NFP) continue;
       if (!VIS && (pnb[r] < 0 || inBlock(pnb[r]))) continue;
+      if constexpr (MFF) {
+#pragma unroll
+        for (int k = 0; k < NF; k++) __builtin_nontemporal_store(tv[k], Fe + k * NFP + q);
+        continue;
+      }
       int base, stride, side, dn;
       fptLine<D, P>(q, base, stride, side, dn);
       const int lb = padIdx<P>(base), ls = padStride<P>(stride);
@@ -1202,9 +1244,9 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
 // contractions, whose accumulators would otherwise take them to 2 waves.
 template <int D, int P, int NF, typename T, bool BND, bool VIS, bool FIN, int GM = 0>
 __global__ __launch_bounds__((ElemGeo<D, P>::NTH),
-                             (FIN && !VIS ? 3
-                              : VIS       ? 1
-                              : (FR_MFMA && D == 3 && P == 3) ? 3
+                             (VIS ? 1
+                              : (FR_MFMA && D == 3 && P == 3) ? FR_MF_WAVES
+                              : FIN                           ? 3
                                                               : FR_STAGE_MINW)) void k_stage(StageArgs<T> a) {
   __shared__ StageSmem<D, P, NF, T> sm;
   // Workgroups are dealt to the 8 XCDs in turn; remap so that each XCD walks a

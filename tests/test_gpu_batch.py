@@ -114,7 +114,13 @@ def test_batch_vs_oracle_and_hex():
         hm.append((cfg, hex_box(3, seed=P), cases.ic_tgv))
     solo, batched, _, _ = _run(hm, 2)
     for a, b in zip(solo, batched):
-        assert rel_err(b.get_state(), a.get_state()).max() <= 1e-14
+        # The batched launch runs the boundary-capable instantiation of the
+        # stage kernel and a lone periodic mesh the one without boundary
+        # faces; the compiler may fuse their multiply-adds differently, so
+        # they agree to rounding: relative to the state's largest value (the
+        # TGV's rho w is 1e-4 of it, where a per-field measure would count
+        # those roundings 1e4 times over).
+        assert rel_err(b.get_state(), a.get_state(), axis=None) <= 1e-14
 
 
 def test_batch_fp32():
