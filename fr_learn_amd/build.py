@@ -14,7 +14,7 @@ LIB = os.path.join(HERE, "libfrhip.so")
 
 # translation units, compiled in parallel (the stage kernels per precision,
 # the adjoint and the batch kernels each in their own)
-SOURCES = ["fr_stage64_3d.hip", "fr_stage32_3d.hip", "fr_adjoint_v3.hip", "fr_adjoint_v2.hip", "fr_stage64.hip",
+SOURCES = ["fr_stage64_3d.hip", "fr_stage32_3d.hip", "fr_stage_dbg.hip", "fr_adjoint_v3.hip", "fr_adjoint_v2.hip", "fr_stage64.hip",
            "fr_stage32.hip", "fr_adjoint.hip",
            "fr_batch.hip", "fr_adjoint_v3ad.hip", "fr_api.hip", "fr_host.cpp", "fr_output.cpp", "fr_tri_ops.cpp",
            "fr_tri.hip", "fr_error.hip"]

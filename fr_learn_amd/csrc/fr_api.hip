@@ -950,6 +950,64 @@ int fr_get_residual(fr_ctx *c, int step, double *out, int layout) {
   return 0;
 }
 
+// Debug view of the residual's intermediate arrays (solver.hpp:55-61), for
+// the current state: one residual evaluation (as fr_calc_residual(ctx, 0))
+// with the stage kernel's (and on the viscous path the gradient kernel's)
+// debug outputs switched on, into buffers allocated for the call.
+int fr_get_array(fr_ctx *c, int which, double *out) {
+  if (!c || !out) return fail(FR_EINVAL, "null argument");
+  if (which < 2 || which > 6) return fail(FR_EINVAL, "which: 2 F_spts, 3 Fn_fpts, 4 disFn_fpts, 5 dU_spts, 6 dU_fpts");
+  if (c->prec != FR_PREC_FP64) return fail(FR_EINVAL, "fr_get_array: FP64 contexts only");
+  if (c->tri) return fail(FR_EINVAL, "fr_get_array: quads and hexes only");
+  if (which >= 5 && !c->cfg.viscous) return fail(FR_EINVAL, "fr_get_array: the gradient exists on the viscous path only");
+  const int D = c->D, NF = c->NF, NS = c->NS, NFP = c->NFP, nE = c->nE;
+  const size_t nSp = (size_t)nE * D * NF * NS, nFp = (size_t)nE * NF * NFP;
+  double *dF = nullptr, *dFn = nullptr, *dJ = nullptr, *dUs = nullptr;
+  auto release = [&]() {
+    for (double *p : {dF, dFn, dJ, dUs})
+      if (p) (void)hipFree(p);
+  };
+  if (hipMalloc(&dF, nSp * 8) != hipSuccess || hipMalloc(&dFn, nFp * 8) != hipSuccess ||
+      hipMalloc(&dJ, nFp * 8) != hipSuccess || (c->cfg.viscous && hipMalloc(&dUs, nSp * 8) != hipSuccess)) {
+    release();
+    return fail(FR_EHIP, "fr_get_array: device allocation failed");
+  }
+  // the residual's face fluxes store new wave speeds, as in residual()
+  if (c->dtType != 0 && !c->dtFirst && !c->dtFrozen) c->dtReady = false;
+  int rc = ensureTraces<double>(c, 0);
+  if (!rc) {
+    frdev::StageArgs<double> a = makeStageArgs<double>(c, 0, c->cur, 0, 0, 0.0);
+    a.dbgF = dF;
+    a.dbgFn = dFn;
+    a.dbgJ = dJ;
+    rc = launchStageDbg(c, a, dUs);
+  }
+  hipError_t err = hipStreamSynchronize(c->stream);
+  std::vector<double> h;
+  if (!rc && err == hipSuccess) {
+    const double *src = which == 2 ? dF : which == 3 ? dFn : which == 4 ? dJ : which == 5 ? dUs : (const double *)c->dUf;
+    h.resize(which == 6 ? (size_t)nE * D * NF * NFP : (which == 2 || which == 5) ? nSp : nFp);
+    err = hipMemcpy(h.data(), src, h.size() * 8, hipMemcpyDeviceToHost);
+  }
+  release();
+  if (rc) return rc;
+  if (err != hipSuccess) return fail(FR_EHIP, std::string("fr_get_array: ") + hipGetErrorString(err));
+  if (which == 3 || which == 4) {
+    devToRef(c, h.data(), out, NF, NFP);
+    return 0;
+  }
+  // (dim, point, ele, field) from the device's [ele][dim][field][point]
+  const int NP = which == 6 ? NFP : NS;
+  for (int d = 0; d < D; d++)
+    for (int p = 0; p < NP; p++)
+      for (int e = 0; e < nE; e++) {
+        const size_t de = (size_t)c->iperm[e];
+        for (int k = 0; k < NF; k++)
+          out[(((size_t)d * NP + p) * nE + e) * NF + k] = h[((de * D + d) * NF + k) * NP + p];
+      }
+  return 0;
+}
+
 int fr_get_traces(fr_ctx *c, double *out) {
   if (!c || !out) return fail(FR_EINVAL, "null argument");
   const size_t n = (size_t)c->nE * c->NF * c->NFP;

@@ -208,6 +208,10 @@ inline void stageMode(const fr_ctx *c, int s, int &mode, double &rk) {
 // Launch of the stage kernel variant for `mode` (fr_stage64.hip / fr_stage32.hip).
 template <typename T>
 int launchStageArgs(fr_ctx *c, const frdev::StageArgs<T> &a, int mode);
+// The residual evaluation of arguments `a` (mode 0, FP64, quads and hexes)
+// with its debug outputs (a.dbgF, a.dbgFn, a.dbgJ; on the viscous path the
+// gradient at the spts into dUs), fr_stage_dbg.hip.
+int launchStageDbg(fr_ctx *c, const frdev::StageArgs<double> &a, double *dUs);
 // (d divF / dU)^T lam at the state Us, device arrays (fr_adjoint.hip).
 int vjpResidualDev(fr_ctx *c, const double *Us, const double *lam, double *out);
 // triangle kernels (fr_tri.hip)

@@ -247,6 +247,11 @@ struct StageArgs {
   int affine;  // geometry layout (GeoAddr)
   int nodal;   // 1: the stage kernel forms its metrics and normals from geoN (GM = 1)
   const T *__restrict__ geoN;  // linear map coefficients [ele][LinearMap<D>::NC]
+  // debug outputs of a residual evaluation (k_stage_dbg only): the
+  // transformed flux F_spts [ele][dim][field][spt], the common normal flux
+  // times dA Fn_fpts [ele][field][fpt] and the corrected jump
+  // disFn_fpts = Fn_fpts - disFn [ele][field][fpt]
+  T *dbgF, *dbgFn, *dbgJ;
 };
 
 // Face f of the two faces normal to reference axis d (hex: x 2,3; y 4,5;
@@ -429,6 +434,7 @@ struct GradArgs {
   int nEles;
   Phys ph;
   int affine;  // geometry layout (GeoAddr)
+  T *dUs;      // debug output (null in production launches): the gradient at the spts [ele][dim][field][spt]
 };
 
 template <int D, int P, int NF, typename T, bool BND>
@@ -510,6 +516,11 @@ __global__ __launch_bounds__((ElemGeo<D, P>::NTH)) void k_grad(GradArgs<T> a) {
     for (int d = 0; d < D; d++)
 #pragma unroll
       for (int k = 0; k < NF; k++) sG[slot][d][k][t] = g[d][k];
+    if (a.dUs)
+#pragma unroll
+      for (int d = 0; d < D; d++)
+#pragma unroll
+        for (int k = 0; k < NF; k++) a.dUs[(((size_t)e * D + d) * NF + k) * NS + t] = g[d][k];
   }
   elementSync<TPE>();
   if (act) {
@@ -667,7 +678,7 @@ __device__ __forceinline__ double lanePull(int addr, double v) {
 // and round-1 flux points of the lane.
 template <int NF, typename T, typename SM>
 __device__ __forceinline__ void mfmaAxes(SM &sm, const int slot, const int t, const T *pJ, const T (&F)[3][NF],
-                                         T (&div)[NF]) {
+                                         T (&div)[NF], T *dbgF) {
   using V = MfmaVec<T>;
   auto &sJ = sm.sJ;
   const MfmaLane<T> L(sm.s1, t);
@@ -681,6 +692,7 @@ __device__ __forceinline__ void mfmaAxes(SM &sm, const int slot, const int t, co
       T ft = (T)0;
 #pragma unroll
       for (int d2 = 0; d2 < 3; d2++) ft += pJ[d * 3 + d2] * F[d2][k];
+      if (dbgF) dbgF[(d * NF + k) * 64 + t] = ft;
       const T b = d < 2 ? lanePull(d == 0 ? L.bx : L.by, ft) : ft;
       T der;
       V::axis(L.aD, d == 0 ? L.aEx : L.aEzy, b, d == 1 ? e1[k] : e0[k], der);
@@ -731,7 +743,7 @@ __device__ __forceinline__ void mfmaFaceRound(const T *s1, const T *S, const int
 // area-weighted normal, whose length is dA; a right-side point uses the
 // negated own normal for the left element's (equal but for rounding).
 // GM = 2 forms the solution-point metrics and reads the face geometry.
-template <int D, int P, int NF, typename T, bool BND, bool VIS, bool FIN, int GM = 0>
+template <int D, int P, int NF, typename T, bool BND, bool VIS, bool FIN, int GM = 0, bool DBG = false>
 __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, StageSmem<D, P, NF, T> &sm,
                                           const int tid) {
   static_assert(GM == 0 || !VIS, "the viscous stage reads the per-point geometry");
@@ -987,7 +999,7 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
   // along d (calcDivF_spts) and its extrapolation to the two faces normal to d
   // (extrapolateNormalFlux), stored negated as the start of the jump
   if constexpr (MF) {
-    if (act) mfmaAxes<NF, T>(sm, slot, t, pJ, F, div);
+    if (act) mfmaAxes<NF, T>(sm, slot, t, pJ, F, div, DBG ? a.dbgF + (size_t)e * D * NF * NS : nullptr);
   } else {
 #pragma unroll
     for (int d = 0; d < D; d++) {
@@ -998,6 +1010,7 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
 #pragma unroll
           for (int d2 = 0; d2 < D; d2++) acc += pJ[d * D + d2] * F[d2][k];
           sFd[slot][k][tP] = acc;
+          if constexpr (DBG) a.dbgF[(((size_t)e * D + d) * NF + k) * NS + t] = acc;
         }
       }
       elementSync<TPE>();
@@ -1143,6 +1156,12 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
       const T scale = right ? -dA : dA;
 #pragma unroll
       for (int k = 0; k < NF; k++) sJ[slot][k][q] = Fn[k] * scale + sJ[slot][k][q];
+      if constexpr (DBG)
+#pragma unroll
+        for (int k = 0; k < NF; k++) {
+          a.dbgFn[((size_t)e * NF + k) * NFP + q] = Fn[k] * scale;
+          a.dbgJ[((size_t)e * NF + k) * NFP + q] = sJ[slot][k][q];
+        }
     }
   }
   elementSync<TPE>();
@@ -1254,6 +1273,16 @@ __global__ __launch_bounds__((ElemGeo<D, P>::NTH),
   // pads the grid to a multiple of 8.
   const int b = (blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);
   stageBody<D, P, NF, T, BND, VIS, FIN, GM>(a, b, sm, (int)threadIdx.x);
+}
+
+// The residual evaluation (mode 0) with the debug outputs of StageArgs
+// (dbgF, dbgFn, dbgJ) written: the slow path behind fr_get_array, FP64, the
+// stored geometry arrays (GM = 0).
+template <int D, int P, int NF, bool BND, bool VIS>
+__global__ __launch_bounds__((ElemGeo<D, P>::NTH), 1) void k_stage_dbg(StageArgs<double> a) {
+  __shared__ StageSmem<D, P, NF, double> sm;
+  const int b = (blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+  stageBody<D, P, NF, double, BND, VIS, false, 0, true>(a, b, sm, (int)threadIdx.x);
 }
 
 // ---------------------------------------------------------------------------

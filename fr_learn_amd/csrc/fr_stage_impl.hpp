@@ -27,7 +27,7 @@ int launchStageArgsD(fr_ctx *c, const frdev::StageArgs<T> &a, int mode) {
     const unsigned nb = (unsigned)(((c->nE + G::EPB - 1) / G::EPB + 7) / 8 * 8);
     if (vis) {
       frdev::GradArgs<T> g{a.Uin,  a.UfIn,  a.JG,   a.detJ,  a.fgeo, a.fnbr,
-                            a.finfo, a.oned, (T *)c->dUf, a.nEles, a.ph, a.affine};
+                            a.finfo, a.oned, (T *)c->dUf, a.nEles, a.ph, a.affine, nullptr};
       {
         TimeScope ts(c, 3);
         if (bnd) hipLaunchKernelGGL((frdev::k_grad<D, P, NF, T, true>), dim3(nb), dim3(G::NTH), 0, c->stream, g);

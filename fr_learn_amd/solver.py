@@ -102,6 +102,23 @@ class Solver:
         _lib.check(self._lib.fr_get_traces(self._h, _lib.dptr(out)))
         return out
 
+    # the residual's intermediate arrays (fr_get_array), with their shapes
+    ARRAYS = {"F_spts": 2, "Fn_fpts": 3, "disFn_fpts": 4, "dU_spts": 5, "dU_fpts": 6}
+
+    def array(self, which):
+        """F_spts (2), Fn_fpts (3), disFn_fpts (4), dU_spts (5), dU_fpts (6)
+        of a residual evaluation of the current state, in the reference
+        layout (a debug path: see fr_get_array); `which` is the number or
+        the name."""
+        which = self.ARRAYS.get(which, which)
+        D, nS, nF, nE, K = self.nDims, self.nSpts, self.nFpts, self.nEles, self.nFields
+        shape = {2: (D, nS, nE, K), 3: (nF, nE, K), 4: (nF, nE, K), 5: (D, nS, nE, K), 6: (D, nF, nE, K)}
+        if which not in shape:
+            raise ValueError(f"no array {which}")
+        out = np.zeros(shape[which])
+        _lib.check(self._lib.fr_get_array(self._h, int(which), _lib.dptr(out)))
+        return out
+
     def update(self, n=1):
         _lib.check(self._lib.fr_update(self._h, n))
 

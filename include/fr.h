@@ -131,9 +131,22 @@ int fr_get_state(fr_ctx *ctx, double *U, int layout);
 int fr_calc_residual(fr_ctx *ctx, int step);
 /* divF_spts[step] (solver.hpp:63) */
 int fr_get_residual(fr_ctx *ctx, int step, double *out, int layout);
-/* Fn_fpts / U_fpts / disFn-free intermediate views are not materialized by the
- * fused kernels; only the face traces U_fpts can be read back: */
+/* U_fpts (solver.hpp:55): the face traces of the current state */
 int fr_get_traces(fr_ctx *ctx, double *out /* U_fpts(fpt,ele,field) */);
+/* The residual's intermediate arrays, the data members solver.hpp:57-60 that
+ * the ele accessors read (ele.cpp:68-97): a debug path.  The fused kernels
+ * keep them in registers and LDS; this call evaluates the residual of the
+ * current state once (as fr_calc_residual(ctx, 0): divF_spts[0] and the traces
+ * are those of the current state afterwards) with debug outputs switched on,
+ * in buffers it allocates and frees.  which, in the reference layout:
+ *   2 F_spts(dim,spt,ele,field)    the transformed total flux JGinv.F
+ *   3 Fn_fpts(fpt,ele,field)       the common normal flux times dA, per side
+ *   4 disFn_fpts(fpt,ele,field)    after correctDivFlux: Fn_fpts - disFn
+ *   5 dU_spts(dim,spt,ele,field)   viscous: the corrected physical gradient
+ *   6 dU_fpts(dim,fpt,ele,field)   viscous: the gradient at the flux points
+ * FP64 quad and hex contexts; FR_EINVAL otherwise, and for 5 / 6 on the
+ * inviscid path. */
+int fr_get_array(fr_ctx *ctx, int which, double *out);
 
 /* solver::update() -- solver.cpp:212-250, repeated nSteps times */
 int fr_update(fr_ctx *ctx, int nSteps);
