@@ -91,6 +91,7 @@ SIGNATURES = {
     "fr_get_residual": (ctypes.c_int, [_P, ctypes.c_int, _DP, ctypes.c_int]),
     "fr_get_traces": (ctypes.c_int, [_P, _DP]),
     "fr_get_array": (ctypes.c_int, [_P, ctypes.c_int, _DP]),
+    "fr_keep_final_residual": (ctypes.c_int, [_P, ctypes.c_int]),
     "fr_update": (ctypes.c_int, [_P, ctypes.c_int]),
     "fr_time_step_a": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_double]),
     "fr_time_step_b": (ctypes.c_int, [_P, ctypes.c_int]),

@@ -126,6 +126,10 @@ frdev::StageArgs<T> makeStageArgs(fr_ctx *c, int in, int cur, int mode, int step
     }
     if (mode == 1) a.Unext = nullptr;
   }
+  // the final stage of a multi-stage update leaves divF_spts[nRK-1] out
+  // unless asked to keep it (stage 1's residual, divF_spts[0], is the one the
+  // residual norms read)
+  if (mode >= 2 && step > 0 && !c->keepFinal) a.divF = nullptr;
   a.mode = mode;
   a.nprev = step;
   a.nEles = c->nE;
@@ -145,6 +149,7 @@ int launchStage(fr_ctx *c, int in, int mode, int step, double rk) {
     c->cur ^= 1;
     c->traceOf = (mode == 1) ? 1 : 0;
   }
+  if (step == c->nRK - 1) c->finalDropped = a.divF == nullptr;
   return 0;
 }
 
@@ -935,6 +940,8 @@ int fr_calc_residual(fr_ctx *c, int step) {
 int fr_get_residual(fr_ctx *c, int step, double *out, int layout) {
   if (!c || !out) return fail(FR_EINVAL, "null argument");
   if (step < 0 || step >= c->nRK) return fail(FR_EINVAL, "step out of range");
+  if (step == c->nRK - 1 && c->finalDropped)
+    return fail(FR_EINVAL, "divF_spts of the final stage was not kept (fr_keep_final_residual)");
   const size_t n = c->nState();
   HIP_TRY(hipStreamSynchronize(c->stream));
   std::vector<double> h(n);
@@ -1478,6 +1485,7 @@ int fr_batch_update(fr_batch *b, int nSteps) {
   for (fr_ctx *c : b->ctx) {
     c->cur = parity;
     c->traceOf = 0;
+    c->finalDropped = !c->keepFinal && c->nRK > 1;
   }
   return 0;
 }
@@ -1518,6 +1526,15 @@ int fr_geometry_mode(fr_ctx *c, int request, int *active) {
     c->gen++;
   }
   if (active) *active = c->geoActive();
+  return 0;
+}
+
+int fr_keep_final_residual(fr_ctx *c, int keep) {
+  if (!c || keep < 0 || keep > 1) return fail(FR_EINVAL, "bad argument");
+  if (c->keepFinal != keep) {
+    c->keepFinal = keep;
+    c->gen++;
+  }
   return 0;
 }
 

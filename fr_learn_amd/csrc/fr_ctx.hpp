@@ -86,6 +86,8 @@ struct fr_ctx {
   void *Uf[2] = {nullptr, nullptr};   // face traces, double-buffered
   int cur = 0;                        // Uf[cur] holds the traces of `traceOf`
   int traceOf = -1;                   // 0: U_spts, 1: stage state Us, -1: stale
+  int keepFinal = 1;                  // the final RK stage stores divF_spts[nRK-1] (fr_keep_final_residual)
+  bool finalDropped = false;          // divF[nRK-1] is stale: the last update did not store it
   void *divF[4] = {nullptr, nullptr, nullptr, nullptr};
   void *dUf = nullptr;                // viscous: gradient traces [ele][dim][field][fpt]
   void *Fv = nullptr;                 // viscous: common viscous normal flux [ele][field][fpt]

@@ -119,6 +119,10 @@ class Solver:
         _lib.check(self._lib.fr_get_array(self._h, int(which), _lib.dptr(out)))
         return out
 
+    def keep_final_residual(self, keep=True):
+        """Whether update() stores the final stage's divF_spts (fr_keep_final_residual)."""
+        _lib.check(self._lib.fr_keep_final_residual(self._h, int(bool(keep))))
+
     def update(self, n=1):
         _lib.check(self._lib.fr_update(self._h, n))
 

@@ -291,6 +291,16 @@ int fr_read_checkpoint(fr_ctx *ctx, const char *path, int *iter, double *time);
  * (ele::calcTransforms, ele.cpp:202-342); the results agree to rounding. */
 int fr_geometry_mode(fr_ctx *ctx, int request, int *active);
 
+/* keep = 1 (the default): the final stage of fr_update stores its
+ * divergence in divF_spts[nRK-1], as solver::update leaves it
+ * (solver.cpp:212-250).  keep = 0: a multi-stage update does not store it --
+ * nothing in the update reads it, and leaving it out saves one state-sized
+ * write per step; fr_get_residual(ctx, nRK-1, ...) then fails with FR_EINVAL
+ * until a residual of that step is computed again (fr_calc_residual).
+ * divF_spts[0], which the residual norms read, is always stored.  No
+ * reference counterpart. */
+int fr_keep_final_residual(fr_ctx *ctx, int keep);
+
 /* Kernel timing: with timing on, every launch of the residual kernel is
  * bracketed by HIP events on the launch stream; fr_kernel_times returns the
  * accumulated milliseconds and launch count since the last reset for
