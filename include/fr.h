@@ -199,9 +199,12 @@ int fr_integrate_error(fr_ctx *ctx, const fr_error_params *p, double *out);
  * static grids), ele::calcDt (1086-1105), solver::calcDt (solver.cpp:293-305);
  * dtType 1: params->dt = min over elements; dtType 2: every element advances
  * with its own dt (timeStepA/B with eles[e]->dt, solver.cpp:307-385) and
- * params->dt = the minimum.  The reference evaluates the wave speeds only
- * from the initial state (solver.cpp:294-297, 1497-1503), so one call after
- * fr_set_state reproduces its dt for the whole run.  Triangles (no reference
+ * params->dt = the minimum.  As the reference does (calcWaveSpFpts at the
+ * first iteration only, solver.cpp:293-297, 1497-1503; the Rusanov face flux
+ * storing the wave speeds, face.cpp:354; calcDt at the start of every update,
+ * solver.cpp:221-222), the first fr_update after this call takes the wave
+ * speeds from the state and every later one from those its stage kernels'
+ * face fluxes stored, so dt follows the solution.  Triangles (no reference
  * counterpart) use the same formula with their dense traces, the element area
  * as vol and the quads' CFL limit.  fr_get_dt: the dt each element advances
  * with, in mesh element order. */
