@@ -4,6 +4,8 @@
 // same order so FP64 results agree with the reference to rounding.
 #pragma once
 
+#include <type_traits>
+
 #include <hip/hip_runtime.h>
 
 namespace frdev {
@@ -33,11 +35,12 @@ constexpr bool hasEnergyOffset() {
   return sizeof(T) == 4;
 }
 
-// 1 / x for the flux routines.  FP64: the hardware reciprocal estimate
-// refined by two Newton-Raphson steps, e = 1 - x r, r <- r + r e (each step
-// squares the relative error), within an ulp of the correctly rounded value
-// for densities of ordinary magnitude -- half the instructions of a full
-// IEEE division.  FP32: the division.
+// 1 / x for the flux routines: the hardware reciprocal estimate refined by
+// Newton-Raphson steps, e = 1 - x r, r <- r + r e (each step squares the
+// relative error), within an ulp of the correctly rounded value for
+// densities of ordinary magnitude.  FP64: two steps from the 2^-22-accurate
+// estimate, half the instructions of a full IEEE division; FP32: one step
+// from the 1-ulp estimate, three instructions instead of ten.
 template <typename T>
 __device__ __forceinline__ T recip(T x) {
   if constexpr (sizeof(T) == 8) {
@@ -46,8 +49,11 @@ __device__ __forceinline__ T recip(T x) {
     r = fma(r, e, r);
     e = fma(-x, r, 1.0);
     return fma(r, e, r);
+  } else if constexpr (std::is_same<T, float>::value) {
+    const float r = __builtin_amdgcn_rcpf(x);
+    return fmaf(fmaf(-x, r, 1.0f), r, r);
   } else {
-    return (T)1 / x;
+    return (T)1 / x;  // dual numbers (the adjoint's derivatives)
   }
 }
 
@@ -59,7 +65,9 @@ __device__ __forceinline__ T recip(T x) {
 // squares the relative error; a Newton correction g <- g + (x - g^2) h
 // after the step leaves g within an ulp of the correctly rounded root.  An x
 // below the smallest normal double is taken as that value (root 1.5e-154
-// instead of 0: the estimate of 0 is infinite).  FP32: the library root.
+// instead of 0: the estimate of 0 is infinite).  FP32: the hardware root
+// and reciprocal (1 ulp each; the IEEE-rounded library root takes about 16
+// instructions for its scaling and corrections).
 __device__ __forceinline__ void rootSteps(double x, double &g, double &h, int steps) {
   x = fmax(x, 2.2250738585072014e-308);
   const double y = __builtin_amdgcn_rsq(x);
@@ -78,8 +86,10 @@ __device__ __forceinline__ T rootPos(T x) {
     double g, h;
     rootSteps(x, g, h, 1);
     return g;
+  } else if constexpr (std::is_same<T, float>::value) {
+    return __builtin_amdgcn_sqrtf(x);
   } else {
-    return sqrt(x);
+    return sqrt(x);  // dual numbers
   }
 }
 template <typename T>
@@ -89,8 +99,11 @@ __device__ __forceinline__ void rootAndInv(T x, T &root, T &inv) {
     rootSteps(x, g, h, 2);
     root = g;
     inv = h + h;
+  } else if constexpr (std::is_same<T, float>::value) {
+    root = __builtin_amdgcn_sqrtf(x);
+    inv = __builtin_amdgcn_rcpf(root);
   } else {
-    root = sqrt(x);
+    root = sqrt(x);  // dual numbers
     inv = (T)1 / root;
   }
 }
