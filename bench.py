@@ -267,10 +267,20 @@ def tgv_ns_adjoint(n=24, steps=200, every=20, eps=1e-6, seed=0):
         return s.kinetic_energy()[0]
 
     s.set_state(U0)
+    s.set_timing(True)
+    s.kernel_times(0)
+    s.kernel_times(3)
     t0 = time.perf_counter()
     s.update(steps)
     s.synchronize()
     t_fwd = time.perf_counter() - t0
+    st_ms, st_n = s.kernel_times(0)
+    gr_ms, gr_n = s.kernel_times(3)
+    s.set_timing(False)
+    # viscous roofline: B_res plus the gradient traffic E*Ns*2*D*Nf*s of a
+    # viscous residual (the gradient written and read once per point)
+    b_vis = s.residual_bytes() + s.nEles * s.nSpts * 2 * 3 * 5 * 8
+    stage_ms = (st_ms + gr_ms) / max(st_n, 1)
     lamN = s.kinetic_energy_gradient()
     s.set_state(U0)
     t0 = time.perf_counter()
@@ -285,18 +295,45 @@ def tgv_ns_adjoint(n=24, steps=200, every=20, eps=1e-6, seed=0):
             "workload": f"{n}^3 hexes P3, Navier-Stokes Re 1600 Pr 0.71 (LDG), RK4 dt 1e-3, {steps} steps, "
                         f"checkpoint every {every}",
             "forward_s": t_fwd, "adjoint_s": t_adj, "adjoint_over_forward": t_adj / t_fwd,
+            "viscous_stage_roofline": {
+                "bound": "hbm", "algorithmic_bytes_per_stage": b_vis, "stage_ms": stage_ms,
+                "gradient_and_face_flux_ms": gr_ms / max(gr_n, 1), "stage_kernel_ms": st_ms / max(st_n, 1),
+                "achieved": b_vis / (stage_ms * 1e-3) / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                "frac": b_vis / (stage_ms * 1e-3) / HBM_PEAK,
+                "note": "k_grad + k_vflux + k_stage<...,VIS> per RK stage (HIP events); bytes = B_res + "
+                        "E*Ns*2*D*Nf*s"},
             "fd_directional": fd, "adjoint_directional": adj, "rel_diff_vs_fd": abs(fd - adj) / abs(fd),
             "eps": eps}
 
 
+def csrc_digest():
+    """sha256 over the library's kernel and host sources (name and bytes of every
+    .hip / .hpp / .cpp under fr_learn_amd/csrc, in name order)."""
+    import hashlib
+    h = hashlib.sha256()
+    csrc = os.path.join(ROOT, "fr_learn_amd", "csrc")
+    for fn in sorted(os.listdir(csrc)):
+        if fn.endswith((".hip", ".hpp", ".cpp")):
+            h.update(fn.encode())
+            with open(os.path.join(csrc, fn), "rb") as fh:
+                h.update(fh.read())
+    return h.hexdigest()
+
+
 def traffic_from_profile(precision, geometry):
-    """HBM bytes per launch of the residual kernel from the committed rocprofv3
-    PMC summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor), if any."""
+    """HBM bytes per RK stage launch of the stage kernel from the committed
+    rocprofv3 PMC summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel
+    factor), if there is one and it was measured on the current sources (its
+    csrc_sha256 equals csrc_digest()); None otherwise, so that a kernel change
+    never reports the traffic of the kernel before it."""
     path = os.path.join(ROOT, "profiles", f"k_stage_tgv48_p3_{precision}_{geometry}_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as fh:
-        return json.load(fh).get("bytes_per_launch")
+        js = json.load(fh)
+    if js.get("csrc_sha256") != csrc_digest():
+        return None
+    return js.get("bytes_per_launch")
 
 
 def main():
