@@ -32,6 +32,9 @@
 #ifndef FR_MF_WAVES
 #define FR_MF_WAVES 3  // waves per SIMD asked of the hex P3 matrix-core stage kernels
 #endif
+#ifndef FR_MF_FIN_WAVES
+#define FR_MF_FIN_WAVES 3  // the same for their final-stage variant
+#endif
 #ifndef FR_STAGE_MINW
 #define FR_STAGE_MINW 1  // waves per SIMD asked of the intermediate inviscid stage kernel
 #endif
@@ -1264,7 +1267,7 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
 template <int D, int P, int NF, typename T, bool BND, bool VIS, bool FIN, int GM = 0>
 __global__ __launch_bounds__((ElemGeo<D, P>::NTH),
                              (VIS ? 1
-                              : (FR_MFMA && D == 3 && P == 3) ? FR_MF_WAVES
+                              : (FR_MFMA && D == 3 && P == 3) ? (FIN ? FR_MF_FIN_WAVES : FR_MF_WAVES)
                               : FIN                           ? 3
                                                               : FR_STAGE_MINW)) void k_stage(StageArgs<T> a) {
   __shared__ StageSmem<D, P, NF, T> sm;
