@@ -762,8 +762,8 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
   // the modes a variant serves: the inviscid FIN variants 2 and 3, the other
   // inviscid variants 0 and 1 (the launches choose so), the viscous one all
   constexpr bool M01 = VIS || !FIN, M23 = VIS || FIN;
-  // hex P3, inviscid: the line contractions of (2) on the matrix cores
-  constexpr bool MF = FR_MFMA && D == 3 && P == 3 && !VIS;
+  // hex P3: the line contractions of (2) on the matrix cores
+  constexpr bool MF = FR_MFMA && D == 3 && P == 3;
   // the face values of the state in (3) and (5) too: FP64 only (three FP32
   // 16x16x4 products per field cost more than the FP32 line sums they replace)
   constexpr bool MFF = MF && sizeof(T) == 8;
