@@ -285,17 +285,26 @@ __global__ __launch_bounds__(256) void k_vjp_face_vis(
     // state directions: dual passes through the inviscid and viscous fluxes
 #pragma unroll 1
     for (int sj = 0; sj < NF; sj++) {
-      DM uo[NF], un[NF], n[D], go[D][NF], gn[D][NF], Fn[NF];
+      DM uo[NF], un[NF], n[D], Fn[NF];
 #pragma unroll
       for (int k = 0; k < NF; k++) uo[k] = (k == sj) ? DM::var(uov[k], 0) : DM(uov[k]);
 #pragma unroll
       for (int c = 0; c < D; c++) n[c] = DM(nv[c]);
+      // the gradient traces do not depend on this pass's direction: plain
+      // doubles (half the registers of constant duals), except at boundaries,
+      // whose ghost gradient may depend on the state
+      double gov[D][NF];
 #pragma unroll
       for (int d = 0; d < D; d++)
 #pragma unroll
-        for (int k = 0; k < NF; k++) go[d][k] = DM(Qf[(((size_t)e * D + d) * NF + k) * NFP + q]);
+        for (int k = 0; k < NF; k++) gov[d][k] = Qf[(((size_t)e * D + d) * NF + k) * NFP + q];
       if (bnd) {
         const int bc = fi - FINFO_BOUNDARY;
+        DM go[D][NF], gn[D][NF];
+#pragma unroll
+        for (int d = 0; d < D; d++)
+#pragma unroll
+          for (int k = 0; k < NF; k++) go[d][k] = DM(gov[d][k]);
         boundaryState<D, NF, DM>(ph, bc, uo, n, un);
         if constexpr (NF > 1) {
           boundaryGradient<D, NF, DM>(ph, bc, uo, un, go, gn);
@@ -317,16 +326,17 @@ __global__ __launch_bounds__(256) void k_vjp_face_vis(
           un[k] = DM(unv[k]);
           g[k] = lj[k] * (right ? -dA : dA) + A[((size_t)en * NF + k) * NFP + qn];
         }
+        double gnv[D][NF];
 #pragma unroll
         for (int d = 0; d < D; d++)
 #pragma unroll
-          for (int k = 0; k < NF; k++) gn[d][k] = DM(Qf[(((size_t)en * D + d) * NF + k) * NFP + qn]);
+          for (int k = 0; k < NF; k++) gnv[d][k] = Qf[(((size_t)en * D + d) * NF + k) * NFP + qn];
         if (right) {
           commonFlux<D, NF, DM>(ph, false, un, uo, n, Fn);
-          ldgViscousFlux<D, NF, DM>(ph, 0, un, uo, gn, go, n, Fn);
+          ldgViscousFlux<D, NF, DM, double>(ph, 0, un, uo, gnv, gov, n, Fn);
         } else {
           commonFlux<D, NF, DM>(ph, false, uo, un, n, Fn);
-          ldgViscousFlux<D, NF, DM>(ph, 0, uo, un, go, gn, n, Fn);
+          ldgViscousFlux<D, NF, DM, double>(ph, 0, uo, un, gov, gnv, n, Fn);
         }
       }
       double acc = 0;
@@ -371,24 +381,72 @@ __global__ __launch_bounds__(256) void k_vjp_face_vis(
           else ldgViscousFlux<D, NF, double>(ph, 0, uo, un, go, gn, nd, F);
         }
       };
-      double go[D][NF], F0[NF], F1[NF];
+      if (!bnd) {
+        // interior: Fn depends on this point's gradient only through this
+        // side's viscous flux F = viscousFlux(uo, go), linearly, with
+        // dFn_k / dF[d][k] = w[d] the same for every field: 1/2 n_d from the
+        // average, plus (left side) pen |n|^2 times the coefficient of F_d
+        // in n.(F_L - F_R) -- with face.cpp:245's substitution of F_L,y for
+        // F_L,z when viscFix = 0 -- or (right side) minus pen |n|^2 n_d.
+        // viscousFlux is linear and homogeneous in the gradient, so the
+        // derivative along e_dk is the flux of the unit gradient itself.
+        double w[D];
+        if constexpr (NF == 1) {
 #pragma unroll
-      for (int d = 0; d < D; d++)
+          for (int d = 0; d < D; d++) w[d] = 0.5 * nd[d];
+        } else {
+          const double pen = ldgPenalty<D, double>(ph, nd);
+          double nn = 0;
 #pragma unroll
-        for (int k = 0; k < NF; k++) go[d][k] = 0.0;
-      viscF(go, F0);
+          for (int d = 0; d < D; d++) nn += nd[d] * nd[d];
+          const double pn = pen * nn;
+#pragma unroll
+          for (int d = 0; d < D; d++) w[d] = 0.5 * nd[d] + (right ? -pn * nd[d] : pn * nd[d]);
+          if constexpr (D == 3)
+            if (!right && !ph.viscFix) {  // F_L,y stands in for F_L,z in n.(F_L - F_R)
+              w[1] += pn * nd[2];
+              w[2] -= pn * nd[2];
+            }
+        }
+#pragma unroll
+        for (int dk = 0; dk < D * NF; dk++) {
+          const int d = dk / NF, sj = dk - d * NF;
+          double ge[D][NF], F[D][NF];
+#pragma unroll
+          for (int d2 = 0; d2 < D; d2++)
+#pragma unroll
+            for (int k = 0; k < NF; k++) ge[d2][k] = (d2 == d && k == sj) ? 1.0 : 0.0;
+          viscousFlux<D, NF, double>(ph, uo, ge, F);
+          double acc = 0;
+#pragma unroll
+          for (int k = 0; k < NF; k++) {
+            double fk = 0;
+#pragma unroll
+            for (int d2 = 0; d2 < D; d2++) fk += w[d2] * F[d2][k];
+            acc += g[k] * fk;
+          }
+          Qo[(((size_t)e * D + d) * NF + sj) * NFP + q] = acc;
+        }
+      } else {
+        double go[D][NF], F0[NF], F1[NF];
+#pragma unroll
+        for (int d = 0; d < D; d++)
+#pragma unroll
+          for (int k = 0; k < NF; k++) go[d][k] = 0.0;
+        viscF(go, F0);
 #pragma unroll 1
-      for (int dk = 0; dk < D * NF; dk++) {
-        const int d = dk / NF, sj = dk - d * NF;
+        for (int dk = 0; dk < D * NF; dk++) {
+          const int d = dk / NF, sj = dk - d * NF;
 #pragma unroll
-        for (int d2 = 0; d2 < D; d2++)
+          for (int d2 = 0; d2 < D; d2++)
 #pragma unroll
-          for (int k = 0; k < NF; k++) go[d2][k] = (d2 == d && k == sj) ? 1.0 : 0.0;
-        viscF(go, F1);
-        double acc = 0;
+            for (int k = 0; k < NF; k++) go[d2][k] = (d2 == d && k == sj) ? 1.0 : 0.0;
+          viscF(go, F1);
+          double acc = 0;
 #pragma unroll
-        for (int k = 0; k < NF; k++) acc += g[k] * (F1[k] - F0[k]);
-        Qo[(((size_t)e * D + d) * NF + sj) * NFP + q] = acc;
+          for (int k = 0; k < NF; k++) acc += g[k] * (F1[k] - F0[k]);
+          Qo[(((size_t)e * D + d) * NF + sj) * NFP + q] = acc;
+        }
       }
     }
     // forward LDG jump of this point (as k_grad forms it)

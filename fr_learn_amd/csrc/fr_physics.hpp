@@ -397,8 +397,10 @@ __device__ __forceinline__ void boundaryState(const Phys &ph, int bc, const T *U
 // uses the physical tauxz = mu(du/dz + dw/dx), tauyz = mu(dv/dz + dw/dy),
 // Fv[0][3] = -tauxz.  Advection-diffusion: Fv = -diffD grad u
 // (viscousFluxAD, flux.cpp:361-366).
-template <int D, int NF, typename T>
-__device__ __forceinline__ void viscousFlux(const Phys &ph, const T *U, const T g[D][NF], T Fv[D][NF]) {
+// TG, the type of the gradient, may differ from T: the adjoint's state
+// passes differentiate through U only and keep the gradient in plain doubles.
+template <int D, int NF, typename T, typename TG = T>
+__device__ __forceinline__ void viscousFlux(const Phys &ph, const T *U, const TG g[D][NF], T Fv[D][NF]) {
   if constexpr (NF == 1) {
 #pragma unroll
     for (int d = 0; d < D; d++) Fv[d][0] = -(T)ph.diffD * g[d][0];
@@ -419,9 +421,9 @@ __device__ __forceinline__ void viscousFlux(const Phys &ph, const T *U, const T 
       w = over(U[3]);
       e -= (T)0.5 * (w * w);
     }
-    const T dRho_dx = g[0][0], dRhoU_dx = g[0][1], dRhoV_dx = g[0][2], dE_dx = g[0][D + 1];
-    const T dRho_dy = g[1][0], dRhoU_dy = g[1][1], dRhoV_dy = g[1][2], dE_dy = g[1][D + 1];
-    T dRho_dz = 0, dRhoU_dz = 0, dRhoV_dz = 0, dRhoW_dx = 0, dRhoW_dy = 0, dRhoW_dz = 0, dE_dz = 0;
+    const TG dRho_dx = g[0][0], dRhoU_dx = g[0][1], dRhoV_dx = g[0][2], dE_dx = g[0][D + 1];
+    const TG dRho_dy = g[1][0], dRhoU_dy = g[1][1], dRhoV_dy = g[1][2], dE_dy = g[1][D + 1];
+    TG dRho_dz = 0, dRhoU_dz = 0, dRhoV_dz = 0, dRhoW_dx = 0, dRhoW_dy = 0, dRhoW_dz = 0, dE_dz = 0;
     if constexpr (D == 3) {
       dRho_dz = g[2][0];
       dRhoU_dz = g[2][1];
@@ -539,15 +541,15 @@ __device__ __forceinline__ T ldgPenalty(const Phys &ph, const T *n) {
 // common flux Fn (face::calcViscousFlux, face.cpp:177-287), in the left
 // element's frame.  bnd: 0 interior, 1 Dirichlet-type boundary, 2 adiabatic
 // wall.
-template <int D, int NF, typename T>
-__device__ __forceinline__ void ldgViscousFlux(const Phys &ph, int bnd, const T *UL, const T *UR, const T gL[D][NF],
-                                               const T gR[D][NF], const T *n, T *Fn) {
+template <int D, int NF, typename T, typename TG = T>
+__device__ __forceinline__ void ldgViscousFlux(const Phys &ph, int bnd, const T *UL, const T *UR, const TG gL[D][NF],
+                                               const TG gR[D][NF], const T *n, T *Fn) {
   T Fc[D][NF];
   if constexpr (NF == 1) {
     // advection-diffusion: central flux of -diffD grad u (face.cpp:270-279)
     T FL[D][NF], FR[D][NF];
-    viscousFlux<D, NF, T>(ph, UL, gL, FL);
-    viscousFlux<D, NF, T>(ph, UR, gR, FR);
+    viscousFlux<D, NF, T, TG>(ph, UL, gL, FL);
+    viscousFlux<D, NF, T, TG>(ph, UR, gR, FR);
     T t = 0;
 #pragma unroll
     for (int d = 0; d < D; d++) t += (T)0.5 * (FL[d][0] + FR[d][0]) * n[d];
@@ -556,16 +558,16 @@ __device__ __forceinline__ void ldgViscousFlux(const Phys &ph, int bnd, const T 
   }
   if (bnd) {
     T Fb[D][NF];
-    if (bnd > 1) viscousFlux<D, NF, T>(ph, UR, gR, Fb);
-    else viscousFlux<D, NF, T>(ph, UL, gL, Fb);
+    if (bnd > 1) viscousFlux<D, NF, T, TG>(ph, UR, gR, Fb);
+    else viscousFlux<D, NF, T, TG>(ph, UL, gL, Fb);
 #pragma unroll
     for (int d = 0; d < D; d++)
 #pragma unroll
       for (int k = 0; k < NF; k++) Fc[d][k] = Fb[d][k] + (T)ph.tau * n[d] * (UL[k] - UR[k]);
   } else {
     T FL[D][NF], FR[D][NF];
-    viscousFlux<D, NF, T>(ph, UL, gL, FL);
-    viscousFlux<D, NF, T>(ph, UR, gR, FR);
+    viscousFlux<D, NF, T, TG>(ph, UL, gL, FL);
+    viscousFlux<D, NF, T, TG>(ph, UR, gR, FR);
     const T pen = ldgPenalty<D, T>(ph, n);
 #pragma unroll
     for (int k = 0; k < NF; k++) {
