@@ -303,6 +303,229 @@ __device__ __forceinline__ void ldgJump(const Phys &ph, bool bnd, bool right, co
   }
 }
 
+// Hex P3 line operations on the matrix cores.  One element is one wavefront,
+// lane t = solution point i + 4j + 16k.  Along each reference axis the
+// element has 16 lines of 4 points, and every line operation of the stage is
+// a product A B of a small operator A (rows: the derivative matrix Dm, the
+// extrapolations l_L and l_R to the line's two faces) and the 4 x 16 matrix
+// B of the lines' values:
+//  - FP64: v_mfma_f64_4x4x4_4b, four independent 4x4x4 products; block b
+//    holds lines 4b .. 4b+3, with A_b[m][k] at lane m + 4b + 16k,
+//    B_b[k][n] at lane n + 4b + 16k and C_b[m][n] at lane n + 4b + 16m.
+//    The derivative (A = Dm) and the extrapolations (A rows l_L, l_R) are
+//    separate products.
+//  - FP32: v_mfma_f32_16x16x4f32, A[m][k] at lane m + 16k, B[k][n] at lane
+//    n + 16k, C[m][n] at lane n + 16 (m / 4), register m % 4.  Rows 4g of A
+//    hold Dm row g (derivative at point g in register 0) and rows 1 + 4s of
+//    A hold the extrapolations (register 1), so one product gives both.
+// In both, line n = n0 + 4 n1 of the axis takes its point k from lane
+// n + 16k and returns its derivative at point g in lane n + 16g: for the z
+// axis (n = i + 4j) that is the lane's own point, so B is the lanes' own
+// values and the derivative lands in place; for x (n = j + 4k, point i) and
+// y (n = i + 4k, point j) the operand comes from the permuted lane (a lane
+// permute, or LDS for state already there) and the derivative goes back
+// through the inverse permutation.  The extrapolations land in the lanes of
+// flux points: z lines put their left face value in lane n and right face
+// value in lane 16 + n, x lines in lanes 32 + n and 48 + n (rows 2, 3 of A
+// in FP64, rows 9, 13 in FP32) and y lines in lanes n and 16 + n, so that
+// one accumulator holds the z and x face values (the 64 flux points of faces
+// 0-3) and another the y values (faces 4, 5) -- lane t handles flux point
+// mfmaFpt(t, r) in round r of the face phases.  The operator entries are
+// those of the VALU line sums (calcDivF_spts, extrapolateNormalFlux,
+// extrapolateU); the matrix cores sum the four products in their own order.
+template <typename T>
+struct MfmaVec;
+template <>
+struct MfmaVec<double> {
+  static __device__ __forceinline__ double mma(double a, double b, double c) {
+    return __builtin_amdgcn_mfma_f64_4x4x4f64(a, b, c, 0, 0, 0);
+  }
+  // derivative (aD) and face values added to acc (aE) of one axis
+  static __device__ __forceinline__ void axis(double aD, double aE, double b, double &acc, double &der) {
+    der = mma(aD, b, 0.0);
+    acc = mma(aE, b, acc);
+  }
+  static __device__ __forceinline__ double ext(double aE, double b, double acc) { return mma(aE, b, acc); }
+  // the derivative alone
+  static __device__ __forceinline__ double der(double aD, double aE, double b) {
+    (void)aE;
+    return mma(aD, b, 0.0);
+  }
+};
+template <>
+struct MfmaVec<float> {
+  typedef float type __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ type mma(float a, float b, float c1) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, type{0.f, c1, 0.f, 0.f}, 0, 0, 0);
+  }
+  static __device__ __forceinline__ void axis(float aD, float aE, float b, float &acc, float &der) {
+    (void)aD;
+    const type c = mma(aE, b, acc);
+    der = c[0];
+    acc = c[1];
+  }
+  static __device__ __forceinline__ float ext(float aE, float b, float acc) { return mma(aE, b, acc)[1]; }
+  static __device__ __forceinline__ float der(float aD, float aE, float b) {
+    (void)aD;
+    return mma(aE, b, 0.f)[0];
+  }
+};
+
+// Flux point of lane t in round r (0, 1) of the face phases of a hex P3
+// element: round 0 holds faces 0-3 (lanes 16f ..), round 1 faces 4 and 5;
+// within a face, the lane of line n has the point fptLine numbers n or, on
+// faces 1, 3 and 4 (whose point order runs against the line order), n ^ 3.
+__device__ __forceinline__ int mfmaFpt(int t, int r) {
+  const int flip = r == 0 ? (t >> 4) & 1 : (t >> 4) == 0;
+  return 64 * r + (t ^ (3 * flip));
+}
+
+// The lane's part of the products of a hex P3 element (lane t): its A
+// entries (the derivative aD; the face values of the z and y axes aEzy and
+// of the x axis aEx), the solution point it stands for in the x and y
+// operands as an LDS index (px, py) and a lane byte address (bx, by), and
+// the lane holding the x / y derivative of its own point (ix, iy).
+template <typename T>
+struct MfmaLane {
+  T aD, aEzy, aEx;
+  int sx, sy, px, py, bx, by, ix, iy;
+  __device__ __forceinline__ MfmaLane(const T *s1, int t) {
+    using O = OneD<3>;
+    const int k = t >> 4;
+    const T lL = s1[O::LL + k], lR = s1[O::LR + k];
+    if constexpr (sizeof(T) == 8) {
+      const int m = t & 3;
+      aD = s1[O::DM + m * 4 + k];
+      aEzy = m == 0 ? lL : m == 1 ? lR : (T)0;
+      aEx = m == 2 ? lL : m == 3 ? lR : (T)0;
+    } else {
+      const int m = t & 15;
+      const T d = (m & 3) == 0 ? s1[O::DM + (m >> 2) * 4 + k] : (T)0;
+      aD = d;
+      aEzy = m == 1 ? lL : m == 5 ? lR : d;
+      aEx = m == 9 ? lL : m == 13 ? lR : d;
+    }
+    const int ln = t & 15, lp = t >> 4;
+    sx = lp + 4 * (ln & 3) + 16 * (ln >> 2);
+    sy = (ln & 3) + 4 * lp + 16 * (ln >> 2);
+    px = padIdx<3>(sx);
+    py = padIdx<3>(sy);
+    bx = 4 * sx;
+    by = 4 * sy;
+    // point t = i + 4j + 16k is point i of x line j + 4k (lane j + 4k + 16i)
+    // and point j of y line i + 4k (lane i + 4k + 16j)
+    ix = 4 * ((t >> 2) + 16 * (t & 3));
+    iy = 4 * ((t & 3) + 4 * (t >> 4) + 16 * ((t >> 2) & 3));
+  }
+};
+
+// Value of v in lane addr / 4 of the wavefront (ds_bpermute: the LDS unit's
+// lane crossbar, no LDS memory).  Every lane of the wave is active here.
+__device__ __forceinline__ float lanePull(int addr, float v) {
+  return __int_as_float(__builtin_amdgcn_ds_bpermute(addr, __float_as_int(v)));
+}
+__device__ __forceinline__ double lanePull(int addr, double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_ds_bpermute(addr, (int)(b & 0xffffffffLL));
+  const int hi = __builtin_amdgcn_ds_bpermute(addr, (int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Phase (2) on the matrix cores: per reference axis d the contravariant
+// flux F~_d = sum_d2 JGinv(d,d2) F_d2, its derivative along d added to div,
+// and its values at the two faces normal to d, stored negated as the start
+// of the jump (the left face's outward flux is -F~_d): sJ at the round-0
+// and round-1 flux points of the lane.
+template <int NF, typename T, typename SM>
+__device__ __forceinline__ void mfmaAxes(SM &sm, const int slot, const int t, const T *pJ, const T (&F)[3][NF],
+                                         T (&div)[NF], T *dbgF) {
+  using V = MfmaVec<T>;
+  auto &sJ = sm.sJ;
+  const MfmaLane<T> L(sm.s1, t);
+  T e0[NF], e1[NF];
+#pragma unroll
+  for (int k = 0; k < NF; k++) e0[k] = e1[k] = (T)0;
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+#pragma unroll
+    for (int k = 0; k < NF; k++) {
+      T ft = (T)0;
+#pragma unroll
+      for (int d2 = 0; d2 < 3; d2++) ft += pJ[d * 3 + d2] * F[d2][k];
+      if (dbgF) dbgF[(d * NF + k) * 64 + t] = ft;
+      const T b = d < 2 ? lanePull(d == 0 ? L.bx : L.by, ft) : ft;
+      T der;
+      V::axis(L.aD, d == 0 ? L.aEx : L.aEzy, b, d == 1 ? e1[k] : e0[k], der);
+      const T dv = d < 2 ? lanePull(d == 0 ? L.ix : L.iy, der) : der;
+      div[k] = d == 0 ? dv : dv + div[k];
+    }
+  }
+  const T sg = ((t >> 4) & 1) ? (T)-1 : (T)1;
+  const int q0 = mfmaFpt(t, 0), q1 = mfmaFpt(t, 1);
+#pragma unroll
+  for (int k = 0; k < NF; k++) {
+    sJ[slot][k][q0] = sg * e0[k];
+    if (t < 32) sJ[slot][k][q1] = sg * e1[k];
+  }
+  elementSync<64>();
+}
+
+// Face values of a state S held in LDS (one element, fields RS apart; padded
+// rows when PADV) at the lane's flux point of round r (mfmaFpt): round 0
+// (faces 0-3) from the z lines -- the lane's own values own[] -- and the x
+// lines, round 1 (faces 4, 5) from the y lines.  All lanes take part.
+template <int NF, int RS, typename T, bool PADV = true>
+__device__ __forceinline__ void mfmaFaceRound(const T *s1, const T *S, const int t, const T (&own)[NF], const int r,
+                                              T (&v)[NF]) {
+  using V = MfmaVec<T>;
+  const MfmaLane<T> L(s1, t);
+  const int ix = PADV ? L.px : L.sx, iy = PADV ? L.py : L.sy;
+#pragma unroll
+  for (int k = 0; k < NF; k++)
+    v[k] = r == 0 ? V::ext(L.aEx, S[k * RS + ix], V::ext(L.aEzy, own[k], (T)0)) : V::ext(L.aEzy, S[k * RS + iy], (T)0);
+}
+
+// correctedGradient (below) for hex P3 with the reference-space derivative on
+// the matrix cores: the state S in LDS (fields RS apart, padded rows when
+// PADV) and the lane's own values own[], the jumps sCe [NF][NFP] at flux
+// points numbered as fptLine numbers them.  Same terms; the matrix cores sum
+// the derivative's four products in their own order.  All lanes take part.
+template <int NF, int RS, typename T, bool PADV>
+__device__ __forceinline__ void mfmaGradient(const T *s1, const T *S, const T (&own)[NF], const T *sCe, const int t,
+                                             const T *J, T detJ, T g[3][NF]) {
+  using V = MfmaVec<T>;
+  using O = OneD<3>;
+  constexpr int NFP = ElemGeo<3, 3>::NFP, NFACE = 6;
+  const MfmaLane<T> L(s1, t);
+  int cq[NFACE], cside[NFACE], cax[NFACE];
+  corrEntries<3, 3>(t, cq, cside, cax);
+  T gr[3][NF];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+#pragma unroll
+    for (int k = 0; k < NF; k++) {
+      const T b = d == 2 ? own[k] : S[k * RS + (d == 0 ? (PADV ? L.px : L.sx) : (PADV ? L.py : L.sy))];
+      const T der = V::der(L.aD, d == 0 ? L.aEx : L.aEzy, b);
+      const T dv = d < 2 ? lanePull(d == 0 ? L.ix : L.iy, der) : der;
+      T corr = (T)0;
+#pragma unroll
+      for (int f = 0; f < NFACE; f++)
+        if (faceAxis<3>(f) == d)
+          corr += (s1[(cside[f] ? O::GR : O::GL) + cax[f]] * (cside[f] ? (T)1 : (T)-1)) * sCe[k * NFP + cq[f]];
+      gr[d][k] = corr + dv;
+    }
+  }
+  const T invDet = (T)1. / detJ;
+#pragma unroll
+  for (int d1 = 0; d1 < 3; d1++)
+#pragma unroll
+    for (int k = 0; k < NF; k++) {
+      T acc = gr[0][k] * J[0 * 3 + d1] + gr[1][k] * J[1 * 3 + d1];
+      acc = acc + gr[2][k] * J[2 * 3 + d1];
+      g[d1][k] = invDet * acc;
+    }
+}
+
 // Corrected physical gradient at solution point t (calcGradU_spts,
 // solver.cpp:808-821; correctGradU 823-879): the reference-space gradient
 // sum_m Dm u along each axis, plus correctU = correction * tNorm applied to the
@@ -454,6 +677,10 @@ __global__ __launch_bounds__((ElemGeo<D, P>::NTH)) void k_grad(GradArgs<T> a) {
   const int slot = threadIdx.x / TPE, t = threadIdx.x - slot * TPE;
   const int e = b * EPB + slot;
   const bool act = e < a.nEles;
+  // hex P3 FP64: the state's face values, the gradient's derivative and the
+  // gradient traces on the matrix cores (lane t handles flux point
+  // mfmaFpt(t, r) in round r)
+  constexpr bool MG = FR_MFMA && D == 3 && P == 3 && sizeof(T) == 8;
   const bool hasS = act && t < NS;
   for (int i = threadIdx.x; i < O::SIZE; i += blockDim.x) s1[i] = a.oned[i];
   if (act) {
@@ -472,18 +699,29 @@ __global__ __launch_bounds__((ElemGeo<D, P>::NTH)) void k_grad(GradArgs<T> a) {
   // LDG jump at every flux point
 #pragma unroll
   for (int r = 0; r < NQ; r++) {
-    const int q = t + r * TPE;
+    const int q = MG ? mfmaFpt(t, r) : t + r * TPE;
+    T uo[NF];
+    if constexpr (MG) {
+      if (act) {
+        T own[NF];
+#pragma unroll
+        for (int k = 0; k < NF; k++) own[k] = sU[slot][k][t];
+        mfmaFaceRound<NF, NS, T, false>(s1, &sU[slot][0][0], t, own, r, uo);
+      }
+    }
     if (act && q < NFP) {
       int base, stride, side, dn;
       fptLine<D, P>(q, base, stride, side, dn);
       const T *l1 = s1 + (side ? O::LR : O::LL);
-      T uo[NF], un[NF], n[D], dUc[NF];
+      T un[NF], n[D], dUc[NF];
+      if constexpr (!MG) {
 #pragma unroll
-      for (int k = 0; k < NF; k++) {
-        T acc = (T)0;
+        for (int k = 0; k < NF; k++) {
+          T acc = (T)0;
 #pragma unroll
-        for (int m = 0; m < P1; m++) acc += l1[m] * sU[slot][k][base + m * stride];
-        uo[k] = acc;
+          for (int m = 0; m < P1; m++) acc += l1[m] * sU[slot][k][base + m * stride];
+          uo[k] = acc;
+        }
       }
 #pragma unroll
       for (int c = 0; c < D; c++) n[c] = a.fgeo[ga.fg(e, c, q)];
@@ -502,7 +740,24 @@ __global__ __launch_bounds__((ElemGeo<D, P>::NTH)) void k_grad(GradArgs<T> a) {
     }
   }
   elementSync<TPE>();
-  if (hasS) {
+  T g[D][NF];
+  if (MG && act) {
+    if constexpr (MG) {
+      T own[NF];
+#pragma unroll
+      for (int k = 0; k < NF; k++) own[k] = sU[slot][k][t];
+      mfmaGradient<NF, NS, T, false>(s1, &sU[slot][0][0], own, &sC[slot][0][0], t, pJ, pdet, g);
+#pragma unroll
+      for (int d = 0; d < D; d++)
+#pragma unroll
+        for (int k = 0; k < NF; k++) sG[slot][d][k][t] = g[d][k];
+      if (a.dUs)
+#pragma unroll
+        for (int d = 0; d < D; d++)
+#pragma unroll
+          for (int k = 0; k < NF; k++) a.dUs[(((size_t)e * D + d) * NF + k) * NS + t] = g[d][k];
+    }
+  } else if (hasS) {
     int si, sj, sk;
     if (D == 3) {
       sk = t / (P1 * P1);
@@ -513,7 +768,6 @@ __global__ __launch_bounds__((ElemGeo<D, P>::NTH)) void k_grad(GradArgs<T> a) {
       sj = t / P1;
       sk = 0;
     }
-    T g[D][NF];
     correctedGradient<D, P, NF, T>(s1, &sU[slot][0][0], &sC[slot][0][0], t, si, sj, sk, pJ, pdet, g);
 #pragma unroll
     for (int d = 0; d < D; d++)
@@ -526,7 +780,23 @@ __global__ __launch_bounds__((ElemGeo<D, P>::NTH)) void k_grad(GradArgs<T> a) {
         for (int k = 0; k < NF; k++) a.dUs[(((size_t)e * D + d) * NF + k) * NS + t] = g[d][k];
   }
   elementSync<TPE>();
-  if (act) {
+  if (MG && act) {
+    if constexpr (MG) {
+      T *Ge = a.dUf + (size_t)e * D * NF * NFP;
+#pragma unroll
+      for (int r = 0; r < NQ; r++) {
+        const int q = mfmaFpt(t, r);
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+          T v[NF];
+          mfmaFaceRound<NF, NS, T, false>(s1, &sG[slot][d][0][0], t, g[d], r, v);
+          if (q < NFP)
+#pragma unroll
+            for (int k = 0; k < NF; k++) Ge[(d * NF + k) * NFP + q] = v[k];
+        }
+      }
+    }
+  } else if (act) {
     T *Ge = a.dUf + (size_t)e * D * NF * NFP;
 #pragma unroll 1
     for (int q = t; q < NFP; q += TPE) {
@@ -555,179 +825,6 @@ struct StageSmem {
   T sJ[G::EPB][NF][G::NFP];   // normal-flux jump Fn*dA - disFn at the flux points
   T s1[OneD<P>::SIZE];        // 1D factor table
 };
-
-// Hex P3 line operations on the matrix cores.  One element is one wavefront,
-// lane t = solution point i + 4j + 16k.  Along each reference axis the
-// element has 16 lines of 4 points, and every line operation of the stage is
-// a product A B of a small operator A (rows: the derivative matrix Dm, the
-// extrapolations l_L and l_R to the line's two faces) and the 4 x 16 matrix
-// B of the lines' values:
-//  - FP64: v_mfma_f64_4x4x4_4b, four independent 4x4x4 products; block b
-//    holds lines 4b .. 4b+3, with A_b[m][k] at lane m + 4b + 16k,
-//    B_b[k][n] at lane n + 4b + 16k and C_b[m][n] at lane n + 4b + 16m.
-//    The derivative (A = Dm) and the extrapolations (A rows l_L, l_R) are
-//    separate products.
-//  - FP32: v_mfma_f32_16x16x4f32, A[m][k] at lane m + 16k, B[k][n] at lane
-//    n + 16k, C[m][n] at lane n + 16 (m / 4), register m % 4.  Rows 4g of A
-//    hold Dm row g (derivative at point g in register 0) and rows 1 + 4s of
-//    A hold the extrapolations (register 1), so one product gives both.
-// In both, line n = n0 + 4 n1 of the axis takes its point k from lane
-// n + 16k and returns its derivative at point g in lane n + 16g: for the z
-// axis (n = i + 4j) that is the lane's own point, so B is the lanes' own
-// values and the derivative lands in place; for x (n = j + 4k, point i) and
-// y (n = i + 4k, point j) the operand comes from the permuted lane (a lane
-// permute, or LDS for state already there) and the derivative goes back
-// through the inverse permutation.  The extrapolations land in the lanes of
-// flux points: z lines put their left face value in lane n and right face
-// value in lane 16 + n, x lines in lanes 32 + n and 48 + n (rows 2, 3 of A
-// in FP64, rows 9, 13 in FP32) and y lines in lanes n and 16 + n, so that
-// one accumulator holds the z and x face values (the 64 flux points of faces
-// 0-3) and another the y values (faces 4, 5) -- lane t handles flux point
-// mfmaFpt(t, r) in round r of the face phases.  The operator entries are
-// those of the VALU line sums (calcDivF_spts, extrapolateNormalFlux,
-// extrapolateU); the matrix cores sum the four products in their own order.
-template <typename T>
-struct MfmaVec;
-template <>
-struct MfmaVec<double> {
-  static __device__ __forceinline__ double mma(double a, double b, double c) {
-    return __builtin_amdgcn_mfma_f64_4x4x4f64(a, b, c, 0, 0, 0);
-  }
-  // derivative (aD) and face values added to acc (aE) of one axis
-  static __device__ __forceinline__ void axis(double aD, double aE, double b, double &acc, double &der) {
-    der = mma(aD, b, 0.0);
-    acc = mma(aE, b, acc);
-  }
-  static __device__ __forceinline__ double ext(double aE, double b, double acc) { return mma(aE, b, acc); }
-};
-template <>
-struct MfmaVec<float> {
-  typedef float type __attribute__((ext_vector_type(4)));
-  static __device__ __forceinline__ type mma(float a, float b, float c1) {
-    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, type{0.f, c1, 0.f, 0.f}, 0, 0, 0);
-  }
-  static __device__ __forceinline__ void axis(float aD, float aE, float b, float &acc, float &der) {
-    (void)aD;
-    const type c = mma(aE, b, acc);
-    der = c[0];
-    acc = c[1];
-  }
-  static __device__ __forceinline__ float ext(float aE, float b, float acc) { return mma(aE, b, acc)[1]; }
-};
-
-// Flux point of lane t in round r (0, 1) of the face phases of a hex P3
-// element: round 0 holds faces 0-3 (lanes 16f ..), round 1 faces 4 and 5;
-// within a face, the lane of line n has the point fptLine numbers n or, on
-// faces 1, 3 and 4 (whose point order runs against the line order), n ^ 3.
-__device__ __forceinline__ int mfmaFpt(int t, int r) {
-  const int flip = r == 0 ? (t >> 4) & 1 : (t >> 4) == 0;
-  return 64 * r + (t ^ (3 * flip));
-}
-
-// The lane's part of the products of a hex P3 element (lane t): its A
-// entries (the derivative aD; the face values of the z and y axes aEzy and
-// of the x axis aEx), the solution point it stands for in the x and y
-// operands as an LDS index (px, py) and a lane byte address (bx, by), and
-// the lane holding the x / y derivative of its own point (ix, iy).
-template <typename T>
-struct MfmaLane {
-  T aD, aEzy, aEx;
-  int px, py, bx, by, ix, iy;
-  __device__ __forceinline__ MfmaLane(const T *s1, int t) {
-    using O = OneD<3>;
-    const int k = t >> 4;
-    const T lL = s1[O::LL + k], lR = s1[O::LR + k];
-    if constexpr (sizeof(T) == 8) {
-      const int m = t & 3;
-      aD = s1[O::DM + m * 4 + k];
-      aEzy = m == 0 ? lL : m == 1 ? lR : (T)0;
-      aEx = m == 2 ? lL : m == 3 ? lR : (T)0;
-    } else {
-      const int m = t & 15;
-      const T d = (m & 3) == 0 ? s1[O::DM + (m >> 2) * 4 + k] : (T)0;
-      aD = d;
-      aEzy = m == 1 ? lL : m == 5 ? lR : d;
-      aEx = m == 9 ? lL : m == 13 ? lR : d;
-    }
-    const int ln = t & 15, lp = t >> 4;
-    const int sx = lp + 4 * (ln & 3) + 16 * (ln >> 2), sy = (ln & 3) + 4 * lp + 16 * (ln >> 2);
-    px = padIdx<3>(sx);
-    py = padIdx<3>(sy);
-    bx = 4 * sx;
-    by = 4 * sy;
-    // point t = i + 4j + 16k is point i of x line j + 4k (lane j + 4k + 16i)
-    // and point j of y line i + 4k (lane i + 4k + 16j)
-    ix = 4 * ((t >> 2) + 16 * (t & 3));
-    iy = 4 * ((t & 3) + 4 * (t >> 4) + 16 * ((t >> 2) & 3));
-  }
-};
-
-// Value of v in lane addr / 4 of the wavefront (ds_bpermute: the LDS unit's
-// lane crossbar, no LDS memory).  Every lane of the wave is active here.
-__device__ __forceinline__ float lanePull(int addr, float v) {
-  return __int_as_float(__builtin_amdgcn_ds_bpermute(addr, __float_as_int(v)));
-}
-__device__ __forceinline__ double lanePull(int addr, double v) {
-  const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_ds_bpermute(addr, (int)(b & 0xffffffffLL));
-  const int hi = __builtin_amdgcn_ds_bpermute(addr, (int)(b >> 32));
-  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
-
-// Phase (2) on the matrix cores: per reference axis d the contravariant
-// flux F~_d = sum_d2 JGinv(d,d2) F_d2, its derivative along d added to div,
-// and its values at the two faces normal to d, stored negated as the start
-// of the jump (the left face's outward flux is -F~_d): sJ at the round-0
-// and round-1 flux points of the lane.
-template <int NF, typename T, typename SM>
-__device__ __forceinline__ void mfmaAxes(SM &sm, const int slot, const int t, const T *pJ, const T (&F)[3][NF],
-                                         T (&div)[NF], T *dbgF) {
-  using V = MfmaVec<T>;
-  auto &sJ = sm.sJ;
-  const MfmaLane<T> L(sm.s1, t);
-  T e0[NF], e1[NF];
-#pragma unroll
-  for (int k = 0; k < NF; k++) e0[k] = e1[k] = (T)0;
-#pragma unroll
-  for (int d = 0; d < 3; d++) {
-#pragma unroll
-    for (int k = 0; k < NF; k++) {
-      T ft = (T)0;
-#pragma unroll
-      for (int d2 = 0; d2 < 3; d2++) ft += pJ[d * 3 + d2] * F[d2][k];
-      if (dbgF) dbgF[(d * NF + k) * 64 + t] = ft;
-      const T b = d < 2 ? lanePull(d == 0 ? L.bx : L.by, ft) : ft;
-      T der;
-      V::axis(L.aD, d == 0 ? L.aEx : L.aEzy, b, d == 1 ? e1[k] : e0[k], der);
-      const T dv = d < 2 ? lanePull(d == 0 ? L.ix : L.iy, der) : der;
-      div[k] = d == 0 ? dv : dv + div[k];
-    }
-  }
-  const T sg = ((t >> 4) & 1) ? (T)-1 : (T)1;
-  const int q0 = mfmaFpt(t, 0), q1 = mfmaFpt(t, 1);
-#pragma unroll
-  for (int k = 0; k < NF; k++) {
-    sJ[slot][k][q0] = sg * e0[k];
-    if (t < 32) sJ[slot][k][q1] = sg * e1[k];
-  }
-  elementSync<64>();
-}
-
-// Face values of a state S held in LDS (the padded layout of one element,
-// fields NSP apart) at the lane's flux point of round r (mfmaFpt): round 0
-// (faces 0-3) from the z lines -- the lane's own values own[] -- and the x
-// lines, round 1 (faces 4, 5) from the y lines.  All lanes take part.
-template <int NF, typename T>
-__device__ __forceinline__ void mfmaFaceRound(const T *s1, const T *S, const int t, const T (&own)[NF], const int r,
-                                              T (&v)[NF]) {
-  using V = MfmaVec<T>;
-  const MfmaLane<T> L(s1, t);
-  constexpr int NSP = ElemGeo<3, 3>::NSP;
-#pragma unroll
-  for (int k = 0; k < NF; k++)
-    v[k] = r == 0 ? V::ext(L.aEx, S[k * NSP + L.px], V::ext(L.aEzy, own[k], (T)0))
-                  : V::ext(L.aEzy, S[k * NSP + L.py], (T)0);
-}
 
 // BND = false is the variant for meshes without boundary faces (no ghost
 // states, which keeps its register allocation small).  VIS adds the viscous
@@ -990,7 +1087,10 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
     inviscidFlux<D, NF, T>(a.ph, u, F);
     if constexpr (VIS) {
       T g[D][NF], Fv[D][NF];
-      correctedGradient<D, P, NF, T, true>(s1, &sU[slot][0][0], &sJ[slot][0][0], t, si, sj, sk, pJ, (T)pdet, g);
+      if constexpr (MFF)
+        mfmaGradient<NF, G::NSP, T, true>(s1, &sU[slot][0][0], u, &sJ[slot][0][0], t, pJ, (T)pdet, g);
+      else
+        correctedGradient<D, P, NF, T, true>(s1, &sU[slot][0][0], &sJ[slot][0][0], t, si, sj, sk, pJ, (T)pdet, g);
       viscousFlux<D, NF, T>(a.ph, u, g, Fv);
 #pragma unroll
       for (int d = 0; d < D; d++)
@@ -1070,7 +1170,7 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
         T own[NF];
 #pragma unroll
         for (int k = 0; k < NF; k++) own[k] = r == 0 ? sU[slot][k][tP] : (T)0;
-        mfmaFaceRound<NF, T>(s1, &sU[slot][0][0], t, own, r, uo);
+        mfmaFaceRound<NF, G::NSP, T>(s1, &sU[slot][0][0], t, own, r, uo);
       }
     }
     if (act && q < NFP) {
@@ -1237,7 +1337,7 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
     for (int r = 0; r < NQ; r++) {
       const int q = fq(r);
       T tv[NF];
-      if constexpr (MFF) mfmaFaceRound<NF, T>(s1, &sFd[slot][0][0], t, utv, r, tv);
+      if constexpr (MFF) mfmaFaceRound<NF, G::NSP, T>(s1, &sFd[slot][0][0], t, utv, r, tv);
       if (q >= NFP) continue;
       if (!VIS && (pnb[r] < 0 || inBlock(pnb[r]))) continue;
       if constexpr (MFF) {
@@ -1264,9 +1364,11 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
 // VGPRs): it holds the most prefetched data, and at 2 waves per SIMD the
 // final stage ran 12 % longer.  So do the hex P3 variants with the MFMA
 // contractions, whose accumulators would otherwise take them to 2 waves.
+// The viscous variant runs one wave per SIMD, except the FP64 hex P3 one,
+// whose matrix-core gradient and line operators fit two without spilling.
 template <int D, int P, int NF, typename T, bool BND, bool VIS, bool FIN, int GM = 0>
 __global__ __launch_bounds__((ElemGeo<D, P>::NTH),
-                             (VIS ? 1
+                             (VIS ? ((FR_MFMA && D == 3 && P == 3 && sizeof(T) == 8) ? 2 : 1)
                               : (FR_MFMA && D == 3 && P == 3) ? (FIN ? FR_MF_FIN_WAVES : FR_MF_WAVES)
                               : FIN                           ? 3
                                                               : FR_STAGE_MINW)) void k_stage(StageArgs<T> a) {
