@@ -485,6 +485,19 @@ __device__ __forceinline__ void mfmaFaceRound(const T *s1, const T *S, const int
     v[k] = r == 0 ? V::ext(L.aEx, S[k * RS + ix], V::ext(L.aEzy, own[k], (T)0)) : V::ext(L.aEzy, S[k * RS + iy], (T)0);
 }
 
+// mfmaFaceRound of values held only in registers (own[], one per lane): the
+// x and y operands come from the permuted lanes.
+template <int NF, typename T>
+__device__ __forceinline__ void mfmaFaceRoundReg(const T *s1, const int t, const T (&own)[NF], const int r,
+                                                 T (&v)[NF]) {
+  using V = MfmaVec<T>;
+  const MfmaLane<T> L(s1, t);
+#pragma unroll
+  for (int k = 0; k < NF; k++)
+    v[k] = r == 0 ? V::ext(L.aEx, lanePull(L.bx, own[k]), V::ext(L.aEzy, own[k], (T)0))
+                  : V::ext(L.aEzy, lanePull(L.by, own[k]), (T)0);
+}
+
 // correctedGradient (below) for hex P3 with the reference-space derivative on
 // the matrix cores: the state S in LDS (fields RS apart, padded rows when
 // PADV) and the lane's own values own[], the jumps sCe [NF][NFP] at flux
@@ -671,7 +684,10 @@ __global__ __launch_bounds__((ElemGeo<D, P>::NTH)) void k_grad(GradArgs<T> a) {
   constexpr int NFACE = G::NFACE, NQ = (NFP + TPE - 1) / TPE;
   __shared__ T sU[EPB][NF][NS];
   __shared__ T sC[EPB][NF][NFP];
-  __shared__ T sG[EPB][D][NF][NS];
+  // the gradient at the solution points for its traces (the matrix-core
+  // variant takes them from the lanes' registers instead)
+  constexpr bool MG0 = FR_MFMA && D == 3 && P == 3 && sizeof(T) == 8;
+  __shared__ T sG[MG0 ? 1 : EPB][D][NF][MG0 ? 1 : NS];
   __shared__ T s1[O::SIZE];
   const int b = (blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);
   const int slot = threadIdx.x / TPE, t = threadIdx.x - slot * TPE;
@@ -747,10 +763,6 @@ __global__ __launch_bounds__((ElemGeo<D, P>::NTH)) void k_grad(GradArgs<T> a) {
 #pragma unroll
       for (int k = 0; k < NF; k++) own[k] = sU[slot][k][t];
       mfmaGradient<NF, NS, T, false>(s1, &sU[slot][0][0], own, &sC[slot][0][0], t, pJ, pdet, g);
-#pragma unroll
-      for (int d = 0; d < D; d++)
-#pragma unroll
-        for (int k = 0; k < NF; k++) sG[slot][d][k][t] = g[d][k];
       if (a.dUs)
 #pragma unroll
         for (int d = 0; d < D; d++)
@@ -789,7 +801,7 @@ __global__ __launch_bounds__((ElemGeo<D, P>::NTH)) void k_grad(GradArgs<T> a) {
 #pragma unroll
         for (int d = 0; d < D; d++) {
           T v[NF];
-          mfmaFaceRound<NF, NS, T, false>(s1, &sG[slot][d][0][0], t, g[d], r, v);
+          mfmaFaceRoundReg<NF, T>(s1, t, g[d], r, v);
           if (q < NFP)
 #pragma unroll
             for (int k = 0; k < NF; k++) Ge[(d * NF + k) * NFP + q] = v[k];
