@@ -552,14 +552,11 @@ __global__ __launch_bounds__((ElemGeo<D, P>::TPE)) void k_vjp_state_vis(
     // state directions: dual passes
 #pragma unroll 1
     for (int sj = 0; sj < NF; sj++) {
-      DM u[NF], qd[D][NF], Fi[D][NF], Fv[D][NF];
+      DM u[NF], Fi[D][NF], Fv[D][NF];
 #pragma unroll
       for (int k = 0; k < NF; k++) u[k] = (k == sj) ? DM::var(sU[k][t], 0) : DM(sU[k][t]);
-#pragma unroll
-      for (int d = 0; d < D; d++)
-#pragma unroll
-        for (int k = 0; k < NF; k++) qd[d][k] = DM(Q[d][k]);
-      viscousFlux<D, NF, DM>(ph, u, qd, Fv);
+      // the gradient held fixed: plain doubles
+      viscousFlux<D, NF, DM, double>(ph, u, Q, Fv);
       inviscidFlux<D, NF, DM>(ph, u, Fi);
       double acc = 0;
 #pragma unroll
@@ -574,7 +571,8 @@ __global__ __launch_bounds__((ElemGeo<D, P>::TPE)) void k_vjp_state_vis(
       double u[NF];
 #pragma unroll
       for (int k = 0; k < NF; k++) u[k] = sU[k][t];
-#pragma unroll 1
+      // (unrolled: each unit gradient's flux folds to a few terms)
+#pragma unroll
       for (int dk = 0; dk < D * NF; dk++) {
         const int d = dk / NF, sj = dk - d * NF;
         double qd[D][NF], Fv[D][NF];
