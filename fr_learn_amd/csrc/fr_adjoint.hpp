@@ -518,7 +518,17 @@ __global__ __launch_bounds__((ElemGeo<D, P>::TPE)) void k_vjp_state_vis(
     for (int c = 0; c < D * D; c++) pJ[c] = JG[((size_t)e * D * D + c) * NS + t];
     const double pdet = detJ[(size_t)e * NS + t];
     double Q[D][NF];
-    correctedGradient<D, P, NF, double>(s1, &sU[0][0], &sC[0][0], t, si, sj, sk, pJ, pdet, Q);
+    // hex P3: the derivatives on the matrix cores (the gradient's, and the
+    // transposed one of the flux cotangent below)
+    constexpr bool MV = FR_MFMA && D == 3 && P == 3;
+    if constexpr (MV) {
+      double own[NF];
+#pragma unroll
+      for (int k = 0; k < NF; k++) own[k] = sU[k][t];
+      mfmaGradient<NF, NS, double, false>(s1, &sU[0][0], own, &sC[0][0], t, pJ, pdet, Q);
+    } else {
+      correctedGradient<D, P, NF, double>(s1, &sU[0][0], &sC[0][0], t, si, sj, sk, pJ, pdet, Q);
+    }
     // cotangent of the total point flux
     double lFt[D][NF], lF[D][NF];
 #pragma unroll
@@ -527,10 +537,17 @@ __global__ __launch_bounds__((ElemGeo<D, P>::TPE)) void k_vjp_state_vis(
 #pragma unroll
       for (int k = 0; k < NF; k++) {
         double acc = 0;
+        if constexpr (MV) {
+          const MfmaLane<double> L(s1, t);
+          const double b = d == 2 ? sL[k][t] : sL[k][d == 0 ? L.sx : L.sy];
+          const double der = MfmaVec<double>::mma(L.aDT, b, 0.0);
+          acc = d < 2 ? lanePull(d == 0 ? L.ix : L.iy, der) : der;
+        } else {
 #pragma unroll
-        for (int m = 0; m < P1; m++) {
-          const int sp = (d == 0) ? m + P1 * (sj + P1 * sk) : (d == 1) ? si + P1 * (m + P1 * sk) : si + P1 * (sj + P1 * m);
-          acc += s1[O::DM + m * P1 + id] * sL[k][sp];
+          for (int m = 0; m < P1; m++) {
+            const int sp = (d == 0) ? m + P1 * (sj + P1 * sk) : (d == 1) ? si + P1 * (m + P1 * sk) : si + P1 * (sj + P1 * m);
+            acc += s1[O::DM + m * P1 + id] * sL[k][sp];
+          }
         }
 #pragma unroll
         for (int f = 0; f < NFACE; f++)

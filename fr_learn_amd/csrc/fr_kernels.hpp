@@ -387,7 +387,7 @@ __device__ __forceinline__ int mfmaFpt(int t, int r) {
 // the lane holding the x / y derivative of its own point (ix, iy).
 template <typename T>
 struct MfmaLane {
-  T aD, aEzy, aEx;
+  T aD, aDT, aEzy, aEx;  // aDT: the transposed derivative matrix (FP64 layout)
   int sx, sy, px, py, bx, by, ix, iy;
   __device__ __forceinline__ MfmaLane(const T *s1, int t) {
     using O = OneD<3>;
@@ -396,12 +396,14 @@ struct MfmaLane {
     if constexpr (sizeof(T) == 8) {
       const int m = t & 3;
       aD = s1[O::DM + m * 4 + k];
+      aDT = s1[O::DM + k * 4 + m];
       aEzy = m == 0 ? lL : m == 1 ? lR : (T)0;
       aEx = m == 2 ? lL : m == 3 ? lR : (T)0;
     } else {
       const int m = t & 15;
       const T d = (m & 3) == 0 ? s1[O::DM + (m >> 2) * 4 + k] : (T)0;
       aD = d;
+      aDT = (T)0;
       aEzy = m == 1 ? lL : m == 5 ? lR : d;
       aEx = m == 9 ? lL : m == 13 ? lR : d;
     }
