@@ -84,6 +84,8 @@ SIGNATURES = {
     "fr_batch_create": (ctypes.c_int, [ctypes.POINTER(_P), ctypes.c_int, ctypes.POINTER(_P)]),
     "fr_batch_update": (ctypes.c_int, [_P, ctypes.c_int]),
     "fr_batch_tuning": (ctypes.c_int, [_P, _IP, _IP, _DP]),
+    "fr_batch_group_tuning": (ctypes.c_int, [_P, ctypes.c_int, _IP, _IP, _IP, _IP, _DP]),
+    "fr_batch_force_tile": (ctypes.c_int, [ctypes.c_int]),
     "fr_batch_destroy": (ctypes.c_int, [_P]),
     "fr_set_state": (ctypes.c_int, [_P, _DP, ctypes.c_int]),
     "fr_get_state": (ctypes.c_int, [_P, _DP, ctypes.c_int]),

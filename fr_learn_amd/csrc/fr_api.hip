@@ -2,6 +2,8 @@
 // device buffers, kernel dispatch and the solver verbs.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -46,13 +48,16 @@ const T *stateIn(fr_ctx *c, int which) {
 // Make Uf[cur] hold the traces of state `which` (0 = U_spts, 1 = Us).
 template <typename T>
 int ensureTraces(fr_ctx *c, int which) {
-  if (c->traceOf == which) return 0;
+  // traces written by launches of another element blocking lack that
+  // blocking's in-block faces
+  if (c->traceOf == which && (c->traceEpb == 0 || c->traceEpb == c->epb)) return 0;
   const T *U = stateIn<T>(c, which);
   const T *o1 = std::is_same<T, float>::value ? (const T *)c->oned32 : (const T *)c->oned64;
   if (c->tri) {
     int rc = launchTriTrace<T>(c, U, (T *)c->Uf[c->cur], o1);
     if (rc) return rc;
     c->traceOf = which;
+    c->traceEpb = 0;
     return 0;
   }
   hipError_t err = hipSuccess;
@@ -67,6 +72,7 @@ int ensureTraces(fr_ctx *c, int which) {
   if (!ok) return fail(FR_EINVAL, "no kernel instance for this (nDims, order, nFields)");
   if (err != hipSuccess) return fail(FR_EHIP, std::string("trace launch: ") + hipGetErrorString(err));
   c->traceOf = which;
+  c->traceEpb = 0;
   return 0;
 }
 
@@ -148,6 +154,7 @@ int launchStage(fr_ctx *c, int in, int mode, int step, double rk) {
   if (mode != 0) {
     c->cur ^= 1;
     c->traceOf = (mode == 1) ? 1 : 0;
+    c->traceEpb = c->tri ? 0 : c->epb;  // the triangle kernels write every face
   }
   if (step == c->nRK - 1) c->finalDropped = a.divF == nullptr;
   return 0;
@@ -339,11 +346,13 @@ int batchMemberCheck(const fr_ctx *c) {
 
 // Before a batched step: every instance holds the traces of U_spts in
 // Uf[parity] (the stage kernels leave only the cross-block traces in HBM,
-// which is what the next step reads).
+// which is what the next step reads: complete, or written by launches of the
+// batch's own element blocking).
 int batchPrepare(fr_batch *b, int parity) {
   for (fr_ctx *c : b->ctx) {
     HIP_TRY(hipStreamSynchronize(c->stream));
-    if (c->traceOf != 0 || c->cur != parity) {
+    const int epb = batchEpb(b, c, b->lay[batchGroup(c)]);
+    if (c->traceOf != 0 || c->cur != parity || (c->traceEpb != 0 && c->traceEpb != epb)) {
       c->cur = parity;
       c->traceOf = -1;
       int rc = (c->prec == FR_PREC_FP32) ? ensureTraces<float>(c, 0) : ensureTraces<double>(c, 0);
@@ -376,7 +385,8 @@ void batchFree(fr_batch *b) {
     if (b->join[grp]) (void)hipEventDestroy(b->join[grp]);
   }
   if (b->fork) (void)hipEventDestroy(b->fork);
-  if (b->items) (void)hipFree(b->items);
+  for (int l = 0; l < 2; l++)
+    if (b->items[l]) (void)hipFree(b->items[l]);
   if (b->stream) (void)hipStreamDestroy(b->stream);
 }
 
@@ -1337,6 +1347,16 @@ int fr_reduce_gradient(fr_ctx *c, int kind, double *out) {
   return 0;
 }
 
+namespace {
+std::atomic<int> gForceTile{-1};  // fr_batch_force_tile
+}
+
+int fr_batch_force_tile(int layout) {
+  if (layout < -1 || layout > 1) return fail(FR_EINVAL, "tile layout is -1 (tuned), 0 or 1");
+  gForceTile.store(layout);
+  return 0;
+}
+
 int fr_batch_create(fr_ctx *const *ctxs, int n, fr_batch **out) {
   if (!ctxs || !out || n < 1) return fail(FR_EINVAL, "bad argument");
   *out = nullptr;
@@ -1352,7 +1372,6 @@ int fr_batch_create(fr_ctx *const *ctxs, int n, fr_batch **out) {
   b->NF = c0->NF;
   b->nRK = c0->nRK;
   b->prec = c0->prec;
-  std::vector<frdev::BatchItem> items;
   for (int i = 0; i < n; i++) {
     fr_ctx *c = ctxs[i];
     if (!c) return bad(FR_EINVAL, "null context");
@@ -1366,26 +1385,39 @@ int fr_batch_create(fr_ctx *const *ctxs, int n, fr_batch **out) {
     b->ctx.push_back(c);
   }
   // work items grouped by kernel (one launch per element type and order),
-  // each group in mesh order
-  for (int grp = 1; grp <= 10; grp++) {
-    b->itemStart[grp] = (int)items.size();
-    for (int i = 0; i < n; i++)
-      if (b->ctx[i]->P + (b->ctx[i]->tri ? 5 : 0) == grp) {
-        const int nb = (b->ctx[i]->nE + b->ctx[i]->epb - 1) / b->ctx[i]->epb;
-        for (int k = 0; k < nb; k++) items.push_back({i, k});
-      }
-    b->itemCount[grp] = (int)items.size() - b->itemStart[grp];
+  // each group in mesh order, for both tile layouts
+  for (int l = 0; l < 2; l++) {
+    std::vector<frdev::BatchItem> items;
+    for (int grp = 1; grp <= 10; grp++) {
+      b->itemStart[l][grp] = (int)items.size();
+      for (int i = 0; i < n; i++)
+        if (batchGroup(b->ctx[i]) == grp) {
+          const int epb = batchEpb(b, b->ctx[i], l);
+          const int nb = (b->ctx[i]->nE + epb - 1) / epb;
+          for (int k = 0; k < nb; k++) items.push_back({i, k});
+        }
+      b->itemCount[l][grp] = (int)items.size() - b->itemStart[l][grp];
+    }
+    int rc;
+    if ((rc = dmalloc(&b->items[l], items.size()))) return bad(rc, g_err);
+    HIP_TRY(hipMemcpy(b->items[l], items.data(), items.size() * sizeof(frdev::BatchItem), hipMemcpyHostToDevice));
   }
-  b->nItems = (int)items.size();
-  int rc;
-  if ((rc = dmalloc(&b->items, items.size()))) return bad(rc, g_err);
-  HIP_TRY(hipMemcpy(b->items, items.data(), items.size() * sizeof(frdev::BatchItem), hipMemcpyHostToDevice));
-  rc = (b->prec == FR_PREC_FP32) ? batchArgs<float>(b) : batchArgs<double>(b);
+  int rc = (b->prec == FR_PREC_FP32) ? batchArgs<float>(b) : batchArgs<double>(b);
   if (rc) return bad(rc, g_err);
   if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) return bad(FR_EHIP, "stream");
-  // autotune the persistent grid on residual-only launches (they write only
-  // divF[0]), with the stage computation of the update
-  if ((rc = batchPrepare(b, 0))) return bad(rc, g_err);
+  // The tuner: per group, the tile layout (threads per element, elements per
+  // block) and the persistent grid (1..32 workgroups per CU) with the
+  // shortest residual-only launch (it writes only divF[0]) of the stage
+  // computation of the update, timed on the group alone.  The members'
+  // traces are made complete first, so that every layout reads valid ones.
+  for (fr_ctx *c : b->ctx) {
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->cur = 0;
+    c->traceOf = -1;
+    if ((rc = (c->prec == FR_PREC_FP32) ? ensureTraces<float>(c, 0) : ensureTraces<double>(c, 0)))
+      return bad(rc, g_err);
+    HIP_TRY(hipStreamSynchronize(c->stream));
+  }
   int cu = 256;
   {
     int dev = 0;
@@ -1395,42 +1427,66 @@ int fr_batch_create(fr_ctx *const *ctxs, int n, fr_batch **out) {
   }
   hipEvent_t e0, e1;
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return bad(FR_EHIP, "event");
-  double best = 1e300;
-  for (int mult : {1, 2, 4, 8, 16, 32}) {
-    const int grid = std::max(1, std::min(cu * mult, b->nItems));
-    if ((rc = batchLaunchStage(b, -1, 0, grid, b->stream))) return bad(rc, g_err);  // warm-up
+  auto timeLaunch = [&](int grp, int lay, int grid, double &ms) -> int {
+    int r;
+    if ((r = batchLaunchGroup(b, -1, 0, b->stream, grp, lay, grid))) return r;  // warm-up
     (void)hipEventRecord(e0, b->stream);
-    for (int r = 0; r < 3; r++)
-      if ((rc = batchLaunchStage(b, -1, 0, grid, b->stream))) return bad(rc, g_err);
+    for (int k = 0; k < 3; k++)
+      if ((r = batchLaunchGroup(b, -1, 0, b->stream, grp, lay, grid))) return r;
     (void)hipEventRecord(e1, b->stream);
-    if (hipEventSynchronize(e1) != hipSuccess) return bad(FR_EHIP, "tuning run");
-    float ms = 0;
-    (void)hipEventElapsedTime(&ms, e0, e1);
-    if (ms / 3 < best) {
-      best = ms / 3;
-      b->grid = grid;
-    }
-    if (grid == b->nItems) break;
+    if (hipEventSynchronize(e1) != hipSuccess) return fail(FR_EHIP, "tuning run");
+    float t = 0;
+    (void)hipEventElapsedTime(&t, e0, e1);
+    ms = t / 3.0;
+    return 0;
+  };
+  b->nItems = 0;
+  b->tunedMs = 0;
+  for (int grp = 1; grp <= 10 && !rc; grp++) {
+    if (!b->itemCount[0][grp]) continue;
+    double best = 1e300;
+    const int P = grp > 5 ? grp - 5 : grp;
+    // layout 1 is a different tile only where it changes threads per element
+    const bool l1 = grp > 5 ? frdev::triTPEL(P, 1) != frdev::triTPEL(P, 0)
+                            : frdev::tensorTPE(b->D, P, 1) != frdev::tensorTPE(b->D, P, 0);
+    const int force = gForceTile.load();
+    const int lay0 = force == 1 && l1 ? 1 : 0, lay1 = force == 0 || !l1 ? 0 : 1;
+    for (int lay = lay0; lay <= lay1 && !rc; lay++)
+      for (int mult : {1, 2, 4, 8, 16, 32}) {
+        const int cnt = b->itemCount[lay][grp], grid = std::max(1, std::min(cu * mult, cnt));
+        double ms = 0;
+        if ((rc = timeLaunch(grp, lay, grid, ms))) break;
+        if (ms < best) {
+          best = ms;
+          b->lay[grp] = lay;
+          b->grid[grp] = grid;
+        }
+        if (grid == cnt) break;
+      }
+    b->groupMs[grp] = best;
+    b->tunedMs += best;
+    b->nItems += b->itemCount[b->lay[grp]][grp];
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  b->tunedMs = best;
+  if (rc) return bad(rc, g_err);
   // One step's launches, captured per trace parity.  Meshes of different
   // element types and orders are independent, so each group's stages form
   // their own branch of the graph (forked from and joined to the batch
   // stream) and run concurrently with the other groups'.
   if (hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess) return bad(FR_EHIP, "event");
   for (int grp = 1; grp <= 10; grp++)
-    if (b->itemCount[grp] && (hipStreamCreateWithFlags(&b->ps[grp], hipStreamNonBlocking) != hipSuccess ||
-                              hipEventCreateWithFlags(&b->join[grp], hipEventDisableTiming) != hipSuccess))
+    if (b->itemCount[0][grp] && (hipStreamCreateWithFlags(&b->ps[grp], hipStreamNonBlocking) != hipSuccess ||
+                                 hipEventCreateWithFlags(&b->join[grp], hipEventDisableTiming) != hipSuccess))
       return bad(FR_EHIP, "stream");
   for (int p = 0; p < ((b->nRK & 1) ? 2 : 1); p++) {
     if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeGlobal) != hipSuccess) return bad(FR_EHIP, "capture");
     bool ok = hipEventRecord(b->fork, b->stream) == hipSuccess;
     for (int grp = 1; grp <= 10 && ok && !rc; grp++) {
-      if (!b->itemCount[grp]) continue;
+      if (!b->itemCount[0][grp]) continue;
       ok = hipStreamWaitEvent(b->ps[grp], b->fork, 0) == hipSuccess;
-      for (int s = 0; s < b->nRK && ok && !rc; s++) rc = batchLaunchStage(b, p, s, b->grid, b->ps[grp], grp);
+      for (int s = 0; s < b->nRK && ok && !rc; s++)
+        rc = batchLaunchGroup(b, p, s, b->ps[grp], grp, b->lay[grp], b->grid[grp]);
       ok = ok && hipEventRecord(b->join[grp], b->ps[grp]) == hipSuccess &&
            hipStreamWaitEvent(b->stream, b->join[grp], 0) == hipSuccess;
     }
@@ -1485,6 +1541,7 @@ int fr_batch_update(fr_batch *b, int nSteps) {
   for (fr_ctx *c : b->ctx) {
     c->cur = parity;
     c->traceOf = 0;
+    c->traceEpb = c->tri ? 0 : batchEpb(b, c, b->lay[batchGroup(c)]);
     c->finalDropped = !c->keepFinal && c->nRK > 1;
   }
   return 0;
@@ -1492,9 +1549,24 @@ int fr_batch_update(fr_batch *b, int nSteps) {
 
 int fr_batch_tuning(fr_batch *b, int *grid, int *nItems, double *msStage0) {
   if (!b) return fail(FR_EINVAL, "null argument");
-  if (grid) *grid = b->grid;
+  if (grid) *grid = *std::max_element(b->grid, b->grid + 11);
   if (nItems) *nItems = b->nItems;
   if (msStage0) *msStage0 = b->tunedMs;
+  return 0;
+}
+
+int fr_batch_group_tuning(fr_batch *b, int group, int *threadsPerElement, int *elementsPerBlock, int *grid,
+                          int *nItems, double *ms) {
+  if (!b) return fail(FR_EINVAL, "null argument");
+  if (group < 1 || group > 10 || (b->D == 3 && group > 5))
+    return fail(FR_EINVAL, "group is an order 1..5, plus 5 for triangles");
+  const int lay = b->lay[group], P = group > 5 ? group - 5 : group;
+  const int tpe = group > 5 ? frdev::triTPEL(P, lay) : frdev::tensorTPE(b->D, P, lay);
+  if (threadsPerElement) *threadsPerElement = tpe;
+  if (elementsPerBlock) *elementsPerBlock = tpe >= 256 ? 1 : 256 / tpe;
+  if (grid) *grid = b->itemCount[lay][group] ? b->grid[group] : 0;
+  if (nItems) *nItems = b->itemCount[lay][group];
+  if (ms) *ms = b->itemCount[lay][group] ? b->groupMs[group] : 0.0;
   return 0;
 }
 

@@ -4,10 +4,41 @@
 
 namespace frimpl {
 
-// Stage s of the meshes of group gOnly (0: every group, one after the
-// other); parity < 0: the residual-only tuning launch.
+// Kernel group of a member: its order P, plus 5 for triangles.
+int batchGroup(const fr_ctx *c) { return c->P + (c->tri ? 5 : 0); }
+
+// Elements per work item of member c in tile layout lay.
+int batchEpb(const fr_batch *b, const fr_ctx *c, int lay) {
+  const int tpe = c->tri ? frdev::triTPEL(c->P, lay) : frdev::tensorTPE(b->D, c->P, lay);
+  return tpe >= 256 ? 1 : 256 / tpe;
+}
+
+namespace {
+
+template <int P, int NF, typename T, int L>
+void triLaunch(bool fin, unsigned g, hipStream_t st, const frdev::StageArgs<T> *args, const frdev::BatchItem *it,
+               int cnt) {
+  constexpr int NTH = frdev::TriGeo<P, L>::NTH;
+  if (fin)
+    hipLaunchKernelGGL((frdev::k_batch_tri_stage<P, NF, T, true, L>), dim3(g), dim3(NTH), 0, st, args, it, cnt);
+  else
+    hipLaunchKernelGGL((frdev::k_batch_tri_stage<P, NF, T, false, L>), dim3(g), dim3(NTH), 0, st, args, it, cnt);
+}
+
+template <int D, int P, int NF, typename T, int L>
+void tensorLaunch(bool fin, unsigned g, hipStream_t st, const frdev::StageArgs<T> *args,
+                  const frdev::BatchItem *it, int cnt) {
+  constexpr int NTH = frdev::ElemGeo<D, P, L>::NTH;
+  if (fin)
+    hipLaunchKernelGGL((frdev::k_batch_stage<D, P, NF, T, true, L>), dim3(g), dim3(NTH), 0, st, args, it, cnt);
+  else
+    hipLaunchKernelGGL((frdev::k_batch_stage<D, P, NF, T, false, L>), dim3(g), dim3(NTH), 0, st, args, it, cnt);
+}
+
+// Stage s of the meshes of group grp in tile layout lay with a persistent
+// grid of `grid` workgroups; parity < 0: the residual-only tuning launch.
 template <typename T>
-int batchLaunch(fr_batch *b, int parity, int s, int grid, hipStream_t st, int gOnly) {
+int launchGroup(fr_batch *b, int parity, int s, hipStream_t st, int grp, int lay, int grid) {
   const size_t n = b->ctx.size();
   const frdev::StageArgs<T> *args = (parity < 0) ? (const frdev::StageArgs<T> *)b->argsTune
                                                  : (const frdev::StageArgs<T> *)b->args[parity] + (size_t)s * n;
@@ -15,50 +46,48 @@ int batchLaunch(fr_batch *b, int parity, int s, int grid, hipStream_t st, int gO
   double rk;
   if (parity >= 0) stageMode(b->ctx[0], s, mode, rk);
   const bool fin = mode >= 2;
-  for (int grp = 10; grp >= 1; grp--) {
-    const int cnt = b->itemCount[grp];
-    if (!cnt || (gOnly && grp != gOnly)) continue;
-    const frdev::BatchItem *it = b->items + b->itemStart[grp];
-    const unsigned g = (unsigned)std::min(grid, cnt);
-    if (grp > 5) {
-      const int P = grp - 5;
-      auto launch = [&](auto Pc, auto NFc) {
-        constexpr int PP = decltype(Pc)::value, NF = decltype(NFc)::value;
-        if (fin)
-          hipLaunchKernelGGL((frdev::k_batch_tri_stage<PP, NF, T, true>), dim3(g), dim3(256), 0, st, args, it, cnt);
-        else
-          hipLaunchKernelGGL((frdev::k_batch_tri_stage<PP, NF, T, false>), dim3(g), dim3(256), 0, st, args, it, cnt);
-      };
-      auto withNF = [&](auto Pc) {
-        if (b->NF == 1) launch(Pc, std::integral_constant<int, 1>{});
-        else launch(Pc, std::integral_constant<int, 4>{});
-      };
-      switch (P) {
-        case 1: withNF(std::integral_constant<int, 1>{}); break;
-        case 2: withNF(std::integral_constant<int, 2>{}); break;
-        case 3: withNF(std::integral_constant<int, 3>{}); break;
-        case 4: withNF(std::integral_constant<int, 4>{}); break;
-        default: withNF(std::integral_constant<int, 5>{}); break;
-      }
-      HIP_TRY(hipGetLastError());
-      continue;
+  const int cnt = b->itemCount[lay][grp];
+  if (!cnt) return 0;
+  const frdev::BatchItem *it = b->items[lay] + b->itemStart[lay][grp];
+  const unsigned g = (unsigned)std::max(1, std::min(grid, cnt));
+  if (grp > 5) {
+    auto withL = [&](auto Pc, auto NFc) {
+      constexpr int PP = decltype(Pc)::value, NF = decltype(NFc)::value;
+      if (lay == 1) triLaunch<PP, NF, T, 1>(fin, g, st, args, it, cnt);
+      else triLaunch<PP, NF, T, 0>(fin, g, st, args, it, cnt);
+    };
+    auto withNF = [&](auto Pc) {
+      if (b->NF == 1) withL(Pc, std::integral_constant<int, 1>{});
+      else withL(Pc, std::integral_constant<int, 4>{});
+    };
+    switch (grp - 5) {
+      case 1: withNF(std::integral_constant<int, 1>{}); break;
+      case 2: withNF(std::integral_constant<int, 2>{}); break;
+      case 3: withNF(std::integral_constant<int, 3>{}); break;
+      case 4: withNF(std::integral_constant<int, 4>{}); break;
+      default: withNF(std::integral_constant<int, 5>{}); break;
     }
-    const int P = grp;
-    dispatch(b->D, P, b->NF, [&](auto Dc, auto Pc, auto NFc) {
+  } else {
+    dispatch(b->D, grp, b->NF, [&](auto Dc, auto Pc, auto NFc) {
       constexpr int D = decltype(Dc)::value, PP = decltype(Pc)::value, NF = decltype(NFc)::value;
-      if (fin)
-        hipLaunchKernelGGL((frdev::k_batch_stage<D, PP, NF, T, true>), dim3(g), dim3(256), 0, st, args, it, cnt);
-      else
-        hipLaunchKernelGGL((frdev::k_batch_stage<D, PP, NF, T, false>), dim3(g), dim3(256), 0, st, args, it, cnt);
+      if constexpr (D == 2) {
+        if (lay == 1) {
+          tensorLaunch<D, PP, NF, T, 1>(fin, g, st, args, it, cnt);
+          return;
+        }
+      }
+      tensorLaunch<D, PP, NF, T, 0>(fin, g, st, args, it, cnt);
     });
-    HIP_TRY(hipGetLastError());
   }
+  HIP_TRY(hipGetLastError());
   return 0;
 }
 
-int batchLaunchStage(fr_batch *b, int parity, int s, int grid, hipStream_t st, int gOnly) {
-  return b->prec == FR_PREC_FP32 ? batchLaunch<float>(b, parity, s, grid, st, gOnly)
-                                 : batchLaunch<double>(b, parity, s, grid, st, gOnly);
+}  // namespace
+
+int batchLaunchGroup(fr_batch *b, int parity, int s, hipStream_t st, int grp, int lay, int grid) {
+  return b->prec == FR_PREC_FP32 ? launchGroup<float>(b, parity, s, st, grp, lay, grid)
+                                 : launchGroup<double>(b, parity, s, st, grp, lay, grid);
 }
 
 }  // namespace frimpl

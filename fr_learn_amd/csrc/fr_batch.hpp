@@ -29,17 +29,18 @@ __device__ __forceinline__ int itemThreadIdx() {
 }
 
 
-// Every order's workgroup is 256 threads (ElemGeo NTH).  One kernel per order
-// keeps each order's register allocation (a single kernel switching over the
-// orders is allocated for their union and spills).
-template <int D, int P, int NF, typename T, bool FIN>
+// One kernel per order keeps each order's register allocation (a single
+// kernel switching over the orders is allocated for their union and spills).
+// L is the tile layout (ElemGeo); a workgroup is ElemGeo<D, P, L>::NTH
+// threads (at most 256), and an item is a block of EPB elements of that
+// layout.
+template <int D, int P, int NF, typename T, bool FIN, int L>
 __global__ __launch_bounds__(256) void k_batch_stage(const StageArgs<T> *__restrict__ args,
                                                      const BatchItem *__restrict__ items, int nItems) {
-  static_assert(ElemGeo<D, P>::NTH == 256, "256-thread workgroups");
-  __shared__ StageSmem<D, P, NF, T> sm;
+  __shared__ StageSmem<D, P, NF, T, L> sm;
   for (int it = blockIdx.x; it < nItems; it += gridDim.x) {
     const BatchItem w = items[it];
-    stageBody<D, P, NF, T, true, false, FIN>(args[w.inst], w.block, sm, itemThreadIdx());
+    stageBody<D, P, NF, T, true, false, FIN, 0, false, L>(args[w.inst], w.block, sm, itemThreadIdx());
     __syncthreads();  // the next item reuses the LDS
   }
 }

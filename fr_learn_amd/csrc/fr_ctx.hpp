@@ -86,6 +86,9 @@ struct fr_ctx {
   void *Uf[2] = {nullptr, nullptr};   // face traces, double-buffered
   int cur = 0;                        // Uf[cur] holds the traces of `traceOf`
   int traceOf = -1;                   // 0: U_spts, 1: stage state Us, -1: stale
+  // element-block size of the launches that wrote Uf[cur] (their in-block
+  // faces are missing from it), 0: every face written
+  int traceEpb = 0;
   int keepFinal = 1;                  // the final RK stage stores divF_spts[nRK-1] (fr_keep_final_residual)
   bool finalDropped = false;          // divF[nRK-1] is stale: the last update did not store it
   void *divF[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -147,10 +150,16 @@ struct fr_batch {
   void *args[2] = {nullptr, nullptr};  // StageArgs per [trace parity][stage][instance]
   void *argsTune = nullptr;            // residual-only (mode 0) StageArgs per instance, for tuning
   // work items grouped by kernel: group g = P (quads / hexes of order P) or
-  // 5 + P (triangles of order P), g in 1..10
-  frdev::BatchItem *items = nullptr;
-  int itemStart[11] = {0}, itemCount[11] = {0};
-  int nItems = 0, grid = 0;           // persistent workgroups per order's launch
+  // 5 + P (triangles of order P), g in 1..10; one list per tile layout
+  // (frdev::ElemGeo / TriGeo L), since an item is a block of that layout's EPB
+  // elements
+  frdev::BatchItem *items[2] = {nullptr, nullptr};
+  int itemStart[2][11] = {}, itemCount[2][11] = {};
+  // per group, chosen at fr_batch_create by timing: tile layout, persistent
+  // grid and the residual-only launch time
+  int lay[11] = {}, grid[11] = {};
+  double groupMs[11] = {};
+  int nItems = 0;  // items of the chosen layouts
   double tunedMs = 0;
   hipGraph_t graph[2] = {nullptr, nullptr};
   hipGraphExec_t exec[2] = {nullptr, nullptr};
@@ -232,6 +241,8 @@ int needPart(fr_ctx *c, size_t n);
 int dtPrepare(fr_ctx *c, hipStream_t st);
 // Stage s of one batched step (fr_batch.hip); parity < 0 is the tuning launch,
 // Ponly != 0 restricts it to the meshes of that order.
-int batchLaunchStage(fr_batch *b, int parity, int s, int grid, hipStream_t st, int gOnly = 0);
+int batchLaunchGroup(fr_batch *b, int parity, int s, hipStream_t st, int grp, int lay, int grid);
+int batchGroup(const fr_ctx *c);
+int batchEpb(const fr_batch *b, const fr_ctx *c, int lay);
 
 }  // namespace frimpl

@@ -267,11 +267,12 @@ __device__ __forceinline__ int dirFace(int d, int which) {
 
 // Synchronisation of the threads of one element between LDS phases: a
 // wavefront-level barrier with wavefront-scope fences when each element's
-// threads lie inside one wavefront (TPE <= 64), else the workgroup barrier.
-// Only the phases that read another element's LDS data need the workgroup.
+// threads lie inside one wavefront (TPE divides 64), else the workgroup
+// barrier.  Only the phases that read another element's LDS data need the
+// workgroup.
 template <int TPE>
 __device__ __forceinline__ void elementSync() {
-  if constexpr (TPE <= 64) {
+  if constexpr (TPE <= 64 && 64 % TPE == 0) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -830,10 +831,10 @@ __global__ __launch_bounds__((ElemGeo<D, P>::NTH)) void k_grad(GradArgs<T> a) {
   }
 }
 
-// LDS of one stage-kernel workgroup.
-template <int D, int P, int NF, typename T>
+// LDS of one stage-kernel workgroup (tile layout L).
+template <int D, int P, int NF, typename T, int L = 0>
 struct StageSmem {
-  using G = ElemGeo<D, P>;
+  using G = ElemGeo<D, P, L>;
   T sU[G::EPB][NF][G::NSP];   // stage state (padded rows, padIdx)
   T sFd[G::EPB][NF][G::NSP];  // contravariant flux F~_d of one reference axis, later the updated state
   T sJ[G::EPB][NF][G::NFP];   // normal-flux jump Fn*dA - disFn at the flux points
@@ -857,11 +858,13 @@ struct StageSmem {
 // area-weighted normal, whose length is dA; a right-side point uses the
 // negated own normal for the left element's (equal but for rounding).
 // GM = 2 forms the solution-point metrics and reads the face geometry.
-template <int D, int P, int NF, typename T, bool BND, bool VIS, bool FIN, int GM = 0, bool DBG = false>
-__device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, StageSmem<D, P, NF, T> &sm,
+// L is the tile layout (ElemGeo): 1 only in the batched 2D kernels.
+template <int D, int P, int NF, typename T, bool BND, bool VIS, bool FIN, int GM = 0, bool DBG = false, int L = 0>
+__device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, StageSmem<D, P, NF, T, L> &sm,
                                           const int tid) {
   static_assert(GM == 0 || !VIS, "the viscous stage reads the per-point geometry");
-  using G = ElemGeo<D, P>;
+  static_assert(L == 0 || (D == 2 && !VIS), "layout 1 is the inviscid 2D batch layout");
+  using G = ElemGeo<D, P, L>;
   using O = OneD<P>;
   constexpr int NS = G::NS, NFP = G::NFP, NFPF = G::NFPF, P1 = G::P1, TPE = G::TPE, EPB = G::EPB;
   constexpr int NFACE = G::NFACE;

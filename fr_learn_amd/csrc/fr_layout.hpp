@@ -35,16 +35,34 @@ constexpr int triTPE(int P) {
   return nextPow2(ns > nf ? ns : nf);
 }
 
-template <int D, int P>
+// Threads per triangle element in tile layout L (as quadTPE).
+constexpr int triTPEL(int P, int L) {
+  const int ns = (P + 1) * (P + 2) / 2, nf = 3 * (P + 1);
+  return L == 1 ? (ns > nf ? ns : nf) : triTPE(P);
+}
+
+// Threads per element of a tensor-product element in tile layout L:
+// 0 (every per-mesh kernel): the solution-point count rounded up to a power of
+// two (at least 8), so that an element's threads never straddle a wavefront;
+// 1 (2D, a choice of the batched kernels' tuner): max(solution points, flux
+// points), packing e.g. seven P5 quads (36 points) into 252 threads where
+// layout 0 gives each 64 threads.
+constexpr int tensorTPE(int D, int P, int L) {
+  const int P1 = P + 1, ns = (D == 3) ? P1 * P1 * P1 : P1 * P1, nf = (D == 3) ? 6 * P1 * P1 : 4 * P1;
+  if (L == 1 && D == 2) return ns > nf ? ns : nf;
+  return nextPow2(ns) < 8 ? 8 : nextPow2(ns);
+}
+
+template <int D, int P, int L = 0>
 struct ElemGeo {
   static constexpr int P1 = P + 1;
   static constexpr int NS = (D == 3) ? P1 * P1 * P1 : P1 * P1;  // solution points
   static constexpr int NFPF = (D == 3) ? P1 * P1 : P1;          // flux points per face
   static constexpr int NFACE = 2 * D;
   static constexpr int NFP = NFACE * NFPF;                      // flux points
-  static constexpr int TPE = nextPow2(NS) < 8 ? 8 : nextPow2(NS);  // threads per element
-  static constexpr int EPB = TPE >= 256 ? 1 : 256 / TPE;           // elements per block
-  static constexpr int NTH = TPE * EPB;
+  static constexpr int TPE = tensorTPE(D, P, L);                // threads per element
+  static constexpr int EPB = TPE >= 256 ? 1 : 256 / TPE;        // elements per block
+  static constexpr int NTH = TPE * EPB;                         // threads per block (<= 256)
   // LDS rows of solution-point data can be padded by one entry per x-line
   // (FR_LDS_PAD), so that lanes walking y- or x-face lines (strides P+1, 1
   // with bases P+1 apart) fall in different banks

@@ -4,7 +4,8 @@
 // points with no tensor lines.  A workgroup of 256 threads holds 256 / TPE
 // elements of TPE = triTPE(P) threads each (one solution point and one flux
 // point per thread: 8 threads at P = 1, 16 at P = 2..4, 32 at P = 5), several
-// elements per wavefront.  The table
+// elements per wavefront; the batched kernels may instead use TriGeo's
+// layout 1 (TPE = max(points), elements straddling wavefronts).  The table
 // Dr | Ds | E | L (at most 1638 values) is staged in LDS once per workgroup.
 // The phases follow the quad kernel's (k_stage): physical and reference flux
 // at the solution points and the common flux at the flux points; the jump
@@ -18,18 +19,21 @@
 
 namespace frdev {
 
-template <int P>
+// LAY: tile layout, 0 = triTPE(P) threads per element (every per-mesh kernel),
+// 1 = max(solution points, flux points) (a choice of the batched kernels'
+// tuner); NTH = EPB * TPE <= 256 threads per workgroup.
+template <int P, int LAY = 0>
 struct TriGeo {
   static constexpr int P1 = P + 1, NS = P1 * (P + 2) / 2, NFPF = P1, NFACE = 3, NFP = 3 * P1;
-  static constexpr int TPE = triTPE(P), NTH = 256, EPB = NTH / TPE;
+  static constexpr int TPE = triTPEL(P, LAY), EPB = 256 / TPE, NTH = EPB * TPE;
   static constexpr int DR = 0, DS = NS * NS, E = 2 * NS * NS, L = 2 * NS * NS + NFP * NS;
   static constexpr int TAB = 2 * NS * NS + 2 * NFP * NS;
   static_assert(NS <= TPE && NFP <= TPE, "one point per thread");
 };
 
-template <int P, int NF, typename T>
+template <int P, int NF, typename T, int LAY = 0>
 struct TriSmem {
-  using G = TriGeo<P>;
+  using G = TriGeo<P, LAY>;
   T tab[G::TAB];
   T sU[G::EPB][NF][G::NS];     // stage state
   T sF[G::EPB][2][NF][G::NS];  // reference flux F~_r, F~_s
@@ -46,10 +50,10 @@ __device__ __forceinline__ void triNormal(int f, T &nx, T &ny) {
 }
 
 // One RK stage of element block b (EPB elements) with the arguments a.
-template <int P, int NF, typename T, bool BND, bool FIN>
-__device__ __forceinline__ void triStageBody(const StageArgs<T> &a, const int b, TriSmem<P, NF, T> &sm,
+template <int P, int NF, typename T, bool BND, bool FIN, int LAY = 0>
+__device__ __forceinline__ void triStageBody(const StageArgs<T> &a, const int b, TriSmem<P, NF, T, LAY> &sm,
                                              const int tid) {
-  using G = TriGeo<P>;
+  using G = TriGeo<P, LAY>;
   constexpr int NS = G::NS, NFP = G::NFP, NFPF = G::NFPF, TPE = G::TPE, EPB = G::EPB, NFACE = G::NFACE;
   const int slot = tid / TPE, t = tid - slot * TPE;
   const int e = b * EPB + slot;
@@ -227,15 +231,16 @@ __global__ __launch_bounds__(256) void k_tri_stage(StageArgs<T> a) {
   triStageBody<P, NF, T, BND, FIN>(a, b, sm, (int)threadIdx.x);
 }
 
-// Batched: a persistent workgroup walks the triangle meshes' element blocks of
-// order P (see k_batch_stage), with each mesh's arguments.
-template <int P, int NF, typename T, bool FIN>
+// Batched: a persistent workgroup of TriGeo<P, LAY>::NTH threads walks the
+// triangle meshes' element blocks of order P (see k_batch_stage), with each
+// mesh's arguments.
+template <int P, int NF, typename T, bool FIN, int LAY>
 __global__ __launch_bounds__(256) void k_batch_tri_stage(const StageArgs<T> *__restrict__ args,
                                                          const BatchItem *__restrict__ items, int nItems) {
-  __shared__ TriSmem<P, NF, T> sm;
+  __shared__ TriSmem<P, NF, T, LAY> sm;
   for (int it = blockIdx.x; it < nItems; it += gridDim.x) {
     const BatchItem w = items[it];
-    triStageBody<P, NF, T, true, FIN>(args[w.inst], w.block, sm, itemThreadIdx());
+    triStageBody<P, NF, T, true, FIN, LAY>(args[w.inst], w.block, sm, itemThreadIdx());
     __syncthreads();  // the next item reuses the LDS
   }
 }

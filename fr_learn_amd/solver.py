@@ -310,11 +310,35 @@ class Batch:
     def update(self, n=1):
         _lib.check(self._lib.fr_batch_update(self._h, int(n)))
 
+    @staticmethod
+    def force_tile(layout=-1):
+        """Tile layout of the batches created afterwards (fr_batch_force_tile):
+        -1 tuned, 0 per-mesh tile, 1 packed 2D tile."""
+        _lib.check(_lib.load().fr_batch_force_tile(int(layout)))
+
     def tuning(self):
-        """(persistent grid, work items, ms per stage-0 launch at that grid)."""
+        """(largest persistent grid, work items, sum over the groups of their
+        residual-only launch time in ms)."""
         g, n, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
         _lib.check(self._lib.fr_batch_tuning(self._h, ctypes.byref(g), ctypes.byref(n), ctypes.byref(ms)))
         return g.value, n.value, ms.value
+
+    def group_tuning(self):
+        """The tuner's choice per kernel group present: {("quad" | "hex" |
+        "tri", P): {"threads_per_element", "elements_per_block", "grid",
+        "items", "ms"}}."""
+        out = {}
+        kind = "hex" if self.solvers[0].nDims == 3 else "quad"
+        for grp in range(1, 11 if kind == "quad" else 6):
+            v = [ctypes.c_int() for _ in range(4)]
+            ms = ctypes.c_double()
+            _lib.check(self._lib.fr_batch_group_tuning(self._h, grp, *[ctypes.byref(x) for x in v],
+                                                       ctypes.byref(ms)))
+            if v[3].value:
+                key = ("tri", grp - 5) if grp > 5 else (kind, grp)
+                out[key] = {"threads_per_element": v[0].value, "elements_per_block": v[1].value,
+                            "grid": v[2].value, "items": v[3].value, "ms": ms.value}
+        return out
 
     def close(self):
         if getattr(self, "_h", None):

@@ -234,14 +234,27 @@ int fr_reduce_gradient(fr_ctx *ctx, int kind, double *out);
  * triangles mixed; inviscid) advanced together, one persistent launch per
  * RK stage, element type and order over all their element blocks, one
  * step's launches replayed from a captured hipGraph.
- * fr_batch_create autotunes the persistent grid on residual evaluations
- * (which overwrite divF_spts[0] only; fr_batch_tuning reports the choice).
+ * fr_batch_create autotunes, per group (element type and order), the tile
+ * shape (threads per element and elements per workgroup) and the persistent
+ * grid on residual evaluations timed group by group (which overwrite
+ * divF_spts[0] only).  fr_batch_tuning reports the largest grid, the work
+ * items (element blocks) and the sum of the groups' stage times;
+ * fr_batch_group_tuning one group's choice (group = P for quads and hexes,
+ * 5 + P for triangles; nItems 0 when the batch has no such mesh).
  * fr_batch_update is fr_update(nSteps) on every member; the contexts stay
  * usable on their own between batch calls. */
 typedef struct fr_batch fr_batch;
 int fr_batch_create(fr_ctx *const *ctxs, int n, fr_batch **out);
 int fr_batch_update(fr_batch *batch, int nSteps);
 int fr_batch_tuning(fr_batch *batch, int *grid, int *nItems, double *msStage0);
+int fr_batch_group_tuning(fr_batch *batch, int group, int *threadsPerElement, int *elementsPerBlock, int *grid,
+                          int *nItems, double *ms);
+/* For the batches created after it (process-wide): -1 lets the tuner choose
+ * the tile layout (the default), 0 fixes the per-mesh kernels' tile (a power
+ * of two threads per element), 1 the packed tile of 2D groups (as many
+ * threads per element as it has solution or flux points, whichever is more;
+ * groups where both coincide keep layout 0).  The grid is tuned either way. */
+int fr_batch_force_tile(int layout);
 int fr_batch_destroy(fr_batch *batch);
 
 /* Stream control.  fr_set_stream(ctx, (void*)hipStream_t) makes the context

@@ -79,6 +79,46 @@ def test_batch_mixes_quads_and_triangles():
         assert err <= 1e-14
 
 
+@pytest.mark.parametrize("layout", [0, 1])
+def test_batch_tile_layouts(layout):
+    """Either tile layout (threads per element a power of two, or packed to
+    the element's point count, elements straddling wavefronts) gives each
+    member's own update; the quads' in-block trace exchange then covers other
+    faces than the per-mesh kernel's, so per-mesh steps after batched ones
+    (and batched after per-mesh) must still agree."""
+    rng = np.random.default_rng(21)
+    members = _mixed_members(rng, 20)
+    from fr_learn_amd.solver import Batch as BatchClass
+    BatchClass.force_tile(layout)
+    try:
+        solo, batched, _, B = _run(members, 2)
+    finally:
+        BatchClass.force_tile(-1)
+    tiles = B.group_tuning()
+    print(tiles)
+    for (kind, P), t in tiles.items():
+        n_pts = (P + 1) ** 2 if kind == "quad" else (P + 1) * (P + 2) // 2
+        n_fpts = 4 * (P + 1) if kind == "quad" else 3 * (P + 1)
+        pow2 = max(8, 1 << (n_pts - 1).bit_length()) if kind == "quad" else \
+            1 << (max(n_pts, n_fpts) - 1).bit_length()
+        want = max(n_pts, n_fpts) if layout == 1 and max(n_pts, n_fpts) != pow2 else pow2
+        assert t["threads_per_element"] == want
+        assert t["elements_per_block"] == 256 // want and t["items"] >= 1 and t["grid"] >= 1
+    for i, (a, b) in enumerate(zip(solo, batched)):
+        err = rel_err(b.get_state(), a.get_state()).max()
+        print(i, "tri" if i % 2 else "quad", "P", members[i][0]["order"], "err", err)
+        assert err <= 1e-14
+    for k in (0, 2, 4):
+        batched[k].update(1)
+        solo[k].update(1)
+    B.update(1)
+    for a in solo:
+        a.update(1)
+    for a, b in zip(solo, batched):
+        assert rel_err(b.get_state(), a.get_state()).max() <= 1e-14
+        assert rel_err(b.residual(0), a.residual(0)).max() <= 1e-12
+
+
 @pytest.mark.parametrize("timeType", [4, 0])
 def test_batch_matches_each_mesh(timeType):
     rng = np.random.default_rng(7)
