@@ -18,11 +18,11 @@ namespace {
 template <int P, int NF, typename T, int L>
 void triLaunch(bool fin, unsigned g, hipStream_t st, const frdev::StageArgs<T> *args, const frdev::BatchItem *it,
                int cnt) {
-  constexpr int NTH = frdev::TriGeo<P, L>::NTH;
+  // 256 threads in either layout (the matrix-core products use whole wavefronts)
   if (fin)
-    hipLaunchKernelGGL((frdev::k_batch_tri_stage<P, NF, T, true, L>), dim3(g), dim3(NTH), 0, st, args, it, cnt);
+    hipLaunchKernelGGL((frdev::k_batch_tri_stage<P, NF, T, true, L>), dim3(g), dim3(256), 0, st, args, it, cnt);
   else
-    hipLaunchKernelGGL((frdev::k_batch_tri_stage<P, NF, T, false, L>), dim3(g), dim3(NTH), 0, st, args, it, cnt);
+    hipLaunchKernelGGL((frdev::k_batch_tri_stage<P, NF, T, false, L>), dim3(g), dim3(256), 0, st, args, it, cnt);
 }
 
 template <int D, int P, int NF, typename T, int L>

@@ -10,9 +10,13 @@
 // The phases follow the quad kernel's (k_stage): physical and reference flux
 // at the solution points and the common flux at the flux points; the jump
 // Fn dA - disFn; divergence plus correction; the stage update; the traces of
-// the new state.  The stage arguments and modes are those of k_stage; every
-// face's traces go through HBM (no in-block exchange).
+// the new state.  In FP64 the operator products run on the matrix cores over
+// all (element, field) columns of the workgroup (triStageMF), in FP32 as each
+// thread's line sums (triStageBody).  The stage arguments and modes are those
+// of k_stage; every face's traces go through HBM (no in-block exchange).
 #pragma once
+
+#include <type_traits>
 
 #include "fr_batch.hpp"
 #include "fr_kernels.hpp"
@@ -21,7 +25,8 @@ namespace frdev {
 
 // LAY: tile layout, 0 = triTPE(P) threads per element (every per-mesh kernel),
 // 1 = max(solution points, flux points) (a choice of the batched kernels'
-// tuner); NTH = EPB * TPE <= 256 threads per workgroup.
+// tuner).  The workgroup is 256 threads, of which NTH = EPB * TPE hold
+// elements.
 template <int P, int LAY = 0>
 struct TriGeo {
   static constexpr int P1 = P + 1, NS = P1 * (P + 2) / 2, NFPF = P1, NFACE = 3, NFP = 3 * P1;
@@ -30,6 +35,13 @@ struct TriGeo {
   static constexpr int TAB = 2 * NS * NS + 2 * NFP * NS;
   static_assert(NS <= TPE && NFP <= TPE, "one point per thread");
 };
+
+// FP64 runs the operator products on the matrix cores (triStageMF), FP32 as
+// line sums of each thread's row (triStageBody).
+template <typename T>
+constexpr bool triMfma() {
+  return sizeof(T) == 8;
+}
 
 template <int P, int NF, typename T, int LAY = 0>
 struct TriSmem {
@@ -49,77 +61,57 @@ __device__ __forceinline__ void triNormal(int f, T &nx, T &ny) {
   ny = (f == 0) ? (T)-1 : (f == 1) ? h : (T)0;
 }
 
-// One RK stage of element block b (EPB elements) with the arguments a.
-template <int P, int NF, typename T, bool BND, bool FIN, int LAY = 0>
-__device__ __forceinline__ void triStageBody(const StageArgs<T> &a, const int b, TriSmem<P, NF, T, LAY> &sm,
-                                             const int tid) {
-  using G = TriGeo<P, LAY>;
-  constexpr int NS = G::NS, NFP = G::NFP, NFPF = G::NFPF, TPE = G::TPE, EPB = G::EPB, NFACE = G::NFACE;
-  const int slot = tid / TPE, t = tid - slot * TPE;
-  const int e = b * EPB + slot;
-  const bool act = e < a.nEles;
-  const bool hasS = act && t < NS, hasQ = act && t < NFP;
-  const size_t ec = (size_t)(act ? e : a.nEles - 1);
-  const int tc = t < NS ? t : NS - 1, qc = t < NFP ? t : NFP - 1;
-
-  for (int i = tid; i < G::TAB; i += G::NTH) sm.tab[i] = a.oned[i];
-  if (act)
-    for (int i = t; i < NF * NS; i += TPE) (&sm.sU[slot][0][0])[i] = a.Uin[ec * NF * NS + i];
-  // this thread's solution point: metrics, detJ, dt, base state
+// The loads of one thread of a triangle's stage (element ec): at its
+// solution point tc the metrics, detJ and base state, the element's time
+// step; at its flux point qc the face record, normal, dA and the neighbour's
+// trace (zero at boundaries).
+template <int P, int NF, typename T, bool BND, bool FIN>
+struct TriPoint {
   T pJ[4];
+  double pdet, pdt, pb[NF];
+  int fi;
+  bool bnd;
+  T n[2], dA, un[NF];
+  __device__ __forceinline__ TriPoint(const StageArgs<T> &a, const size_t ec, const int tc, const int qc) {
+    using G = TriGeo<P>;
+    constexpr int NS = G::NS, NFP = G::NFP, NFPF = G::NFPF, NFACE = G::NFACE;
 #pragma unroll
-  for (int c = 0; c < 4; c++) pJ[c] = a.JG[(ec * 4 + c) * NS + tc];
-  const double pdet = a.detJ[ec * NS + tc];
-  const double pdt = a.dtE[ec * a.dtStride];
-  double pb[NF];
+    for (int c = 0; c < 4; c++) pJ[c] = a.JG[(ec * 4 + c) * NS + tc];
+    pdet = a.detJ[ec * NS + tc];
+    pdt = a.dtE[ec * a.dtStride];
 #pragma unroll
-  for (int k = 0; k < NF; k++)
-    pb[k] = FIN ? a.Ubase[(ec * NF + k) * NS + tc] : (double)a.UbaseT[(ec * NF + k) * NS + tc];
-  // this thread's flux point: face record, normal, dA, neighbour trace
-  const int fi = a.finfo[ec * NFACE + qc / NFPF];
-  const int nbe = a.fnbr[ec * NFACE + qc / NFPF];
-  const bool bnd = BND && fi >= FINFO_BOUNDARY;
-  T n[2], un[NF];
-  n[0] = a.fgeo[(ec * 3 + 0) * NFP + qc];
-  n[1] = a.fgeo[(ec * 3 + 1) * NFP + qc];
-  const T dA = a.fgeo[(ec * 3 + 2) * NFP + qc];
-  if (!bnd && nbe >= 0) {
-    const T *Ur = a.UfIn + (size_t)nbe * NF * NFP + neighbourFpt<2, P>(qc, fi);
+    for (int k = 0; k < NF; k++)
+      pb[k] = FIN ? a.Ubase[(ec * NF + k) * NS + tc] : (double)a.UbaseT[(ec * NF + k) * NS + tc];
+    fi = a.finfo[ec * NFACE + qc / NFPF];
+    const int nbe = a.fnbr[ec * NFACE + qc / NFPF];
+    bnd = BND && fi >= FINFO_BOUNDARY;
+    n[0] = a.fgeo[(ec * 3 + 0) * NFP + qc];
+    n[1] = a.fgeo[(ec * 3 + 1) * NFP + qc];
+    dA = a.fgeo[(ec * 3 + 2) * NFP + qc];
+    if (!bnd && nbe >= 0) {
+      const T *Ur = a.UfIn + (size_t)nbe * NF * NFP + neighbourFpt<2, P>(qc, fi);
 #pragma unroll
-    for (int k = 0; k < NF; k++) un[k] = Ur[k * NFP];
-  } else {
+      for (int k = 0; k < NF; k++) un[k] = Ur[k * NFP];
+    } else {
 #pragma unroll
-    for (int k = 0; k < NF; k++) un[k] = (T)0;
+      for (int k = 0; k < NF; k++) un[k] = (T)0;
+    }
   }
-  __syncthreads();
-
-  const T *Dr = sm.tab + G::DR, *Ds = sm.tab + G::DS, *Em = sm.tab + G::E, *Lm = sm.tab + G::L;
-  // (1) flux at the solution point, common flux at the flux point
-  if (hasS) {
-    T u[NF], F[2][NF];
-#pragma unroll
-    for (int k = 0; k < NF; k++) u[k] = sm.sU[slot][k][t];
-    inviscidFlux<2, NF, T>(a.ph, u, F);
+  // contravariant flux F~_d at the solution point from the state u
+  __device__ __forceinline__ void refFlux(const Phys &ph, const T (&u)[NF], T (&Fr)[2][NF]) const {
+    T F[2][NF];
+    inviscidFlux<2, NF, T>(ph, u, F);
 #pragma unroll
     for (int d = 0; d < 2; d++)
 #pragma unroll
-      for (int k = 0; k < NF; k++) sm.sF[slot][d][k][t] = pJ[d * 2 + 0] * F[0][k] + pJ[d * 2 + 1] * F[1][k];
+      for (int k = 0; k < NF; k++) Fr[d][k] = pJ[d * 2 + 0] * F[0][k] + pJ[d * 2 + 1] * F[1][k];
   }
-  T Fn[NF];
-  if (hasQ) {
-    T uo[NF], ub[NF], Fc[NF];
-    // operator entries read once for all fields (the LDS reads bound these loops)
-#pragma unroll
-    for (int k = 0; k < NF; k++) uo[k] = (T)0;
-#pragma unroll 3
-    for (int s = 0; s < NS; s++) {
-      const T w = Em[t * NS + s];
-#pragma unroll
-      for (int k = 0; k < NF; k++) uo[k] += w * sm.sU[slot][k][s];
-    }
+  // common normal flux times dA at the flux point from the own trace uo
+  __device__ __forceinline__ void normalFlux(const Phys &ph, const T (&uo)[NF], T (&Fn)[NF]) const {
+    T ub[NF], Fc[NF];
     const bool right = !bnd && ((fi >> 5) & 1);
     if (bnd) {
-      boundaryState<2, NF, T>(a.ph, fi - FINFO_BOUNDARY, uo, n, ub);
+      boundaryState<2, NF, T>(ph, fi - FINFO_BOUNDARY, uo, n, ub);
     } else {
 #pragma unroll
       for (int k = 0; k < NF; k++) ub[k] = un[k];
@@ -132,10 +124,85 @@ __device__ __forceinline__ void triStageBody(const StageArgs<T> &a, const int b,
       sL[k] = right ? ub[k] : uo[k];
       sR[k] = right ? uo[k] : ub[k];
     }
-    commonFlux<2, NF, T>(a.ph, bnd, sL, sR, n, Fc);
+    commonFlux<2, NF, T>(ph, bnd, sL, sR, n, Fc);
     const T scale = right ? -dA : dA;
 #pragma unroll
     for (int k = 0; k < NF; k++) Fn[k] = Fc[k] * scale;
+  }
+  // the residual dv of field k at the solution point (global index gi): stored
+  // in divF, and (modes 1-3) the stage update, returned and stored
+  __device__ __forceinline__ T update(const StageArgs<T> &a, const int k, const size_t gi, const T dv) const {
+    if (a.divF) a.divF[gi] = dv;
+    if (a.mode == 0) return dv;
+    const double dtJ = pdt / pdet;
+    double u = pb[k];
+    if (a.mode == 2) {
+      const T r0 = (a.nprev > 0) ? a.dprev0[gi] : (T)0;
+      const T r1 = (a.nprev > 1) ? a.dprev1[gi] : (T)0;
+      const T r2 = (a.nprev > 2) ? a.dprev2[gi] : (T)0;
+      if (a.nprev > 0) u -= a.c0 * (double)r0 * dtJ;
+      if (a.nprev > 1) u -= a.c1 * (double)r1 * dtJ;
+      if (a.nprev > 2) u -= a.c2 * (double)r2 * dtJ;
+      const double cl = (a.nprev == 0) ? a.c0 : (a.nprev == 1) ? a.c1 : (a.nprev == 2) ? a.c2 : a.c3;
+      u -= cl * (double)dv * dtJ;
+    } else {
+      u = u - a.rk * (double)dv * dtJ;
+    }
+    if (a.mode == 1) {
+      a.Unext[gi] = (T)u;
+    } else {
+      a.Uout[gi] = u;
+      if (a.Uout32) a.Uout32[gi] = (float)u;
+    }
+    return (T)u;
+  }
+};
+
+// One RK stage of element block b (EPB elements) with the arguments a, the
+// operator products as each thread's line sums (FP32).
+template <int P, int NF, typename T, bool BND, bool FIN, int LAY = 0>
+__device__ __forceinline__ void triStageBody(const StageArgs<T> &a, const int b, TriSmem<P, NF, T, LAY> &sm,
+                                             const int tid) {
+  using G = TriGeo<P, LAY>;
+  constexpr int NS = G::NS, NFP = G::NFP, NFPF = G::NFPF, TPE = G::TPE, EPB = G::EPB;
+  const int slot = tid / TPE, t = tid - slot * TPE;
+  const int e = b * EPB + slot;
+  const bool act = slot < EPB && e < a.nEles;
+  const bool hasS = act && t < NS, hasQ = act && t < NFP;
+  const size_t ec = (size_t)(act ? e : a.nEles - 1);
+  const int tc = t < NS ? t : NS - 1, qc = t < NFP ? t : NFP - 1;
+
+  for (int i = tid; i < G::TAB; i += 256) sm.tab[i] = a.oned[i];
+  if (act)
+    for (int i = t; i < NF * NS; i += TPE) (&sm.sU[slot][0][0])[i] = a.Uin[ec * NF * NS + i];
+  const TriPoint<P, NF, T, BND, FIN> pt(a, ec, tc, qc);
+  __syncthreads();
+
+  const T *Dr = sm.tab + G::DR, *Ds = sm.tab + G::DS, *Em = sm.tab + G::E, *Lm = sm.tab + G::L;
+  // (1) flux at the solution point, common flux at the flux point
+  if (hasS) {
+    T u[NF], F[2][NF];
+#pragma unroll
+    for (int k = 0; k < NF; k++) u[k] = sm.sU[slot][k][t];
+    pt.refFlux(a.ph, u, F);
+#pragma unroll
+    for (int d = 0; d < 2; d++)
+#pragma unroll
+      for (int k = 0; k < NF; k++) sm.sF[slot][d][k][t] = F[d][k];
+  }
+  T Fn[NF];
+  if (hasQ) {
+    T uo[NF];
+    // operator entries read once for all fields
+#pragma unroll
+    for (int k = 0; k < NF; k++) uo[k] = (T)0;
+#pragma unroll 3
+    for (int s = 0; s < NS; s++) {
+      const T w = Em[t * NS + s];
+#pragma unroll
+      for (int k = 0; k < NF; k++) uo[k] += w * sm.sU[slot][k][s];
+    }
+    pt.normalFlux(a.ph, uo, Fn);
   }
   elementSync<TPE>();
   // (2) jump at the flux point: Fn dA - (E F~) . tNorm
@@ -177,33 +244,7 @@ __device__ __forceinline__ void triStageBody(const StageArgs<T> &a, const int b,
       for (int k = 0; k < NF; k++) cor[k] += wl * sm.sJ[slot][k][q];
     }
 #pragma unroll
-    for (int k = 0; k < NF; k++) {
-      const T dv = dvs[k] + cor[k];
-      const size_t gi = eo + (size_t)k * NS + t;
-      if (a.divF) a.divF[gi] = dv;
-      if (a.mode == 0) continue;
-      const double dtJ = pdt / pdet;
-      double u = pb[k];
-      if (a.mode == 2) {
-        const T r0 = (a.nprev > 0) ? a.dprev0[gi] : (T)0;
-        const T r1 = (a.nprev > 1) ? a.dprev1[gi] : (T)0;
-        const T r2 = (a.nprev > 2) ? a.dprev2[gi] : (T)0;
-        if (a.nprev > 0) u -= a.c0 * (double)r0 * dtJ;
-        if (a.nprev > 1) u -= a.c1 * (double)r1 * dtJ;
-        if (a.nprev > 2) u -= a.c2 * (double)r2 * dtJ;
-        const double cl = (a.nprev == 0) ? a.c0 : (a.nprev == 1) ? a.c1 : (a.nprev == 2) ? a.c2 : a.c3;
-        u -= cl * (double)dv * dtJ;
-      } else {
-        u = u - a.rk * (double)dv * dtJ;
-      }
-      if (a.mode == 1) {
-        a.Unext[gi] = (T)u;
-      } else {
-        a.Uout[gi] = u;
-        if (a.Uout32) a.Uout32[gi] = (float)u;
-      }
-      sm.sN[slot][k][t] = (T)u;
-    }
+    for (int k = 0; k < NF; k++) sm.sN[slot][k][t] = pt.update(a, k, eo + (size_t)k * NS + t, dvs[k] + cor[k]);
   }
   if (a.mode == 0) return;
   elementSync<TPE>();
@@ -224,23 +265,262 @@ __device__ __forceinline__ void triStageBody(const StageArgs<T> &a, const int b,
   }
 }
 
-template <int P, int NF, typename T, bool BND, bool FIN>
-__global__ __launch_bounds__(256) void k_tri_stage(StageArgs<T> a) {
-  __shared__ TriSmem<P, NF, T> sm;
-  const int b = (blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);  // XCD-contiguous blocks
-  triStageBody<P, NF, T, BND, FIN>(a, b, sm, (int)threadIdx.x);
+// Entries read from an NR x K operator table (row stride K) by the products:
+// rows up to the last tile of 4, k up to the last step of 4.
+constexpr int triOpSize(int NR, int K) { return (4 * ((NR + 3) / 4) - 1) * K + 4 * ((K + 3) / 4); }
+
+// LDS of the FP64 triangle stage, whose operator products run on the matrix
+// cores (triStageMF).  The data are stored zero-padded, the point dimension to
+// a multiple of 4 and the columns (element slot, field) to a multiple of 16,
+// so that every operand of the products is read at a constant offset from the
+// lane's base, without predication.  The padding of the point dimension stays
+// zero (written once per workgroup; the stage writes only the points
+// themselves), so the operator tables need none along k: what a product reads
+// past a row's end (the next row, or the zero tail) meets zero data.  Rows past
+// the last give results that are not stored.  The data rows (columns of the
+// products) have one more entry, an odd stride, so that the 16 columns one
+// instruction reads fall in different LDS banks.
+template <int P, int NF, int LAY>
+struct TriSmemMF {
+  using G = TriGeo<P, LAY>;
+  static constexpr int NS = G::NS, NFP = G::NFP;
+  static constexpr int NS4 = (NS + 3) & ~3, NQ4 = (NFP + 3) & ~3;
+  static constexpr int SS = NS4 + 1, SQ = NQ4 + 1;     // data row strides
+  static constexpr int NCP = (G::EPB * NF + 15) & ~15;  // columns
+  double opE[triOpSize(NFP, NS)];  // extrapolation E (flux points x solution points)
+  double opDr[triOpSize(NS, NS)];  // Dr
+  double opDs[triOpSize(NS, NS)];  // Ds
+  double opL[triOpSize(NS, NFP)];  // lifting L (solution points x flux points)
+  double sU[NCP][SS];              // stage state of column (slot, field); later the divergence + correction
+  double sF[2][NCP][SS];           // reference flux F~_r, F~_s; later the updated state in [0]
+  double sJ[NCP][SQ];              // traces E U, then Fn dA, then the jump; later the updated state's traces
+};
+
+// Row tiles per work item of a product with nrt tiles: a divisor, so that the
+// items need no guards.
+constexpr int triRpi(int nrt) { return nrt % 3 == 0 ? 3 : nrt % 2 == 0 ? 2 : 1; }
+
+// One segment of a product on the matrix cores: the operator at op (row
+// stride AST) against the data at dat (column stride BST) over KS steps of 4
+// along k (a multiple of 4 at or past the segment's length).
+template <int AST, int BST, int KS>
+struct TriSeg {
+  const double *op, *dat;
+};
+
+// acc_j += A(rows of tile rt0 + j) B(columns of group cg) over the segment,
+// FP64 4x4x4 on the matrix cores (four 4x4 blocks per instruction).  Lane l:
+// A row 4 rt + (l & 3), k 4 ks + (l >> 4) (the same in every block); B column
+// 16 cg + (l & 15), k 4 ks + (l >> 4).  Every operand is at a constant offset
+// from the lane's base.  rt0 is uniform per wavefront.
+template <int NRT, int RPI, int AST, int BST, int KS>
+__device__ __forceinline__ void triAcc(const TriSeg<AST, BST, KS> &sg, const int rt0, const int cg, const int lane,
+                                       double (&acc)[RPI]) {
+  const double *ap = sg.op + (4 * rt0 + (lane & 3)) * AST + (lane >> 4);
+  const double *bp = sg.dat + (16 * cg + (lane & 15)) * BST + (lane >> 4);
+#pragma unroll
+  for (int ks = 0; ks < KS; ks++) {
+    const double bv = bp[4 * ks];
+#pragma unroll
+    for (int j = 0; j < RPI; j++) acc[j] = __builtin_amdgcn_mfma_f64_4x4x4f64(ap[4 * j * AST + 4 * ks], bv, acc[j], 0, 0, 0);
+  }
 }
 
-// Batched: a persistent workgroup of TriGeo<P, LAY>::NTH threads walks the
-// triangle meshes' element blocks of order P (see k_batch_stage), with each
-// mesh's arguments.
+// A product of the workgroup on the matrix cores: for every group of 16
+// columns (element slot, field) and tile of 4 rows, the sum over the segments
+// of A B, handed to fin(cg, rt, acc) (lane l holds column 16 cg + (l & 15),
+// row 4 rt + (l >> 4)).  A work item is a column group and RPI row tiles (of
+// NRT), taken by the NW wavefronts in turn; every lane of a wavefront takes
+// part (the instructions read all 64 lanes' operands).
+template <int NRT, int NCG, int NW, typename Fin, typename... Segs>
+__device__ __forceinline__ void triProduct(const int wave, const int lane, Fin fin, const Segs &...segs) {
+  constexpr int RPI = triRpi(NRT), NRG = NRT / RPI;
+  for (int item = wave; item < NCG * NRG; item += NW) {
+    const int cg = item / NRG, rt0 = (item - cg * NRG) * RPI;
+    double acc[RPI];
+#pragma unroll
+    for (int j = 0; j < RPI; j++) acc[j] = 0.0;
+    (triAcc<NRT, RPI>(segs, rt0, cg, lane, acc), ...);
+#pragma unroll
+    for (int j = 0; j < RPI; j++) fin(cg, rt0 + j, acc[j]);
+  }
+}
+
+// One RK stage of element block b, FP64: the phases of triStageBody with the
+// operator products (E U, (E F~) . tNorm, [Dr | Ds | L] [F~_r; F~_s; J] and
+// E U') on the matrix cores over the block's (slot, field) columns, and the
+// pointwise work (fluxes, common flux, update) one point per thread.
+template <int P, int NF, bool BND, bool FIN, int LAY>
+__device__ __forceinline__ void triStageMF(const StageArgs<double> &a, const int b, TriSmemMF<P, NF, LAY> &sm,
+                                           const int tid) {
+  using G = TriGeo<P, LAY>;
+  using S = TriSmemMF<P, NF, LAY>;
+  using T = double;
+  constexpr int NS = G::NS, NFP = G::NFP, NFPF = G::NFPF, TPE = G::TPE, EPB = G::EPB;
+  constexpr int NS4 = S::NS4, NQ4 = S::NQ4, SS = S::SS, SQ = S::SQ, NCG = S::NCP / 16;
+  // the workgroup is 256 threads; those past EPB * TPE idle but for the products
+  const int slot = tid / TPE, t = tid - slot * TPE;
+  const int e = b * EPB + slot;
+  const bool act = slot < EPB && e < a.nEles;
+  const bool hasS = act && t < NS, hasQ = act && t < NFP;
+  const size_t ec = (size_t)(act ? e : a.nEles - 1);
+  const int tc = t < NS ? t : NS - 1, qc = t < NFP ? t : NFP - 1;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+  const int lk = lane >> 4, lc = lane & 15;
+  if (act)
+    for (int i = t; i < NF * NS; i += TPE) {
+      const int k = i / NS;
+      sm.sU[slot * NF + k][i - k * NS] = a.Uin[ec * NF * NS + i];
+    }
+  const TriPoint<P, NF, T, BND, FIN> pt(a, ec, tc, qc);
+  __syncthreads();
+
+  // (1) flux at the solution point; the traces E U
+  if (hasS) {
+    T u[NF], F[2][NF];
+#pragma unroll
+    for (int k = 0; k < NF; k++) u[k] = sm.sU[slot * NF + k][t];
+    pt.refFlux(a.ph, u, F);
+#pragma unroll
+    for (int d = 0; d < 2; d++)
+#pragma unroll
+      for (int k = 0; k < NF; k++) sm.sF[d][slot * NF + k][t] = F[d][k];
+  }
+  triProduct<NQ4 / 4, NCG, 4>(
+      wave, lane,
+      [&](int cg, int rt, double v) {
+        if (4 * rt + lk < NFP) sm.sJ[16 * cg + lc][4 * rt + lk] = v;
+      },
+      TriSeg<NS, SS, NS4 / 4>{sm.opE, &sm.sU[0][0]});
+  __syncthreads();
+  // (2) common flux at the flux point, Fn dA in place of the trace
+  if (hasQ) {
+    T uo[NF], Fn[NF];
+#pragma unroll
+    for (int k = 0; k < NF; k++) uo[k] = sm.sJ[slot * NF + k][t];
+    pt.normalFlux(a.ph, uo, Fn);
+#pragma unroll
+    for (int k = 0; k < NF; k++) sm.sJ[slot * NF + k][t] = Fn[k];
+  }
+  __syncthreads();
+  // (3) the jump Fn dA - (E F~) . tNorm, tNorm the reference normal of the
+  // flux point's edge: E F~_r and E F~_s share the operand A
+  {
+    constexpr int NRT = NQ4 / 4, RPI = triRpi(NRT), NRG = NRT / RPI;
+    for (int item = wave; item < NCG * NRG; item += 4) {
+      const int cg = item / NRG, rt0 = (item - cg * NRG) * RPI;
+      double ar[RPI], as[RPI];
+#pragma unroll
+      for (int j = 0; j < RPI; j++) ar[j] = as[j] = 0.0;
+      const double *ap = sm.opE + (4 * rt0 + (lane & 3)) * NS + lk;
+      const double *br = &sm.sF[0][16 * cg + lc][lk], *bs = &sm.sF[1][16 * cg + lc][lk];
+#pragma unroll
+      for (int ks = 0; ks < NS4 / 4; ks++) {
+        const double vr = br[4 * ks], vs = bs[4 * ks];
+#pragma unroll
+        for (int j = 0; j < RPI; j++) {
+          const double av = ap[4 * j * NS + 4 * ks];
+          ar[j] = __builtin_amdgcn_mfma_f64_4x4x4f64(av, vr, ar[j], 0, 0, 0);
+          as[j] = __builtin_amdgcn_mfma_f64_4x4x4f64(av, vs, as[j], 0, 0, 0);
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < RPI; j++) {
+        const int q = 4 * (rt0 + j) + lk;
+        if (q < NFP) {
+          T tx, ty;
+          triNormal<T>(q / NFPF, tx, ty);
+          double &J = sm.sJ[16 * cg + lc][q];
+          J = J - (tx * ar[j] + ty * as[j]);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  // (4) divergence + correction [Dr | Ds | L] [F~_r; F~_s; J] into sU
+  triProduct<NS4 / 4, NCG, 4>(
+      wave, lane,
+      [&](int cg, int rt, double v) {
+        if (4 * rt + lk < NS) sm.sU[16 * cg + lc][4 * rt + lk] = v;
+      },
+      TriSeg<NS, SS, NS4 / 4>{sm.opDr, &sm.sF[0][0][0]}, TriSeg<NS, SS, NS4 / 4>{sm.opDs, &sm.sF[1][0][0]},
+      TriSeg<NFP, SQ, NQ4 / 4>{sm.opL, &sm.sJ[0][0]});
+  __syncthreads();
+  // (5) stage update, the new state into sF[0]
+  if (hasS) {
+    const size_t eo = ec * NF * NS;
+#pragma unroll
+    for (int k = 0; k < NF; k++)
+      sm.sF[0][slot * NF + k][t] = pt.update(a, k, eo + (size_t)k * NS + t, sm.sU[slot * NF + k][t]);
+  }
+  if (a.mode == 0) return;
+  __syncthreads();
+  // (6) traces of the updated state E U'
+  triProduct<NQ4 / 4, NCG, 4>(
+      wave, lane,
+      [&](int cg, int rt, double v) {
+        if (4 * rt + lk < NFP) sm.sJ[16 * cg + lc][4 * rt + lk] = v;
+      },
+      TriSeg<NS, SS, NS4 / 4>{sm.opE, &sm.sF[0][0][0]});
+  __syncthreads();
+  if (hasQ) {
+    T *Fe = a.UfOut + ec * NF * NFP;
+#pragma unroll
+    for (int k = 0; k < NF; k++) Fe[k * NFP + t] = sm.sJ[slot * NF + k][t];
+  }
+}
+
+// The operator tables of TriSmemMF from the table Dr | Ds | E | L (zero
+// tails), and zero data arrays (their point padding stays zero).
+template <int P, int NF, int LAY>
+__device__ __forceinline__ void triSetupMF(const double *tab, TriSmemMF<P, NF, LAY> &sm, const int tid) {
+  using G = TriGeo<P, LAY>;
+  using S = TriSmemMF<P, NF, LAY>;
+  constexpr int NS = G::NS, NFP = G::NFP;
+  for (int i = tid; i < triOpSize(NFP, NS); i += 256) sm.opE[i] = i < NFP * NS ? tab[G::E + i] : 0.0;
+  for (int i = tid; i < triOpSize(NS, NS); i += 256) {
+    sm.opDr[i] = i < NS * NS ? tab[G::DR + i] : 0.0;
+    sm.opDs[i] = i < NS * NS ? tab[G::DS + i] : 0.0;
+  }
+  for (int i = tid; i < triOpSize(NS, NFP); i += 256) sm.opL[i] = i < NS * NFP ? tab[G::L + i] : 0.0;
+  for (int i = tid; i < S::NCP * S::SS; i += 256) {
+    (&sm.sU[0][0])[i] = 0.0;
+    (&sm.sF[0][0][0])[i] = 0.0;
+    (&sm.sF[1][0][0])[i] = 0.0;
+  }
+  for (int i = tid; i < S::NCP * S::SQ; i += 256) (&sm.sJ[0][0])[i] = 0.0;
+  __syncthreads();
+}
+
+// The stage's LDS for T: FP64 the matrix-core layout.
+template <int P, int NF, typename T, int LAY>
+using TriSmemFor = typename std::conditional<triMfma<T>(), TriSmemMF<P, NF, LAY>, TriSmem<P, NF, T, LAY>>::type;
+
+template <int P, int NF, typename T, bool BND, bool FIN>
+__global__ __launch_bounds__(256) void k_tri_stage(StageArgs<T> a) {
+  __shared__ TriSmemFor<P, NF, T, 0> sm;
+  const int b = (blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);  // XCD-contiguous blocks
+  if constexpr (triMfma<T>()) {
+    triSetupMF<P, NF, 0>(a.oned, sm, (int)threadIdx.x);
+    triStageMF<P, NF, BND, FIN, 0>(a, b, sm, (int)threadIdx.x);
+  } else {
+    triStageBody<P, NF, T, BND, FIN>(a, b, sm, (int)threadIdx.x);
+  }
+}
+
+// Batched: a persistent workgroup of 256 threads walks the triangle meshes'
+// element blocks of order P (see k_batch_stage), with each mesh's arguments.
 template <int P, int NF, typename T, bool FIN, int LAY>
 __global__ __launch_bounds__(256) void k_batch_tri_stage(const StageArgs<T> *__restrict__ args,
                                                          const BatchItem *__restrict__ items, int nItems) {
-  __shared__ TriSmem<P, NF, T, LAY> sm;
+  __shared__ TriSmemFor<P, NF, T, LAY> sm;
+  // the members share the order's operator table
+  if constexpr (triMfma<T>())
+    if (blockIdx.x < nItems) triSetupMF<P, NF, LAY>(args[items[blockIdx.x].inst].oned, sm, (int)threadIdx.x);
   for (int it = blockIdx.x; it < nItems; it += gridDim.x) {
     const BatchItem w = items[it];
-    triStageBody<P, NF, T, true, FIN, LAY>(args[w.inst], w.block, sm, itemThreadIdx());
+    if constexpr (triMfma<T>()) triStageMF<P, NF, true, FIN, LAY>(args[w.inst], w.block, sm, itemThreadIdx());
+    else triStageBody<P, NF, T, true, FIN, LAY>(args[w.inst], w.block, sm, itemThreadIdx());
     __syncthreads();  // the next item reuses the LDS
   }
 }
