@@ -208,8 +208,7 @@ def run_batch(n_inst=256, steps=20, warmup=3, seed=0):
     solvers = []
     dof = 0
     for cfg, m, ic, _ in batch_members(n_inst, seed):
-        s = Solver(cfg, m)
-        s.geometry_mode(0)  # per-point metrics, as on the main workload
+        s = Solver(cfg, m)  # default geometry: compact (every member is affine)
         s.initialize(ic)
         solvers.append(s)
         dof += s.nEles * s.nSpts

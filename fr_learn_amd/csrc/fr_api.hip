@@ -645,8 +645,8 @@ int createImpl(const fr_config *cfg, const fr_mesh *m, int precision, fr_ctx *c)
     if (neg) return fail(FR_EJACOBIAN, "Negative Jacobian at solution points.");
   }
   // affine meshes: compact per-element / per-face geometry for the stage
-  // kernels (quads and hexes; the triangle kernel reads the per-point arrays)
-  if (!c->tri) {
+  // kernels (every linear triangle is affine)
+  {
     const size_t nJ = (size_t)nE * D * D, nG = (size_t)nE * (D + 1) * c->NFACE;
     int8_t *dOk;
     if ((rc = dmalloc(&c->JGc64, nJ)) || (rc = dmalloc(&c->detJc, nE)) || (rc = dmalloc(&c->fgeoc64, nG)) ||
@@ -655,10 +655,10 @@ int createImpl(const fr_config *cfg, const fr_mesh *m, int precision, fr_ctx *c)
     const double tol = 1e-13;
     if (D == 3)
       hipLaunchKernelGGL(frdev::k_affine<3>, dim3(nblk(nE)), dim3(256), 0, c->stream, c->JG64, c->detJ, c->fgeo64,
-                         NS, NFP, c->NFPF, nE, tol, c->JGc64, c->detJc, c->fgeoc64, dOk);
+                         NS, NFP, c->NFPF, c->NFACE, nE, tol, c->JGc64, c->detJc, c->fgeoc64, dOk);
     else
       hipLaunchKernelGGL(frdev::k_affine<2>, dim3(nblk(nE)), dim3(256), 0, c->stream, c->JG64, c->detJ, c->fgeo64,
-                         NS, NFP, c->NFPF, nE, tol, c->JGc64, c->detJc, c->fgeoc64, dOk);
+                         NS, NFP, c->NFPF, c->NFACE, nE, tol, c->JGc64, c->detJc, c->fgeoc64, dOk);
     HIP_TRY(hipGetLastError());
     std::vector<int8_t> ok(nE);
     HIP_TRY(hipMemcpyAsync(ok.data(), dOk, nE, hipMemcpyDeviceToHost, c->stream));

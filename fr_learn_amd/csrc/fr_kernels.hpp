@@ -1659,9 +1659,9 @@ static __global__ __launch_bounds__(256) void k_dt_min(const double *__restrict_
 // first value (detJ, dA).  The compact arrays take the first point's values.
 template <int D>
 __global__ void k_affine(const double *__restrict__ JG, const double *__restrict__ detJ,
-                         const double *__restrict__ fgeo, int NS, int NFP, int NFPF, int nEles, double tol,
-                         double *__restrict__ JGc, double *__restrict__ detJc, double *__restrict__ fgeoc,
-                         int8_t *__restrict__ ok) {
+                         const double *__restrict__ fgeo, int NS, int NFP, int NFPF, int nFace, int nEles,
+                         double tol, double *__restrict__ JGc, double *__restrict__ detJc,
+                         double *__restrict__ fgeoc, int8_t *__restrict__ ok) {
   const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= nEles) return;
   bool good = true;
@@ -1675,10 +1675,10 @@ __global__ void k_affine(const double *__restrict__ JG, const double *__restrict
   const double *dj = detJ + e * NS;
   detJc[e] = dj[0];
   for (int p = 1; p < NS; p++) good = good && fabs(dj[p] - dj[0]) <= tol * fabs(dj[0]);
-  for (int f = 0; f < 2 * D; f++)
+  for (int f = 0; f < nFace; f++)
     for (int c = 0; c <= D; c++) {
       const double *v = fgeo + (e * (D + 1) + c) * NFP + f * NFPF;
-      fgeoc[(e * (D + 1) + c) * 2 * D + f] = v[0];
+      fgeoc[(e * (D + 1) + c) * nFace + f] = v[0];
       const double s = (c < D) ? 1.0 : fabs(v[0]);
       for (int j = 1; j < NFPF; j++) good = good && fabs(v[j] - v[0]) <= tol * s;
     }

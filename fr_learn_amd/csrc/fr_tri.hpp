@@ -64,7 +64,9 @@ __device__ __forceinline__ void triNormal(int f, T &nx, T &ny) {
 // The loads of one thread of a triangle's stage (element ec): at its
 // solution point tc the metrics, detJ and base state, the element's time
 // step; at its flux point qc the face record, normal, dA and the neighbour's
-// trace (zero at boundaries).
+// trace (zero at boundaries).  The geometry comes from the per-point arrays,
+// or (a.affine) from the compact ones: metrics and detJ per element, normal
+// and dA per edge (a linear triangle's are constant).
 template <int P, int NF, typename T, bool BND, bool FIN>
 struct TriPoint {
   T pJ[4];
@@ -75,9 +77,10 @@ struct TriPoint {
   __device__ __forceinline__ TriPoint(const StageArgs<T> &a, const size_t ec, const int tc, const int qc) {
     using G = TriGeo<P>;
     constexpr int NS = G::NS, NFP = G::NFP, NFPF = G::NFPF, NFACE = G::NFACE;
+    const int gS = a.affine ? 1 : NS, gP = a.affine ? 0 : tc;
 #pragma unroll
-    for (int c = 0; c < 4; c++) pJ[c] = a.JG[(ec * 4 + c) * NS + tc];
-    pdet = a.detJ[ec * NS + tc];
+    for (int c = 0; c < 4; c++) pJ[c] = a.JG[(ec * 4 + c) * gS + gP];
+    pdet = a.detJ[ec * gS + gP];
     pdt = a.dtE[ec * a.dtStride];
 #pragma unroll
     for (int k = 0; k < NF; k++)
@@ -85,9 +88,10 @@ struct TriPoint {
     fi = a.finfo[ec * NFACE + qc / NFPF];
     const int nbe = a.fnbr[ec * NFACE + qc / NFPF];
     bnd = BND && fi >= FINFO_BOUNDARY;
-    n[0] = a.fgeo[(ec * 3 + 0) * NFP + qc];
-    n[1] = a.fgeo[(ec * 3 + 1) * NFP + qc];
-    dA = a.fgeo[(ec * 3 + 2) * NFP + qc];
+    const int fS = a.affine ? NFACE : NFP, fP = a.affine ? qc / NFPF : qc;
+    n[0] = a.fgeo[(ec * 3 + 0) * fS + fP];
+    n[1] = a.fgeo[(ec * 3 + 1) * fS + fP];
+    dA = a.fgeo[(ec * 3 + 2) * fS + fP];
     if (!bnd && nbe >= 0) {
       const T *Ur = a.UfIn + (size_t)nbe * NF * NFP + neighbourFpt<2, P>(qc, fi);
 #pragma unroll

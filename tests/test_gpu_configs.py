@@ -160,10 +160,8 @@ def test_config4_batched_256_meshes():
     solo, batched, U0s = [], [], []
     for cfg, m, ic, _ in members:
         a = Solver(cfg, m)
-        a.geometry_mode(0)
         U0 = a.initialize(ic)
         b = Solver(cfg, m)
-        b.geometry_mode(0)
         b.set_state(U0)
         solo.append(a)
         batched.append(b)

@@ -60,12 +60,17 @@ def test_operators_geometry_faces(P):
 
 @pytest.mark.parametrize("P", [1, 2, 3, 4, 5])
 @pytest.mark.parametrize("riemann", [0, 1])
-def test_euler_residual_periodic(P, riemann):
+@pytest.mark.parametrize("geo", [0, 1])
+def test_euler_residual_periodic(P, riemann, geo):
+    """geo 0: the per-point metrics and normals; 1 (the default): one metric
+    set per element and one normal per edge (every linear triangle is
+    affine)."""
     cfg = euler_cfg(2, P, riemannType=riemann)
     m = _mesh(4)
     g = Solver(cfg, m)
+    assert g.geometry_mode(geo) == geo
     U = _state(g, cfg)
-    print(f"tri case P={P} riemann={riemann} nEles={m.nEles}", flush=True)
+    print(f"tri case P={P} riemann={riemann} geometry={geo} nEles={m.nEles}", flush=True)
     ops, JG, detJ, nrm, dA, faces = _checker(g, cfg, m)
     g.calc_residual(0)
     err = rel_err(g.residual(0), tri.residual(U, cfg, ops, JG, nrm, dA, faces))
@@ -74,10 +79,12 @@ def test_euler_residual_periodic(P, riemann):
 
 
 @pytest.mark.parametrize("P", [2, 4])
-def test_euler_residual_boundaries(P):
+@pytest.mark.parametrize("geo", [0, 1])
+def test_euler_residual_boundaries(P, geo):
     cfg = euler_cfg(2, P, riemannType=1)
     m = _mesh(4, periodic=False, bc=QUAD_BCS)
     g = Solver(cfg, m)
+    assert g.geometry_mode(geo) == geo
     U = _state(g, cfg)
     ops, JG, detJ, nrm, dA, faces = _checker(g, cfg, m)
     g.calc_residual(0)
