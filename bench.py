@@ -207,13 +207,16 @@ def run_batch(n_inst=256, steps=20, warmup=3, seed=0):
     from fr_learn_amd import Batch, Solver
     solvers = []
     dof = 0
+    b_res = 0.0
     for cfg, m, ic, _ in batch_members(n_inst, seed):
         s = Solver(cfg, m)  # default geometry: compact (every member is affine)
         s.initialize(ic)
         solvers.append(s)
         dof += s.nEles * s.nSpts
+        b_res += s.residual_bytes()
     b = Batch(solvers)
     grid, items, _ = b.tuning()
+    tiles = {f"{k} P{P}": v for (k, P), v in sorted(b.group_tuning().items())}
     b.update(warmup)
     t0 = time.perf_counter()
     b.update(steps)
@@ -234,11 +237,18 @@ def run_batch(n_inst=256, steps=20, warmup=3, seed=0):
         s.close()
     v_b = dof * 4 * steps / wall_b / 1e9
     v_s = dof * 4 * steps / wall_s / 1e9
+    # batched roofline: the members' algorithmic bytes of one residual (B_res,
+    # as for the main workload) per RK stage, over the measured stage time
+    stage_s = wall_b / steps / 4
+    achieved = b_res / stage_s / 1e9
     return {"workload": f"{n_inst} periodic meshes 8^2..64^2 cells, quads or triangles (p = 0.5), P 1..5, "
                         "2D Euler vortices, Roe, RK4",
             "instances": n_inst, "dof_per_field": dof, "steps": steps, "value": v_b, "unit": "GDOF/s",
             "ms_per_step": wall_b / steps * 1e3, "per_mesh_launches_GDOF_per_s": v_s,
-            "speedup_vs_per_mesh_launches": v_b / v_s, "autotuned_grid": grid, "work_items": items}
+            "speedup_vs_per_mesh_launches": v_b / v_s, "largest_group_grid": grid, "work_items": items,
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                         "frac": achieved / (HBM_PEAK / 1e9), "bytes_per_stage": b_res},
+            "tiles": tiles}
 
 
 def tgv_ns_adjoint(n=24, steps=200, every=20, eps=1e-6, seed=0):
@@ -355,6 +365,8 @@ def main():
                     help="skip the extra FP64 runs with the other geometry layouts and the perturbed mesh")
     ap.add_argument("--copy-calibration", action="store_true", help="only the 256 MiB copy for PMC scaling")
     ap.add_argument("--copy-bandwidth", action="store_true", help="only measure the 1 GiB copy bandwidth")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="only the batched 256-mesh workload (--steps / --warmup steps), its own JSON line")
     ap.add_argument("--adjoint", action="store_true",
                     help="only the Navier-Stokes TGV adjoint workload (24^3 P3, 200 steps), its own JSON line")
     args = ap.parse_args()
@@ -374,6 +386,9 @@ def main():
         return
     if args.adjoint:
         print(json.dumps(tgv_ns_adjoint()), flush=True)
+        return
+    if args.batch_only:
+        print(json.dumps(run_batch(steps=args.steps, warmup=args.warmup)), flush=True)
         return
 
     copy_gbps = copy_bandwidth_child()  # before this process touches the GPU

@@ -1347,6 +1347,10 @@ int fr_reduce_gradient(fr_ctx *c, int kind, double *out) {
   return 0;
 }
 
+#ifndef FR_BATCH_STREAMS
+#define FR_BATCH_STREAMS 4  // graph branches of a batched step (hardware queues per process by default)
+#endif
+
 namespace {
 std::atomic<int> gForceTile{-1};  // fr_batch_force_tile
 }
@@ -1471,24 +1475,39 @@ int fr_batch_create(fr_ctx *const *ctxs, int n, fr_batch **out) {
   (void)hipEventDestroy(e1);
   if (rc) return bad(rc, g_err);
   // One step's launches, captured per trace parity.  Meshes of different
-  // element types and orders are independent, so each group's stages form
-  // their own branch of the graph (forked from and joined to the batch
-  // stream) and run concurrently with the other groups'.
-  if (hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess) return bad(FR_EHIP, "event");
+  // element types and orders are independent, so the groups' stages form
+  // FR_BATCH_STREAMS branches of the graph (forked from and joined to the
+  // batch stream) that run concurrently, as many as the device has hardware
+  // queues by default.  The groups go to the branches longest first, each to
+  // the branch with the least tuned time so far, so that the branches end
+  // together; a branch runs its groups one after the other.
+  std::vector<int> order;
   for (int grp = 1; grp <= 10; grp++)
-    if (b->itemCount[0][grp] && (hipStreamCreateWithFlags(&b->ps[grp], hipStreamNonBlocking) != hipSuccess ||
-                                 hipEventCreateWithFlags(&b->join[grp], hipEventDisableTiming) != hipSuccess))
+    if (b->itemCount[0][grp]) order.push_back(grp);
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return b->groupMs[x] > b->groupMs[y]; });
+  const int nBr = std::max(1, std::min<int>(FR_BATCH_STREAMS, (int)order.size()));
+  std::vector<std::vector<int>> branch(nBr);
+  std::vector<double> load(nBr, 0.0);
+  for (int grp : order) {
+    const int br = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    branch[br].push_back(grp);
+    load[br] += b->groupMs[grp];
+  }
+  if (hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess) return bad(FR_EHIP, "event");
+  for (int br = 0; br < nBr; br++)
+    if (hipStreamCreateWithFlags(&b->ps[br], hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&b->join[br], hipEventDisableTiming) != hipSuccess)
       return bad(FR_EHIP, "stream");
   for (int p = 0; p < ((b->nRK & 1) ? 2 : 1); p++) {
     if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeGlobal) != hipSuccess) return bad(FR_EHIP, "capture");
     bool ok = hipEventRecord(b->fork, b->stream) == hipSuccess;
-    for (int grp = 1; grp <= 10 && ok && !rc; grp++) {
-      if (!b->itemCount[0][grp]) continue;
-      ok = hipStreamWaitEvent(b->ps[grp], b->fork, 0) == hipSuccess;
-      for (int s = 0; s < b->nRK && ok && !rc; s++)
-        rc = batchLaunchGroup(b, p, s, b->ps[grp], grp, b->lay[grp], b->grid[grp]);
-      ok = ok && hipEventRecord(b->join[grp], b->ps[grp]) == hipSuccess &&
-           hipStreamWaitEvent(b->stream, b->join[grp], 0) == hipSuccess;
+    for (int br = 0; br < nBr && ok && !rc; br++) {
+      ok = hipStreamWaitEvent(b->ps[br], b->fork, 0) == hipSuccess;
+      for (int grp : branch[br])
+        for (int s = 0; s < b->nRK && ok && !rc; s++)
+          rc = batchLaunchGroup(b, p, s, b->ps[br], grp, b->lay[grp], b->grid[grp]);
+      ok = ok && hipEventRecord(b->join[br], b->ps[br]) == hipSuccess &&
+           hipStreamWaitEvent(b->stream, b->join[br], 0) == hipSuccess;
     }
     if (!ok || rc) {
       hipGraph_t g = nullptr;

@@ -163,7 +163,7 @@ struct fr_batch {
   double tunedMs = 0;
   hipGraph_t graph[2] = {nullptr, nullptr};
   hipGraphExec_t exec[2] = {nullptr, nullptr};
-  hipStream_t ps[11] = {nullptr};  // one per group (capture)
+  hipStream_t ps[11] = {nullptr};  // one per graph branch (capture)
   hipEvent_t fork = nullptr, join[11] = {nullptr};
 };
 
