@@ -500,8 +500,9 @@ __device__ __forceinline__ void triSetupMF(const double *tab, TriSmemMF<P, NF, L
 template <int P, int NF, typename T, int LAY>
 using TriSmemFor = typename std::conditional<triMfma<T>(), TriSmemMF<P, NF, LAY>, TriSmem<P, NF, T, LAY>>::type;
 
+// Four waves per SIMD: 128 registers (the FP64 kernels need no more).
 template <int P, int NF, typename T, bool BND, bool FIN>
-__global__ __launch_bounds__(256) void k_tri_stage(StageArgs<T> a) {
+__global__ __launch_bounds__(256, 4) void k_tri_stage(StageArgs<T> a) {
   __shared__ TriSmemFor<P, NF, T, 0> sm;
   const int b = (blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);  // XCD-contiguous blocks
   if constexpr (triMfma<T>()) {
@@ -515,7 +516,7 @@ __global__ __launch_bounds__(256) void k_tri_stage(StageArgs<T> a) {
 // Batched: a persistent workgroup of 256 threads walks the triangle meshes'
 // element blocks of order P (see k_batch_stage), with each mesh's arguments.
 template <int P, int NF, typename T, bool FIN, int LAY>
-__global__ __launch_bounds__(256) void k_batch_tri_stage(const StageArgs<T> *__restrict__ args,
+__global__ __launch_bounds__(256, 4) void k_batch_tri_stage(const StageArgs<T> *__restrict__ args,
                                                          const BatchItem *__restrict__ items, int nItems) {
   __shared__ TriSmemFor<P, NF, T, LAY> sm;
   // the members share the order's operator table
