@@ -17,6 +17,10 @@
 
 using namespace frimpl;
 
+#ifndef FR_STAGE_REVERSE
+#define FR_STAGE_REVERSE 1  // stage launches alternate their element-block order
+#endif
+
 
 namespace {
 
@@ -148,7 +152,11 @@ template <typename T>
 int launchStage(fr_ctx *c, int in, int mode, int step, double rk) {
   int rc = ensureTraces<T>(c, in);
   if (rc) return rc;
-  const frdev::StageArgs<T> a = makeStageArgs<T>(c, in, c->cur, mode, step, rk);
+  frdev::StageArgs<T> a = makeStageArgs<T>(c, in, c->cur, mode, step, rk);
+  if (FR_STAGE_REVERSE) {  // alternate the block order from launch to launch
+    a.rev = c->revNext;
+    c->revNext ^= 1;
+  }
   int rc2 = launchStageArgs<T>(c, a, mode);
   if (rc2) return rc2;
   if (mode != 0) {

@@ -17,22 +17,22 @@ namespace {
 
 template <int P, int NF, typename T, int L>
 void triLaunch(bool fin, unsigned g, hipStream_t st, const frdev::StageArgs<T> *args, const frdev::BatchItem *it,
-               int cnt) {
+               int cnt, int rev) {
   // 256 threads in either layout (the matrix-core products use whole wavefronts)
   if (fin)
-    hipLaunchKernelGGL((frdev::k_batch_tri_stage<P, NF, T, true, L>), dim3(g), dim3(256), 0, st, args, it, cnt);
+    hipLaunchKernelGGL((frdev::k_batch_tri_stage<P, NF, T, true, L>), dim3(g), dim3(256), 0, st, args, it, cnt, rev);
   else
-    hipLaunchKernelGGL((frdev::k_batch_tri_stage<P, NF, T, false, L>), dim3(g), dim3(256), 0, st, args, it, cnt);
+    hipLaunchKernelGGL((frdev::k_batch_tri_stage<P, NF, T, false, L>), dim3(g), dim3(256), 0, st, args, it, cnt, rev);
 }
 
 template <int D, int P, int NF, typename T, int L>
 void tensorLaunch(bool fin, unsigned g, hipStream_t st, const frdev::StageArgs<T> *args,
-                  const frdev::BatchItem *it, int cnt) {
+                  const frdev::BatchItem *it, int cnt, int rev) {
   constexpr int NTH = frdev::ElemGeo<D, P, L>::NTH;
   if (fin)
-    hipLaunchKernelGGL((frdev::k_batch_stage<D, P, NF, T, true, L>), dim3(g), dim3(NTH), 0, st, args, it, cnt);
+    hipLaunchKernelGGL((frdev::k_batch_stage<D, P, NF, T, true, L>), dim3(g), dim3(NTH), 0, st, args, it, cnt, rev);
   else
-    hipLaunchKernelGGL((frdev::k_batch_stage<D, P, NF, T, false, L>), dim3(g), dim3(NTH), 0, st, args, it, cnt);
+    hipLaunchKernelGGL((frdev::k_batch_stage<D, P, NF, T, false, L>), dim3(g), dim3(NTH), 0, st, args, it, cnt, rev);
 }
 
 // Stage s of the meshes of group grp in tile layout lay with a persistent
@@ -46,6 +46,7 @@ int launchGroup(fr_batch *b, int parity, int s, hipStream_t st, int grp, int lay
   double rk;
   if (parity >= 0) stageMode(b->ctx[0], s, mode, rk);
   const bool fin = mode >= 2;
+  const int rev = parity >= 0 ? (s & 1) : 0;  // the stages alternate their item order
   const int cnt = b->itemCount[lay][grp];
   if (!cnt) return 0;
   const frdev::BatchItem *it = b->items[lay] + b->itemStart[lay][grp];
@@ -53,8 +54,8 @@ int launchGroup(fr_batch *b, int parity, int s, hipStream_t st, int grp, int lay
   if (grp > 5) {
     auto withL = [&](auto Pc, auto NFc) {
       constexpr int PP = decltype(Pc)::value, NF = decltype(NFc)::value;
-      if (lay == 1) triLaunch<PP, NF, T, 1>(fin, g, st, args, it, cnt);
-      else triLaunch<PP, NF, T, 0>(fin, g, st, args, it, cnt);
+      if (lay == 1) triLaunch<PP, NF, T, 1>(fin, g, st, args, it, cnt, rev);
+      else triLaunch<PP, NF, T, 0>(fin, g, st, args, it, cnt, rev);
     };
     auto withNF = [&](auto Pc) {
       if (b->NF == 1) withL(Pc, std::integral_constant<int, 1>{});
@@ -72,11 +73,11 @@ int launchGroup(fr_batch *b, int parity, int s, hipStream_t st, int grp, int lay
       constexpr int D = decltype(Dc)::value, PP = decltype(Pc)::value, NF = decltype(NFc)::value;
       if constexpr (D == 2) {
         if (lay == 1) {
-          tensorLaunch<D, PP, NF, T, 1>(fin, g, st, args, it, cnt);
+          tensorLaunch<D, PP, NF, T, 1>(fin, g, st, args, it, cnt, rev);
           return;
         }
       }
-      tensorLaunch<D, PP, NF, T, 0>(fin, g, st, args, it, cnt);
+      tensorLaunch<D, PP, NF, T, 0>(fin, g, st, args, it, cnt, rev);
     });
   }
   HIP_TRY(hipGetLastError());

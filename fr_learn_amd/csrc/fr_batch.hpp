@@ -33,13 +33,14 @@ __device__ __forceinline__ int itemThreadIdx() {
 // kernel switching over the orders is allocated for their union and spills).
 // L is the tile layout (ElemGeo); a workgroup is ElemGeo<D, P, L>::NTH
 // threads (at most 256), and an item is a block of EPB elements of that
-// layout.
+// layout.  rev walks the items backwards (the stages alternate, so that a
+// stage reads first what the one before wrote last).
 template <int D, int P, int NF, typename T, bool FIN, int L>
 __global__ __launch_bounds__(256) void k_batch_stage(const StageArgs<T> *__restrict__ args,
-                                                     const BatchItem *__restrict__ items, int nItems) {
+                                                     const BatchItem *__restrict__ items, int nItems, int rev) {
   __shared__ StageSmem<D, P, NF, T, L> sm;
   for (int it = blockIdx.x; it < nItems; it += gridDim.x) {
-    const BatchItem w = items[it];
+    const BatchItem w = items[rev ? nItems - 1 - it : it];
     stageBody<D, P, NF, T, true, false, FIN, 0, false, L>(args[w.inst], w.block, sm, itemThreadIdx());
     __syncthreads();  // the next item reuses the LDS
   }

@@ -89,6 +89,7 @@ struct fr_ctx {
   // element-block size of the launches that wrote Uf[cur] (their in-block
   // faces are missing from it), 0: every face written
   int traceEpb = 0;
+  int revNext = 0;  // block order of the next stage launch (FR_STAGE_REVERSE)
   int keepFinal = 1;                  // the final RK stage stores divF_spts[nRK-1] (fr_keep_final_residual)
   bool finalDropped = false;          // divF[nRK-1] is stale: the last update did not store it
   void *divF[4] = {nullptr, nullptr, nullptr, nullptr};

@@ -255,6 +255,7 @@ struct StageArgs {
   // times dA Fn_fpts [ele][field][fpt] and the corrected jump
   // disFn_fpts = Fn_fpts - disFn [ele][field][fpt]
   T *dbgF, *dbgFn, *dbgJ;
+  int rev;  // walk each XCD's range of element blocks backwards (k_stage)
 };
 
 // Face f of the two faces normal to reference axis d (hex: x 2,3; y 4,5;
@@ -1392,8 +1393,11 @@ __global__ __launch_bounds__((ElemGeo<D, P>::NTH),
   __shared__ StageSmem<D, P, NF, T> sm;
   // Workgroups are dealt to the 8 XCDs in turn; remap so that each XCD walks a
   // contiguous range of elements and face neighbours share its L2.  The host
-  // pads the grid to a multiple of 8.
-  const int b = (blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+  // pads the grid to a multiple of 8.  rev walks the range backwards, so that
+  // a launch after a forward one reads first what that one wrote last (still
+  // in the memory-side cache).
+  const int R = gridDim.x >> 3, j = blockIdx.x >> 3;
+  const int b = (blockIdx.x & 7) * R + (a.rev ? R - 1 - j : j);
   stageBody<D, P, NF, T, BND, VIS, FIN, GM>(a, b, sm, (int)threadIdx.x);
 }
 

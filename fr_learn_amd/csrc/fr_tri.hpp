@@ -514,16 +514,18 @@ __global__ __launch_bounds__(256, 4) void k_tri_stage(StageArgs<T> a) {
 }
 
 // Batched: a persistent workgroup of 256 threads walks the triangle meshes'
-// element blocks of order P (see k_batch_stage), with each mesh's arguments.
+// element blocks of order P (see k_batch_stage; rev walks them backwards),
+// with each mesh's arguments.
 template <int P, int NF, typename T, bool FIN, int LAY>
 __global__ __launch_bounds__(256, 4) void k_batch_tri_stage(const StageArgs<T> *__restrict__ args,
-                                                         const BatchItem *__restrict__ items, int nItems) {
+                                                         const BatchItem *__restrict__ items, int nItems,
+                                                         int rev) {
   __shared__ TriSmemFor<P, NF, T, LAY> sm;
   // the members share the order's operator table
   if constexpr (triMfma<T>())
     if (blockIdx.x < nItems) triSetupMF<P, NF, LAY>(args[items[blockIdx.x].inst].oned, sm, (int)threadIdx.x);
   for (int it = blockIdx.x; it < nItems; it += gridDim.x) {
-    const BatchItem w = items[it];
+    const BatchItem w = items[rev ? nItems - 1 - it : it];
     if constexpr (triMfma<T>()) triStageMF<P, NF, true, FIN, LAY>(args[w.inst], w.block, sm, itemThreadIdx());
     else triStageBody<P, NF, T, true, FIN, LAY>(args[w.inst], w.block, sm, itemThreadIdx());
     __syncthreads();  // the next item reuses the LDS
