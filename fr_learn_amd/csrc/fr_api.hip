@@ -388,7 +388,7 @@ void batchFree(fr_batch *b) {
     if (b->args[p]) (void)hipFree(b->args[p]);
   }
   if (b->argsTune) (void)hipFree(b->argsTune);
-  for (int grp = 0; grp < 11; grp++) {
+  for (int grp = 0; grp < 16; grp++) {
     if (b->ps[grp]) (void)hipStreamDestroy(b->ps[grp]);
     if (b->join[grp]) (void)hipEventDestroy(b->join[grp]);
   }
@@ -1380,15 +1380,13 @@ int fr_batch_create(fr_ctx *const *ctxs, int n, fr_batch **out) {
   };
   const fr_ctx *c0 = ctxs[0];
   if (!c0) return bad(FR_EINVAL, "null context");
-  b->D = c0->D;
-  b->NF = c0->NF;
   b->nRK = c0->nRK;
   b->prec = c0->prec;
   for (int i = 0; i < n; i++) {
     fr_ctx *c = ctxs[i];
     if (!c) return bad(FR_EINVAL, "null context");
-    if (c->D != b->D || c->NF != b->NF || c->prec != b->prec || c->cfg.timeType != c0->cfg.timeType)
-      return bad(FR_EINVAL, "batched meshes must share nDims, equation, precision and timeType");
+    if (c->cfg.equation != c0->cfg.equation || c->prec != b->prec || c->cfg.timeType != c0->cfg.timeType)
+      return bad(FR_EINVAL, "batched meshes must share equation, precision and timeType");
     if (c->cfg.viscous) return bad(FR_EINVAL, "the batched kernel is inviscid");
     if (c->P < 1 || c->P > 5) return bad(FR_EINVAL, "batched meshes need P in 1..5");
     for (fr_ctx *o : b->ctx)
@@ -1400,10 +1398,11 @@ int fr_batch_create(fr_ctx *const *ctxs, int n, fr_batch **out) {
   // each group in mesh order, for both tile layouts
   for (int l = 0; l < 2; l++) {
     std::vector<frdev::BatchItem> items;
-    for (int grp = 1; grp <= 10; grp++) {
+    for (int grp = 1; grp <= 15; grp++) {
       b->itemStart[l][grp] = (int)items.size();
       for (int i = 0; i < n; i++)
         if (batchGroup(b->ctx[i]) == grp) {
+          b->groupNF[grp] = b->ctx[i]->NF;
           const int epb = batchEpb(b, b->ctx[i], l);
           const int nb = (b->ctx[i]->nE + epb - 1) / epb;
           for (int k = 0; k < nb; k++) items.push_back({i, k});
@@ -1454,13 +1453,15 @@ int fr_batch_create(fr_ctx *const *ctxs, int n, fr_batch **out) {
   };
   b->nItems = 0;
   b->tunedMs = 0;
-  for (int grp = 1; grp <= 10 && !rc; grp++) {
+  for (int grp = 1; grp <= 15 && !rc; grp++) {
     if (!b->itemCount[0][grp]) continue;
     double best = 1e300;
-    const int P = grp > 5 ? grp - 5 : grp;
+    int gD, P;
+    bool gTri;
+    batchGroupKind(grp, gD, P, gTri);
     // layout 1 is a different tile only where it changes threads per element
-    const bool l1 = grp > 5 ? frdev::triTPEL(P, 1) != frdev::triTPEL(P, 0)
-                            : frdev::tensorTPE(b->D, P, 1) != frdev::tensorTPE(b->D, P, 0);
+    const bool l1 = gTri ? frdev::triTPEL(P, 1) != frdev::triTPEL(P, 0)
+                         : frdev::tensorTPE(gD, P, 1) != frdev::tensorTPE(gD, P, 0);
     const int force = gForceTile.load();
     const int lay0 = force == 1 && l1 ? 1 : 0, lay1 = force == 0 || !l1 ? 0 : 1;
     for (int lay = lay0; lay <= lay1 && !rc; lay++)
@@ -1490,7 +1491,7 @@ int fr_batch_create(fr_ctx *const *ctxs, int n, fr_batch **out) {
   // the branch with the least tuned time so far, so that the branches end
   // together; a branch runs its groups one after the other.
   std::vector<int> order;
-  for (int grp = 1; grp <= 10; grp++)
+  for (int grp = 1; grp <= 15; grp++)
     if (b->itemCount[0][grp]) order.push_back(grp);
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return b->groupMs[x] > b->groupMs[y]; });
   const int nBr = std::max(1, std::min<int>(FR_BATCH_STREAMS, (int)order.size()));
@@ -1576,7 +1577,7 @@ int fr_batch_update(fr_batch *b, int nSteps) {
 
 int fr_batch_tuning(fr_batch *b, int *grid, int *nItems, double *msStage0) {
   if (!b) return fail(FR_EINVAL, "null argument");
-  if (grid) *grid = *std::max_element(b->grid, b->grid + 11);
+  if (grid) *grid = *std::max_element(b->grid, b->grid + 16);
   if (nItems) *nItems = b->nItems;
   if (msStage0) *msStage0 = b->tunedMs;
   return 0;
@@ -1585,10 +1586,13 @@ int fr_batch_tuning(fr_batch *b, int *grid, int *nItems, double *msStage0) {
 int fr_batch_group_tuning(fr_batch *b, int group, int *threadsPerElement, int *elementsPerBlock, int *grid,
                           int *nItems, double *ms) {
   if (!b) return fail(FR_EINVAL, "null argument");
-  if (group < 1 || group > 10 || (b->D == 3 && group > 5))
-    return fail(FR_EINVAL, "group is an order 1..5, plus 5 for triangles");
-  const int lay = b->lay[group], P = group > 5 ? group - 5 : group;
-  const int tpe = group > 5 ? frdev::triTPEL(P, lay) : frdev::tensorTPE(b->D, P, lay);
+  if (group < 1 || group > 15)
+    return fail(FR_EINVAL, "group is an order 1..5, plus 5 for triangles, plus 10 for hexes");
+  int gD, P;
+  bool gTri;
+  batchGroupKind(group, gD, P, gTri);
+  const int lay = b->lay[group];
+  const int tpe = gTri ? frdev::triTPEL(P, lay) : frdev::tensorTPE(gD, P, lay);
   if (threadsPerElement) *threadsPerElement = tpe;
   if (elementsPerBlock) *elementsPerBlock = tpe >= 256 ? 1 : 256 / tpe;
   if (grid) *grid = b->itemCount[lay][group] ? b->grid[group] : 0;

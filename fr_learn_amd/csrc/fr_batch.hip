@@ -4,12 +4,19 @@
 
 namespace frimpl {
 
-// Kernel group of a member: its order P, plus 5 for triangles.
-int batchGroup(const fr_ctx *c) { return c->P + (c->tri ? 5 : 0); }
+// Kernel group of a member: its order P, plus 5 for triangles, plus 10 for
+// hexes.
+int batchGroup(const fr_ctx *c) { return c->P + (c->tri ? 5 : c->D == 3 ? 10 : 0); }
+
+void batchGroupKind(int g, int &D, int &P, bool &tri) {
+  tri = g > 5 && g <= 10;
+  D = g > 10 ? 3 : 2;
+  P = g - (g > 10 ? 10 : g > 5 ? 5 : 0);
+}
 
 // Elements per work item of member c in tile layout lay.
-int batchEpb(const fr_batch *b, const fr_ctx *c, int lay) {
-  const int tpe = c->tri ? frdev::triTPEL(c->P, lay) : frdev::tensorTPE(b->D, c->P, lay);
+int batchEpb(const fr_batch *, const fr_ctx *c, int lay) {
+  const int tpe = c->tri ? frdev::triTPEL(c->P, lay) : frdev::tensorTPE(c->D, c->P, lay);
   return tpe >= 256 ? 1 : 256 / tpe;
 }
 
@@ -51,17 +58,21 @@ int launchGroup(fr_batch *b, int parity, int s, hipStream_t st, int grp, int lay
   if (!cnt) return 0;
   const frdev::BatchItem *it = b->items[lay] + b->itemStart[lay][grp];
   const unsigned g = (unsigned)std::max(1, std::min(grid, cnt));
-  if (grp > 5) {
+  int gD, gP;
+  bool gTri;
+  batchGroupKind(grp, gD, gP, gTri);
+  const int gNF = b->groupNF[grp];
+  if (gTri) {
     auto withL = [&](auto Pc, auto NFc) {
       constexpr int PP = decltype(Pc)::value, NF = decltype(NFc)::value;
       if (lay == 1) triLaunch<PP, NF, T, 1>(fin, g, st, args, it, cnt, rev);
       else triLaunch<PP, NF, T, 0>(fin, g, st, args, it, cnt, rev);
     };
     auto withNF = [&](auto Pc) {
-      if (b->NF == 1) withL(Pc, std::integral_constant<int, 1>{});
+      if (gNF == 1) withL(Pc, std::integral_constant<int, 1>{});
       else withL(Pc, std::integral_constant<int, 4>{});
     };
-    switch (grp - 5) {
+    switch (gP) {
       case 1: withNF(std::integral_constant<int, 1>{}); break;
       case 2: withNF(std::integral_constant<int, 2>{}); break;
       case 3: withNF(std::integral_constant<int, 3>{}); break;
@@ -69,7 +80,7 @@ int launchGroup(fr_batch *b, int parity, int s, hipStream_t st, int grp, int lay
       default: withNF(std::integral_constant<int, 5>{}); break;
     }
   } else {
-    dispatch(b->D, grp, b->NF, [&](auto Dc, auto Pc, auto NFc) {
+    dispatch(gD, gP, gNF, [&](auto Dc, auto Pc, auto NFc) {
       constexpr int D = decltype(Dc)::value, PP = decltype(Pc)::value, NF = decltype(NFc)::value;
       if constexpr (D == 2) {
         if (lay == 1) {

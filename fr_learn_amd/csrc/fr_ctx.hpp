@@ -146,26 +146,28 @@ struct fr_batch {
   std::vector<unsigned> gen;  // each member's settings generation the arguments were built at
   std::vector<bool> alive;    // members not yet destroyed
   bool broken = false;        // a member was destroyed
-  int D = 2, NF = 4, nRK = 1, prec = FR_PREC_FP64;
+  int nRK = 1, prec = FR_PREC_FP64;
   hipStream_t stream = nullptr;
   void *args[2] = {nullptr, nullptr};  // StageArgs per [trace parity][stage][instance]
   void *argsTune = nullptr;            // residual-only (mode 0) StageArgs per instance, for tuning
-  // work items grouped by kernel: group g = P (quads / hexes of order P) or
-  // 5 + P (triangles of order P), g in 1..10; one list per tile layout
+  // work items grouped by kernel: group g = P (quads of order P), 5 + P
+  // (triangles) or 10 + P (hexes), g in 1..15 (frimpl::batchGroup); one list
+  // per tile layout
   // (frdev::ElemGeo / TriGeo L), since an item is a block of that layout's EPB
   // elements
   frdev::BatchItem *items[2] = {nullptr, nullptr};
-  int itemStart[2][11] = {}, itemCount[2][11] = {};
+  int itemStart[2][16] = {}, itemCount[2][16] = {};
+  int groupNF[16] = {};  // fields of the group's meshes
   // per group, chosen at fr_batch_create by timing: tile layout, persistent
   // grid and the residual-only launch time
-  int lay[11] = {}, grid[11] = {};
-  double groupMs[11] = {};
+  int lay[16] = {}, grid[16] = {};
+  double groupMs[16] = {};
   int nItems = 0;  // items of the chosen layouts
   double tunedMs = 0;
   hipGraph_t graph[2] = {nullptr, nullptr};
   hipGraphExec_t exec[2] = {nullptr, nullptr};
-  hipStream_t ps[11] = {nullptr};  // one per graph branch (capture)
-  hipEvent_t fork = nullptr, join[11] = {nullptr};
+  hipStream_t ps[16] = {nullptr};  // one per graph branch (capture)
+  hipEvent_t fork = nullptr, join[16] = {nullptr};
 };
 
 namespace frimpl {
@@ -244,6 +246,8 @@ int dtPrepare(fr_ctx *c, hipStream_t st);
 // Ponly != 0 restricts it to the meshes of that order.
 int batchLaunchGroup(fr_batch *b, int parity, int s, hipStream_t st, int grp, int lay, int grid);
 int batchGroup(const fr_ctx *c);
+// Element type of batch group g: nDims, order and whether triangles.
+void batchGroupKind(int g, int &D, int &P, bool &tri);
 int batchEpb(const fr_batch *b, const fr_ctx *c, int lay);
 
 }  // namespace frimpl

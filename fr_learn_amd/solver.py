@@ -296,8 +296,9 @@ class Solver:
 
 class Batch:
     """Several Solvers advanced together (fr_batch_*): one persistent launch
-    per RK stage over all their element blocks, replayed from a hipGraph.
-    The solvers must share nDims, equation, precision and timeType."""
+    per RK stage, element type and order over all their element blocks,
+    replayed from a hipGraph.  The solvers must share equation, precision and
+    timeType (quads, triangles and hexes may mix)."""
 
     def __init__(self, solvers):
         self.solvers = list(solvers)
@@ -328,14 +329,13 @@ class Batch:
         "tri", P): {"threads_per_element", "elements_per_block", "grid",
         "items", "ms"}}."""
         out = {}
-        kind = "hex" if self.solvers[0].nDims == 3 else "quad"
-        for grp in range(1, 11 if kind == "quad" else 6):
+        for grp in range(1, 16):
             v = [ctypes.c_int() for _ in range(4)]
             ms = ctypes.c_double()
             _lib.check(self._lib.fr_batch_group_tuning(self._h, grp, *[ctypes.byref(x) for x in v],
                                                        ctypes.byref(ms)))
             if v[3].value:
-                key = ("tri", grp - 5) if grp > 5 else (kind, grp)
+                key = ("hex", grp - 10) if grp > 10 else ("tri", grp - 5) if grp > 5 else ("quad", grp)
                 out[key] = {"threads_per_element": v[0].value, "elements_per_block": v[1].value,
                             "grid": v[2].value, "items": v[3].value, "ms": ms.value}
         return out

@@ -229,18 +229,18 @@ int fr_vjp_step(fr_ctx *ctx, const double *lam1, double *lam0);
 int fr_vjp_steps(fr_ctx *ctx, int nSteps, int every, const double *lamN, double *lam0);
 int fr_reduce_gradient(fr_ctx *ctx, int kind, double *out);
 
-/* Batches (no reference counterpart): many contexts of one nDims, equation,
- * precision and timeType (sizes, orders, boundaries and, in 2D, quads and
- * triangles mixed; inviscid) advanced together, one persistent launch per
- * RK stage, element type and order over all their element blocks, one
- * step's launches replayed from a captured hipGraph.
+/* Batches (no reference counterpart): many contexts of one equation,
+ * precision and timeType (sizes, orders, boundaries, and quads, triangles and
+ * hexes mixed; inviscid) advanced together, one persistent launch per RK
+ * stage, element type and order over all their element blocks, one step's
+ * launches replayed from a captured hipGraph.
  * fr_batch_create autotunes, per group (element type and order), the tile
  * shape (threads per element and elements per workgroup) and the persistent
  * grid on residual evaluations timed group by group (which overwrite
  * divF_spts[0] only).  fr_batch_tuning reports the largest grid, the work
  * items (element blocks) and the sum of the groups' stage times;
- * fr_batch_group_tuning one group's choice (group = P for quads and hexes,
- * 5 + P for triangles; nItems 0 when the batch has no such mesh).
+ * fr_batch_group_tuning one group's choice (group = P for quads, 5 + P for
+ * triangles, 10 + P for hexes; nItems 0 when the batch has no such mesh).
  * fr_batch_update is fr_update(nSteps) on every member; the contexts stay
  * usable on their own between batch calls. */
 typedef struct fr_batch fr_batch;

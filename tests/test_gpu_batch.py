@@ -7,7 +7,7 @@ import pytest
 from fr_learn_amd import Batch, Solver, cases, mesh
 from oracle import Oracle
 
-from helpers import QUAD_BCS, TOL_FP64, euler_cfg, hex_box, quad_box, rel_err
+from helpers import MIXED_BCS, QUAD_BCS, TOL_FP64, euler_cfg, hex_box, perturbed_state, quad_box, rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -136,6 +136,38 @@ def test_batch_matches_each_mesh(timeType):
         a.update(2)
     for a, b in zip(solo, batched):
         assert rel_err(b.get_state(), a.get_state()).max() <= 1e-14
+
+
+def test_batch_mixes_dimensions():
+    """Quads, triangles and hexes (3D Euler, five fields against the 2D
+    four) in one batch: each group its own kernel and field count; every
+    member equals its own update, and a hex member the oracle."""
+    rng = np.random.default_rng(31)
+    members = _mixed_members(rng, 6)
+    for P in (1, 2, 3):
+        if P % 2:
+            members.append((euler_cfg(3, P, dt=1e-3), hex_box(3, seed=P), cases.ic_tgv))
+        else:  # boundaries: a perturbed flow near the boundary states
+            ic = lambda x, c: perturbed_state(cases.ic_uniform(x, c, vel=[0.3, 0.1, -0.05], p=1.0 / 1.4),
+                                              seed=4, amp=2e-2)
+            members.append((euler_cfg(3, P, dt=1e-3), hex_box(3, seed=P, periodic=False, bc=MIXED_BCS, L=1.0), ic))
+    solo, batched, U0s, B = _run(members, 2)
+    tiles = B.group_tuning()
+    print(tiles)
+    assert {k for k in tiles if k[0] == "hex"} == {("hex", 1), ("hex", 2), ("hex", 3)}
+    for i, (a, b) in enumerate(zip(solo, batched)):
+        err = rel_err(b.get_state(), a.get_state(), axis=None)
+        print(i, members[i][0]["nDims"], "P", members[i][0]["order"], "err", err)
+        assert err <= 1e-14
+    # the hex member with boundaries against the oracle (its fields are all of
+    # order 0.1..1, unlike the Taylor-Green state's rho w)
+    i = len(members) - 2
+    cfg, m, _ = members[i]
+    o = Oracle(cfg, m)
+    o.set_state(U0s[i])
+    o.update(2)
+    assert rel_err(batched[i].get_state(), o.get_state()).max() <= TOL_FP64
+    o.close()
 
 
 def test_batch_vs_oracle_and_hex():
