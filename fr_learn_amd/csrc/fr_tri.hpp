@@ -32,7 +32,10 @@ struct TriGeo {
   static constexpr int P1 = P + 1, NS = P1 * (P + 2) / 2, NFPF = P1, NFACE = 3, NFP = 3 * P1;
   static constexpr int TPE = triTPEL(P, LAY), EPB = 256 / TPE, NTH = EPB * TPE;
   static constexpr int DR = 0, DS = NS * NS, E = 2 * NS * NS, L = 2 * NS * NS + NFP * NS;
-  static constexpr int TAB = 2 * NS * NS + 2 * NFP * NS;
+  static constexpr int TAB = 2 * NS * NS + 2 * NFP * NS;  // Dr | Ds | E | L
+  // past TAB: Dr - L tNorm_r E and Ds - L tNorm_s E (the jump folded into the
+  // divergence, fr_tri_ops.cpp), for the FP64 kernel
+  static constexpr int DRF = TAB, DSF = TAB + NS * NS;
   static_assert(NS <= TPE && NFP <= TPE, "one point per thread");
 };
 
@@ -305,12 +308,12 @@ struct TriSmemMF {
   static constexpr int SS = NS4 + 1, SQ = NQ4 + 1;     // data row strides
   static constexpr int NCP = (G::EPB * NF + 15) & ~15;  // columns
   double opE[triOpSize(NFP, NS)];  // extrapolation E (flux points x solution points)
-  double opDr[triOpSize(NS, NS)];  // Dr
-  double opDs[triOpSize(NS, NS)];  // Ds
+  double opDr[triOpSize(NS, NS)];  // Dr - L tNorm_r E (the jump folded in, TriGeo::DRF)
+  double opDs[triOpSize(NS, NS)];  // Ds - L tNorm_s E
   double opL[triOpSize(NS, NFP)];  // lifting L (solution points x flux points)
   double sU[NCP][SS];              // stage state of column (slot, field); later the divergence + correction
   double sF[2][NCP][SS];           // reference flux F~_r, F~_s; later the updated state in [0]
-  double sJ[NCP][SQ];              // traces E U, then Fn dA, then the jump; later the updated state's traces
+  double sJ[NCP][SQ];              // traces E U, then Fn dA; later the updated state's traces
 };
 
 // Row tiles per work item of a product with nrt tiles: a divisor, so that the
@@ -364,8 +367,10 @@ __device__ __forceinline__ void triProduct(const int wave, const int lane, Fin f
 }
 
 // One RK stage of element block b, FP64: the phases of triStageBody with the
-// operator products (E U, (E F~) . tNorm, [Dr | Ds | L] [F~_r; F~_s; J] and
-// E U') on the matrix cores over the block's (slot, field) columns, and the
+// operator products on the matrix cores over the block's (slot, field)
+// columns -- the traces E U, the divergence plus correction with the jump
+// folded in, [Dr - L tNorm_r E | Ds - L tNorm_s E | L] [F~_r; F~_s; Fn dA]
+// (no product for (E F~) . tNorm), and the new traces E U' -- and the
 // pointwise work (fluxes, common flux, update) one point per thread.
 template <int P, int NF, bool BND, bool FIN, int LAY>
 __device__ __forceinline__ void triStageMF(const StageArgs<double> &a, const int b, TriSmemMF<P, NF, LAY> &sm,
@@ -420,41 +425,8 @@ __device__ __forceinline__ void triStageMF(const StageArgs<double> &a, const int
     for (int k = 0; k < NF; k++) sm.sJ[slot * NF + k][t] = Fn[k];
   }
   __syncthreads();
-  // (3) the jump Fn dA - (E F~) . tNorm, tNorm the reference normal of the
-  // flux point's edge: E F~_r and E F~_s share the operand A
-  {
-    constexpr int NRT = triRowTiles(NFP), RPI = triRpi(NRT), NRG = NRT / RPI;
-    for (int item = wave; item < NCG * NRG; item += 4) {
-      const int cg = item / NRG, rt0 = (item - cg * NRG) * RPI;
-      double ar[RPI], as[RPI];
-#pragma unroll
-      for (int j = 0; j < RPI; j++) ar[j] = as[j] = 0.0;
-      const double *ap = sm.opE + (4 * rt0 + (lane & 3)) * NS + lk;
-      const double *br = &sm.sF[0][16 * cg + lc][lk], *bs = &sm.sF[1][16 * cg + lc][lk];
-#pragma unroll
-      for (int ks = 0; ks < NS4 / 4; ks++) {
-        const double vr = br[4 * ks], vs = bs[4 * ks];
-#pragma unroll
-        for (int j = 0; j < RPI; j++) {
-          const double av = ap[4 * j * NS + 4 * ks];
-          ar[j] = __builtin_amdgcn_mfma_f64_4x4x4f64(av, vr, ar[j], 0, 0, 0);
-          as[j] = __builtin_amdgcn_mfma_f64_4x4x4f64(av, vs, as[j], 0, 0, 0);
-        }
-      }
-#pragma unroll
-      for (int j = 0; j < RPI; j++) {
-        const int q = 4 * (rt0 + j) + lk;
-        if (q < NFP) {
-          T tx, ty;
-          triNormal<T>(q / NFPF, tx, ty);
-          double &J = sm.sJ[16 * cg + lc][q];
-          J = J - (tx * ar[j] + ty * as[j]);
-        }
-      }
-    }
-  }
-  __syncthreads();
-  // (4) divergence + correction [Dr | Ds | L] [F~_r; F~_s; J] into sU
+  // (3) divergence + correction with the jump folded in:
+  // [Dr - L tNorm_r E | Ds - L tNorm_s E | L] [F~_r; F~_s; Fn dA] into sU
   triProduct<triRowTiles(NS), NCG, 4>(
       wave, lane,
       [&](int cg, int rt, double v) {
@@ -463,7 +435,7 @@ __device__ __forceinline__ void triStageMF(const StageArgs<double> &a, const int
       TriSeg<NS, SS, NS4 / 4>{sm.opDr, &sm.sF[0][0][0]}, TriSeg<NS, SS, NS4 / 4>{sm.opDs, &sm.sF[1][0][0]},
       TriSeg<NFP, SQ, NQ4 / 4>{sm.opL, &sm.sJ[0][0]});
   __syncthreads();
-  // (5) stage update, the new state into sF[0]
+  // (4) stage update, the new state into sF[0]
   if (hasS) {
     const size_t eo = ec * NF * NS;
 #pragma unroll
@@ -472,7 +444,7 @@ __device__ __forceinline__ void triStageMF(const StageArgs<double> &a, const int
   }
   if (a.mode == 0) return;
   __syncthreads();
-  // (6) traces of the updated state E U'
+  // (5) traces of the updated state E U'
   triProduct<triRowTiles(NFP), NCG, 4>(
       wave, lane,
       [&](int cg, int rt, double v) {
@@ -496,8 +468,8 @@ __device__ __forceinline__ void triSetupMF(const double *tab, TriSmemMF<P, NF, L
   constexpr int NS = G::NS, NFP = G::NFP;
   for (int i = tid; i < triOpSize(NFP, NS); i += 256) sm.opE[i] = i < NFP * NS ? tab[G::E + i] : 0.0;
   for (int i = tid; i < triOpSize(NS, NS); i += 256) {
-    sm.opDr[i] = i < NS * NS ? tab[G::DR + i] : 0.0;
-    sm.opDs[i] = i < NS * NS ? tab[G::DS + i] : 0.0;
+    sm.opDr[i] = i < NS * NS ? tab[G::DRF + i] : 0.0;
+    sm.opDs[i] = i < NS * NS ? tab[G::DSF + i] : 0.0;
   }
   for (int i = tid; i < triOpSize(NS, NFP); i += 256) sm.opL[i] = i < NS * NFP ? tab[G::L + i] : 0.0;
   for (int i = tid; i < S::NCP * S::SS; i += 256) {

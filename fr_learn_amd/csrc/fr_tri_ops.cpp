@@ -296,12 +296,25 @@ std::string buildTriOperators(const fr_config &cfg, Operators &o) {
   fill(o.locFpts, nF, o.dshapeFpts);
   o.shapeSpts.assign((size_t)nS * 3, 0.0);
   for (int p = 0; p < nS; p++) shapeFunctions(2, &o.locSpts[3 * p], &o.shapeSpts[(size_t)p * 3], 3);
-  // kernel table: Dr | Ds | E | L
+  // kernel table: Dr | Ds | E | L | Dr - L tNorm_r E | Ds - L tNorm_s E.  The
+  // last two fold the jump into the divergence: with J = Fn dA - (tNorm_r E
+  // F~_r + tNorm_s E F~_s), Dr F~_r + Ds F~_s + L J equals
+  // (Dr - L tNorm_r E) F~_r + (Ds - L tNorm_s E) F~_s + L Fn dA, tNorm_d the
+  // diagonal of the flux points' reference normals (the FP64 kernel's form).
   o.oned.clear();
   o.oned.insert(o.oned.end(), Dr.begin(), Dr.end());
   o.oned.insert(o.oned.end(), Ds.begin(), Ds.end());
   o.oned.insert(o.oned.end(), E.begin(), E.end());
   o.oned.insert(o.oned.end(), L.begin(), L.end());
+  for (int d = 0; d < 2; d++) {
+    const std::vector<double> &D = d == 0 ? Dr : Ds;
+    for (int r = 0; r < nS; r++)
+      for (int k = 0; k < nS; k++) {
+        double acc = 0.0;
+        for (int q = 0; q < nF; q++) acc += L[(size_t)r * nF + q] * o.tNorm[3 * q + d] * E[(size_t)q * nS + k];
+        o.oned.push_back(D[(size_t)r * nS + k] - acc);
+      }
+  }
   return "";
 }
 
