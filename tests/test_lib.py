@@ -48,6 +48,25 @@ def test_validation_errors():
     _expect(-2, cases.config(nDims=3), m2)
 
 
+def test_batch_argument_checks():
+    """The batch calls' host-side checks (no device work): the tile switch
+    takes -1, 0 or 1; a null batch is an invalid argument for every batch
+    call."""
+    lib = _lib.load()
+    assert lib.fr_batch_force_tile(2) == -1
+    assert lib.fr_batch_force_tile(-2) == -1
+    assert lib.fr_batch_force_tile(1) == 0
+    assert lib.fr_batch_force_tile(-1) == 0
+    i = ctypes.c_int()
+    d = ctypes.c_double()
+    assert lib.fr_batch_group_tuning(None, 1, ctypes.byref(i), ctypes.byref(i), ctypes.byref(i),
+                                     ctypes.byref(i), ctypes.byref(d)) == -1
+    assert lib.fr_batch_tuning(None, ctypes.byref(i), ctypes.byref(i), ctypes.byref(d)) == -1
+    assert lib.fr_batch_update(None, 1) == -1
+    assert lib.fr_batch_create(None, 0, None) == -1
+    assert lib.fr_batch_destroy(None) == 0
+
+
 def test_device_error_without_gpu():
     """Valid input reaches the device stage; without a GPU that is FR_EHIP.
     The GPU is detected by the kernel driver's device node, not by a HIP
