@@ -405,7 +405,7 @@ std::string buildOperators(const fr_config &cfg, Operators &o, int nNodes) {
   // 1D factors for the kernels and a check that the kernels' index maps
   // (fr_layout.hpp) enumerate exactly the nonzeros of the dense operators
   {
-    const int L = 5 * P1 + P1 * P1;
+    const int L = 5 * P1 + 2 * P1 * P1;
     o.oned.assign(L, 0.0);
     for (int m = 0; m < P1; m++) {
       o.oned[4 * P1 + P1 * P1 + m] = x[m];
@@ -415,6 +415,11 @@ std::string buildOperators(const fr_config &cfg, Operators &o, int nNodes) {
       o.oned[3 * P1 + m] = dvcjh(x[m], 1, P, eta);
       for (int i = 0; i < P1; i++) o.oned[4 * P1 + i * P1 + m] = dlag(x, x[i], m);
     }
+    // folded derivative (OneD::DMF)
+    for (int i = 0; i < P1; i++)
+      for (int m = 0; m < P1; m++)
+        o.oned[5 * P1 + P1 * P1 + i * P1 + m] = o.oned[4 * P1 + i * P1 + m] + o.oned[2 * P1 + i] * o.oned[m] -
+                                                o.oned[3 * P1 + i] * o.oned[P1 + m];
     std::string err = verifyIndexMaps(o);
     if (!err.empty()) return err;
   }

@@ -26,6 +26,9 @@
 #ifndef FR_MFMA
 #define FR_MFMA 1  // hex P3 inviscid stages: the line contractions of F~ on the matrix cores
 #endif
+#ifndef FR_FOLD
+#define FR_FOLD 1  // inviscid stage variants: the jump correction folded into the derivative
+#endif
 #ifndef FR_INBLOCK
 #define FR_INBLOCK 1  // neighbours in the same workgroup: traces extrapolated from their LDS state (VALU variants)
 #endif
@@ -391,19 +394,22 @@ template <typename T>
 struct MfmaLane {
   T aD, aDT, aEzy, aEx;  // aDT: the transposed derivative matrix (FP64 layout)
   int sx, sy, px, py, bx, by, ix, iy;
-  __device__ __forceinline__ MfmaLane(const T *s1, int t) {
+  // FD: the derivative entries (aD, and the derivative rows of the FP32
+  // operands) from the folded derivative OneD::DMF
+  __device__ __forceinline__ MfmaLane(const T *s1, int t, bool FD = false) {
     using O = OneD<3>;
     const int k = t >> 4;
     const T lL = s1[O::LL + k], lR = s1[O::LR + k];
+    const int dm = FD ? O::DMF : O::DM;
     if constexpr (sizeof(T) == 8) {
       const int m = t & 3;
-      aD = s1[O::DM + m * 4 + k];
+      aD = s1[dm + m * 4 + k];
       aDT = s1[O::DM + k * 4 + m];
       aEzy = m == 0 ? lL : m == 1 ? lR : (T)0;
       aEx = m == 2 ? lL : m == 3 ? lR : (T)0;
     } else {
       const int m = t & 15;
-      const T d = (m & 3) == 0 ? s1[O::DM + (m >> 2) * 4 + k] : (T)0;
+      const T d = (m & 3) == 0 ? s1[dm + (m >> 2) * 4 + k] : (T)0;
       aD = d;
       aDT = (T)0;
       aEzy = m == 1 ? lL : m == 5 ? lR : d;
@@ -439,13 +445,14 @@ __device__ __forceinline__ double lanePull(int addr, double v) {
 // flux F~_d = sum_d2 JGinv(d,d2) F_d2, its derivative along d added to div,
 // and its values at the two faces normal to d, stored negated as the start
 // of the jump (the left face's outward flux is -F~_d): sJ at the round-0
-// and round-1 flux points of the lane.
-template <int NF, typename T, typename SM>
+// and round-1 flux points of the lane.  FD: the folded derivative
+// (OneD::DMF) alone, no face values (the jump is the common flux alone).
+template <int NF, typename T, bool FD = false, typename SM>
 __device__ __forceinline__ void mfmaAxes(SM &sm, const int slot, const int t, const T *pJ, const T (&F)[3][NF],
                                          T (&div)[NF], T *dbgF) {
   using V = MfmaVec<T>;
   auto &sJ = sm.sJ;
-  const MfmaLane<T> L(sm.s1, t);
+  const MfmaLane<T> L(sm.s1, t, FD);
   T e0[NF], e1[NF];
 #pragma unroll
   for (int k = 0; k < NF; k++) e0[k] = e1[k] = (T)0;
@@ -459,17 +466,20 @@ __device__ __forceinline__ void mfmaAxes(SM &sm, const int slot, const int t, co
       if (dbgF) dbgF[(d * NF + k) * 64 + t] = ft;
       const T b = d < 2 ? lanePull(d == 0 ? L.bx : L.by, ft) : ft;
       T der;
-      V::axis(L.aD, d == 0 ? L.aEx : L.aEzy, b, d == 1 ? e1[k] : e0[k], der);
+      if constexpr (FD) der = V::der(L.aD, d == 0 ? L.aEx : L.aEzy, b);
+      else V::axis(L.aD, d == 0 ? L.aEx : L.aEzy, b, d == 1 ? e1[k] : e0[k], der);
       const T dv = d < 2 ? lanePull(d == 0 ? L.ix : L.iy, der) : der;
       div[k] = d == 0 ? dv : dv + div[k];
     }
   }
-  const T sg = ((t >> 4) & 1) ? (T)-1 : (T)1;
-  const int q0 = mfmaFpt(t, 0), q1 = mfmaFpt(t, 1);
+  if constexpr (!FD) {
+    const T sg = ((t >> 4) & 1) ? (T)-1 : (T)1;
+    const int q0 = mfmaFpt(t, 0), q1 = mfmaFpt(t, 1);
 #pragma unroll
-  for (int k = 0; k < NF; k++) {
-    sJ[slot][k][q0] = sg * e0[k];
-    if (t < 32) sJ[slot][k][q1] = sg * e1[k];
+    for (int k = 0; k < NF; k++) {
+      sJ[slot][k][q0] = sg * e0[k];
+      if (t < 32) sJ[slot][k][q1] = sg * e1[k];
+    }
   }
   elementSync<64>();
 }
@@ -882,6 +892,11 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
   // the face values of the state in (3) and (5) too: FP64 only (three FP32
   // 16x16x4 products per field cost more than the FP32 line sums they replace)
   constexpr bool MFF = MF && sizeof(T) == 8;
+  // (2) with the correction of the extrapolated flux folded into the
+  // derivative (OneD::DMF): no face values of F~, and the jump of (3) is the
+  // common flux alone (the inviscid variants; the viscous one keeps the LDG
+  // jumps of (0) in sJ, and the debug instance the jump it reports)
+  constexpr bool FOLD = FR_FOLD && !VIS && !DBG;
   const int slot = tid / TPE, t = tid - slot * TPE;
   const int tP = padIdx<P>(t);  // padded LDS index of this thread's solution point
   // flux point of this thread in round r of the face phases: the matrix-core
@@ -1120,7 +1135,7 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
   // along d (calcDivF_spts) and its extrapolation to the two faces normal to d
   // (extrapolateNormalFlux), stored negated as the start of the jump
   if constexpr (MF) {
-    if (act) mfmaAxes<NF, T>(sm, slot, t, pJ, F, div, DBG ? a.dbgF + (size_t)e * D * NF * NS : nullptr);
+    if (act) mfmaAxes<NF, T, FOLD>(sm, slot, t, pJ, F, div, DBG ? a.dbgF + (size_t)e * D * NF * NS : nullptr);
   } else {
 #pragma unroll
     for (int d = 0; d < D; d++) {
@@ -1146,12 +1161,12 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
             const int sp = padIdx<P>((d == 0) ? m + P1 * (sj + P1 * sk)
                                      : (d == 1) ? si + P1 * (m + P1 * sk)
                                                 : si + P1 * (sj + P1 * m));
-            acc += s1[O::DM + id * P1 + m] * sFd[slot][k][sp];
+            acc += s1[(FOLD ? O::DMF : O::DM) + id * P1 + m] * sFd[slot][k][sp];
           }
           div[k] = (d == 0) ? acc : acc + div[k];
         }
       }
-      if (act && t < 2 * NFPF) {
+      if (!FOLD && act && t < 2 * NFPF) {
         const int q = dirFace<D>(d, t / NFPF) * NFPF + t % NFPF;
         int base, stride, side, dn;
         fptLine<D, P>(q, base, stride, side, dn);
@@ -1276,7 +1291,7 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
       }
       const T scale = right ? -dA : dA;
 #pragma unroll
-      for (int k = 0; k < NF; k++) sJ[slot][k][q] = Fn[k] * scale + sJ[slot][k][q];
+      for (int k = 0; k < NF; k++) sJ[slot][k][q] = FOLD ? Fn[k] * scale : Fn[k] * scale + sJ[slot][k][q];
       if constexpr (DBG)
 #pragma unroll
         for (int k = 0; k < NF; k++) {

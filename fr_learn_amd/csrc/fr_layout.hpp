@@ -178,7 +178,12 @@ template <int P>
 struct OneD {
   static constexpr int P1 = P + 1;
   static constexpr int LL = 0, LR = P1, GL = 2 * P1, GR = 3 * P1, DM = 4 * P1, X1 = 4 * P1 + P1 * P1;
-  static constexpr int SIZE = 5 * P1 + P1 * P1;
+  // the derivative with the correction of the extrapolated flux folded in,
+  // DMF[i][m] = DM[i][m] + GL[i] LL[m] - GR[i] LR[m]: along a line, DM F~ plus
+  // the correction of the jump (common flux less the extrapolated outward
+  // flux) is DMF F~ plus the correction of the common flux alone
+  static constexpr int DMF = 5 * P1 + P1 * P1;
+  static constexpr int SIZE = 5 * P1 + 2 * P1 * P1;
 };
 
 }  // namespace frdev
