@@ -27,7 +27,7 @@
 #define FR_MFMA 1  // hex P3 inviscid stages: the line contractions of F~ on the matrix cores
 #endif
 #ifndef FR_FOLD
-#define FR_FOLD 1  // inviscid stage variants: the jump correction folded into the derivative
+#define FR_FOLD 1  // stage kernels: the jump correction folded into the derivative
 #endif
 #ifndef FR_INBLOCK
 #define FR_INBLOCK 1  // neighbours in the same workgroup: traces extrapolated from their LDS state (VALU variants)
@@ -894,9 +894,10 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
   constexpr bool MFF = MF && sizeof(T) == 8;
   // (2) with the correction of the extrapolated flux folded into the
   // derivative (OneD::DMF): no face values of F~, and the jump of (3) is the
-  // common flux alone (the inviscid variants; the viscous one keeps the LDG
-  // jumps of (0) in sJ, and the debug instance the jump it reports)
-  constexpr bool FOLD = FR_FOLD && !VIS && !DBG;
+  // common flux alone (the viscous variant's LDG jumps of (0), held in sJ, are
+  // read in (1), before (2)'s barriers; the debug instance keeps the jump it
+  // reports)
+  constexpr bool FOLD = FR_FOLD && !DBG;
   const int slot = tid / TPE, t = tid - slot * TPE;
   const int tP = padIdx<P>(t);  // padded LDS index of this thread's solution point
   // flux point of this thread in round r of the face phases: the matrix-core

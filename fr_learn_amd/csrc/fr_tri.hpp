@@ -166,7 +166,9 @@ struct TriPoint {
 };
 
 // One RK stage of element block b (EPB elements) with the arguments a, the
-// operator products as each thread's line sums (FP32).
+// operator products as each thread's line sums (FP32).  With FR_FOLD the
+// divergence operators are the folded ones (TriGeo::DRF, DSF, copied into the
+// table's Dr and Ds places) and the jump is the common flux alone.
 template <int P, int NF, typename T, bool BND, bool FIN, int LAY = 0>
 __device__ __forceinline__ void triStageBody(const StageArgs<T> &a, const int b, TriSmem<P, NF, T, LAY> &sm,
                                              const int tid) {
@@ -179,7 +181,7 @@ __device__ __forceinline__ void triStageBody(const StageArgs<T> &a, const int b,
   const size_t ec = (size_t)(act ? e : a.nEles - 1);
   const int tc = t < NS ? t : NS - 1, qc = t < NFP ? t : NFP - 1;
 
-  for (int i = tid; i < G::TAB; i += 256) sm.tab[i] = a.oned[i];
+  for (int i = tid; i < G::TAB; i += 256) sm.tab[i] = a.oned[(FR_FOLD && i < 2 * NS * NS) ? G::DRF + i : i];
   if (act)
     for (int i = t; i < NF * NS; i += TPE) (&sm.sU[slot][0][0])[i] = a.Uin[ec * NF * NS + i];
   const TriPoint<P, NF, T, BND, FIN> pt(a, ec, tc, qc);
@@ -212,8 +214,12 @@ __device__ __forceinline__ void triStageBody(const StageArgs<T> &a, const int b,
     pt.normalFlux(a.ph, uo, Fn);
   }
   elementSync<TPE>();
-  // (2) jump at the flux point: Fn dA - (E F~) . tNorm
-  if (hasQ) {
+  // (2) jump at the flux point: Fn dA - (E F~) . tNorm, or Fn dA alone with
+  // the folded divergence operators
+  if (FR_FOLD && hasQ) {
+#pragma unroll
+    for (int k = 0; k < NF; k++) sm.sJ[slot][k][t] = Fn[k];
+  } else if (hasQ) {
     T tx, ty;
     triNormal<T>(t / NFPF, tx, ty);
     T fr[NF], fs[NF];
