@@ -907,14 +907,15 @@ __device__ __forceinline__ void stageBody(const StageArgs<T> &a, const int b, St
   const bool act = e < a.nEles;
   const bool hasS = act && t < NS;
   // Faces between two elements of this block take the neighbour's trace from
-  // its state in LDS, and their traces are not written to HBM (the element
-  // order groups face neighbours into blocks).  The viscous variant keeps the
-  // full trace exchange, which k_grad reads, and so does the matrix-core
-  // variant, whose face values come out of the products at the lanes' own
-  // flux points (the LDS line sums for the neighbour's points cost more there
-  // than the trace reads they save).
+  // its state in LDS (line sums), and their traces are not written to HBM (the
+  // element order groups face neighbours into blocks).  The viscous variant
+  // keeps the full trace exchange, which k_grad reads, and so does the FP32
+  // matrix-core variant (hex P3 FP32 31.97 against 30.38 GDOF/s with it; the
+  // FP64 one gains, 20.47 -> 21.02, since its step is bound by its bytes).
   const int blockBase = b * EPB;
-  auto inBlock = [&](int en) { return FR_INBLOCK && !MF && !VIS && (unsigned)(en - blockBase) < (unsigned)EPB; };
+  auto inBlock = [&](int en) {
+    return FR_INBLOCK && (!MF || MFF) && !VIS && (unsigned)(en - blockBase) < (unsigned)EPB;
+  };
   // Loads.  Vector-memory counters retire in issue order, so the face records
   // (which the neighbour-trace addresses depend on) go first, the state (or
   // the residual it is formed from) and the base state, detJ and dt (needed
