@@ -77,6 +77,9 @@ struct TriPoint {
   int fi;
   bool bnd;
   T n[2], dA, un[NF];
+  // final stage: the earlier stages' residuals at tc, issued with the other
+  // loads so that the update at the end of the stage does not wait on them
+  T pd[FIN ? 3 : 1][NF];
   __device__ __forceinline__ TriPoint(const StageArgs<T> &a, const size_t ec, const int tc, const int qc) {
     using G = TriGeo<P>;
     constexpr int NS = G::NS, NFP = G::NFP, NFPF = G::NFPF, NFACE = G::NFACE;
@@ -88,6 +91,14 @@ struct TriPoint {
 #pragma unroll
     for (int k = 0; k < NF; k++)
       pb[k] = FIN ? a.Ubase[(ec * NF + k) * NS + tc] : (double)a.UbaseT[(ec * NF + k) * NS + tc];
+    if constexpr (FIN) {
+      const T *dp[3] = {a.dprev0, a.dprev1, a.dprev2};
+#pragma unroll
+      for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int k = 0; k < NF; k++)
+          pd[r][k] = (a.mode == 2 && r < a.nprev) ? dp[r][(ec * NF + k) * NS + tc] : (T)0;
+    }
     fi = a.finfo[ec * NFACE + qc / NFPF];
     const int nbe = a.fnbr[ec * NFACE + qc / NFPF];
     bnd = BND && fi >= FINFO_BOUNDARY;
@@ -144,9 +155,10 @@ struct TriPoint {
     const double dtJ = pdt / pdet;
     double u = pb[k];
     if (a.mode == 2) {
-      const T r0 = (a.nprev > 0) ? a.dprev0[gi] : (T)0;
-      const T r1 = (a.nprev > 1) ? a.dprev1[gi] : (T)0;
-      const T r2 = (a.nprev > 2) ? a.dprev2[gi] : (T)0;
+      // gi is this thread's point tc of field k (the prefetched residuals')
+      const T r0 = FIN ? pd[0][k] : (a.nprev > 0) ? a.dprev0[gi] : (T)0;
+      const T r1 = FIN ? pd[FIN ? 1 : 0][k] : (a.nprev > 1) ? a.dprev1[gi] : (T)0;
+      const T r2 = FIN ? pd[FIN ? 2 : 0][k] : (a.nprev > 2) ? a.dprev2[gi] : (T)0;
       if (a.nprev > 0) u -= a.c0 * (double)r0 * dtJ;
       if (a.nprev > 1) u -= a.c1 * (double)r1 * dtJ;
       if (a.nprev > 2) u -= a.c2 * (double)r2 * dtJ;
