@@ -1356,7 +1356,7 @@ int fr_reduce_gradient(fr_ctx *c, int kind, double *out) {
 }
 
 #ifndef FR_BATCH_STREAMS
-#define FR_BATCH_STREAMS 4  // graph branches of a batched step (hardware queues per process by default)
+#define FR_BATCH_STREAMS 8  // graph branches of a batched step (twice the hardware queues a process has by default)
 #endif
 
 namespace {
@@ -1486,10 +1486,10 @@ int fr_batch_create(fr_ctx *const *ctxs, int n, fr_batch **out) {
   // One step's launches, captured per trace parity.  Meshes of different
   // element types and orders are independent, so the groups' stages form
   // FR_BATCH_STREAMS branches of the graph (forked from and joined to the
-  // batch stream) that run concurrently, as many as the device has hardware
-  // queues by default.  The groups go to the branches longest first, each to
-  // the branch with the least tuned time so far, so that the branches end
-  // together; a branch runs its groups one after the other.
+  // batch stream) that run concurrently (the runtime spreads the branches'
+  // streams over its hardware queues).  The groups go to the branches longest
+  // first, each to the branch with the least tuned time so far, so that the
+  // branches end together; a branch runs its groups one after the other.
   std::vector<int> order;
   for (int grp = 1; grp <= 15; grp++)
     if (b->itemCount[0][grp]) order.push_back(grp);
