@@ -102,11 +102,13 @@ def test_advection_residual(P):
     assert rel_err(g.residual(0), tri.residual(U, cfg, ops, JG, nrm, dA, faces)).max() <= TOL_FP64
 
 
+@pytest.mark.parametrize("P", [1, 3, 5])
 @pytest.mark.parametrize("timeType", [4, 5])
-def test_rk_steps(timeType):
+def test_rk_steps(timeType, P):
     """Classical RK4 and a Jameson scheme (input.cpp:424-461) through the
-    fused stage kernel against the checker's stage-by-stage update."""
-    cfg = euler_cfg(2, 3, timeType=timeType, dt=2e-3)
+    fused stage kernel against the checker's stage-by-stage update (the final
+    RK4 stage combines the three earlier residuals it loads up front)."""
+    cfg = euler_cfg(2, P, timeType=timeType, dt=2e-3)
     m = _mesh(4)
     g = Solver(cfg, m)
     U = _state(g, cfg)
